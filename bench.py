@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the fused LBM time step on one MI355X.
+
+Default workload: D2Q9 BGK channel with a cylinder (Karman street), 1024x256 nodes, FP32,
+cylinder r = 20 at (256,128), parabolic Zou-He inlet u_max = 0.1, reference-pressure outlet,
+no-slip walls, Re = 150, 1e-4 seeded noise (seed 0x5EED0002).  A "step" is one lattice time
+step over the whole lattice; MLUPS counts every node (solid included), 1e6 lattice updates/s.
+
+Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes of the fused kernel
+(2 * Q * bytes-per-population per node update) over the per-launch device time measured with
+HIP events on the lattice's stream; ``cpu_baseline`` times the CPU oracle (a restatement of the
+reference step, OpenMP) on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "lbm-arrayfire_amd")]
+
+METRIC = "MLUPS & % HBM roofline; batched jobs/s + p99 at ≤1e-3 rel-L2 vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def workloads():
+    from lbm_amd import cases
+    from lbm_amd.capi import BF16S, F16S, F32, MRT_GS
+
+    W = {
+        "d2q9_bgk_karman_1024x256_fp32": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
+        "d2q9_bgk_karman_1024x256_f16s": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F16S),
+        "d3q27_mrt_gs_lid_256_fp32": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32),
+        "d3q27_mrt_gs_lid_256_f16s": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S),
+        "d3q27_bgk_sphere_512x256x256_fp32": lambda: cases.d3q27_sphere_channel(storage=F32),
+        "d3q27_bgk_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S),
+        "d3q27_bgk_sphere_512x256x256_bf16s": lambda: cases.d3q27_sphere_channel(storage=BF16S),
+        "d3q27_mrt_gs_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S, collision=MRT_GS),
+    }
+    return W
+
+
+def measure(name, steps, warmup):
+    import numpy as np  # noqa: F401
+    from lbm_amd import Lattice
+    from lbm_amd.capi import STORAGE_BYTES
+
+    cfg, types = workloads()[name]()
+    with Lattice(cfg, types) as lat:
+        u0 = (cfg.get("u_in", 0.0), 0.0, 0.0)
+        lat.init_equilibrium(1.0, u0, 0x5EED0002, 1e-4)
+        lat.step(warmup)
+        lat.sync()
+        t0 = time.perf_counter()
+        ms = lat.step_timed(steps)
+        wall = time.perf_counter() - t0
+        _, _, bad = lat.monitors()
+    N = lat.N
+    Q = cfg["lattice"]
+    bytes_per_launch = 2 * Q * STORAGE_BYTES[cfg["storage"]] * N
+    per_launch_s = ms * 1e-3 / steps
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    return dict(cfg=cfg, N=N, Q=Q, steps=steps, ms=ms, wall_s=wall, mlups=N * steps / (ms * 1e3),
+                ms_per_step=ms / steps, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
+                nonfinite=int(bad))
+
+
+def traffic_for(name):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get(name, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(name, target_s=10.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    cfg, types = workloads()[name]()
+    types = types.reshape(-1)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    t = orc.time_steps(cfg, types, 5, threads=threads)
+    n = max(5, int(target_s / max(t / 5, 1e-6)))
+    t = orc.time_steps(cfg, types, n, threads=threads)
+    N = types.size
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": N * n / (t * 1e6), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{name}: {n} steps of the CPU oracle (restated reference step, same boundary "
+                      f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", default="d2q9_bgk_karman_1024x256_fp32")
+    ap.add_argument("--also", default="default",
+                    help="comma list of extra workloads timed briefly, 'none', or 'default'")
+    ap.add_argument("--also-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "
+              "multi-GPU path in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    r = measure(args.workload, args.steps, args.warmup)
+    also = {}
+    names = [] if args.also == "none" else (
+        [k for k in workloads() if k != args.workload] if args.also == "default" else args.also.split(","))
+    for k in names:
+        try:
+            m = measure(k, args.also_steps, 20)
+            also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
+                       "achieved_gbs": round(m["achieved_gbs"], 1),
+                       "frac_hbm": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4), "steps": m["steps"],
+                       "nonfinite": m["nonfinite"]}
+        except Exception as e:  # noqa: BLE001 — report, keep the headline line
+            also[k] = {"error": str(e)}
+    cpu = None if args.no_cpu else cpu_baseline(args.workload)
+    cfg = r["cfg"]
+    out = {
+        "metric": METRIC,
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if cfg["storage"] in (1, 2, 3) else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "nx": cfg["nx"], "ny": cfg["ny"], "nz": cfg.get("nz", 1),
+                   "lattice": f"D{2 if cfg['lattice'] == 9 else 3}Q{cfg['lattice']}",
+                   "collision": ["BGK", "MRT_RAW", "MRT_GS"][cfg.get("collision", 0)],
+                   "storage": ["F64", "F32", "F16S", "BF16S"][cfg["storage"]],
+                   "bc_mode": ["REFERENCE", "PHYSICAL"][cfg.get("bc_mode", 0)],
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_for(args.workload),
+                     "algorithmic_bytes_per_launch": r["bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "wall_s_timed": round(r["wall_s"], 4),
+        "nonfinite": r["nonfinite"],
+        "also": also,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

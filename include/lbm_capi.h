@@ -1,0 +1,172 @@
+/*
+ * lbm_capi.h — C ABI of the MI355X lattice Boltzmann time-step library (liblbm_amd.so).
+ *
+ * The reference (michaltakac/lbm-arrayfire) has no library boundary: each example program runs
+ * its time loop inline against ArrayFire's array API (e.g. cpp/d2q9_bgk_lid.cpp:121-180,
+ * rust/examples/d3q27_mrt_lid.rs:302-444).  Each entry point below replaces one group of those
+ * ArrayFire calls; the replaced call sites are cited per declaration.  The calling convention
+ * mirrors ArrayFire's own C API (af_err status codes + a last-error string, asynchronous work on
+ * one device queue fenced by an explicit sync).
+ *
+ * Conventions
+ *  - Plain C types only.  The caller owns every host buffer; the library owns device memory.
+ *  - Layout is the reference's column-major, x fastest: node n = x + nx*(y + ny*z); population i
+ *    of node n lives at n + N*i (N = nx*ny*nz), cpp/d2q9_bgk_lid.cpp:86.
+ *  - Every call returns lbm_status; on failure lbm_last_error() holds a thread-local message.
+ *    No C++ exception crosses the ABI.
+ *  - One handle is bound to one device and one stream.  Calls on a handle must be serialised
+ *    by the caller; distinct handles may be driven from distinct threads.
+ */
+#ifndef LBM_CAPI_H
+#define LBM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+typedef struct lbm_ctx *lbm_handle;
+
+typedef enum {
+  LBM_OK = 0,
+  LBM_EINVAL = 1,     /* bad argument or descriptor */
+  LBM_ENOMEM = 2,     /* device allocation failed / over budget */
+  LBM_EDEVICE = 3,    /* HIP runtime error or no device */
+  LBM_EUNSTABLE = 4,  /* non-finite populations detected by the monitors */
+  LBM_EINTERNAL = 5,
+  LBM_EAGAIN = 6      /* server: job does not fit now, retry later */
+} lbm_status;
+
+/* Velocity sets (rows of the reference tables cpp/d2q9_bgk_lid.cpp:72-79 and
+ * rust/examples/d3q27_mrt_lid.rs:98-104). */
+typedef enum { LBM_D2Q9 = 9, LBM_D3Q27 = 27 } lbm_lattice;
+
+/* Collision operators. BGK = cpp/d2q9_bgk_lid.cpp:149-153.  MRT_RAW = the reference's raw
+ * monomial moment matrix (rust/examples/d3q27_mrt_lid.rs:167-199).  MRT_GS = the same rows
+ * Gram-Schmidt orthogonalised (not in the reference). */
+typedef enum { LBM_BGK = 0, LBM_MRT_RAW = 1, LBM_MRT_GS = 2 } lbm_collision;
+
+/* Population storage in HBM.  F16S/BF16S store the shifted value f - w_i with FP32 collision
+ * arithmetic; F64 computes in FP64, all others in FP32. */
+typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_storage;
+
+/* Boundary semantics.  REFERENCE reproduces the reference: every axis periodic, full-way
+ * bounce-back at solid nodes, Dirichlet overrides of rho/u before the equilibrium
+ * (cpp/d2q9_bgk_lid.cpp:94-98,141-155; rust/examples/d2q9_bgk_channel.rs:190-201).
+ * PHYSICAL: half-way (and moving-wall) bounce-back at links into solid nodes, non-equilibrium
+ * bounce-back (Zou-He) velocity inlet, reference-pressure outlet with zero-gradient tangential
+ * velocity (no reference counterpart). */
+typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
+
+/* How the MRT moment transforms are evaluated. */
+typedef enum {
+  LBM_MRT_AUTO = 0,         /* MFMA_F32 for FP32-arithmetic storages, SCALAR for F64 */
+  LBM_MRT_SCALAR = 1,       /* per-node unrolled FMA code with compile-time matrices */
+  LBM_MRT_MFMA_F32 = 2,     /* 27x27 transforms on v_mfma_f32_16x16x4_f32 over 64-node tiles */
+  LBM_MRT_MFMA_BF16X3 = 3   /* transforms on bf16 MFMA, population operand split in 3 */
+} lbm_mrt_engine;
+
+/* Equilibrium moments for MRT_RAW: REFERENCE = the reference's hand-written table
+ * (rust/examples/d3q27_mrt_lid.rs:357-383, row 26 = |u|^2 + rho/27); PROJECTED = M * f_eq.
+ * MRT_GS always uses PROJECTED. */
+typedef enum { LBM_MEQ_REFERENCE = 0, LBM_MEQ_PROJECTED = 1 } lbm_meq_mode;
+
+typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profile;
+
+/* Node-type byte, one per node (the reference's BOUND mask, cpp/d2q9_bgk_lid.cpp:90-94,
+ * generalised).  Bits combine. */
+#define LBM_NODE_FLUID 0x00u
+#define LBM_NODE_SOLID 0x01u   /* REFERENCE: full-way bounce-back here, rho=u=0 override.
+                                  PHYSICAL: wall; links into it bounce back half-way. */
+#define LBM_NODE_MOVING 0x02u  /* REFERENCE: u := u_wall override (lid, :141-142).
+                                  PHYSICAL (with SOLID): the wall moves with u_wall. */
+#define LBM_NODE_INLET 0x04u   /* REFERENCE: ux := inlet velocity override (channel :190).
+                                  PHYSICAL: Zou-He velocity inlet on the x = 0 face. */
+#define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: outlet on the x = nx-1 face at density rho_bc, with
+                                  zero normal gradient of the tangential velocity. */
+#define LBM_NODE_RHO_FIX 0x10u /* REFERENCE: rho := rho_bc after the solid override
+                                  (channel set_row, rust/examples/d2q9_bgk_channel.rs:197-201). */
+
+typedef struct {
+  int32_t lattice;        /* lbm_lattice: 9 or 27 */
+  int32_t nx, ny, nz;     /* nz must be 1 for D2Q9 */
+  int32_t collision;      /* lbm_collision */
+  int32_t storage;        /* lbm_storage */
+  int32_t bc_mode;        /* lbm_bc_mode */
+  int32_t mrt_engine;     /* lbm_mrt_engine */
+  int32_t meq_mode;       /* lbm_meq_mode */
+  int32_t inlet_profile;  /* lbm_inlet_profile */
+  double omega;           /* BGK relaxation rate 1/tau */
+  double s[27];           /* MRT rates, one per moment row */
+  double u_wall[3];       /* lid / moving-wall velocity */
+  double u_in;            /* inlet velocity (peak value for PARABOLIC) */
+  double rho_bc;          /* density imposed by RHO_FIX and used for moving walls (1.0) */
+} lbm_desc;
+
+/* ---- device / library ------------------------------------------------------------------- */
+int lbm_abi_version(void);
+/* Selects the device later lbm_create calls bind to.  Replaces af::setDevice(argv[1])
+ * (cpp/d2q9_bgk_lid.cpp:211-214). */
+lbm_status lbm_set_device(int device);
+/* Thread-local message of the last failed call ("" if none). */
+const char *lbm_last_error(void);
+/* Device bytes a lattice with this descriptor holds (both population buffers, node types,
+ * monitor scratch).  Used for admission by HBM footprint. */
+size_t lbm_footprint_bytes(const lbm_desc *d);
+
+/* ---- one lattice ------------------------------------------------------------------------ */
+/* Allocates the lattice and uploads the node-type mask.  Replaces the setup of
+ * cpp/d2q9_bgk_lid.cpp:81-98 (index maps, BOUND, ON, TO_REFLECT/REFLECTED), which the fused
+ * kernel does not need.  node_type: N bytes, x fastest. */
+lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *out);
+lbm_status lbm_destroy(lbm_handle h);
+
+/* F = f_eq(rho, u) at every node.  Any field pointer may be NULL (rho -> 1, u -> 0).  Replaces
+ * cpp/d2q9_bgk_lid.cpp:100-107 and the channel variant cpp/d2q9_bgk_channel.cpp:107-118. */
+lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, const double *uy,
+                           const double *uz);
+/* F = f_eq(rho0, u0 + noise).  SplitMix64(seed) draws D numbers per node for every node in node
+ * order (x component, then y, then z); draw r maps to noise_amp * (2 * (r >> 11) * 2^-53 - 1),
+ * uniform in [-noise_amp, noise_amp), and is added only at non-solid nodes.  No reference
+ * counterpart (the reference starts noise-free). */
+lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], uint64_t seed,
+                                double noise_amp);
+/* MRT only: f = rho0/27 everywhere followed by one moment-space collision with rho = rho0,
+ * u = 0 and no bounce-back (rust/examples/d3q27_mrt_lid.rs:228-287).  BGK: f = f_eq(rho0, 0). */
+lbm_status lbm_init_reference(lbm_handle h, double rho0);
+
+/* Enqueues nsteps fused stream+collide steps on the handle's stream and returns.  Replaces the
+ * loop body cpp/d2q9_bgk_lid.cpp:124-155 / rust/examples/d3q27_mrt_lid.rs:304-389. */
+lbm_status lbm_step(lbm_handle h, int64_t nsteps);
+/* As lbm_step, then waits; *ms = device time between HIP events recorded on the handle's
+ * stream around the launches. */
+lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms);
+/* Waits for the handle's stream.  Replaces af::sync() (cpp/d2q9_bgk_lid.cpp:118,182). */
+lbm_status lbm_sync(lbm_handle h);
+int64_t lbm_steps_done(lbm_handle h);
+
+/* Macroscopic fields the reference holds after a step (DENSITY/UX/UY[/UZ] of
+ * cpp/d2q9_bgk_lid.cpp:131-146): moments of the pulled populations with the boundary overrides
+ * applied (REFERENCE) or of the post-boundary populations (PHYSICAL).  N doubles each; any
+ * pointer may be NULL.  Blocks. */
+lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, double *uz);
+/* Current post-collision populations as doubles (decoded for shifted storage), N*Q.  Blocks. */
+lbm_status lbm_get_populations(lbm_handle h, double *f);
+lbm_status lbm_set_populations(lbm_handle h, const double *f);
+/* Raw storage-format bytes of the current populations (bitwise checkpoint / restore). */
+size_t lbm_state_bytes(lbm_handle h);
+lbm_status lbm_get_state(lbm_handle h, void *dst);
+lbm_status lbm_set_state(lbm_handle h, const void *src);
+/* Wavefront reductions over the current populations with FP64 accumulation:
+ * mass = sum_n sum_i f, kinetic_energy = sum_n |sum_i c_i f|^2 / (2 sum_i f) over non-solid
+ * nodes, nonfinite = count of non-finite populations.  Any pointer may be NULL.  Blocks. */
+lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_CAPI_H */

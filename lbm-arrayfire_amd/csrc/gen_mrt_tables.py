@@ -1,0 +1,118 @@
+#!/usr/bin/env python3
+"""Generates mrt_tables.h: exact D3Q27 moment matrices as integer tables.
+
+* MRAW  — the raw monomial moment matrix, rows in the order of
+  rust/examples/d3q27_mrt_lid.rs:167-193 (row 0 = 1, 1-3 = c, 4-6 = products, 7 = |c|^2, ...).
+* MRAW_INV24 — 24 * MRAW^-1, which is an integer matrix (MRAW^-1 = MRAW_INV24 / 24).
+* MGS  — the same rows Gram-Schmidt orthogonalised in row order (unweighted inner product over
+  the 27 velocities), each row scaled to primitive integers.  MGS_NORM2[k] = |row k|^2, so
+  MGS^-1 = MGS^T diag(1 / MGS_NORM2).
+
+Exact rational arithmetic (fractions.Fraction); run from this directory:
+    python3 gen_mrt_tables.py > mrt_tables.h
+"""
+from fractions import Fraction
+from math import gcd
+import sys
+
+# D3Q27 velocities in the reference's order (rust/examples/d3q27_mrt_lid.rs:98-100).
+CX = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1]
+CY = [0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1]
+CZ = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1]
+
+
+def raw_rows():
+    rows = []
+    fns = [
+        lambda x, y, z: 1, lambda x, y, z: x, lambda x, y, z: y, lambda x, y, z: z,
+        lambda x, y, z: x * y, lambda x, y, z: x * z, lambda x, y, z: y * z,
+        lambda x, y, z: x * x + y * y + z * z, lambda x, y, z: x * x - y * y,
+        lambda x, y, z: x * x - z * z, lambda x, y, z: x * y * y, lambda x, y, z: x * z * z,
+        lambda x, y, z: y * x * x, lambda x, y, z: z * x * x, lambda x, y, z: y * z * z,
+        lambda x, y, z: z * y * y, lambda x, y, z: x * y * z, lambda x, y, z: x * x * y * y,
+        lambda x, y, z: x * x * z * z, lambda x, y, z: y * y * z * z,
+        lambda x, y, z: x * x * y * z, lambda x, y, z: x * y * y * z,
+        lambda x, y, z: x * y * z * z, lambda x, y, z: x * y * y * z * z,
+        lambda x, y, z: x * x * y * z * z, lambda x, y, z: x * x * y * y * z,
+        lambda x, y, z: x * x * y * y * z * z,
+    ]
+    for f in fns:
+        rows.append([Fraction(f(CX[i], CY[i], CZ[i])) for i in range(27)])
+    return rows
+
+
+def inverse(A):
+    n = len(A)
+    a = [list(A[r]) + [Fraction(int(r == c)) for c in range(n)] for r in range(n)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if a[r][c] != 0)
+        a[c], a[p] = a[p], a[c]
+        piv = a[c][c]
+        a[c] = [v / piv for v in a[c]]
+        for r in range(n):
+            if r != c and a[r][c] != 0:
+                f = a[r][c]
+                a[r] = [vr - f * vc for vr, vc in zip(a[r], a[c])]
+    return [row[n:] for row in a]
+
+
+def gram_schmidt(rows):
+    out = []
+    for r in rows:
+        v = list(r)
+        for q in out:
+            num = sum(a * b for a, b in zip(r, q))
+            den = sum(b * b for b in q)
+            c = num / den
+            v = [a - c * b for a, b in zip(v, q)]
+        out.append(v)
+    return out
+
+
+def primitive(row):
+    den = 1
+    for v in row:
+        den = den * v.denominator // gcd(den, v.denominator)
+    ints = [int(v * den) for v in row]
+    g = 0
+    for v in ints:
+        g = gcd(g, abs(v))
+    g = g or 1
+    ints = [v // g for v in ints]
+    # sign convention: first non-zero entry in the same direction as the raw row
+    return ints
+
+
+def fmt(name, ctype, mat):
+    lines = [f"constexpr {ctype} {name}[27][27] = {{"]
+    for row in mat:
+        lines.append("    {" + ", ".join(str(v) for v in row) + "},")
+    lines.append("};")
+    return "\n".join(lines)
+
+
+def main():
+    M = raw_rows()
+    Mi = inverse(M)
+    Mi24 = [[v * 24 for v in row] for row in Mi]
+    assert all(v.denominator == 1 for row in Mi24 for v in row), "24*M^-1 is not integral"
+    G = gram_schmidt(M)
+    Gi = [primitive(row) for row in G]
+    # rows must be mutually orthogonal
+    for a in range(27):
+        for b in range(a):
+            assert sum(x * y for x, y in zip(Gi[a], Gi[b])) == 0
+    norm2 = [sum(v * v for v in row) for row in Gi]
+    out = sys.stdout
+    out.write("// Generated by gen_mrt_tables.py — exact D3Q27 moment matrices.  Do not edit.\n")
+    out.write("// MRAW rows follow rust/examples/d3q27_mrt_lid.rs:167-193 of the reference.\n")
+    out.write("#pragma once\n\nnamespace lbm {\n\n")
+    out.write(fmt("MRAW", "int", [[int(v) for v in row] for row in M]) + "\n\n")
+    out.write(fmt("MRAW_INV24", "int", [[int(v) for v in row] for row in Mi24]) + "\n\n")
+    out.write(fmt("MGS", "int", Gi) + "\n\n")
+    out.write("constexpr long long MGS_NORM2[27] = {" + ", ".join(str(v) for v in norm2) + "};\n\n")
+    out.write("}  // namespace lbm\n")
+
+
+if __name__ == "__main__":
+    main()

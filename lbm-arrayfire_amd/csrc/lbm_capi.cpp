@@ -1,0 +1,413 @@
+// lbm_capi.cpp — host implementation of the C ABI declared in include/lbm_capi.h.
+//
+// A handle owns two population buffers (A-B pull scheme), the node-type bytes, monitor scratch
+// and one HIP stream.  All kernels of a handle are enqueued on its stream; the calls that return
+// host data synchronise it (af::sync() in the reference, cpp/d2q9_bgk_lid.cpp:118,182).
+#include "lbm_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lbm_internal.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = 0;
+
+lbm_status fail(lbm_status s, const std::string &m) {
+  g_err = m;
+  return s;
+}
+
+#define HIPCHK(call)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail(LBM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+double weight(int Q, int i) { return Q == 9 ? Lat<9>::w[i] : Lat<27>::w[i]; }
+int cxq(int Q, int i) { return Q == 9 ? Lat<9>::cx[i] : Lat<27>::cx[i]; }
+int cyq(int Q, int i) { return Q == 9 ? Lat<9>::cy[i] : Lat<27>::cy[i]; }
+int czq(int Q, int i) { return Q == 9 ? Lat<9>::cz[i] : Lat<27>::cz[i]; }
+
+// Host-side codecs for shifted storage: the value stored is f - w_i rounded to nearest even.
+uint16_t enc_half(double v, double w) {
+  const float g = (float)(v - (double)(float)w);
+  return __builtin_bit_cast(uint16_t, (_Float16)g);
+}
+double dec_half(uint16_t h, double w) {
+  return (double)(float)__builtin_bit_cast(_Float16, h) + (double)(float)w;
+}
+uint16_t enc_bf16(double v, double w) {
+  const float g = (float)(v - (double)(float)w);
+  return __builtin_bit_cast(uint16_t, (__bf16)g);
+}
+double dec_bf16(uint16_t b, double w) {
+  return (double)__builtin_bit_cast(float, (uint32_t)b << 16) + (double)(float)w;
+}
+
+uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+std::string check_desc(const lbm_desc *d) {
+  if (!d) return "descriptor is NULL";
+  if (d->lattice != 9 && d->lattice != 27) return "lattice must be 9 (D2Q9) or 27 (D3Q27)";
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return "lattice dimensions must be >= 1";
+  if (d->lattice == 9 && d->nz != 1) return "D2Q9 requires nz == 1";
+  const uint64_t N = (uint64_t)d->nx * (uint64_t)d->ny * (uint64_t)d->nz;
+  if (N >= (1ull << 32)) return "lattice has 2^32 or more nodes";
+  if (d->collision < 0 || d->collision > 2) return "unknown collision operator";
+  if (d->collision != LBM_BGK && d->lattice != 27) return "MRT is implemented for D3Q27 only";
+  if (d->storage < 0 || d->storage > 3) return "unknown storage format";
+  if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
+  if (d->mrt_engine < 0 || d->mrt_engine > 3) return "unknown MRT engine";
+  if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
+  if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
+  if (!std::isfinite(d->omega)) return "omega is not finite";
+  for (int k = 0; k < 27; ++k)
+    if (!std::isfinite(d->s[k])) return "MRT rate is not finite";
+  if (d->inlet_profile == LBM_INLET_PARABOLIC && (d->ny < 3 || (d->lattice == 27 && d->nz < 3)))
+    return "parabolic inlet needs at least 3 nodes across the channel";
+  return "";
+}
+
+}  // namespace
+
+struct lbm_ctx {
+  lbm_desc d{};
+  int device = 0;
+  int Q = 9;
+  int64_t N = 0;
+  int esize = 4;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *f[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint8_t *types = nullptr;
+  double *partials = nullptr;
+  double *red = nullptr;
+  double *fields = nullptr;  // 4N doubles, allocated on first use
+  int64_t steps = 0;
+  std::vector<uint8_t> host_types;
+  Config cfg{};
+  StepParams base{};
+
+  size_t pop_bytes() const { return (size_t)N * (size_t)Q * (size_t)esize; }
+
+  StepParams params(int from) const {
+    StepParams p = base;
+    p.fin = f[from];
+    p.fout = f[from ^ 1];
+    return p;
+  }
+};
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+const char *lbm_last_error(void) { return g_err.c_str(); }
+
+lbm_status lbm_set_device(int device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(LBM_EINVAL, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  return LBM_OK;
+}
+
+size_t lbm_footprint_bytes(const lbm_desc *d) {
+  if (!check_desc(d).empty()) return 0;
+  const size_t N = (size_t)d->nx * d->ny * d->nz;
+  int32_t gx, gy, gz, nt;
+  tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
+  return 2 * N * (size_t)d->lattice * (size_t)storage_bytes(d->storage) + N +
+         (size_t)nt * 3 * sizeof(double) + 4 * sizeof(double);
+}
+
+lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *out) {
+  if (!out) return fail(LBM_EINVAL, "out is NULL");
+  *out = nullptr;
+  const std::string why = check_desc(d);
+  if (!why.empty()) return fail(LBM_EINVAL, why);
+  if (!node_type) return fail(LBM_EINVAL, "node_type is NULL");
+  lbm_ctx *h = new (std::nothrow) lbm_ctx();
+  if (!h) return fail(LBM_ENOMEM, "host allocation failed");
+  h->d = *d;
+  h->device = g_device;
+  h->Q = d->lattice;
+  h->N = (int64_t)d->nx * d->ny * d->nz;
+  h->esize = storage_bytes(d->storage);
+  h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, d->mrt_engine};
+
+  // Node types; PHYSICAL mode marks the non-solid nodes whose pull needs the boundary path.
+  h->host_types.assign(node_type, node_type + h->N);
+  for (auto &t : h->host_types) t &= 0x1f;
+  if (d->bc_mode == LBM_BC_PHYSICAL) {
+    const int nx = d->nx, ny = d->ny, nz = d->nz, Q = h->Q;
+    std::vector<uint8_t> slow(h->N, 0);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const int64_t n = x + (int64_t)nx * (y + (int64_t)ny * z);
+          const uint8_t t = h->host_types[n];
+          if (t & NODE_SOLID) continue;
+          bool s = (t & (NODE_INLET | NODE_OUTLET)) != 0;
+          for (int i = 1; i < Q && !s; ++i) {
+            const int sx = ((x - cxq(Q, i)) % nx + nx) % nx, sy = ((y - cyq(Q, i)) % ny + ny) % ny,
+                      sz = ((z - czq(Q, i)) % nz + nz) % nz;
+            if (h->host_types[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) s = true;
+          }
+          slow[n] = s;
+        }
+    for (int64_t n = 0; n < h->N; ++n)
+      if (slow[n]) h->host_types[n] |= NODE_SLOW;
+  }
+
+  StepParams &p = h->base;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.nz = d->nz;
+  tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
+  p.inlet_profile = d->inlet_profile;
+  p.meq_mode = d->meq_mode;
+  p.N = h->N;
+  p.omega = d->omega;
+  for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
+  p.u_in = d->u_in;
+  p.rho_bc = d->rho_bc;
+  for (int k = 0; k < 27; ++k) p.s[k] = d->s[k];
+
+  auto cleanup = [&](lbm_status s, const std::string &m) {
+    lbm_destroy(h);
+    return fail(s, m);
+  };
+  if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "hipEventCreate failed");
+  for (int b = 0; b < 2; ++b)
+    if (hipMalloc(&h->f[b], h->pop_bytes()) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->pop_bytes()) + " bytes)");
+  if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
+  if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
+    return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
+  if (hipMemcpy(h->types, h->host_types.data(), (size_t)h->N, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "node-type upload failed");
+  p.type = h->types;
+  *out = h;
+  return LBM_OK;
+}
+
+lbm_status lbm_destroy(lbm_handle h) {
+  if (!h) return LBM_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int b = 0; b < 2; ++b)
+    if (h->f[b]) (void)hipFree(h->f[b]);
+  if (h->types) (void)hipFree(h->types);
+  if (h->partials) (void)hipFree(h->partials);
+  if (h->red) (void)hipFree(h->red);
+  if (h->fields) (void)hipFree(h->fields);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return LBM_OK;
+}
+
+static lbm_status ensure_fields(lbm_handle h) {
+  if (!h->fields) HIPCHK(hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)));
+  return LBM_OK;
+}
+
+lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, const double *uy,
+                           const double *uz) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  lbm_status st = ensure_fields(h);
+  if (st != LBM_OK) return st;
+  const double *src[4] = {rho, ux, uy, uz};
+  const double *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int k = 0; k < 4; ++k)
+    if (src[k]) {
+      double *dst = h->fields + (size_t)k * h->N;
+      HIPCHK(hipMemcpyAsync(dst, src[k], (size_t)h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      dev[k] = dst;
+    }
+  HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->steps = 0;
+  return LBM_OK;
+}
+
+lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], uint64_t seed,
+                                double noise_amp) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  const int D = h->Q == 9 ? 2 : 3;
+  std::vector<double> rho(h->N, rho0), u[3];
+  for (int a = 0; a < 3; ++a) u[a].assign(h->N, u0 ? u0[a] : 0.0);
+  uint64_t s = seed;
+  for (int64_t n = 0; n < h->N; ++n)
+    for (int a = 0; a < D; ++a) {
+      const double r = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+      if (!(h->host_types[n] & NODE_SOLID)) u[a][n] += noise_amp * (2.0 * r - 1.0);
+    }
+  return lbm_init_fields(h, rho.data(), u[0].data(), u[1].data(), D == 3 ? u[2].data() : nullptr);
+}
+
+lbm_status lbm_init_reference(lbm_handle h, double rho0) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->steps = 0;
+  return LBM_OK;
+}
+
+static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
+  for (int64_t t = 0; t < nsteps; ++t) {
+    HIPCHK(launch_step(h->cfg, h->params(h->cur), h->stream));
+    h->cur ^= 1;
+    h->steps++;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_step(lbm_handle h, int64_t nsteps) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+  HIPCHK(hipSetDevice(h->device));
+  return enqueue_steps(h, nsteps);
+}
+
+lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  lbm_status st = enqueue_steps(h, nsteps);
+  if (st != LBM_OK) return st;
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  if (ms) *ms = (double)t;
+  return LBM_OK;
+}
+
+lbm_status lbm_sync(lbm_handle h) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+int64_t lbm_steps_done(lbm_handle h) { return h ? h->steps : -1; }
+
+lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, double *uz) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  lbm_status st = ensure_fields(h);
+  if (st != LBM_OK) return st;
+  double *dst[4] = {rho, ux, uy, uz};
+  double *dev[4];
+  for (int k = 0; k < 4; ++k) dev[k] = dst[k] ? h->fields + (size_t)k * h->N : nullptr;
+  HIPCHK(launch_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->stream));
+  for (int k = 0; k < 4; ++k)
+    if (dst[k])
+      HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+size_t lbm_state_bytes(lbm_handle h) { return h ? h->pop_bytes() : 0; }
+
+lbm_status lbm_get_state(lbm_handle h, void *dst) {
+  if (!h || !dst) return fail(LBM_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipMemcpyAsync(dst, h->f[h->cur], h->pop_bytes(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_set_state(lbm_handle h, const void *src) {
+  if (!h || !src) return fail(LBM_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(h->device));
+  // Both buffers: in PHYSICAL mode solid nodes are never rewritten by a step.
+  for (int b = 0; b < 2; ++b)
+    HIPCHK(hipMemcpyAsync(h->f[b], src, h->pop_bytes(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_get_populations(lbm_handle h, double *f) {
+  if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
+  std::vector<unsigned char> raw(h->pop_bytes());
+  lbm_status st = lbm_get_state(h, raw.data());
+  if (st != LBM_OK) return st;
+  const int64_t N = h->N;
+  for (int i = 0; i < h->Q; ++i) {
+    const double w = weight(h->Q, i);
+    for (int64_t n = 0; n < N; ++n) {
+      const size_t k = (size_t)(n + N * i);
+      switch (h->d.storage) {
+        case LBM_F64: f[k] = reinterpret_cast<const double *>(raw.data())[k]; break;
+        case LBM_F32: f[k] = reinterpret_cast<const float *>(raw.data())[k]; break;
+        case LBM_F16S: f[k] = dec_half(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
+        default: f[k] = dec_bf16(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
+      }
+    }
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_set_populations(lbm_handle h, const double *f) {
+  if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
+  std::vector<unsigned char> raw(h->pop_bytes());
+  const int64_t N = h->N;
+  for (int i = 0; i < h->Q; ++i) {
+    const double w = weight(h->Q, i);
+    for (int64_t n = 0; n < N; ++n) {
+      const size_t k = (size_t)(n + N * i);
+      switch (h->d.storage) {
+        case LBM_F64: reinterpret_cast<double *>(raw.data())[k] = f[k]; break;
+        case LBM_F32: reinterpret_cast<float *>(raw.data())[k] = (float)f[k]; break;
+        case LBM_F16S: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_half(f[k], w); break;
+        default: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_bf16(f[k], w); break;
+      }
+    }
+  }
+  return lbm_set_state(h, raw.data());
+}
+
+lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite) {
+  if (!h) return fail(LBM_EINVAL, "handle is NULL");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(launch_monitors(h->cfg, h->params(h->cur), h->partials, h->red, h->stream));
+  double out[3];
+  HIPCHK(hipMemcpyAsync(out, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (mass) *mass = out[0];
+  if (kinetic_energy) *kinetic_energy = out[1];
+  if (nonfinite) *nonfinite = (int64_t)out[2];
+  return out[2] > 0 ? fail(LBM_EUNSTABLE, "non-finite populations") : LBM_OK;
+}
+
+}  // extern "C"

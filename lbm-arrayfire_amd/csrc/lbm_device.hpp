@@ -1,0 +1,417 @@
+// lbm_device.hpp — device building blocks of the fused pull stream + collide step (gfx950).
+//
+// One thread owns one node.  It pulls its Q populations from the neighbours (periodic wrap as
+// the reference's af::shift, cpp/d2q9_bgk_lid.cpp:18-28), applies the boundary rules, takes the
+// moments, collides and writes Q populations: 2*Q*s bytes of HBM traffic per node update.
+//
+// Arithmetic follows the reference's operation order (cpp/d2q9_bgk_lid.cpp:131-153,
+// rust/examples/d3q27_mrt_lid.rs:311-387) term by term, with every product and sum rounded
+// separately (-ffp-contract=off), so the FP64 kernels reproduce the CPU restatement bit for bit.
+// Only the FP32 MRT moment transforms use explicit FMAs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <utility>
+
+#include "lbm_internal.hpp"
+#include "mrt_tables.h"
+
+namespace lbm {
+
+template <int V> using IC = std::integral_constant<int, V>;
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(IC<I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// ---- storage codecs ----------------------------------------------------------------------
+template <int ST, int Q> struct Codec;
+template <int Q> struct Codec<ST_F64, Q> {
+  using T = double;
+  using C = double;
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = v; }
+};
+template <int Q> struct Codec<ST_F32, Q> {
+  using T = float;
+  using C = float;
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = v; }
+};
+// Shifted half storage: the stored value is f - w_i, rounded to nearest even.
+template <int Q> struct Codec<ST_F16S, Q> {
+  using T = uint16_t;
+  using C = float;
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
+    return (float)__builtin_bit_cast(_Float16, p[k]) + (float)Lat<Q>::w[I];
+  }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) {
+    p[k] = __builtin_bit_cast(uint16_t, (_Float16)(v - (float)Lat<Q>::w[I]));
+  }
+};
+template <int Q> struct Codec<ST_BF16S, Q> {
+  using T = uint16_t;
+  using C = float;
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
+    return __builtin_bit_cast(float, (uint32_t)p[k] << 16) + (float)Lat<Q>::w[I];
+  }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) {
+    p[k] = __builtin_bit_cast(uint16_t, (__bf16)(v - (float)Lat<Q>::w[I]));
+  }
+};
+
+// ---- XCD-aware tile order ---------------------------------------------------------------
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8).  Remapping
+// b -> tile so that each XCD owns one contiguous range of tiles keeps a slab of the lattice, and
+// the neighbour rows its pulls touch, in that XCD's L2 from one step to the next.
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int q = ntiles >> 3, r = ntiles & 7, k = b & 7, j = b >> 3;
+  return k * q + (k < r ? k : r) + j;
+}
+
+struct NodePos {
+  int x, y, z;
+  uint32_t n;
+  bool valid;
+};
+
+__device__ __forceinline__ NodePos node_of_thread(const StepParams &p) {
+  const int tile = xcd_tile((int)blockIdx.x, p.ntiles);
+  const int bx = tile % p.gx, t2 = tile / p.gx, by = t2 % p.gy, bz = t2 / p.gy;
+  NodePos q;
+  q.x = bx * TILE_X + (int)(threadIdx.x & (TILE_X - 1));
+  q.y = by * TILE_Y + (int)(threadIdx.x / TILE_X);
+  q.z = bz;
+  q.valid = q.x < p.nx && q.y < p.ny;
+  q.n = (uint32_t)q.x + (uint32_t)p.nx * ((uint32_t)q.y + (uint32_t)p.ny * (uint32_t)q.z);
+  return q;
+}
+
+// ---- pulls -------------------------------------------------------------------------------
+// Periodic pull: f_i(x) = F_i(x - c_i mod dims).
+template <int Q, int ST>
+__device__ __forceinline__ void pull_periodic(const StepParams &p, const NodePos &q,
+                                              typename Codec<ST, Q>::C *f) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  const T *fin = (const T *)p.fin;
+  const int xm = q.x == 0 ? p.nx - 1 : q.x - 1, xp = q.x == p.nx - 1 ? 0 : q.x + 1;
+  const int ym = q.y == 0 ? p.ny - 1 : q.y - 1, yp = q.y == p.ny - 1 ? 0 : q.y + 1;
+  const int zm = q.z == 0 ? p.nz - 1 : q.z - 1, zp = q.z == p.nz - 1 ? 0 : q.z + 1;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+    const int sx = cx == 1 ? xm : (cx == -1 ? xp : q.x);
+    const int sy = cy == 1 ? ym : (cy == -1 ? yp : q.y);
+    const int sz = cz == 1 ? zm : (cz == -1 ? zp : q.z);
+    const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
+    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, src);
+  });
+}
+
+template <class C>
+__device__ __forceinline__ C inlet_velocity(const StepParams &p, int D, int y, int z) {
+  if (p.inlet_profile == 0) return (C)p.u_in;
+  const C H = (C)(p.ny - 2), eta = (C)y - (C)0.5;
+  C u = (((C)4 * (C)p.u_in) * eta * (H - eta)) / (H * H);
+  if (D == 3) {
+    const C Hz = (C)(p.nz - 2), ez = (C)z - (C)0.5;
+    u = u * (((C)4 * ez * (Hz - ez)) / (Hz * Hz));
+  }
+  return u;
+}
+
+template <int Q, class C>
+__device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz);
+template <int Q, int i, class C>
+__device__ __forceinline__ C cdotu(C ux, C uy, C uz);
+
+// Non-equilibrium bounce-back closure of the unknowns (c_x == SGN) of an x-face node with
+// prescribed rho, u; transverse momentum corrected through N_a = sum_{c_x=0} f c_a - (2/3) rho u_a
+// (reduces to Zou-He's formulas on D2Q9).
+template <int Q, int SGN, class C>
+__device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
+  C ny_ = 0, nz_ = 0;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Lat<Q>::cx[i] == 0) {
+      ny_ += (C)Lat<Q>::cy[i] * f[i];
+      nz_ += (C)Lat<Q>::cz[i] * f[i];
+    }
+  });
+  ny_ = ny_ - (((C)2 / (C)3) * rho) * uy;
+  nz_ = nz_ - (((C)2 / (C)3) * rho) * uz;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Lat<Q>::cx[i] == SGN) {
+      constexpr int io = Lat<Q>::opp[i];
+      const C cu = cdotu<Q, i>(ux, uy, uz);
+      f[i] = (f[io] + (((C)6 * (C)Lat<Q>::w[i]) * rho) * cu) -
+             ((C)18 * (C)Lat<Q>::w[i]) * ((C)Lat<Q>::cy[i] * ny_ + (C)Lat<Q>::cz[i] * nz_);
+    }
+  });
+}
+
+// PHYSICAL-mode pull: half-way bounce-back on links whose source is solid (plus the moving-wall
+// momentum term 6 w_i rho_w c_i.u_w); inlet nodes close their c_x = +1 unknowns for the
+// prescribed velocity; outlet nodes close their c_x = -1 unknowns for rho = rho_bc, the normal
+// velocity the known populations carry, and the upstream neighbour's tangential velocity.
+template <int Q, int ST>
+__device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos &q, uint8_t t,
+                                             typename Codec<ST, Q>::C *f) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  using C = typename CD::C;
+  if (!(t & NODE_SLOW)) {
+    pull_periodic<Q, ST>(p, q, f);
+    return;
+  }
+  const T *fin = (const T *)p.fin;
+  const C rw = (C)p.rho_bc;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+    constexpr int io = Lat<Q>::opp[i];
+    int sx = q.x - cx;
+    sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
+    int sy = q.y - cy;
+    sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
+    int sz = q.z - cz;
+    sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
+    const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
+    const uint8_t ts = p.type[src];
+    if (ts & NODE_SOLID) {
+      C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.N, q.n);
+      if (ts & NODE_MOVING) {
+        C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
+        if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
+        v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+      }
+      f[i] = v;
+    } else {
+      f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, src);
+    }
+  });
+  if (t & NODE_INLET) {
+    const C uin = inlet_velocity<C>(p, Lat<Q>::D, q.y, q.z);
+    C s0 = 0, sm = 0;
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
+      if constexpr (Lat<Q>::cx[i] == -1) sm += f[i];
+    });
+    const C rho = (s0 + (C)2 * sm) / ((C)1 - uin);
+    nebb_close<Q, 1>(f, rho, uin, (C)0, (C)0);
+  } else if (t & NODE_OUTLET) {
+    C uy = 0, uz = 0;
+    if (q.x > 0 && !(p.type[q.n - 1] & NODE_SOLID)) {
+      C g[Q], r, vx, vy, vz;
+      static_for<Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        g[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, q.n - 1);
+      });
+      moments<Q>(g, r, vx, vy, vz);
+      uy = vy;
+      uz = vz;
+    }
+    C s0 = 0, sp = 0;
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
+      if constexpr (Lat<Q>::cx[i] == 1) sp += f[i];
+    });
+    const C ux = (s0 + (C)2 * sp) / rw - (C)1;
+    nebb_close<Q, -1>(f, rw, ux, uy, uz);
+  }
+}
+
+// ---- moments, overrides, collisions ------------------------------------------------------
+// rho = sum_i f_i; u = (sum_i c_i f_i) / rho, ascending i (cpp/d2q9_bgk_lid.cpp:131-138).
+template <int Q, class C>
+__device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz) {
+  C r = 0, jx = 0, jy = 0, jz = 0;
+  static_for<Q>([&](auto I) { r = r + f[decltype(I)::value]; });
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Lat<Q>::cx[i] == 1) jx = jx + f[i];
+    if constexpr (Lat<Q>::cx[i] == -1) jx = jx - f[i];
+  });
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Lat<Q>::cy[i] == 1) jy = jy + f[i];
+    if constexpr (Lat<Q>::cy[i] == -1) jy = jy - f[i];
+  });
+  if constexpr (Lat<Q>::D == 3) {
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      if constexpr (Lat<Q>::cz[i] == 1) jz = jz + f[i];
+      if constexpr (Lat<Q>::cz[i] == -1) jz = jz - f[i];
+    });
+  }
+  rho = r;
+  ux = jx / r;
+  uy = jy / r;
+  uz = (Lat<Q>::D == 3) ? jz / r : (C)0;
+}
+
+// REFERENCE Dirichlet overrides, in the reference's order: lid velocity, inlet x-velocity,
+// solid rho = u = 0, fixed-density columns (cpp/d2q9_bgk_lid.cpp:141-146,
+// rust/examples/d2q9_bgk_channel.rs:190-201, rust/examples/d3q27_mrt_lid.rs:323-330).
+template <int Q, class C>
+__device__ __forceinline__ void overrides(const StepParams &p, uint8_t t, int y, int z, C &rho, C &ux,
+                                          C &uy, C &uz) {
+  if (t & NODE_MOVING) { ux = (C)p.u_wall[0]; uy = (C)p.u_wall[1]; uz = (C)p.u_wall[2]; }
+  if (t & NODE_INLET) ux = inlet_velocity<C>(p, Lat<Q>::D, y, z);
+  if (t & NODE_SOLID) { ux = 0; uy = 0; uz = 0; rho = 0; }
+  if (t & NODE_RHO_FIX) rho = (C)p.rho_bc;
+}
+
+// c_i . u evaluated as the reference does: (cx ux + cy uy) + cz uz with exact +-1/0 products.
+template <int Q, int i, class C>
+__device__ __forceinline__ C cdotu(C ux, C uy, C uz) {
+  C e = (C)Lat<Q>::cx[i] * ux + (C)Lat<Q>::cy[i] * uy;
+  if constexpr (Lat<Q>::D == 3) e = e + (C)Lat<Q>::cz[i] * uz;
+  return e;
+}
+
+template <int Q, int i, class C>
+__device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq) {
+  const C eu = cdotu<Q, i>(ux, uy, uz);
+  return ((C)Lat<Q>::w[i] * rho) * ((((C)1 + (C)3 * eu) + (C)4.5 * (eu * eu)) - (C)1.5 * usq);
+}
+
+template <int Q, class C>
+__device__ __forceinline__ C usq_of(C ux, C uy, C uz) {
+  C u2 = ux * ux + uy * uy;
+  if constexpr (Lat<Q>::D == 3) u2 = u2 + uz * uz;
+  return u2;
+}
+
+// F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153)
+template <int Q, class C>
+__device__ __forceinline__ void collide_bgk(const StepParams &p, const C *f, C rho, C ux, C uy, C uz,
+                                            C *out) {
+  const C omega = (C)p.omega, om1 = (C)1 - omega;
+  const C usq = usq_of<Q>(ux, uy, uz);
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    out[i] = omega * feq_i<Q, i>(rho, ux, uy, uz, usq) + om1 * f[i];
+  });
+}
+
+// ---- MRT (D3Q27) -------------------------------------------------------------------------
+template <int CO> __device__ __forceinline__ constexpr int mcoef(int k, int i) {
+  return CO == CO_MRT_RAW ? MRAW[k][i] : MGS[k][i];
+}
+template <class C, int CO> __device__ __forceinline__ constexpr C minv_coef(int i, int k) {
+  return CO == CO_MRT_RAW ? (C)MRAW_INV24[i][k] / (C)24 : (C)MGS[k][i] / (C)MGS_NORM2[k];
+}
+
+// acc + a*c with c a compile-time coefficient; +-1 are exact adds, others a rounded product
+// (FP64) or one FMA (FP32).
+template <class C, int ci>
+__device__ __forceinline__ C add_int_term(C acc, C a) {
+  if constexpr (ci == 1) return acc + a;
+  else if constexpr (ci == -1) return acc - a;
+  else if constexpr (ci == 0) return acc;
+  else if constexpr (sizeof(C) == 4) return __builtin_fmaf(a, (C)ci, acc);
+  else return acc + a * (C)ci;
+}
+
+// out_k = sum_i M[k][i] v_i, ascending i
+template <class C, int CO>
+__device__ __forceinline__ void mrt_forward(const C *v, C *out) {
+  static_for<27>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    C acc = 0;
+    static_for<27>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      acc = add_int_term<C, mcoef<CO>(k, i)>(acc, v[i]);
+    });
+    out[k] = acc;
+  });
+}
+
+// out_i = sum_k Minv[i][k] m_k, ascending k
+template <class C, int CO>
+__device__ __forceinline__ void mrt_inverse(const C *m, C *out) {
+  static_for<27>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    C acc = 0;
+    static_for<27>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr C c = minv_coef<C, CO>(i, k);
+      if constexpr (c != (C)0) {
+        if constexpr (sizeof(C) == 4) acc = __builtin_fmaf(m[k], c, acc);
+        else acc = acc + m[k] * c;
+      }
+    });
+    out[i] = acc;
+  });
+}
+
+// Reference equilibrium-moment table (rust/examples/d3q27_mrt_lid.rs:341-383), row 26 quirk kept.
+template <class C>
+__device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
+  const C cs2 = (C)1 / (C)3, cs4 = cs2 * cs2, cs6 = cs2 * cs2 * cs2;
+  const C rvx = r * ux, rvy = r * uy, rvz = r * uz, rc2 = cs2 * r, rc4 = cs4 * r, rc6 = cs6 * r;
+  const C vx2 = ux * ux, vy2 = uy * uy, vz2 = uz * uz, v2 = vx2 + vy2 + vz2;
+  meq[0] = r; meq[1] = rvx; meq[2] = rvy; meq[3] = rvz;
+  meq[4] = rvx * uy; meq[5] = rvx * uz; meq[6] = rvy * uz;
+  meq[7] = r + rvx * ux + rvy * uy + rvz * uz;
+  meq[8] = r * (vx2 - vy2); meq[9] = r * (vx2 - vz2);
+  meq[10] = rc2 * ux; meq[11] = rc2 * ux; meq[12] = rc2 * uy; meq[13] = rc2 * uz;
+  meq[14] = rc2 * uy; meq[15] = rc2 * uz; meq[16] = 0;
+  meq[17] = rc2 * (cs2 + vx2 + vy2); meq[18] = rc2 * (cs2 + vx2 + vz2); meq[19] = rc2 * (cs2 + vy2 + vz2);
+  meq[20] = rc2 * uy * uz; meq[21] = rc2 * ux * uz; meq[22] = rc2 * ux * uy;
+  meq[23] = rc4 * ux; meq[24] = rc4 * uy; meq[25] = rc4 * uz;
+  meq[26] = v2 + rc6;
+}
+
+// m = M f; m* = m - s (m - meq); out = M^-1 m*   (rust/examples/d3q27_mrt_lid.rs:336,385-387)
+template <int CO, class C>
+__device__ __forceinline__ void collide_mrt_scalar(const StepParams &p, const C *f, C rho, C ux, C uy,
+                                                   C uz, C *out) {
+  C m[27], meq[27];
+  mrt_forward<C, CO>(f, m);
+  if (CO == CO_MRT_RAW && p.meq_mode == 0) {
+    meq_reference<C>(rho, ux, uy, uz, meq);
+  } else {
+    C fe[27];
+    const C usq = ux * ux + uy * uy + uz * uz;
+    static_for<27>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      fe[i] = feq_i<27, i>(rho, ux, uy, uz, usq);
+    });
+    mrt_forward<C, CO>(fe, meq);
+  }
+  static_for<27>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    m[k] = m[k] - (m[k] - meq[k]) * (C)p.s[k];
+  });
+  mrt_inverse<C, CO>(m, out);
+}
+
+template <int Q, int CO, class C>
+__device__ __forceinline__ void collide(const StepParams &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
+  if constexpr (CO == CO_BGK) collide_bgk<Q>(p, f, rho, ux, uy, uz, out);
+  else collide_mrt_scalar<CO>(p, f, rho, ux, uy, uz, out);
+}
+
+template <int Q, int ST>
+__device__ __forceinline__ void store_all(const StepParams &p, uint32_t n, const typename Codec<ST, Q>::C *out) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  T *fout = (T *)p.fout;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    CD::template st<i>(fout + (size_t)i * (size_t)p.N, n, out[i]);
+  });
+}
+
+}  // namespace lbm

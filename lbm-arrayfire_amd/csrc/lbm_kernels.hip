@@ -1,0 +1,274 @@
+// lbm_kernels.hip — node-per-thread kernels: the fused step, field extraction, initial states
+// and the mass / kinetic-energy monitors, plus the host-side dispatch over
+// (lattice, storage, boundary mode, collision).
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+// One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
+// solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
+template <int Q, int ST, int BC, int CO>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_thread(p);
+  if (!q.valid) return;
+  const uint8_t t = p.type[q.n];
+  C f[Q], out[Q], rho, ux, uy, uz;
+  if constexpr (BC == BC_REF) {
+    pull_periodic<Q, ST>(p, q, f);
+    moments<Q>(f, rho, ux, uy, uz);
+    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+    if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK: out[opp(j)] = f[j], j >= 1
+      static_for<Q - 1>([&](auto J) {
+        constexpr int j = decltype(J)::value + 1;
+        out[Lat<Q>::opp[j]] = f[j];
+      });
+    }
+  } else {
+    if (t & NODE_SOLID) return;  // walls keep their populations; nothing reads or writes them
+    pull_physical<Q, ST>(p, q, t, f);
+    moments<Q>(f, rho, ux, uy, uz);
+    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+  }
+  store_all<Q, ST>(p, q.n, out);
+}
+
+// rho, u the reference holds after a step: moments of the pulled populations with the overrides
+// (REFERENCE), or of the boundary-completed pulled populations (PHYSICAL; 0 at solid nodes).
+template <int Q, int ST, int BC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fields(StepParams p, double *rho_o, double *ux_o,
+                                                          double *uy_o, double *uz_o) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_thread(p);
+  if (!q.valid) return;
+  const uint8_t t = p.type[q.n];
+  C f[Q], rho = 0, ux = 0, uy = 0, uz = 0;
+  if constexpr (BC == BC_REF) {
+    pull_periodic<Q, ST>(p, q, f);
+    moments<Q>(f, rho, ux, uy, uz);
+    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+  } else if (!(t & NODE_SOLID)) {
+    pull_physical<Q, ST>(p, q, t, f);
+    moments<Q>(f, rho, ux, uy, uz);
+  }
+  if (rho_o) rho_o[q.n] = (double)rho;
+  if (ux_o) ux_o[q.n] = (double)ux;
+  if (uy_o) uy_o[q.n] = (double)uy;
+  if (uz_o) uz_o[q.n] = (double)uz;
+}
+
+// f = f_eq(rho, u) (cpp/d2q9_bgk_lid.cpp:100-107) into both buffers.
+template <int Q, int ST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_fields(StepParams p, const double *rho_i,
+                                                               const double *ux_i, const double *uy_i,
+                                                               const double *uz_i) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_thread(p);
+  if (!q.valid) return;
+  const C r = rho_i ? (C)rho_i[q.n] : (C)1, vx = ux_i ? (C)ux_i[q.n] : (C)0,
+          vy = uy_i ? (C)uy_i[q.n] : (C)0, vz = (uz_i && Lat<Q>::D == 3) ? (C)uz_i[q.n] : (C)0;
+  const C usq = usq_of<Q>(vx, vy, vz);
+  C out[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    out[i] = feq_i<Q, i>(r, vx, vy, vz, usq);
+  });
+  store_all<Q, ST>(p, q.n, out);
+  StepParams p2 = p;
+  p2.fout = const_cast<void *>(p.fin);
+  store_all<Q, ST>(p2, q.n, out);
+}
+
+// MRT: f = rho0/27 then one collision at rho = rho0, u = 0, no bounce-back
+// (rust/examples/d3q27_mrt_lid.rs:228-287).  BGK: f = f_eq(rho0, 0).
+template <int Q, int ST, int CO>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, double rho0) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_thread(p);
+  if (!q.valid) return;
+  C out[Q];
+  if constexpr (CO == CO_BGK) {
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      out[i] = feq_i<Q, i>((C)rho0, (C)0, (C)0, (C)0, (C)0);
+    });
+  } else {
+    C f[Q];
+    static_for<Q>([&](auto I) { f[decltype(I)::value] = (C)rho0 / (C)27; });
+    collide<Q, CO>(p, f, (C)rho0, (C)0, (C)0, (C)0, out);
+  }
+  store_all<Q, ST>(p, q.n, out);
+  StepParams p2 = p;
+  p2.fout = const_cast<void *>(p.fin);
+  store_all<Q, ST>(p2, q.n, out);
+}
+
+// Wavefront + workgroup reduction of (mass, kinetic energy, nonfinite) in FP64 over the
+// non-solid nodes of one tile; one partial triple per tile (deterministic order).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <int Q, int ST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_monitors(StepParams p, double *partials) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  __shared__ double red[3][BLOCK_THREADS / 64];
+  const NodePos q = node_of_thread(p);
+  double mass = 0, ke = 0, bad = 0;
+  if (q.valid && !(p.type[q.n] & NODE_SOLID)) {
+    const T *fin = (const T *)p.fin;
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    bool finite = true;
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.N, q.n);
+      finite = finite && __builtin_isfinite(v);
+      r += v;
+      jx += (double)Lat<Q>::cx[i] * v;
+      jy += (double)Lat<Q>::cy[i] * v;
+      jz += (double)Lat<Q>::cz[i] * v;
+    });
+    if (finite) {
+      mass = r;
+      ke = (jx * jx + jy * jy + jz * jz) / (2.0 * r);
+    } else {
+      bad = 1;
+    }
+  }
+  mass = wave_sum(mass);
+  ke = wave_sum(ke);
+  bad = wave_sum(bad);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = mass; red[1][wv] = ke; red[2][wv] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < BLOCK_THREADS / 64; ++k) { a += red[0][k]; b += red[1][k]; c += red[2][k]; }
+    const int tile = xcd_tile((int)blockIdx.x, p.ntiles);
+    partials[3 * tile + 0] = a;
+    partials[3 * tile + 1] = b;
+    partials[3 * tile + 2] = c;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK_THREADS) k_reduce_partials(const double *partials, int n, double *out) {
+  __shared__ double red[3][BLOCK_THREADS / 64];
+  double a = 0, b = 0, c = 0;
+  for (int k = threadIdx.x; k < n; k += BLOCK_THREADS) {
+    a += partials[3 * k + 0];
+    b += partials[3 * k + 1];
+    c += partials[3 * k + 2];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = a; red[1][wv] = b; red[2][wv] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0, z = 0;
+    for (int k = 0; k < BLOCK_THREADS / 64; ++k) { x += red[0][k]; y += red[1][k]; z += red[2][k]; }
+    out[0] = x;
+    out[1] = y;
+    out[2] = z;
+  }
+}
+
+// ---- dispatch ----------------------------------------------------------------------------
+template <class F>
+static hipError_t with_storage(const Config &c, F &&f) {
+  auto byQ = [&](auto Qc) -> hipError_t {
+    switch (c.storage) {
+      case ST_F64: return f(Qc, IC<ST_F64>{});
+      case ST_F32: return f(Qc, IC<ST_F32>{});
+      case ST_F16S: return f(Qc, IC<ST_F16S>{});
+      case ST_BF16S: return f(Qc, IC<ST_BF16S>{});
+    }
+    return hipErrorInvalidValue;
+  };
+  if (c.Q == 9) return byQ(IC<9>{});
+  if (c.Q == 27) return byQ(IC<27>{});
+  return hipErrorInvalidValue;
+}
+
+template <class F>
+static hipError_t with_full(const Config &c, F &&f) {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    constexpr int Q = decltype(Qc)::value;
+    auto byBC = [&](auto BCc) -> hipError_t {
+      if (c.collision == CO_BGK) return f(Qc, STc, BCc, IC<CO_BGK>{});
+      if constexpr (Q == 27) {
+        if (c.collision == CO_MRT_RAW) return f(Qc, STc, BCc, IC<CO_MRT_RAW>{});
+        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
+      }
+      return hipErrorInvalidValue;
+    };
+    return c.bc == BC_REF ? byBC(IC<BC_REF>{}) : byBC(IC<BC_PHYS>{});
+  });
+}
+
+hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s) {
+  return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
+    hipLaunchKernelGGL((k_step<decltype(Qc)::value, decltype(STc)::value, decltype(BCc)::value,
+                               decltype(COc)::value>),
+                       dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
+                         double *uz, hipStream_t s) {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    constexpr int Q = decltype(Qc)::value, ST = decltype(STc)::value;
+    if (c.bc == BC_REF)
+      hipLaunchKernelGGL((k_fields<Q, ST, BC_REF>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p, rho,
+                         ux, uy, uz);
+    else
+      hipLaunchKernelGGL((k_fields<Q, ST, BC_PHYS>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p, rho,
+                         ux, uy, uz);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_init_fields(const Config &c, const StepParams &p, const double *rho, const double *ux,
+                              const double *uy, const double *uz, hipStream_t s) {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    hipLaunchKernelGGL((k_init_fields<decltype(Qc)::value, decltype(STc)::value>), dim3(p.ntiles),
+                       dim3(BLOCK_THREADS), 0, s, p, rho, ux, uy, uz);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_init_reference(const Config &c, const StepParams &p, double rho0, hipStream_t s) {
+  Config c2 = c;
+  c2.bc = BC_REF;
+  return with_full(c2, [&](auto Qc, auto STc, auto, auto COc) -> hipError_t {
+    hipLaunchKernelGGL((k_init_reference<decltype(Qc)::value, decltype(STc)::value, decltype(COc)::value>),
+                       dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p, rho0);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_monitors(const Config &c, const StepParams &p, double *partials, double *out,
+                           hipStream_t s) {
+  hipError_t e = with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    hipLaunchKernelGGL((k_monitors<decltype(Qc)::value, decltype(STc)::value>), dim3(p.ntiles),
+                       dim3(BLOCK_THREADS), 0, s, p, partials);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK_THREADS), 0, s, partials, p.ntiles, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbm
+
+namespace lbm {
+hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s) {
+  return launch_step_scalar(c, p, s);
+}
+}  // namespace lbm

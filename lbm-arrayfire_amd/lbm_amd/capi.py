@@ -1,0 +1,224 @@
+"""ctypes binding of liblbm_amd.so (the C ABI declared in include/lbm_capi.h).
+
+``Lattice`` mirrors the shape of the reference examples' ``lbm()`` (cpp/d2q9_bgk_lid.cpp:30-207):
+descriptor + node-type mask in, ``step`` for the time loop, rho/u fields out.  Every call goes to
+the HIP library; there is no CPU fallback — if the library is missing, ``load()`` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "liblbm_amd.so")
+
+# enums of lbm_capi.h
+D2Q9, D3Q27 = 9, 27
+BGK, MRT_RAW, MRT_GS = 0, 1, 2
+F64, F32, F16S, BF16S = 0, 1, 2, 3
+BC_REFERENCE, BC_PHYSICAL = 0, 1
+MRT_AUTO, MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_BF16X3 = 0, 1, 2, 3
+MEQ_REFERENCE, MEQ_PROJECTED = 0, 1
+INLET_UNIFORM, INLET_PARABOLIC = 0, 1
+NODE_FLUID, NODE_SOLID, NODE_MOVING, NODE_INLET, NODE_OUTLET, NODE_RHO_FIX = 0, 1, 2, 4, 8, 16
+
+STATUS = {0: "LBM_OK", 1: "LBM_EINVAL", 2: "LBM_ENOMEM", 3: "LBM_EDEVICE", 4: "LBM_EUNSTABLE",
+          5: "LBM_EINTERNAL", 6: "LBM_EAGAIN"}
+STORAGE_BYTES = {F64: 8, F32: 4, F16S: 2, BF16S: 2}
+
+# Every symbol include/lbm_capi.h declares (checked by tests/test_capi_exports.py).
+EXPORTS = [
+    "lbm_abi_version", "lbm_set_device", "lbm_last_error", "lbm_footprint_bytes", "lbm_create",
+    "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
+    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_get_fields", "lbm_get_populations",
+    "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
+]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class LbmDesc(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32), ("collision", ctypes.c_int32), ("storage", ctypes.c_int32),
+        ("bc_mode", ctypes.c_int32), ("mrt_engine", ctypes.c_int32), ("meq_mode", ctypes.c_int32),
+        ("inlet_profile", ctypes.c_int32), ("omega", ctypes.c_double),
+        ("s", ctypes.c_double * 27), ("u_wall", ctypes.c_double * 3), ("u_in", ctypes.c_double),
+        ("rho_bc", ctypes.c_double),
+    ]
+
+
+def make_desc(cfg):
+    """Builds an LbmDesc from a dict with the lbm_desc field names."""
+    d = LbmDesc()
+    d.lattice = cfg["lattice"]
+    d.nx, d.ny, d.nz = cfg["nx"], cfg["ny"], cfg.get("nz", 1)
+    d.collision = cfg.get("collision", BGK)
+    d.storage = cfg.get("storage", F32)
+    d.bc_mode = cfg.get("bc_mode", BC_REFERENCE)
+    d.mrt_engine = cfg.get("mrt_engine", MRT_AUTO)
+    d.meq_mode = cfg.get("meq_mode", MEQ_REFERENCE)
+    d.inlet_profile = cfg.get("inlet_profile", INLET_UNIFORM)
+    d.omega = cfg.get("omega", 1.0)
+    s = cfg.get("s", [1.0] * 27)
+    for k in range(27):
+        d.s[k] = s[k]
+    uw = cfg.get("u_wall", (0.0, 0.0, 0.0))
+    for k in range(3):
+        d.u_wall[k] = uw[k]
+    d.u_in = cfg.get("u_in", 0.0)
+    d.rho_bc = cfg.get("rho_bc", 1.0)
+    return d
+
+
+_lib = None
+
+
+def load(path=None):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or os.environ.get("LBM_AMD_LIB", LIB_PATH)
+    if not os.path.exists(p):
+        raise RuntimeError(f"liblbm_amd.so not found at {p}: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(p)
+    H = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.lbm_abi_version.restype = ctypes.c_int
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_footprint_bytes.argtypes = [ctypes.POINTER(LbmDesc)]
+    L.lbm_footprint_bytes.restype = ctypes.c_size_t
+    L.lbm_set_device.argtypes = [ctypes.c_int]
+    L.lbm_create.argtypes = [ctypes.POINTER(LbmDesc), ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(H)]
+    L.lbm_destroy.argtypes = [H]
+    L.lbm_init_fields.argtypes = [H, dp, dp, dp, dp]
+    L.lbm_init_equilibrium.argtypes = [H, ctypes.c_double, dp, ctypes.c_uint64, ctypes.c_double]
+    L.lbm_init_reference.argtypes = [H, ctypes.c_double]
+    L.lbm_step.argtypes = [H, ctypes.c_int64]
+    L.lbm_step_timed.argtypes = [H, ctypes.c_int64, dp]
+    L.lbm_sync.argtypes = [H]
+    L.lbm_steps_done.argtypes = [H]
+    L.lbm_steps_done.restype = ctypes.c_int64
+    L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
+    L.lbm_get_populations.argtypes = [H, dp]
+    L.lbm_set_populations.argtypes = [H, dp]
+    L.lbm_state_bytes.argtypes = [H]
+    L.lbm_state_bytes.restype = ctypes.c_size_t
+    L.lbm_get_state.argtypes = [H, ctypes.c_void_p]
+    L.lbm_set_state.argtypes = [H, ctypes.c_void_p]
+    L.lbm_monitors.argtypes = [H, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def check(status):
+    if status != 0:
+        raise LbmError(status, load().lbm_last_error().decode())
+
+
+def footprint_bytes(cfg):
+    d = make_desc(cfg)
+    return load().lbm_footprint_bytes(ctypes.byref(d))
+
+
+class Lattice:
+    """One lattice on the GPU: a handle of the C ABI."""
+
+    def __init__(self, cfg, node_type):
+        self.cfg = dict(cfg)
+        self.lib = load()
+        self.desc = make_desc(cfg)
+        self.Q = cfg["lattice"]
+        self.shape = (cfg["nx"], cfg["ny"], cfg.get("nz", 1))
+        self.N = self.shape[0] * self.shape[1] * self.shape[2]
+        self.node_type = np.ascontiguousarray(node_type, dtype=np.uint8).reshape(-1)
+        if self.node_type.size != self.N:
+            raise ValueError(f"node_type has {self.node_type.size} entries, lattice has {self.N} nodes")
+        self.h = ctypes.c_void_p()
+        check(self.lib.lbm_create(ctypes.byref(self.desc),
+                                  self.node_type.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init_fields(self, rho=None, ux=None, uy=None, uz=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                for a in (rho, ux, uy, uz)]
+        check(self.lib.lbm_init_fields(self.h, *[_dp(a) for a in arrs]))
+
+    def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), seed=0, noise_amp=0.0):
+        u = (ctypes.c_double * 3)(*u0)
+        check(self.lib.lbm_init_equilibrium(self.h, rho0, u, seed, noise_amp))
+
+    def init_reference(self, rho0=1.0):
+        check(self.lib.lbm_init_reference(self.h, rho0))
+
+    def step(self, n=1):
+        check(self.lib.lbm_step(self.h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        check(self.lib.lbm_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(self.lib.lbm_sync(self.h))
+
+    @property
+    def steps_done(self):
+        return self.lib.lbm_steps_done(self.h)
+
+    def fields(self):
+        out = [np.zeros(self.N) for _ in range(4)]
+        check(self.lib.lbm_get_fields(self.h, *[_dp(a) for a in out]))
+        return out
+
+    def populations(self):
+        f = np.zeros(self.N * self.Q)
+        check(self.lib.lbm_get_populations(self.h, _dp(f)))
+        return f
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(self.lib.lbm_set_populations(self.h, _dp(f)))
+
+    def state(self):
+        b = np.zeros(self.lib.lbm_state_bytes(self.h), dtype=np.uint8)
+        check(self.lib.lbm_get_state(self.h, b.ctypes.data_as(ctypes.c_void_p)))
+        return b
+
+    def set_state(self, b):
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        check(self.lib.lbm_set_state(self.h, b.ctypes.data_as(ctypes.c_void_p)))
+
+    def monitors(self):
+        """(mass, kinetic_energy, nonfinite); does not raise on non-finite populations."""
+        m, k, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        st = self.lib.lbm_monitors(self.h, ctypes.byref(m), ctypes.byref(k), ctypes.byref(n))
+        if st not in (0, 4):
+            check(st)
+        return m.value, k.value, n.value

@@ -1,0 +1,98 @@
+"""Descriptor builders for the solver cases: the reference examples and the benchmark workloads.
+
+omega follows the reference's unit convention nu = (tau - 1/2)/3, omega = 1/tau
+(cpp/d2q9_bgk_lid.cpp:45-49)."""
+from . import geometry as g
+from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q27, F32, INLET_PARABOLIC,
+                   INLET_UNIFORM, MEQ_REFERENCE, MRT_AUTO, MRT_GS, MRT_RAW)
+
+
+def omega_from_nu(nu):
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+# Rates of the reference MRT lid case, one per moment row (rust/examples/d3q27_mrt_lid.rs:111-150).
+def reference_mrt_rates():
+    s0, s1, sv, sb, s3, s3b, s4, s4b, s5, s6 = 1.3, 1.3, 1.1, 0.9, 1.2, 0.9, 1.2, 1.2, 1.1, 1.1
+    return [s0, s1, s1, s1, sv, sv, sv, sb, sv, sv, s3, s3, s3, s3, s3, s3, s3b,
+            s4, s4, s4, s4b, s4b, s4b, s5, s5, s5, s6]
+
+
+def shear_mrt_rates(omega, conserved=1.0, bulk=1.0, third=1.2, fourth=1.2, fifth=1.1, sixth=1.1):
+    """Rates for a given shear viscosity: s_nu = omega on the shear rows (4-6, 8, 9), the other
+    groups fixed (recorded with each result)."""
+    s = [conserved] * 4 + [omega] * 3 + [bulk] + [omega] * 2 + [third] * 7 + [fourth] * 6 + \
+        [fifth] * 3 + [sixth]
+    assert len(s) == 27
+    return s
+
+
+def d2q9_lid_reference(n=128, u_lid=0.05, re=100.0, storage=F32):
+    """D2Q9 BGK lid cavity, reference semantics; nu = u*nx/Re as cpp/d2q9_bgk_lid.cpp:45."""
+    nu = u_lid * n / re
+    cfg = dict(lattice=D2Q9, nx=n, ny=n, nz=1, collision=BGK, storage=storage, bc_mode=BC_REFERENCE,
+               omega=omega_from_nu(nu), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, g.lid_cavity_2d_reference(n, n)
+
+
+def d2q9_lid_physical(n=128, u_lid=0.05, re=100.0, storage=F32):
+    nu = u_lid * n / re
+    cfg = dict(lattice=D2Q9, nx=n, ny=n, nz=1, collision=BGK, storage=storage, bc_mode=BC_PHYSICAL,
+               omega=omega_from_nu(nu), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, g.lid_cavity_2d_physical(n, n)
+
+
+def d2q9_channel_reference(nx=300, ny=100, u_max=0.1, re=220.0, storage=F32):
+    """rust/examples/d2q9_bgk_channel.rs:36-60: nu = u_max * 2r / Re with r = ny/10+1."""
+    r = ny // 10 + 1
+    nu = u_max * 2.0 * r / re
+    cfg = dict(lattice=D2Q9, nx=nx, ny=ny, nz=1, collision=BGK, storage=storage, bc_mode=BC_REFERENCE,
+               omega=omega_from_nu(nu), u_in=u_max, inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_2d_reference(nx, ny)
+
+
+def d2q9_karman(nx=1024, ny=256, cx=256, cy=128, r=20, u_max=0.1, re=150.0, storage=F32):
+    """Channel with a cylinder: Zou-He parabolic inlet, reference-pressure outlet
+    (zero-gradient tangential velocity), no-slip walls.
+    Re = u_max * D / nu with D = 2r."""
+    nu = u_max * 2 * r / re
+    cfg = dict(lattice=D2Q9, nx=nx, ny=ny, nz=1, collision=BGK, storage=storage, bc_mode=BC_PHYSICAL,
+               omega=omega_from_nu(nu), u_in=u_max, inlet_profile=INLET_PARABOLIC)
+    return cfg, g.channel_2d_physical(nx, ny, cx, cy, r)
+
+
+def d3q27_mrt_lid_reference(n=64, u_lid=0.1, storage=F32, engine=MRT_AUTO):
+    """rust/examples/d3q27_mrt_lid.rs: raw monomial M, reference rates and meq table."""
+    cfg = dict(lattice=D3Q27, nx=n, ny=n, nz=n, collision=MRT_RAW, storage=storage,
+               bc_mode=BC_REFERENCE, mrt_engine=engine, meq_mode=MEQ_REFERENCE,
+               s=reference_mrt_rates(), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, g.lid_cavity_3d_reference(n, n, n)
+
+
+def d3q27_mrt_lid_gs(n=128, u_lid=0.05, re=400.0, storage=F32, engine=MRT_AUTO, bc_mode=BC_REFERENCE):
+    """D3Q27 MRT lid cavity with the Gram-Schmidt basis, s_nu = omega, nu = u*n/Re."""
+    nu = u_lid * n / re
+    om = omega_from_nu(nu)
+    geo = g.lid_cavity_3d_reference(n, n, n) if bc_mode == BC_REFERENCE else g.lid_cavity_3d_physical(n, n, n)
+    cfg = dict(lattice=D3Q27, nx=n, ny=n, nz=n, collision=MRT_GS, storage=storage, bc_mode=bc_mode,
+               mrt_engine=engine, omega=om, s=shear_mrt_rates(om), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, geo
+
+
+def d3q27_bgk_lid_reference(n=64, u_lid=0.1, re=None, nu=None, storage=F32):
+    if nu is None:
+        nu = u_lid * n / (re if re else 100.0)
+    cfg = dict(lattice=D3Q27, nx=n, ny=n, nz=n, collision=BGK, storage=storage, bc_mode=BC_REFERENCE,
+               omega=omega_from_nu(nu), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, g.lid_cavity_3d_reference(n, n, n)
+
+
+def d3q27_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
+                         storage=F32, collision=BGK):
+    """Duct with a sphere: uniform Zou-He inlet, reference-pressure outlet; Re = u * D / nu, D = 2r."""
+    nu = u_in * 2 * r / re
+    om = omega_from_nu(nu)
+    cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
+               bc_mode=BC_PHYSICAL, omega=om, s=shear_mrt_rates(om), u_in=u_in,
+               inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_3d_physical(nx, ny, nz, center[0], center[1], center[2], r)

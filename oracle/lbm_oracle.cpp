@@ -1,0 +1,727 @@
+// lbm_oracle.cpp — CPU restatement of the reference lattice Boltzmann time step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP kernels are checked against and
+// the CPU baseline timed beside them.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it; the product library never does.
+//
+// What it restates (paths relative to the reference tree):
+//  * lattice tables     cpp/d2q9_bgk_lid.cpp:72-84, rust/examples/d3q27_mrt_lid.rs:98-105,
+//                       cpp/d3q27_bgk_lid.cpp:86-89 (D3Q27 weights)
+//  * pull map           stream() cpp/d2q9_bgk_lid.cpp:18-28,86-87 (af::shift = circular shift,
+//                       out[x] = in[x - c]); rust/examples/d3q27_mrt_lid.rs:12-48,107-108
+//  * REFERENCE step     cpp/d2q9_bgk_lid.cpp:124-155 in the same pass order: gather through a
+//                       materialised index array, bounce-back capture (TO_REFLECT), rho/u
+//                       reductions, Dirichlet overrides (lid, inlet, solid, density columns:
+//                       rust/examples/d2q9_bgk_channel.rs:190-201), equilibrium + BGK relaxation
+//                       at every node, bounce-back scatter (REFLECTED)
+//  * MRT                rust/examples/d3q27_mrt_lid.rs:167-199 (raw monomial M, inverse),
+//                       :111-150 (rates), :336-387 (m = M f, meq table, relax, f = M^-1 m*),
+//                       :228-287 (initial state rho0/27 + one collision)
+//  * f32 index rounding the FP32 builds index through float arrays (cpp/d2q9_bgk_lid.cpp:86,
+//                       rust/examples/d2q9_bgk_lid.rs:97); f32_index_compat reproduces it.
+//  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
+//                       (non-equilibrium bounce-back) velocity inlet, reference-pressure outlet
+//                       with zero normal gradient of the tangential velocity.
+//
+// Summation order inside a node is ascending population index; ArrayFire's reduction order is
+// not observable without ArrayFire, so FP32 bitwise parity with ArrayFire is unpinnable (FP64
+// differences stay at round-off).  Pinned in tests/test_oracle.py by closed forms, brute force
+// on tiny grids against an independent numpy restatement, and exact-algebra identities.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#include <type_traits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+typedef struct {
+  int32_t lattice, nx, ny, nz, collision, storage, bc_mode, mrt_engine, meq_mode, inlet_profile;
+  double omega;
+  double s[27];
+  double u_wall[3];
+  double u_in;
+  double rho_bc;
+} oracle_desc;  // field-for-field the lbm_desc of the library's C ABI
+}
+
+namespace {
+
+enum { F64 = 0, F32 = 1, F16S = 2, BF16S = 3 };
+enum { BGK = 0, MRT_RAW = 1, MRT_GS = 2 };
+enum { SOLID = 1, MOVING = 2, INLET = 4, OUTLET = 8, RHO_FIX = 16 };
+
+// ---- lattice tables (transcribed from the reference arrays) ------------------------------
+const int CX9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+const int CY9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+const int NBIDX9[8] = {2, 3, 0, 1, 6, 7, 4, 5};
+const int CX27[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+const int CY27[27] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1};
+const int CZ27[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1};
+const int NBIDX27[26] = {1, 0, 3, 2, 5, 4, 9, 8, 7, 6, 13, 12, 11, 10, 17, 16, 15, 14, 25, 24, 23, 22, 21, 20, 19, 18};
+
+struct Lattice {
+  int Q, D;
+  int cx[27], cy[27], cz[27], opp[27];
+  double w[27];  // exact value as double; cast to T where the reference computes in T
+};
+
+Lattice make_lattice(int Q) {
+  Lattice L{};
+  L.Q = Q;
+  if (Q == 9) {
+    L.D = 2;
+    for (int i = 0; i < 9; ++i) { L.cx[i] = CX9[i]; L.cy[i] = CY9[i]; L.cz[i] = 0; }
+    L.opp[0] = 0;
+    for (int j = 0; j < 8; ++j) L.opp[j + 1] = NBIDX9[j] + 1;
+    const double t1 = 4. / 9., t2 = 1. / 9., t3 = 1. / 36.;
+    const double w[9] = {t1, t2, t2, t2, t2, t3, t3, t3, t3};
+    for (int i = 0; i < 9; ++i) L.w[i] = w[i];
+  } else {
+    L.D = 3;
+    for (int i = 0; i < 27; ++i) { L.cx[i] = CX27[i]; L.cy[i] = CY27[i]; L.cz[i] = CZ27[i]; }
+    L.opp[0] = 0;
+    for (int j = 0; j < 26; ++j) L.opp[j + 1] = NBIDX27[j] + 1;
+    const double t1 = 8. / 27., t2 = 2. / 27., t3 = 1. / 54., t4 = 1. / 216.;
+    for (int i = 0; i < 27; ++i) L.w[i] = (i == 0) ? t1 : (i <= 6 ? t2 : (i <= 18 ? t3 : t4));
+  }
+  return L;
+}
+
+// ---- storage codecs (round to nearest even) ----------------------------------------------
+uint16_t f2h(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // >= 65520 rounds to inf
+  if (ax < 0x38800000u) {                                    // half subnormal range
+    float v;
+    std::memcpy(&v, &ax, 4);
+    uint32_t m = (uint32_t)std::nearbyint((double)v * 16777216.0);  // exact scaling, RNE
+    return (uint16_t)(sign | m);
+  }
+  uint32_t e = (ax >> 23) - 127 + 15, mant = ax & 0x7fffffu;
+  uint32_t h = (e << 10) | (mant >> 13), rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return (uint16_t)(sign | h);
+}
+float h2f(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float v;
+  if (e == 0) {
+    v = (float)std::ldexp((double)m, -24);
+    return sign ? -v : v;
+  }
+  uint32_t x;
+  if (e == 31) x = sign | 0x7f800000u | (m << 13);
+  else x = sign | ((e - 15 + 127) << 23) | (m << 13);
+  std::memcpy(&v, &x, 4);
+  return v;
+}
+uint16_t f2bf(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  uint32_t lsb = (x >> 16) & 1u;
+  x += 0x7fffu + lsb;
+  return (uint16_t)(x >> 16);
+}
+float bf2f(uint16_t b) {
+  uint32_t x = (uint32_t)b << 16;
+  float v;
+  std::memcpy(&v, &x, 4);
+  return v;
+}
+
+// ---- MRT matrices ------------------------------------------------------------------------
+// Raw monomial rows in the reference's order (rust/examples/d3q27_mrt_lid.rs:167-193).
+void raw_moment_matrix(const Lattice &L, double M[27][27]) {
+  for (int i = 0; i < 27; ++i) {
+    const double x = L.cx[i], y = L.cy[i], z = L.cz[i];
+    const double r[27] = {1, x, y, z, x * y, x * z, y * z, x * x + y * y + z * z, x * x - y * y,
+                          x * x - z * z, x * y * y, x * z * z, y * x * x, z * x * x, y * z * z,
+                          z * y * y, x * y * z, x * x * y * y, x * x * z * z, y * y * z * z,
+                          x * x * y * z, x * y * y * z, x * y * z * z, x * y * y * z * z,
+                          x * x * y * z * z, x * x * y * y * z, x * x * y * y * z * z};
+    for (int k = 0; k < 27; ++k) M[k][i] = r[k];
+  }
+}
+// Classical Gram-Schmidt of the raw rows, in row order, unweighted inner product over the 27
+// velocities.  Rows are left unnormalised (any row scaling leaves the collision unchanged).
+void gs_moment_matrix(const Lattice &L, double M[27][27]) {
+  double R[27][27];
+  raw_moment_matrix(L, R);
+  long double G[27][27];
+  for (int k = 0; k < 27; ++k) {
+    for (int i = 0; i < 27; ++i) G[k][i] = R[k][i];
+    for (int j = 0; j < k; ++j) {
+      long double num = 0, den = 0;
+      for (int i = 0; i < 27; ++i) { num += (long double)R[k][i] * G[j][i]; den += G[j][i] * G[j][i]; }
+      const long double c = num / den;
+      for (int i = 0; i < 27; ++i) G[k][i] -= c * G[j][i];
+    }
+  }
+  for (int k = 0; k < 27; ++k)
+    for (int i = 0; i < 27; ++i) M[k][i] = (double)G[k][i];
+}
+// Gauss-Jordan inverse with partial pivoting in long double (stands in for the reference's
+// LAPACK inverse, rust/examples/d3q27_mrt_lid.rs:196).
+bool invert27(const double A[27][27], double Ainv[27][27]) {
+  long double a[27][54];
+  for (int r = 0; r < 27; ++r)
+    for (int c = 0; c < 54; ++c) a[r][c] = (c < 27) ? A[r][c] : (c - 27 == r ? 1.0L : 0.0L);
+  for (int c = 0; c < 27; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 27; ++r)
+      if (std::fabs((double)a[r][c]) > std::fabs((double)a[p][c])) p = r;
+    if (std::fabs((double)a[p][c]) < 1e-12) return false;
+    if (p != c)
+      for (int k = 0; k < 54; ++k) std::swap(a[p][k], a[c][k]);
+    const long double piv = a[c][c];
+    for (int k = 0; k < 54; ++k) a[c][k] /= piv;
+    for (int r = 0; r < 27; ++r) {
+      if (r == c || a[r][c] == 0) continue;
+      const long double f = a[r][c];
+      for (int k = 0; k < 54; ++k) a[r][k] -= f * a[c][k];
+    }
+  }
+  for (int r = 0; r < 27; ++r)
+    for (int c = 0; c < 27; ++c) Ainv[r][c] = (double)a[r][c + 27];
+  return true;
+}
+
+// ---- simulation state ---------------------------------------------------------------------
+template <class T>
+struct Sim {
+  oracle_desc d;
+  Lattice L;
+  int64_t N;
+  int nx, ny, nz;
+  const uint8_t *type;
+  std::vector<T> F;          // current post-collision populations (decoded for shifted storage)
+  std::vector<int64_t> nb;   // materialised pull map (REFERENCE mode)
+  std::vector<int64_t> on;   // linear indices of solid nodes, ascending (af::where)
+  // MRT
+  T M[27][27], Minv[27][27], s[27];
+  double Md[27][27];
+  bool mrt = false;
+  T w[27];
+
+  Sim(const oracle_desc &dd, const uint8_t *types) : d(dd), L(make_lattice(dd.lattice)), type(types) {
+    nx = d.nx; ny = d.ny; nz = d.nz;
+    N = (int64_t)nx * ny * nz;
+    for (int i = 0; i < L.Q; ++i) w[i] = (T)L.w[i];
+    if (d.collision != BGK) {
+      mrt = true;
+      double Mi[27][27];
+      if (d.collision == MRT_RAW) raw_moment_matrix(L, Md);
+      else gs_moment_matrix(L, Md);
+      invert27(Md, Mi);
+      for (int k = 0; k < 27; ++k)
+        for (int i = 0; i < 27; ++i) { M[k][i] = (T)Md[k][i]; Minv[k][i] = (T)Mi[k][i]; }
+      for (int k = 0; k < 27; ++k) s[k] = (T)d.s[k];
+    }
+  }
+
+  int64_t idx(int x, int y, int z) const { return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z); }
+  static int wrap(int a, int n) { return ((a % n) + n) % n; }
+
+  // nb_index[p(x,i)] = p((x - c_i) mod dims, i); f32 builds round the index through float.
+  void build_pull_map(bool f32_compat) {
+    nb.resize((size_t)(N * L.Q));
+    for (int i = 0; i < L.Q; ++i)
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const int64_t src = idx(wrap(x - L.cx[i], nx), wrap(y - L.cy[i], ny), wrap(z - L.cz[i], nz)) + N * i;
+            nb[(size_t)(idx(x, y, z) + N * i)] = f32_compat ? (int64_t)(float)src : src;
+          }
+    on.clear();
+    for (int64_t n = 0; n < N; ++n)
+      if (type[n] & SOLID) on.push_back(n);
+  }
+
+  T inlet_u(int y, int z) const {
+    if (d.inlet_profile == 0) return (T)d.u_in;
+    const T H = (T)(ny - 2), eta = (T)y - (T)0.5;
+    T u = (((T)4 * (T)d.u_in) * eta * (H - eta)) / (H * H);
+    if (L.D == 3) {
+      const T Hz = (T)(nz - 2), ez = (T)z - (T)0.5;
+      u = u * (((T)4 * ez * (Hz - ez)) / (Hz * Hz));
+    }
+    return u;
+  }
+
+  // Reference overrides in the reference's order (cpp/d2q9_bgk_lid.cpp:141-146,
+  // rust/examples/d2q9_bgk_channel.rs:190-201, rust/examples/d3q27_mrt_lid.rs:323-330).
+  void overrides(int64_t n, int y, int z, T &rho, T &ux, T &uy, T &uz) const {
+    const uint8_t t = type[n];
+    if (t & MOVING) { ux = (T)d.u_wall[0]; uy = (T)d.u_wall[1]; uz = (T)d.u_wall[2]; }
+    if (t & INLET) ux = inlet_u(y, z);
+    if (t & SOLID) { ux = 0; uy = 0; uz = 0; rho = 0; }
+    if (t & RHO_FIX) rho = (T)d.rho_bc;
+  }
+
+  // BGK equilibrium + relaxation at one node (cpp/d2q9_bgk_lid.cpp:149-153 operation order).
+  void bgk(const T *S, T rho, T ux, T uy, T uz, T *out) const {
+    const T omega = (T)d.omega, om1 = (T)1 - omega;
+    T usq = ux * ux + uy * uy;
+    if (L.D == 3) usq = usq + uz * uz;
+    for (int i = 0; i < L.Q; ++i) {
+      T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
+      if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
+      const T feq = (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+      out[i] = omega * feq + om1 * S[i];
+    }
+  }
+
+  // Equilibrium moments (rust/examples/d3q27_mrt_lid.rs:341-383 operation order).
+  void meq_reference(T r, T ux, T uy, T uz, T *meq) const {
+    const T cs2 = (T)1 / (T)3, cs4 = cs2 * cs2, cs6 = cs2 * cs2 * cs2;
+    const T rvx = r * ux, rvy = r * uy, rvz = r * uz, rc2 = cs2 * r, rc4 = cs4 * r, rc6 = cs6 * r;
+    const T vx2 = ux * ux, vy2 = uy * uy, vz2 = uz * uz, v2 = vx2 + vy2 + vz2;
+    meq[0] = r; meq[1] = rvx; meq[2] = rvy; meq[3] = rvz;
+    meq[4] = rvx * uy; meq[5] = rvx * uz; meq[6] = rvy * uz;
+    meq[7] = r + rvx * ux + rvy * uy + rvz * uz;
+    meq[8] = r * (vx2 - vy2); meq[9] = r * (vx2 - vz2);
+    meq[10] = rc2 * ux; meq[11] = rc2 * ux; meq[12] = rc2 * uy; meq[13] = rc2 * uz;
+    meq[14] = rc2 * uy; meq[15] = rc2 * uz; meq[16] = 0;
+    meq[17] = rc2 * (cs2 + vx2 + vy2); meq[18] = rc2 * (cs2 + vx2 + vz2); meq[19] = rc2 * (cs2 + vy2 + vz2);
+    meq[20] = rc2 * uy * uz; meq[21] = rc2 * ux * uz; meq[22] = rc2 * ux * uy;
+    meq[23] = rc4 * ux; meq[24] = rc4 * uy; meq[25] = rc4 * uz;
+    meq[26] = v2 + rc6;
+  }
+
+  // m = M S; m* = m - s (m - meq); out = M^-1 m*  (rust/examples/d3q27_mrt_lid.rs:336,385-387).
+  void mrt_collide(const T *S, T rho, T ux, T uy, T uz, T *out) const {
+    T m[27], meq[27], mc[27];
+    for (int k = 0; k < 27; ++k) {
+      T acc = 0;
+      for (int i = 0; i < 27; ++i) acc += S[i] * M[k][i];
+      m[k] = acc;
+    }
+    if (d.collision == MRT_RAW && d.meq_mode == 0) {
+      meq_reference(rho, ux, uy, uz, meq);
+    } else {  // projected: meq = M f_eq(rho, u)
+      T feq[27];
+      const T usq = ux * ux + uy * uy + uz * uz;
+      for (int i = 0; i < 27; ++i) {
+        const T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy + (T)L.cz[i] * uz;
+        feq[i] = (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+      }
+      for (int k = 0; k < 27; ++k) {
+        T acc = 0;
+        for (int i = 0; i < 27; ++i) acc += feq[i] * M[k][i];
+        meq[k] = acc;
+      }
+    }
+    for (int k = 0; k < 27; ++k) mc[k] = m[k] - (m[k] - meq[k]) * s[k];
+    for (int i = 0; i < 27; ++i) {
+      T acc = 0;
+      for (int k = 0; k < 27; ++k) acc += mc[k] * Minv[i][k];
+      out[i] = acc;
+    }
+  }
+
+  void moments(const T *S, T &rho, T &ux, T &uy, T &uz) const {
+    T r = 0, jx = 0, jy = 0, jz = 0;
+    for (int i = 0; i < L.Q; ++i) r += S[i];
+    for (int i = 0; i < L.Q; ++i) jx += (T)L.cx[i] * S[i];
+    for (int i = 0; i < L.Q; ++i) jy += (T)L.cy[i] * S[i];
+    if (L.D == 3)
+      for (int i = 0; i < L.Q; ++i) jz += (T)L.cz[i] * S[i];
+    rho = r; ux = jx / r; uy = jy / r; uz = (L.D == 3) ? jz / r : (T)0;
+  }
+
+  // ---- REFERENCE step, multi-pass as the reference's ArrayFire program ----
+  std::vector<T> S, B, RHO, UX, UY, UZ;
+  void ref_step() {
+    const int Q = L.Q;
+    const int64_t NQ = N * Q, nON = (int64_t)on.size();
+    S.resize((size_t)NQ);
+    B.resize((size_t)(nON * (Q - 1)));
+    RHO.resize((size_t)N); UX.resize((size_t)N); UY.resize((size_t)N); UZ.resize((size_t)N);
+    // K1 gather: F_streamed = F(nb_index)
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < NQ; ++p) S[(size_t)p] = F[(size_t)nb[(size_t)p]];
+    // K2 capture: BOUNCEDBACK = F_streamed(TO_REFLECT), k = s + nON*(j-1)
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nON * (Q - 1); ++k) {
+      const int64_t sidx = k % nON, j = k / nON + 1;
+      B[(size_t)k] = S[(size_t)(on[(size_t)sidx] + N * j)];
+    }
+    // K3 reductions + K5 overrides
+#pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < N; ++n) {
+      T Sn[27];
+      for (int i = 0; i < Q; ++i) Sn[i] = S[(size_t)(n + N * i)];
+      T rho, ux, uy, uz;
+      moments(Sn, rho, ux, uy, uz);
+      const int y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
+      overrides(n, y, z, rho, ux, uy, uz);
+      RHO[(size_t)n] = rho; UX[(size_t)n] = ux; UY[(size_t)n] = uy; UZ[(size_t)n] = uz;
+    }
+    // K4/K7 collision at every node
+#pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < N; ++n) {
+      T Sn[27], out[27];
+      for (int i = 0; i < Q; ++i) Sn[i] = S[(size_t)(n + N * i)];
+      if (mrt) mrt_collide(Sn, RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
+      else bgk(Sn, RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
+      for (int i = 0; i < Q; ++i) F[(size_t)(n + N * i)] = out[i];
+    }
+    // K6 scatter: F(REFLECTED) = BOUNCEDBACK
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nON * (Q - 1); ++k) {
+      const int64_t sidx = k % nON, j = k / nON + 1;
+      F[(size_t)(on[(size_t)sidx] + N * L.opp[j])] = B[(size_t)k];
+    }
+  }
+
+  // ---- PHYSICAL step: node loop ----
+  // Non-equilibrium bounce-back closure of the unknown populations (c_x == sgn) of an x-face
+  // node with prescribed rho, u; transverse momentum corrected through
+  // N_a = sum_{c_x=0} f c_a - (2/3) rho u_a (Zou-He in 2D).
+  void nebb(T *f, int sgn, T rho, T ux, T uy, T uz) const {
+    T ny_ = 0, nz_ = 0;
+    for (int i = 0; i < L.Q; ++i)
+      if (L.cx[i] == 0) { ny_ += (T)L.cy[i] * f[i]; nz_ += (T)L.cz[i] * f[i]; }
+    ny_ = ny_ - (((T)2 / (T)3) * rho) * uy;
+    nz_ = nz_ - (((T)2 / (T)3) * rho) * uz;
+    for (int i = 0; i < L.Q; ++i)
+      if (L.cx[i] == sgn) {
+        T cu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
+        if (L.D == 3) cu = cu + (T)L.cz[i] * uz;
+        f[i] = (f[L.opp[i]] + (((T)6 * w[i]) * rho) * cu) -
+               ((T)18 * w[i]) * ((T)L.cy[i] * ny_ + (T)L.cz[i] * nz_);
+      }
+  }
+
+  // Pulled populations of fluid node (x,y,z) with the physical boundary rules applied.
+  void phys_pull(const std::vector<T> &Fin, int x, int y, int z, T *f) const {
+    const int64_t n = idx(x, y, z);
+    const uint8_t t = type[n];
+    const T rw = (T)d.rho_bc;
+    for (int i = 0; i < L.Q; ++i) {
+      const int64_t src = idx(wrap(x - L.cx[i], nx), wrap(y - L.cy[i], ny), wrap(z - L.cz[i], nz));
+      if (type[src] & SOLID) {
+        T v = Fin[(size_t)(n + N * L.opp[i])];
+        if (type[src] & MOVING) {
+          T cu = (T)L.cx[i] * (T)d.u_wall[0] + (T)L.cy[i] * (T)d.u_wall[1];
+          if (L.D == 3) cu = cu + (T)L.cz[i] * (T)d.u_wall[2];
+          v = v + (((T)6 * w[i]) * rw) * cu;
+        }
+        f[i] = v;
+      } else {
+        f[i] = Fin[(size_t)(src + N * i)];
+      }
+    }
+    if (t & INLET) {  // velocity inlet, unknowns c_x = +1
+      const T uin = inlet_u(y, z);
+      T s0 = 0, sm = 0;
+      for (int i = 0; i < L.Q; ++i) {
+        if (L.cx[i] == 0) s0 += f[i];
+        if (L.cx[i] == -1) sm += f[i];
+      }
+      const T rho = (s0 + (T)2 * sm) / ((T)1 - uin);
+      nebb(f, 1, rho, uin, (T)0, (T)0);
+    } else if (t & OUTLET) {  // reference-pressure outlet, unknowns c_x = -1
+      T uy = 0, uz = 0;
+      if (x > 0 && !(type[n - 1] & SOLID)) {  // tangential velocity of the upstream neighbour
+        T g[27], r, vx, vy, vz;
+        for (int i = 0; i < L.Q; ++i) g[i] = Fin[(size_t)(n - 1 + N * i)];
+        moments(g, r, vx, vy, vz);
+        uy = vy;
+        uz = vz;
+      }
+      T s0 = 0, sp = 0;
+      for (int i = 0; i < L.Q; ++i) {
+        if (L.cx[i] == 0) s0 += f[i];
+        if (L.cx[i] == 1) sp += f[i];
+      }
+      const T rho = rw;
+      const T ux = (s0 + (T)2 * sp) / rho - (T)1;
+      nebb(f, -1, rho, ux, uy, uz);
+    }
+  }
+
+  std::vector<T> Fnext;
+  void phys_step() {
+    const int Q = L.Q;
+    Fnext.resize(F.size());
+#pragma omp parallel for schedule(static)
+    for (int64_t n = 0; n < N; ++n) {
+      const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
+      if (type[n] & SOLID) {
+        for (int i = 0; i < Q; ++i) Fnext[(size_t)(n + N * i)] = F[(size_t)(n + N * i)];
+        continue;
+      }
+      T f[27], out[27], rho, ux, uy, uz;
+      phys_pull(F, x, y, z, f);
+      moments(f, rho, ux, uy, uz);
+      if (mrt) mrt_collide(f, rho, ux, uy, uz, out);
+      else bgk(f, rho, ux, uy, uz, out);
+      for (int i = 0; i < Q; ++i) Fnext[(size_t)(n + N * i)] = out[i];
+    }
+    F.swap(Fnext);
+  }
+
+  // storage codec applied to the whole state (shifted FP16/BF16)
+  void codec() {
+    if constexpr (std::is_same<T, float>::value) {
+      if (d.storage != F16S && d.storage != BF16S) return;
+      const int Q = L.Q;
+#pragma omp parallel for schedule(static)
+      for (int64_t n = 0; n < N; ++n)
+        for (int i = 0; i < Q; ++i) {
+          float &v = F[(size_t)(n + N * i)];
+          const float g = v - w[i];
+          v = (d.storage == F16S ? h2f(f2h(g)) : bf2f(f2bf(g))) + w[i];
+        }
+    }
+  }
+
+  void step() {
+    if (d.bc_mode == 0) ref_step();
+    else phys_step();
+    codec();
+  }
+
+  // state in/out as doubles; shifted storages use decode = half + w in double (exact)
+  void load(const double *f) {
+    F.resize((size_t)(N * L.Q));
+    for (int i = 0; i < L.Q; ++i)
+      for (int64_t n = 0; n < N; ++n) {
+        const double v = f[n + N * i];
+        if (d.storage == F16S || d.storage == BF16S) {
+          const float g = (float)(v - (double)(float)L.w[i]);
+          const float hv = (d.storage == F16S) ? h2f(f2h(g)) : bf2f(f2bf(g));
+          F[(size_t)(n + N * i)] = (T)(hv + (float)L.w[i]);
+        } else {
+          F[(size_t)(n + N * i)] = (T)v;
+        }
+      }
+  }
+  void store(double *f) const {
+    for (int i = 0; i < L.Q; ++i)
+      for (int64_t n = 0; n < N; ++n) {
+        const T v = F[(size_t)(n + N * i)];
+        if (d.storage == F16S || d.storage == BF16S) {
+          const float fv = (float)v;
+          const float g = fv - (float)L.w[i];
+          const float hv = (d.storage == F16S) ? h2f(f2h(g)) : bf2f(f2bf(g));
+          f[n + N * i] = (double)hv + (double)(float)L.w[i];
+        } else {
+          f[n + N * i] = (double)v;
+        }
+      }
+  }
+};
+
+bool desc_ok(const oracle_desc *d) {
+  if (!d) return false;
+  if (d->lattice != 9 && d->lattice != 27) return false;
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return false;
+  if (d->lattice == 9 && d->nz != 1) return false;
+  if (d->collision != BGK && d->lattice != 27) return false;
+  return true;
+}
+
+template <class T>
+int run_impl(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32c) {
+  Sim<T> sim(*d, types);
+  sim.load(f);
+  if (d->bc_mode == 0) sim.build_pull_map(f32c != 0);
+  for (int64_t t = 0; t < nsteps; ++t) sim.step();
+  sim.store(f);
+  return 0;
+}
+
+template <class T>
+int init_fields_impl(const oracle_desc *d, const uint8_t *types, const double *rho, const double *ux,
+                     const double *uy, const double *uz, double *f) {
+  Sim<T> sim(*d, types);
+  const int Q = sim.L.Q;
+  sim.F.assign((size_t)(sim.N * Q), (T)0);
+  for (int64_t n = 0; n < sim.N; ++n) {
+    const T r = rho ? (T)rho[n] : (T)1, vx = ux ? (T)ux[n] : (T)0, vy = uy ? (T)uy[n] : (T)0,
+            vz = (uz && sim.L.D == 3) ? (T)uz[n] : (T)0;
+    T zero[27] = {0}, out[27];
+    // F = f_eq: the BGK relaxation with omega = 1 evaluates w*rho*(...) exactly as f_eq
+    T usq = vx * vx + vy * vy;
+    if (sim.L.D == 3) usq = usq + vz * vz;
+    for (int i = 0; i < Q; ++i) {
+      T eu = (T)sim.L.cx[i] * vx + (T)sim.L.cy[i] * vy;
+      if (sim.L.D == 3) eu = eu + (T)sim.L.cz[i] * vz;
+      out[i] = (sim.w[i] * r) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+    }
+    (void)zero;
+    for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
+  }
+  sim.codec();
+  sim.store(f);
+  return 0;
+}
+
+template <class T>
+int init_reference_impl(const oracle_desc *d, const uint8_t *types, double rho0, double *f) {
+  Sim<T> sim(*d, types);
+  const int Q = sim.L.Q;
+  sim.F.assign((size_t)(sim.N * Q), (T)0);
+  for (int64_t n = 0; n < sim.N; ++n) {
+    T out[27];
+    if (sim.mrt) {
+      T S[27];
+      for (int i = 0; i < 27; ++i) S[i] = (T)rho0 / (T)27;
+      sim.mrt_collide(S, (T)rho0, 0, 0, 0, out);
+    } else {
+      for (int i = 0; i < Q; ++i) out[i] = (sim.w[i] * (T)rho0) * ((((T)1 + (T)0) + (T)0) - (T)0);
+    }
+    for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
+  }
+  sim.codec();
+  sim.store(f);
+  return 0;
+}
+
+template <class T>
+int fields_impl(const oracle_desc *d, const uint8_t *types, const double *f, double *rho, double *ux,
+                double *uy, double *uz) {
+  Sim<T> sim(*d, types);
+  sim.load(f);
+  const int Q = sim.L.Q;
+  for (int z = 0; z < sim.nz; ++z)
+    for (int y = 0; y < sim.ny; ++y)
+      for (int x = 0; x < sim.nx; ++x) {
+        const int64_t n = sim.idx(x, y, z);
+        T S[27], r, vx, vy, vz;
+        if (d->bc_mode == 0) {
+          for (int i = 0; i < Q; ++i) {
+            const int64_t src = sim.idx(Sim<T>::wrap(x - sim.L.cx[i], sim.nx), Sim<T>::wrap(y - sim.L.cy[i], sim.ny),
+                                        Sim<T>::wrap(z - sim.L.cz[i], sim.nz));
+            S[i] = sim.F[(size_t)(src + sim.N * i)];
+          }
+          sim.moments(S, r, vx, vy, vz);
+          sim.overrides(n, y, z, r, vx, vy, vz);
+        } else if (types[n] & SOLID) {
+          r = 0; vx = 0; vy = 0; vz = 0;
+        } else {
+          sim.phys_pull(sim.F, x, y, z, S);
+          sim.moments(S, r, vx, vy, vz);
+        }
+        if (rho) rho[n] = (double)r;
+        if (ux) ux[n] = (double)vx;
+        if (uy) uy[n] = (double)vy;
+        if (uz) uz[n] = (double)vz;
+      }
+  return 0;
+}
+
+template <class T>
+double time_impl(const oracle_desc *d, const uint8_t *types, int64_t nsteps) {
+  Sim<T> sim(*d, types);
+  const int Q = sim.L.Q;
+  sim.F.assign((size_t)(sim.N * Q), (T)0);
+  for (int i = 0; i < Q; ++i)
+    for (int64_t n = 0; n < sim.N; ++n) sim.F[(size_t)(n + sim.N * i)] = sim.w[i];
+  if (d->bc_mode == 0) sim.build_pull_map(false);
+  sim.step();  // warm-up (page in temporaries)
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int64_t t = 0; t < nsteps; ++t) sim.step();
+  const auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+bool is_f64(const oracle_desc *d) { return d->storage == F64; }
+
+}  // namespace
+
+extern "C" {
+
+int oracle_abi(void) { return 1; }
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Lattice tables: c (3*Q ints, x then y then z rows), w (Q), opp (Q).
+int oracle_tables(int Q, int32_t *c, double *w, int32_t *opp) {
+  if (Q != 9 && Q != 27) return 1;
+  Lattice L = make_lattice(Q);
+  for (int i = 0; i < Q; ++i) {
+    c[i] = L.cx[i]; c[Q + i] = L.cy[i]; c[2 * Q + i] = L.cz[i];
+    w[i] = L.w[i]; opp[i] = L.opp[i];
+  }
+  return 0;
+}
+
+// basis 1 = raw monomial (reference), 2 = Gram-Schmidt.  Row-major 27x27.
+int oracle_mrt_matrices(int basis, double *M, double *Minv) {
+  Lattice L = make_lattice(27);
+  double A[27][27], Ai[27][27];
+  if (basis == 1) raw_moment_matrix(L, A);
+  else if (basis == 2) gs_moment_matrix(L, A);
+  else return 1;
+  if (!invert27(A, Ai)) return 2;
+  for (int k = 0; k < 27; ++k)
+    for (int i = 0; i < 27; ++i) { M[k * 27 + i] = A[k][i]; Minv[k * 27 + i] = Ai[k][i]; }
+  return 0;
+}
+
+int oracle_init_fields(const oracle_desc *d, const uint8_t *types, const double *rho, const double *ux,
+                       const double *uy, const double *uz, double *f) {
+  if (!desc_ok(d)) return 1;
+  return is_f64(d) ? init_fields_impl<double>(d, types, rho, ux, uy, uz, f)
+                   : init_fields_impl<float>(d, types, rho, ux, uy, uz, f);
+}
+
+int oracle_init_reference(const oracle_desc *d, const uint8_t *types, double rho0, double *f) {
+  if (!desc_ok(d)) return 1;
+  return is_f64(d) ? init_reference_impl<double>(d, types, rho0, f)
+                   : init_reference_impl<float>(d, types, rho0, f);
+}
+
+int oracle_run(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32_index_compat) {
+  if (!desc_ok(d)) return 1;
+  return is_f64(d) ? run_impl<double>(d, types, f, nsteps, f32_index_compat)
+                   : run_impl<float>(d, types, f, nsteps, f32_index_compat);
+}
+
+int oracle_fields(const oracle_desc *d, const uint8_t *types, const double *f, double *rho, double *ux,
+                  double *uy, double *uz) {
+  if (!desc_ok(d)) return 1;
+  return is_f64(d) ? fields_impl<double>(d, types, f, rho, ux, uy, uz)
+                   : fields_impl<float>(d, types, f, rho, ux, uy, uz);
+}
+
+// Seconds for nsteps steps from a rest state (CPU baseline timing).
+double oracle_time_steps(const oracle_desc *d, const uint8_t *types, int64_t nsteps) {
+  if (!desc_ok(d)) return -1.0;
+  return is_f64(d) ? time_impl<double>(d, types, nsteps) : time_impl<float>(d, types, nsteps);
+}
+
+// Codecs exposed for tests (checked against numpy's float16 conversion).
+uint16_t oracle_f2h(float f) { return f2h(f); }
+float oracle_h2f(uint16_t h) { return h2f(h); }
+uint16_t oracle_f2bf(float f) { return f2bf(f); }
+float oracle_bf2f(uint16_t b) { return bf2f(b); }
+
+}  // extern "C"

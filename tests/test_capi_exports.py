@@ -1,0 +1,64 @@
+"""The C-ABI library loads and exports every symbol include/lbm_capi.h declares; descriptor
+validation (pure host code) rejects bad input.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from lbm_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "lbm_capi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert declared_functions() == sorted(capi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol(lbm_lib):
+    missing = [s for s in declared_functions() if not hasattr(lbm_lib, s)]
+    assert not missing, missing
+    assert lbm_lib.lbm_abi_version() == 1
+
+
+def test_desc_layout_matches_header():
+    # 10 int32 fields, then 1 + 27 + 3 + 1 + 1 doubles
+    assert ctypes.sizeof(capi.LbmDesc) == 40 + 8 * 33
+    assert capi.LbmDesc.omega.offset == 40
+
+
+@pytest.mark.parametrize("bad", [
+    dict(lattice=7),
+    dict(nz=2),
+    dict(collision=1),
+    dict(storage=9),
+    dict(bc_mode=3),
+    dict(nx=0),
+    dict(omega=float("nan")),
+])
+def test_create_rejects_bad_descriptors(lbm_lib, bad):
+    cfg = dict(lattice=9, nx=8, ny=8, nz=1, omega=1.0)
+    cfg.update(bad)
+    d = capi.make_desc(cfg)
+    types = np.zeros(8 * 8 * max(cfg["nz"], 1), np.uint8)
+    h = ctypes.c_void_p()
+    st = lbm_lib.lbm_create(ctypes.byref(d), types.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                            ctypes.byref(h))
+    assert st == 1  # LBM_EINVAL, before any device call
+    assert lbm_lib.lbm_last_error()
+
+
+def test_footprint_bytes(lbm_lib):
+    cfg = dict(lattice=27, nx=64, ny=32, nz=16, collision=0, storage=capi.F16S)
+    N = 64 * 32 * 16
+    fp = capi.footprint_bytes(cfg)
+    assert fp >= 2 * N * 27 * 2 + N
+    assert fp < 2 * N * 27 * 2 + N + 64 * 1024
+    assert capi.footprint_bytes(dict(cfg, lattice=5)) == 0

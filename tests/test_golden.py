@@ -1,0 +1,44 @@
+"""Golden vectors (tests/golden/*.npz, written by make_golden.py from the oracle).
+
+CPU: the oracle still reproduces every fixture bit for bit (regression pin).
+GPU: the FP64 kernels reproduce every fixture bit for bit through the C ABI.
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+IDS = [os.path.basename(f)[:-4] for f in FILES]
+
+
+def load(path):
+    z = np.load(path, allow_pickle=False)
+    return json.loads(str(z["cfg"])), z["types"], z["f0"], z["f1"], int(z["steps"])
+
+
+def test_fixtures_present():
+    assert len(FILES) >= 8
+
+
+@pytest.mark.parametrize("path", FILES, ids=IDS)
+def test_oracle_reproduces_golden(path):
+    cfg, types, f0, f1, steps = load(path)
+    assert np.array_equal(orc.run(cfg, types, f0, steps), f1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FILES, ids=IDS)
+def test_kernel_reproduces_golden_bitwise(path):
+    from lbm_amd import Lattice
+    cfg, types, f0, f1, steps = load(path)
+    with Lattice(cfg, types) as lat:
+        lat.set_populations(f0)
+        lat.step(steps)
+        got = lat.populations()
+    assert np.array_equal(got, f1), float(np.abs(got - f1).max())

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
+
+Tolerances: FP64 within 1e-12 relative L2 on rho/u and populations; FP32 storage
+against the FP32 oracle within 1e-5 (same operation order, usually bit-identical); shifted
+FP16/BF16 storage within 1e-3 relative L2 on rho/u against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from lbm_amd import Lattice, cases, geometry as geo
+from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_RAW, MRT_SCALAR
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    b = np.asarray(b)
+    return float(np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def noisy_fields(types, Q, seed, u0=(0.0, 0.0, 0.0), amp=1e-3):
+    rng = np.random.default_rng(seed)
+    N = types.size
+    u = np.zeros((3, N))
+    for a in range(2 if Q == 9 else 3):
+        u[a] = u0[a] + amp * rng.standard_normal(N)
+    return 1.0 + 0.1 * amp * rng.standard_normal(N), u
+
+
+def run_both(cfg, types, steps, seed=0, u0=(0.0, 0.0, 0.0), amp=1e-3, storage=None):
+    """Same initial fields for kernel and oracle; returns (gpu f, oracle f, gpu fields, oracle fields)."""
+    types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1)
+    rho, u = noisy_fields(types, cfg["lattice"], seed, u0, amp)
+    with Lattice(cfg, types) as lat:
+        lat.init_fields(rho, *u)
+        lat.step(steps)
+        fg = lat.populations()
+        flds_g = lat.fields()
+    f0 = orc.init_fields(cfg, types, rho, *u)
+    fo = orc.run(cfg, types, f0, steps)
+    flds_o = orc.fields(cfg, types, fo)
+    return fg, fo, flds_g, flds_o
+
+
+PARITY = [
+    ("d2q9_lid_ref_128", lambda st: cases.d2q9_lid_reference(128, storage=st), 1000),
+    ("d2q9_channel_ref_300x100", lambda st: cases.d2q9_channel_reference(300, 100, storage=st), 500),
+    ("d2q9_karman_phys_256x64", lambda st: cases.d2q9_karman(256, 64, 64, 32, 6, storage=st), 500),
+    ("d2q9_lid_phys_64", lambda st: cases.d2q9_lid_physical(64, storage=st), 500),
+    ("d3q27_bgk_lid_ref_24", lambda st: cases.d3q27_bgk_lid_reference(24, u_lid=0.05, re=100.0, storage=st), 100),
+    ("d3q27_mrt_raw_lid_ref_20", lambda st: cases.d3q27_mrt_lid_reference(20, storage=st, engine=MRT_SCALAR), 60),
+    ("d3q27_mrt_gs_lid_ref_20", lambda st: cases.d3q27_mrt_lid_gs(20, re=100.0, storage=st, engine=MRT_SCALAR), 60),
+    ("d3q27_sphere_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, storage=st), 100),
+    ("d3q27_sphere_mrt_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(
+        40, 20, 20, (12, 10, 10), 4, storage=st, collision=MRT_GS), 60),
+]
+
+
+@pytest.mark.parametrize("name,make,steps", PARITY, ids=[p[0] for p in PARITY])
+def test_fp64_parity(name, make, steps):
+    cfg, types = make(F64)
+    cfg = dict(cfg, mrt_engine=MRT_SCALAR) if cfg.get("collision", 0) else cfg
+    fg, fo, flg, flo = run_both(cfg, types, steps, seed=11)
+    e = rel_l2(fg, fo)
+    print(f"{name}: FP64 populations rel-L2 {e:.3e}, bitwise {np.array_equal(fg, fo)}")
+    assert e <= 1e-12
+    for k in range(3 if cfg["lattice"] == 9 else 4):
+        if np.linalg.norm(flo[k]) > 0:
+            assert rel_l2(flg[k], flo[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("name,make,steps", PARITY, ids=[p[0] for p in PARITY])
+def test_fp32_parity(name, make, steps):
+    cfg, types = make(F32)
+    cfg = dict(cfg, mrt_engine=MRT_SCALAR) if cfg.get("collision", 0) else cfg
+    fg, fo, flg, flo = run_both(cfg, types, steps, seed=12)
+    e = rel_l2(fg, fo)
+    print(f"{name}: FP32 populations rel-L2 {e:.3e}, bitwise {np.array_equal(fg, fo)}")
+    assert e <= 1e-5
+
+
+SHIFTED = [
+    ("d2q9_lid_ref_128", lambda st: cases.d2q9_lid_reference(128, storage=st), 2000),
+    ("d2q9_karman_phys_256x64", lambda st: cases.d2q9_karman(256, 64, 64, 32, 6, storage=st), 2000),
+    ("d3q27_bgk_lid_ref_32", lambda st: cases.d3q27_bgk_lid_reference(32, u_lid=0.05, re=100.0, storage=st), 300),
+    ("d3q27_mrt_gs_lid_ref_24", lambda st: cases.d3q27_mrt_lid_gs(24, re=100.0, storage=st, engine=MRT_SCALAR), 200),
+]
+
+
+@pytest.mark.parametrize("storage", [F16S, BF16S], ids=["f16s", "bf16s"])
+@pytest.mark.parametrize("name,make,steps", SHIFTED, ids=[p[0] for p in SHIFTED])
+def test_shifted_storage_against_fp64(name, make, steps, storage):
+    cfg16, types = make(storage)
+    cfg64 = dict(cfg16, storage=F64)
+    types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1)
+    rho, u = noisy_fields(types, cfg16["lattice"], 13)
+    out = {}
+    for key, cfg in (("s", cfg16), ("d", cfg64)):
+        with Lattice(cfg, types) as lat:
+            lat.init_fields(rho, *u)
+            lat.step(steps)
+            out[key] = lat.fields()
+    errs = [rel_l2(out["s"][k], out["d"][k]) for k in range(3 if cfg16["lattice"] == 9 else 4)]
+    print(f"{name} storage {storage}: rel-L2 rho/u vs FP64 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= 1e-3
+
+
+@pytest.mark.parametrize("storage", [F16S, BF16S], ids=["f16s", "bf16s"])
+def test_shifted_storage_matches_oracle_emulation(storage):
+    cfg, types = cases.d2q9_karman(128, 48, 32, 24, 5, storage=storage)
+    fg, fo, _, _ = run_both(cfg, types, 300, seed=14, u0=(0.05, 0, 0))
+    e = rel_l2(fg, fo)
+    print(f"storage {storage}: kernel vs oracle codec emulation rel-L2 {e:.3e}")
+    assert e <= 1e-6
+
+
+def test_init_equilibrium_noise_generator():
+    """lbm_init_equilibrium draws SplitMix64 noise exactly as its contract states."""
+    cfg, types = cases.d2q9_karman(64, 32, 16, 16, 4, storage=F64)
+    types = types.reshape(-1)
+    N = types.size
+    mask = 2 ** 64 - 1
+    s = 0x5EED0002
+    u = np.zeros((3, N))
+    u[0] = 0.1
+    for n in range(N):
+        for a in range(2):
+            s = (s + 0x9E3779B97F4A7C15) & mask
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+            z ^= z >> 31
+            r = (z >> 11) * 2.0 ** -53
+            if not (types[n] & 1):
+                u[a, n] += 1e-4 * (2 * r - 1)
+    with Lattice(cfg, types) as a, Lattice(cfg, types) as b:
+        a.init_equilibrium(1.0, (0.1, 0.0, 0.0), 0x5EED0002, 1e-4)
+        b.init_fields(np.ones(N), u[0], u[1])
+        assert np.array_equal(a.populations(), b.populations())
+
+
+def test_monitors_match_host_sums():
+    cfg, types = cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, storage=F64)
+    types = types.reshape(-1)
+    with Lattice(cfg, types) as lat:
+        lat.init_equilibrium(1.0, (0.05, 0, 0), 7, 1e-3)
+        lat.step(20)
+        mass, ke, bad = lat.monitors()
+        f = lat.populations().reshape(27, -1)
+    fluid = (types & 1) == 0
+    c, _, _ = orc.tables(27)
+    rho = f[:, fluid].sum(0)
+    j = c.astype(float) @ f[:, fluid]
+    assert bad == 0
+    assert abs(mass - rho.sum()) <= 1e-12 * rho.sum()
+    assert abs(ke - ((j * j).sum(0) / (2 * rho)).sum()) <= 1e-10 * ke
+
+
+def test_monitors_flag_nonfinite():
+    cfg, types = cases.d2q9_lid_reference(32, storage=F32)
+    with Lattice(cfg, types) as lat:
+        lat.init_reference(1.0)
+        f = lat.populations()
+        f[5 + 32 * 10 + 1024 * 3] = np.nan  # a fluid node
+        lat.set_populations(f)
+        _, _, bad = lat.monitors()
+    assert bad == 1
+
+
+@pytest.mark.parametrize("storage", [F64, F32, F16S, BF16S])
+def test_checkpoint_resume_is_bitwise(storage):
+    cfg, types = cases.d2q9_karman(128, 48, 32, 24, 5, storage=storage)
+    with Lattice(cfg, types) as a:
+        a.init_equilibrium(1.0, (0.1, 0, 0), 3, 1e-4)
+        a.step(37)
+        snap = a.state()
+        a.step(41)
+        ref = a.state()
+    with Lattice(cfg, types) as b:
+        b.set_state(snap)
+        b.step(41)
+        assert np.array_equal(b.state(), ref)
+
+
+def test_mrt_reference_init_matches_oracle():
+    cfg, types = cases.d3q27_mrt_lid_reference(16, storage=F64, engine=MRT_SCALAR)
+    types = types.reshape(-1)
+    with Lattice(cfg, types) as lat:
+        lat.init_reference(1.0)
+        g = lat.populations()
+    assert np.array_equal(g, orc.init_reference(cfg, types, 1.0))

@@ -1,0 +1,360 @@
+"""Pins the CPU oracle (oracle/lbm_oracle.cpp) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake (wrong sign of c, wrong opposite map, wrong
+weight, dropped override, transposed moment matrix, wrong rounding) fails one of them:
+closed forms of the lattice, brute-force streaming permutations, an independent numpy
+restatement on tiny grids, exact MRT algebra, and codecs against numpy/torch conversions.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import np_reference as npr
+import oracle as orc
+from lbm_amd import cases, geometry as geo
+from lbm_amd.capi import BC_PHYSICAL, BC_REFERENCE, BGK, F16S, F32, F64, BF16S, MRT_GS, MRT_RAW
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.mark.parametrize("Q", [9, 27])
+def test_lattice_closed_forms(Q):
+    c, w, opp = orc.tables(Q)
+    cT = c.T.astype(float)
+    assert abs(w.sum() - 1) < 1e-15
+    assert np.abs(w @ cT).max() < 1e-16
+    D = 2 if Q == 9 else 3
+    second = np.einsum("i,ia,ib->ab", w, cT, cT)
+    assert np.allclose(second[:D, :D], np.eye(D) / 3, atol=1e-16)
+    # fourth-order isotropy: sum w c_a^4 = 1/3, sum w c_a^2 c_b^2 = 1/9
+    assert abs(np.sum(w * cT[:, 0] ** 4) - 1 / 3) < 1e-15
+    assert abs(np.sum(w * cT[:, 0] ** 2 * cT[:, 1] ** 2) - 1 / 9) < 1e-15
+    for i in range(Q):
+        assert (c[:, opp[i]] == -c[:, i]).all()
+    # the oracle's tables are the reference's listing
+    cr, wr, oppr = npr.lattice(Q)
+    assert (cr.T == c).all() and np.allclose(wr, w, rtol=0, atol=0) and (oppr == opp).all()
+
+
+@pytest.mark.parametrize("Q,shape", [(9, (1, 4, 5)), (27, (3, 4, 3))])
+def test_pure_streaming_is_pull_permutation(Q, shape):
+    """omega = 0 in a fully periodic fluid box: the step is exactly the pull permutation
+    f_i(x) <- f_i(x - c_i mod dims), checked slot by slot with distinct values."""
+    nz, ny, nx = shape
+    cfg = dict(lattice=Q, nx=nx, ny=ny, nz=nz, collision=BGK, storage=F64, bc_mode=BC_REFERENCE, omega=0.0)
+    types = geo.periodic_box(nx, ny, nz).reshape(-1)
+    N = nx * ny * nz
+    f = np.arange(N * Q, dtype=np.float64) + 1.0
+    out = orc.run(cfg, types, f, 1).reshape(Q, nz, ny, nx)
+    F = f.reshape(Q, nz, ny, nx)
+    c, _, _ = orc.tables(Q)
+    for i in range(Q):
+        for z in range(nz):
+            for y in range(ny):
+                for x in range(nx):
+                    sx, sy, sz = (x - c[0, i]) % nx, (y - c[1, i]) % ny, (z - c[2, i]) % nz
+                    assert out[i, z, y, x] == F[i, sz, sy, sx]
+    assert out.sum() == F.sum()
+
+
+@pytest.mark.parametrize("Q", [9, 27])
+def test_equilibrium_moments(Q):
+    nx, ny, nz = (6, 5, 1) if Q == 9 else (4, 3, 3)
+    N = nx * ny * nz
+    rng = np.random.default_rng(1)
+    rho = 1 + 0.1 * rng.standard_normal(N)
+    u = 0.05 * rng.standard_normal((3, N))
+    if Q == 9:
+        u[2] = 0
+    cfg = dict(lattice=Q, nx=nx, ny=ny, nz=nz, storage=F64)
+    f = orc.init_fields(cfg, np.zeros(N, np.uint8), rho, u[0], u[1], u[2]).reshape(Q, N)
+    c, w, _ = orc.tables(Q)
+    cf = c.astype(float)
+    assert np.allclose(f.sum(0), rho, rtol=1e-14)
+    assert np.allclose(cf @ f, rho * u, atol=1e-15)
+    D = 2 if Q == 9 else 3
+    for a in range(D):
+        for b in range(D):
+            Pab = (cf[a] * cf[b]) @ f
+            assert np.allclose(Pab, rho * (u[a] * u[b] + (a == b) / 3), atol=1e-14)
+
+
+@pytest.mark.parametrize("Q", [9, 27])
+def test_bgk_conserves_mass_and_momentum(Q):
+    nx, ny, nz = (8, 6, 1) if Q == 9 else (5, 4, 4)
+    N = nx * ny * nz
+    rng = np.random.default_rng(2)
+    u = 0.03 * rng.standard_normal((3, N))
+    if Q == 9:
+        u[2] = 0
+    cfg = dict(lattice=Q, nx=nx, ny=ny, nz=nz, collision=BGK, storage=F64, bc_mode=BC_REFERENCE, omega=1.7)
+    types = np.zeros(N, np.uint8)
+    f0 = orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(N), u[0], u[1], u[2])
+    f1 = orc.run(cfg, types, f0, 20)
+    c, _, _ = orc.tables(Q)
+    F0, F1 = f0.reshape(Q, N), f1.reshape(Q, N)
+    assert abs(F1.sum() - F0.sum()) < 1e-12
+    assert np.allclose(c.astype(float) @ F1.sum(1), c.astype(float) @ F0.sum(1), atol=1e-13)
+
+
+REF_CASES = {
+    "d2q9_lid": lambda: (dict(cases.d2q9_lid_reference(8, re=5.0, storage=F64)[0]),
+                         geo.lid_cavity_2d_reference(8, 6)),
+    "d2q9_channel": lambda: (dict(cases.d2q9_channel_reference(14, 10, storage=F64)[0]),
+                             geo.channel_2d_reference(14, 10)),
+    "d3q27_bgk_lid": lambda: (dict(cases.d3q27_bgk_lid_reference(5, nu=0.05, storage=F64)[0]),
+                              geo.lid_cavity_3d_reference(5, 5, 5)),
+    "d3q27_mrt_raw": lambda: (dict(cases.d3q27_mrt_lid_reference(5, storage=F64)[0]),
+                              geo.lid_cavity_3d_reference(5, 5, 5)),
+    "d3q27_mrt_gs": lambda: (dict(cases.d3q27_mrt_lid_gs(5, re=10.0, storage=F64)[0]),
+                             geo.lid_cavity_3d_reference(5, 5, 5)),
+}
+
+
+def _fix_dims(cfg, types):
+    nz, ny, nx = types.shape
+    cfg.update(nx=nx, ny=ny, nz=nz)
+    return cfg
+
+
+@pytest.mark.parametrize("name", list(REF_CASES))
+def test_oracle_matches_numpy_restatement_reference_mode(name):
+    cfg, types = REF_CASES[name]()
+    cfg = _fix_dims(cfg, types)
+    Q = cfg["lattice"]
+    rng = np.random.default_rng(3)
+    shape = types.shape
+    u = np.zeros((3,) + shape)
+    u[0] = 0.02 * rng.standard_normal(shape)
+    u[1] = 0.02 * rng.standard_normal(shape)
+    if Q == 27:
+        u[2] = 0.02 * rng.standard_normal(shape)
+    rho = 1 + 0.01 * rng.standard_normal(shape)
+    F0 = npr.init_feq(shape, cfg, rho, u)
+    f_or = orc.run(cfg, types.reshape(-1), F0.reshape(-1), 4)
+    f_np = npr.run(F0, types, cfg, 4).reshape(-1)
+    assert rel_l2(f_or, f_np) < 1e-13, rel_l2(f_or, f_np)
+
+
+PHYS_CASES = {
+    "d2q9_lid_phys": lambda: (dict(cases.d2q9_lid_physical(9, re=5.0, storage=F64)[0]),
+                              geo.lid_cavity_2d_physical(9, 7)),
+    "d2q9_karman": lambda: (dict(cases.d2q9_karman(24, 12, 7, 6, 2, storage=F64)[0]),
+                            geo.channel_2d_physical(24, 12, 7, 6, 2)),
+    "d3q27_sphere": lambda: (dict(cases.d3q27_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64)[0]),
+                             geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+    "d3q27_sphere_mrt": lambda: (dict(cases.d3q27_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64,
+                                                                  collision=MRT_GS)[0]),
+                                 geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+}
+
+
+@pytest.mark.parametrize("name", list(PHYS_CASES))
+def test_oracle_matches_numpy_restatement_physical_mode(name):
+    cfg, types = PHYS_CASES[name]()
+    cfg = _fix_dims(cfg, types)
+    shape = types.shape
+    rng = np.random.default_rng(4)
+    u = np.zeros((3,) + shape)
+    u[0] = 0.05 + 0.01 * rng.standard_normal(shape)
+    u[1] = 0.01 * rng.standard_normal(shape)
+    if cfg["lattice"] == 27:
+        u[2] = 0.01 * rng.standard_normal(shape)
+    F0 = npr.init_feq(shape, cfg, None, u)
+    f_or = orc.run(cfg, types.reshape(-1), F0.reshape(-1), 5)
+    f_np = npr.run(F0, types, cfg, 5).reshape(-1)
+    assert rel_l2(f_or, f_np) < 1e-13, rel_l2(f_or, f_np)
+
+
+def _read_int_table(name):
+    s = open(os.path.join(ROOT, "lbm-arrayfire_amd", "csrc", "mrt_tables.h")).read()
+    body = re.search(name + r"\[27\]\[27\] = \{(.*?)\};", s, re.S).group(1)
+    return np.array([[int(v) for v in r.split(",") if v.strip()] for r in re.findall(r"\{([^{}]*)\}", body)])
+
+
+def test_mrt_matrices_exact():
+    M, Mi = orc.mrt_matrices(1)
+    assert np.array_equal(M, npr.raw_moment_matrix())
+    assert np.allclose(M @ Mi, np.eye(27), atol=1e-13)
+    # 24 M^-1 is an integer matrix; the kernels' compile-time table equals it
+    assert np.abs(24 * Mi - np.round(24 * Mi)).max() < 1e-12
+    assert np.array_equal(np.round(24 * Mi).astype(int), _read_int_table("MRAW_INV24"))
+    assert np.array_equal(M.astype(int), _read_int_table("MRAW"))
+    assert (M != 0).sum() == 339 and (np.round(24 * Mi) != 0).sum() == 226
+    # Gram-Schmidt basis: orthogonal rows spanning the same row space, integer table is a row
+    # scaling of the oracle's rows
+    G, Gi = orc.mrt_matrices(2)
+    gram = G @ G.T
+    assert np.abs(gram - np.diag(np.diag(gram))).max() < 1e-10
+    Gint = _read_int_table("MGS")
+    for k in range(27):
+        ratio = Gint[k] @ G[k] / (G[k] @ G[k])
+        assert np.allclose(Gint[k], ratio * G[k], atol=1e-9)
+
+
+def _mrt_cfg(s, meq_mode=0, coll=MRT_RAW):
+    return dict(lattice=27, nx=4, ny=3, nz=3, collision=coll, storage=F64, bc_mode=BC_REFERENCE,
+                meq_mode=meq_mode, s=list(s))
+
+
+def test_mrt_zero_rates_is_pure_streaming():
+    cfg = _mrt_cfg([0.0] * 27)
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(5)
+    f = 1 / 27 + 0.01 * rng.random(36 * 27)
+    cfg_bgk = dict(cfg, collision=BGK, omega=0.0)
+    assert rel_l2(orc.run(cfg, types, f, 2), orc.run(cfg_bgk, types, f, 2)) < 1e-14
+
+
+def test_mrt_unit_rates_returns_equilibrium():
+    """s = 1 with meq = M f_eq: f_out = M^-1 M f_eq(rho, u) = f_eq, i.e. BGK with omega = 1."""
+    cfg = _mrt_cfg([1.0] * 27, meq_mode=1)
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(6)
+    u = 0.03 * rng.standard_normal((3, 36))
+    f = orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(36), *u)
+    f = f + 1e-3 * rng.standard_normal(f.size)
+    out = orc.run(cfg, types, f, 1)
+    assert rel_l2(out, orc.run(dict(cfg, collision=BGK, omega=1.0), types, f, 1)) < 1e-13
+
+
+@pytest.mark.parametrize("basis", [MRT_RAW, MRT_GS])
+def test_mrt_with_projected_meq_and_uniform_rate_is_bgk(basis):
+    """With meq = M f_eq and every rate = omega, MRT is BGK: M^-1 (I - omega)(M f) + omega f_eq."""
+    om = 1.37
+    cfg = _mrt_cfg([om] * 27, meq_mode=1, coll=basis)
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(7)
+    f = orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(36), *(0.03 * rng.standard_normal((3, 36))))
+    f = f + 1e-4 * rng.standard_normal(f.size)
+    a = orc.run(cfg, types, f, 3)
+    b = orc.run(dict(cfg, collision=BGK, omega=om), types, f, 3)
+    assert rel_l2(a, b) < 1e-13
+
+
+def test_reference_meq_differs_from_projection_only_in_row_26():
+    M = npr.raw_moment_matrix()
+    rng = np.random.default_rng(8)
+    rho = 1 + 0.1 * rng.standard_normal(5)
+    u = 0.05 * rng.standard_normal((3, 5))
+    c, w, _ = npr.lattice(27)
+    proj = M @ npr.feq(rho[None, None], u[:, None, None], c, w).reshape(27, -1)
+    ref = npr.meq_reference(rho, u)
+    assert np.allclose(proj[:26], ref[:26], atol=1e-14)
+    # row 26: projection gives rho (1 + 3|u|^2) / 27; the reference table |u|^2 + rho/27
+    usq = (u * u).sum(0)
+    assert np.allclose(proj[26], rho * (1 + 3 * usq) / 27, atol=1e-15)
+    assert np.allclose(ref[26], usq + rho / 27, atol=1e-15)
+
+
+def test_mrt_reference_initial_state():
+    """rho0/27 followed by one collision at rho0, u = 0 (rust/examples/d3q27_mrt_lid.rs:228-287)."""
+    cfg = dict(cases.d3q27_mrt_lid_reference(4, storage=F64)[0])
+    types = geo.lid_cavity_3d_reference(4, 4, 4).reshape(-1)
+    f = orc.init_reference(cfg, types, 1.0).reshape(27, -1)
+    M = npr.raw_moment_matrix()
+    m = M @ np.full(27, 1 / 27)
+    me = npr.meq_reference(np.array([1.0]), np.zeros((3, 1)))[:, 0]
+    s = np.array(cfg["s"])
+    expect = np.linalg.inv(M) @ (m - s * (m - me))
+    assert np.allclose(f, expect[:, None], atol=1e-15)
+
+
+def test_half_codec_matches_numpy():
+    rng = np.random.default_rng(9)
+    vals = np.concatenate([rng.standard_normal(3000) * 10.0 ** rng.integers(-9, 5, 3000),
+                           [0.0, -0.0, 6.1e-5, 6.0e-8, 2.98e-8, 65504.0, 65519.0, 65520.0, 1e6, -1e-10],
+                           # exact ties between half values
+                           np.float16(np.arange(1, 200)).astype(np.float32) + np.float32(2.0 ** -11)])
+    for v in vals.astype(np.float32):
+        h = orc.f2h(v)
+        assert h == np.float32(v).astype(np.float16).view(np.uint16), v
+        assert orc.h2f(h) == np.float32(np.uint16(h).view(np.float16))
+
+
+def test_bf16_codec_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(10)
+    vals = np.concatenate([rng.standard_normal(3000) * 10.0 ** rng.integers(-30, 30, 3000),
+                           [0.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -3.0e-39]]).astype(np.float32)
+    t = torch.from_numpy(vals).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    for v, ref in zip(vals, t):
+        assert orc.f2bf(v) == ref, v
+        assert orc.bf2f(ref) == np.float32((np.uint32(ref) << 16).view(np.float32))
+
+
+@pytest.mark.parametrize("storage", [F16S, BF16S])
+def test_shifted_storage_roundtrip_bound(storage):
+    """decode(encode(f)) differs from f by at most half an ulp of (f - w_i) in the codec plus the
+    float rounding of the shift."""
+    cfg = dict(lattice=9, nx=6, ny=5, nz=1, storage=storage, collision=BGK, omega=0.0, bc_mode=BC_REFERENCE)
+    types = np.zeros(30, np.uint8)
+    rng = np.random.default_rng(11)
+    u = 0.05 * rng.standard_normal((3, 30))
+    f64 = orc.init_fields(dict(cfg, storage=F64), types, 1 + 0.02 * rng.standard_normal(30), *u)
+    _, w, _ = orc.tables(9)
+    g = f64 - np.repeat(w.astype(np.float32).astype(np.float64), 30)
+    bits = 11 if storage == F16S else 8
+    rt = orc.run(cfg, types, f64, 0)  # load encodes, store decodes
+    err = np.abs(rt - f64)
+    assert (err <= np.abs(g) * 2.0 ** -bits + 2.0 ** -24 * np.abs(f64) + 2.0 ** -24).all()
+    assert err.max() > 0
+
+
+def test_zou_he_inlet_recovers_prescribed_velocity():
+    cfg, types = cases.d2q9_karman(32, 14, 10, 7, 2, storage=F64)
+    cfg = dict(cfg)
+    types = types.reshape(-1)
+    rng = np.random.default_rng(12)
+    u = np.zeros((3, types.size))
+    u[0] = 0.05 + 0.01 * rng.standard_normal(types.size)
+    u[1] = 0.01 * rng.standard_normal(types.size)
+    f = orc.run(cfg, types, orc.init_fields(cfg, types, None, *u), 3)
+    rho, ux, uy, _ = orc.fields(cfg, types, f)
+    inlet = (types & 4) != 0
+    y = np.arange(types.size) // 32
+    H = 14 - 2
+    expect = 4 * 0.1 * (y - 0.5) * (H - (y - 0.5)) / H ** 2
+    assert np.abs(ux[inlet] - expect[inlet]).max() < 1e-14
+    assert np.abs(uy[inlet]).max() < 1e-14
+
+
+def test_poiseuille_profile_develops():
+    """Empty channel, parabolic velocity inlet, reference-pressure outlet, half-way walls: the
+    flow becomes steady with a parabolic cross-section (walls at y = 0.5 and ny - 1.5), the mass
+    flux sum_y rho u_x is the same at every section, and the density falls to rho_bc at the
+    outlet."""
+    nx, ny = 60, 12
+    cfg = dict(lattice=9, nx=nx, ny=ny, nz=1, collision=BGK, storage=F64, bc_mode=BC_PHYSICAL,
+               omega=1.0, u_in=0.04, inlet_profile=1)
+    types = geo.channel_2d_physical(nx, ny, -100, -100, 1).reshape(-1)
+    f = orc.run(cfg, types, orc.init_fields(cfg, types), 4000)
+    rho, ux, uy, _ = orc.fields(cfg, types, f)
+    rho, ux = rho.reshape(ny, nx), ux.reshape(ny, nx)
+    y = np.arange(1, ny - 1)
+    H = ny - 2
+    shape = (y - 0.5) * (H - (y - 0.5))
+    for x in (30, nx - 10):
+        prof = ux[1:-1, x]
+        assert np.abs(prof / prof.max() - shape / shape.max()).max() < 2e-2
+    flux = (rho[1:-1] * ux[1:-1]).sum(0)
+    inflow = (4 * 0.04 * shape / H ** 2).sum() * rho[1:-1, 0].mean()
+    assert np.abs(flux[1:nx // 2] / flux[1] - 1).max() < 1e-4
+    assert np.abs(flux[1:] / flux[1] - 1).max() < 5e-3  # odd-even ripple next to the outlet
+    assert abs(flux[1] / inflow - 1) < 2e-2
+    assert np.abs(rho[1:-1, nx - 1] - 1.0).max() < 1e-12
+
+
+def test_f32_index_compat_only_changes_large_lattices():
+    cfg = dict(lattice=9, nx=8, ny=6, nz=1, collision=BGK, storage=F32, bc_mode=BC_REFERENCE, omega=1.2)
+    types = geo.lid_cavity_2d_reference(8, 6).reshape(-1)
+    f = orc.init_fields(cfg, types)
+    a = orc.run(cfg, types, f, 3, f32_index_compat=False)
+    b = orc.run(cfg, types, f, 3, f32_index_compat=True)
+    assert np.array_equal(a, b)
