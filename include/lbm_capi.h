@@ -50,8 +50,9 @@ typedef enum { LBM_D2Q9 = 9, LBM_D3Q27 = 27 } lbm_lattice;
  * Gram-Schmidt orthogonalised (not in the reference). */
 typedef enum { LBM_BGK = 0, LBM_MRT_RAW = 1, LBM_MRT_GS = 2 } lbm_collision;
 
-/* Population storage in HBM.  F16S/BF16S store the shifted value f - w_i with FP32 collision
- * arithmetic; F64 computes in FP64, all others in FP32. */
+/* Population storage in HBM.  F16S stores the shifted value (f - w_i) * 2^15 as IEEE half,
+ * BF16S stores f - w_i as bfloat16 (both round to nearest even) with FP32 collision arithmetic;
+ * F64 computes in FP64, all others in FP32. */
 typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_storage;
 
 /* Boundary semantics.  REFERENCE reproduces the reference: every axis periodic, full-way

@@ -39,13 +39,14 @@ int cxq(int Q, int i) { return Q == 9 ? Lat<9>::cx[i] : Lat<27>::cx[i]; }
 int cyq(int Q, int i) { return Q == 9 ? Lat<9>::cy[i] : Lat<27>::cy[i]; }
 int czq(int Q, int i) { return Q == 9 ? Lat<9>::cz[i] : Lat<27>::cz[i]; }
 
-// Host-side codecs for shifted storage: the value stored is f - w_i rounded to nearest even.
+// Host-side codecs for shifted storage: F16S stores (f - w_i) * 2^15 as IEEE half, BF16S stores
+// f - w_i as bfloat16, both rounded to nearest even.
 uint16_t enc_half(double v, double w) {
   const float g = (float)(v - (double)(float)w);
-  return __builtin_bit_cast(uint16_t, (_Float16)g);
+  return __builtin_bit_cast(uint16_t, (_Float16)(g * 32768.0f));
 }
 double dec_half(uint16_t h, double w) {
-  return (double)(float)__builtin_bit_cast(_Float16, h) + (double)(float)w;
+  return (double)(float)__builtin_bit_cast(_Float16, h) / 32768.0 + (double)(float)w;
 }
 uint16_t enc_bf16(double v, double w) {
   const float g = (float)(v - (double)(float)w);
@@ -74,6 +75,9 @@ std::string check_desc(const lbm_desc *d) {
   if (d->storage < 0 || d->storage > 3) return "unknown storage format";
   if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
   if (d->mrt_engine < 0 || d->mrt_engine > 3) return "unknown MRT engine";
+  if (d->mrt_engine == LBM_MRT_MFMA_BF16X3) return "the BF16x3 MFMA engine is not available in this build";
+  if (d->mrt_engine == LBM_MRT_MFMA_F32 && d->storage == LBM_F64)
+    return "the FP32 MFMA engine needs FP32 arithmetic (storage F32, F16S or BF16S)";
   if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
   if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
   if (!std::isfinite(d->omega)) return "omega is not finite";
@@ -100,6 +104,7 @@ struct lbm_ctx {
   double *partials = nullptr;
   double *red = nullptr;
   double *fields = nullptr;  // 4N doubles, allocated on first use
+  float *mrt_tab = nullptr;  // MFMA MRT per-lane operands
   int64_t steps = 0;
   std::vector<uint8_t> host_types;
   Config cfg{};
@@ -211,6 +216,15 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   if (hipMemcpy(h->types, h->host_types.data(), (size_t)h->N, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(LBM_EDEVICE, "node-type upload failed");
   p.type = h->types;
+  if (d->collision != LBM_BGK) {
+    std::vector<float> tab(64 * MFMA_TAB);
+    build_mfma_table(d->collision, d->meq_mode, d->s, tab.data());
+    if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "MRT table allocation failed");
+    if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "MRT table upload failed");
+    p.mrt_tab = h->mrt_tab;
+  }
   *out = h;
   return LBM_OK;
 }
@@ -225,6 +239,7 @@ lbm_status lbm_destroy(lbm_handle h) {
   if (h->partials) (void)hipFree(h->partials);
   if (h->red) (void)hipFree(h->red);
   if (h->fields) (void)hipFree(h->fields);
+  if (h->mrt_tab) (void)hipFree(h->mrt_tab);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);

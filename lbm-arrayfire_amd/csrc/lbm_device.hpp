@@ -43,17 +43,21 @@ template <int Q> struct Codec<ST_F32, Q> {
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
   template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = v; }
 };
-// Shifted half storage: the stored value is f - w_i, rounded to nearest even.
+// Shifted half storage: the stored value is (f - w_i) * 2^15 rounded to nearest even.  The exact
+// power-of-two scale keeps the small deviations of the low-weight D3Q27 directions out of the
+// half-precision subnormal range, so every stored value carries 11 significant bits.
+constexpr float F16S_SCALE = 32768.0f, F16S_INV_SCALE = 1.0f / 32768.0f;
 template <int Q> struct Codec<ST_F16S, Q> {
   using T = uint16_t;
   using C = float;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
-    return (float)__builtin_bit_cast(_Float16, p[k]) + (float)Lat<Q>::w[I];
+    return (float)__builtin_bit_cast(_Float16, p[k]) * F16S_INV_SCALE + (float)Lat<Q>::w[I];
   }
   template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) {
-    p[k] = __builtin_bit_cast(uint16_t, (_Float16)(v - (float)Lat<Q>::w[I]));
+    p[k] = __builtin_bit_cast(uint16_t, (_Float16)((v - (float)Lat<Q>::w[I]) * F16S_SCALE));
   }
 };
+// Shifted bfloat16 storage: the stored value is f - w_i rounded to nearest even.
 template <int Q> struct Codec<ST_BF16S, Q> {
   using T = uint16_t;
   using C = float;
@@ -167,34 +171,34 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   using C = typename CD::C;
-  if (!(t & NODE_SLOW)) {
-    pull_periodic<Q, ST>(p, q, f);
-    return;
-  }
+  // The periodic pull is issued by every node before its type is known; nodes flagged SLOW then
+  // replace the links whose source is solid and close their inlet/outlet unknowns.
+  pull_periodic<Q, ST>(p, q, f);
+  if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
   const C rw = (C)p.rho_bc;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
     constexpr int io = Lat<Q>::opp[i];
-    int sx = q.x - cx;
-    sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
-    int sy = q.y - cy;
-    sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
-    int sz = q.z - cz;
-    sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
-    const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
-    const uint8_t ts = p.type[src];
-    if (ts & NODE_SOLID) {
-      C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.N, q.n);
-      if (ts & NODE_MOVING) {
-        C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
-        if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
-        v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+    if constexpr (i != 0) {
+      int sx = q.x - cx;
+      sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
+      int sy = q.y - cy;
+      sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
+      int sz = q.z - cz;
+      sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
+      const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
+      const uint8_t ts = p.type[src];
+      if (ts & NODE_SOLID) {
+        C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.N, q.n);
+        if (ts & NODE_MOVING) {
+          C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
+          if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
+          v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+        }
+        f[i] = v;
       }
-      f[i] = v;
-    } else {
-      f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, src);
     }
   });
   if (t & NODE_INLET) {

@@ -19,6 +19,7 @@ struct StepParams {
   int32_t meq_mode;
   int32_t pad_;
   int64_t N;
+  const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
   double omega;
   double u_wall[3];
   double u_in;
@@ -33,6 +34,8 @@ struct Config {
 // Tile shape of the node-per-thread kernels: one wavefront per 64-node row segment,
 // four rows per 256-thread workgroup.
 constexpr int TILE_X = 64, TILE_Y = 4, BLOCK_THREADS = 256;
+// Per-lane constants of the MFMA MRT kernel: M (2x8), s.E (2x4), 1 - s (2x4), M^-1 (2x8).
+constexpr int MFMA_TAB = 48;
 
 inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t &gz, int32_t &nt) {
   gx = (nx + TILE_X - 1) / TILE_X;
@@ -42,6 +45,14 @@ inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t 
 }
 
 hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s);
+hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s);
+hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s);
+// True when the step of this configuration runs on the MFMA MRT kernel.
+inline bool uses_mfma(const Config &c) {
+  return c.collision != CO_BGK && c.storage != ST_F64 && (c.engine == 0 || c.engine == 2);
+}
+// Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice.
+void build_mfma_table(int collision, int meq_mode, const double *s, float *tab);
 // Reference-style fields (rho, u after the boundary overrides) of the state p.fin.
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);

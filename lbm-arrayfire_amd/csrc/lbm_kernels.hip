@@ -26,8 +26,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
       });
     }
   } else {
-    if (t & NODE_SOLID) return;  // walls keep their populations; nothing reads or writes them
     pull_physical<Q, ST>(p, q, t, f);
+    if (t & NODE_SOLID) return;  // walls keep their populations; nothing writes them
     moments<Q>(f, rho, ux, uy, uz);
     collide<Q, CO>(p, f, rho, ux, uy, uz, out);
   }
@@ -269,6 +269,7 @@ hipError_t launch_monitors(const Config &c, const StepParams &p, double *partial
 
 namespace lbm {
 hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s) {
+  if (uses_mfma(c)) return launch_step_mrt_mfma(c, p, s);
   return launch_step_scalar(c, p, s);
 }
 }  // namespace lbm

@@ -23,6 +23,8 @@
 //                       (non-equilibrium bounce-back) velocity inlet, reference-pressure outlet
 //                       with zero normal gradient of the tangential velocity.
 //
+// Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
+//
 // Summation order inside a node is ascending population index; ArrayFire's reduction order is
 // not observable without ArrayFire, so FP32 bitwise parity with ArrayFire is unpinnable (FP64
 // differences stay at round-off).  Pinned in tests/test_oracle.py by closed forms, brute force
@@ -482,7 +484,7 @@ struct Sim {
         for (int i = 0; i < Q; ++i) {
           float &v = F[(size_t)(n + N * i)];
           const float g = v - w[i];
-          v = (d.storage == F16S ? h2f(f2h(g)) : bf2f(f2bf(g))) + w[i];
+          v = (d.storage == F16S ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g))) + w[i];
         }
     }
   }
@@ -501,7 +503,7 @@ struct Sim {
         const double v = f[n + N * i];
         if (d.storage == F16S || d.storage == BF16S) {
           const float g = (float)(v - (double)(float)L.w[i]);
-          const float hv = (d.storage == F16S) ? h2f(f2h(g)) : bf2f(f2bf(g));
+          const float hv = (d.storage == F16S) ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g));
           F[(size_t)(n + N * i)] = (T)(hv + (float)L.w[i]);
         } else {
           F[(size_t)(n + N * i)] = (T)v;
@@ -515,7 +517,7 @@ struct Sim {
         if (d.storage == F16S || d.storage == BF16S) {
           const float fv = (float)v;
           const float g = fv - (float)L.w[i];
-          const float hv = (d.storage == F16S) ? h2f(f2h(g)) : bf2f(f2bf(g));
+          const float hv = (d.storage == F16S) ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g));
           f[n + N * i] = (double)hv + (double)(float)L.w[i];
         } else {
           f[n + N * i] = (double)v;

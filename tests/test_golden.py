@@ -1,7 +1,8 @@
 """Golden vectors (tests/golden/*.npz, written by make_golden.py from the oracle).
 
 CPU: the oracle still reproduces every fixture bit for bit (regression pin).
-GPU: the FP64 kernels reproduce every fixture bit for bit through the C ABI.
+GPU: the FP64 kernels reproduce every fixture bit for bit through the C ABI (Gram-Schmidt MRT
+fixtures to round-off, see the test).
 """
 import glob
 import json
@@ -41,4 +42,9 @@ def test_kernel_reproduces_golden_bitwise(path):
         lat.set_populations(f0)
         lat.step(steps)
         got = lat.populations()
-    assert np.array_equal(got, f1), float(np.abs(got - f1).max())
+    if cfg.get("collision") == 2:
+        # Gram-Schmidt basis: the kernels use integer-scaled rows, the oracle unscaled ones, so the
+        # two agree to round-off rather than bit for bit
+        assert np.linalg.norm(got - f1) <= 1e-14 * np.linalg.norm(f1)
+    else:
+        assert np.array_equal(got, f1), float(np.abs(got - f1).max())
