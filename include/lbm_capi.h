@@ -30,6 +30,8 @@ extern "C" {
 #define LBM_ABI_VERSION 1
 
 typedef struct lbm_ctx *lbm_handle;
+typedef struct lbm_batch_ctx *lbm_batch;
+typedef struct lbm_server_ctx *lbm_server;
 
 typedef enum {
   LBM_OK = 0,
@@ -166,6 +168,71 @@ lbm_status lbm_set_state(lbm_handle h, const void *src);
  * mass = sum_n sum_i f, kinetic_energy = sum_n |sum_i c_i f|^2 / (2 sum_i f) over non-solid
  * nodes, nonfinite = count of non-finite populations.  Any pointer may be NULL.  Blocks. */
 lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite);
+
+/* ---- batching: many lattices, one persistent launch per configuration per step -------- */
+/* No reference counterpart (each reference program steps one lattice).  Members keep their own
+ * state and stay readable through their handles (lbm_get_*, lbm_monitors, lbm_set_*); while a
+ * lattice is a member its work runs on the batch's stream and lbm_step on it is refused.  Stepping
+ * a batch gives every member exactly the populations it would have reached stepped alone. */
+lbm_status lbm_batch_create(lbm_batch *out);
+lbm_status lbm_batch_add(lbm_batch b, lbm_handle h);
+lbm_status lbm_batch_remove(lbm_batch b, lbm_handle h);
+/* Enqueues nsteps steps of every member (replays of a captured two-step hipGraph). */
+lbm_status lbm_batch_step(lbm_batch b, int64_t nsteps);
+lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms);
+lbm_status lbm_batch_sync(lbm_batch b);
+/* Autotuner: times each configuration group's persistent grid size over `trials` launches per
+ * candidate and keeps the fastest.  Members' states are not changed. */
+lbm_status lbm_batch_tune(lbm_batch b, int32_t trials);
+int32_t lbm_batch_size(lbm_batch b);
+lbm_status lbm_batch_destroy(lbm_batch b);
+
+/* ---- serving: continuous batching of simulation jobs ------------------------------------ */
+/* No reference counterpart.  Jobs are admitted while their footprint (lbm_footprint_bytes) fits
+ * the HBM budget; admitted jobs stay resident and are stepped together, and jobs that arrive
+ * while others run join at the next tick.  A job's populations are exactly those it would reach
+ * run alone. */
+typedef struct {
+  lbm_desc desc;
+  const uint8_t *node_type; /* N bytes, copied at submit */
+  double rho0;              /* initial state: lbm_init_equilibrium(rho0, u0, seed, noise_amp) */
+  double u0[3];
+  uint64_t seed;
+  double noise_amp;
+  int64_t nsteps;
+  int32_t want_fields;      /* keep rho/u at completion for lbm_server_take */
+  int32_t reserved;
+} lbm_job_spec;
+
+typedef struct {
+  int64_t job_id;
+  int32_t status;           /* LBM_OK, LBM_EUNSTABLE (non-finite populations) or a create error */
+  int32_t reserved;
+  double submit_s, admit_s, finish_s; /* server clock (lbm_server_clock), seconds */
+  double mass, kinetic_energy;        /* monitors at completion */
+  int64_t nodes, steps;
+} lbm_job_result;
+
+/* budget_bytes: device memory the server may hold for resident jobs; steps_per_tick: most steps
+ * a tick advances the resident jobs by. */
+lbm_status lbm_server_create(size_t budget_bytes, int32_t steps_per_tick, lbm_server *out);
+/* Queues a job (LBM_ENOMEM if it could never fit the budget). */
+lbm_status lbm_server_submit(lbm_server s, const lbm_job_spec *spec, int64_t *job_id);
+/* Admits queued jobs that fit, advances all resident jobs by min(steps_per_tick, fewest steps
+ * left) in one batched launch sequence, retires finished (or non-finite) jobs and admits again.
+ * Blocks until the tick's steps are done. */
+lbm_status lbm_server_tick(lbm_server s, int32_t *active, int32_t *queued);
+/* Moves up to cap finished results into out; *n = number moved. */
+lbm_status lbm_server_poll(lbm_server s, lbm_job_result *out, int32_t cap, int32_t *n);
+/* Fields of a finished job submitted with want_fields (N doubles each, NULL ok); frees them. */
+lbm_status lbm_server_take(lbm_server s, int64_t job_id, double *rho, double *ux, double *uy, double *uz);
+/* Drops a queued or resident job without a result. */
+lbm_status lbm_server_cancel(lbm_server s, int64_t job_id);
+/* Monitors of every resident job are read every `ticks` ticks (0 = only at completion). */
+lbm_status lbm_server_set_check_every(lbm_server s, int32_t ticks);
+size_t lbm_server_used_bytes(lbm_server s);
+double lbm_server_clock(lbm_server s);
+lbm_status lbm_server_destroy(lbm_server s);
 
 #ifdef __cplusplus
 }

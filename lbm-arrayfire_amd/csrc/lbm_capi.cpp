@@ -9,12 +9,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
-#include "lbm_internal.hpp"
+#include "lbm_handle.hpp"
 
 using namespace lbm;
 
@@ -90,36 +91,6 @@ std::string check_desc(const lbm_desc *d) {
 
 }  // namespace
 
-struct lbm_ctx {
-  lbm_desc d{};
-  int device = 0;
-  int Q = 9;
-  int64_t N = 0;
-  int esize = 4;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  void *f[2] = {nullptr, nullptr};
-  int cur = 0;
-  uint8_t *types = nullptr;
-  double *partials = nullptr;
-  double *red = nullptr;
-  double *fields = nullptr;  // 4N doubles, allocated on first use
-  float *mrt_tab = nullptr;  // MFMA MRT per-lane operands
-  int64_t steps = 0;
-  std::vector<uint8_t> host_types;
-  Config cfg{};
-  StepParams base{};
-
-  size_t pop_bytes() const { return (size_t)N * (size_t)Q * (size_t)esize; }
-
-  StepParams params(int from) const {
-    StepParams p = base;
-    p.fin = f[from];
-    p.fout = f[from ^ 1];
-    return p;
-  }
-};
-
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -140,7 +111,8 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
   const size_t N = (size_t)d->nx * d->ny * d->nz;
   int32_t gx, gy, gz, nt;
   tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
-  return 2 * N * (size_t)d->lattice * (size_t)storage_bytes(d->storage) + N +
+  return 2 * (size_t)population_stride((int64_t)N, storage_bytes(d->storage)) * (size_t)d->lattice *
+             (size_t)storage_bytes(d->storage) + N +
          (size_t)nt * 3 * sizeof(double) + 4 * sizeof(double);
 }
 
@@ -157,6 +129,7 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   h->Q = d->lattice;
   h->N = (int64_t)d->nx * d->ny * d->nz;
   h->esize = storage_bytes(d->storage);
+  h->stride = population_stride(h->N, h->esize);
   h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, d->mrt_engine};
 
   // Node types; PHYSICAL mode marks the non-solid nodes whose pull needs the boundary path.
@@ -191,6 +164,7 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   p.inlet_profile = d->inlet_profile;
   p.meq_mode = d->meq_mode;
   p.N = h->N;
+  p.stride = h->stride;
   p.omega = d->omega;
   for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
   p.u_in = d->u_in;
@@ -207,8 +181,8 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return cleanup(LBM_EDEVICE, "hipEventCreate failed");
   for (int b = 0; b < 2; ++b)
-    if (hipMalloc(&h->f[b], h->pop_bytes()) != hipSuccess)
-      return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->pop_bytes()) + " bytes)");
+    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
   if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
   if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
@@ -231,6 +205,8 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
 
 lbm_status lbm_destroy(lbm_handle h) {
   if (!h) return LBM_OK;
+  if (h->batch) return fail(LBM_EINVAL, "remove the lattice from its batch before destroying it");
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (int b = 0; b < 2; ++b)
@@ -263,11 +239,11 @@ lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, co
   for (int k = 0; k < 4; ++k)
     if (src[k]) {
       double *dst = h->fields + (size_t)k * h->N;
-      HIPCHK(hipMemcpyAsync(dst, src[k], (size_t)h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync(dst, src[k], (size_t)h->N * sizeof(double), hipMemcpyHostToDevice, h->strm()));
       dev[k] = dst;
     }
-  HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   h->steps = 0;
   return LBM_OK;
 }
@@ -290,17 +266,54 @@ lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], u
 lbm_status lbm_init_reference(lbm_handle h, double rho0) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   h->steps = 0;
   return LBM_OK;
 }
 
+// Steps are enqueued as replays of a captured two-step graph (cur -> other -> cur); an odd step
+// or a graph that starts from the other buffer is launched directly.
 static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
-  for (int64_t t = 0; t < nsteps; ++t) {
-    HIPCHK(launch_step(h->cfg, h->params(h->cur), h->stream));
+  auto direct = [&]() -> lbm_status {
+    HIPCHK(launch_step(h->cfg, h->params(h->cur), h->strm()));
     h->cur ^= 1;
     h->steps++;
+    return LBM_OK;
+  };
+  if (nsteps >= 4 && !h->graph) {
+    hipStream_t cs = h->strm();
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = launch_step(h->cfg, h->params(h->cur), cs);
+    hipError_t e2 = launch_step(h->cfg, h->params(h->cur ^ 1), cs);
+    hipError_t e3 = hipStreamEndCapture(cs, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(LBM_EDEVICE, "step graph capture failed");
+    }
+    hipError_t e4 = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e4 != hipSuccess) {
+      h->graph = nullptr;
+      return fail(LBM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e4));
+    }
+    h->graph_cur = h->cur;
+  }
+  int64_t left = nsteps;
+  if (h->graph && left >= 2 && h->cur != h->graph_cur) {
+    lbm_status st = direct();
+    if (st != LBM_OK) return st;
+    --left;
+  }
+  if (h->graph)
+    for (; left >= 2; left -= 2) {
+      HIPCHK(hipGraphLaunch(h->graph, h->strm()));
+      h->steps += 2;
+    }
+  for (; left > 0; --left) {
+    lbm_status st = direct();
+    if (st != LBM_OK) return st;
   }
   return LBM_OK;
 }
@@ -308,6 +321,7 @@ static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
 lbm_status lbm_step(lbm_handle h, int64_t nsteps) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
   if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+  if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
   HIPCHK(hipSetDevice(h->device));
   return enqueue_steps(h, nsteps);
 }
@@ -315,11 +329,12 @@ lbm_status lbm_step(lbm_handle h, int64_t nsteps) {
 lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
   if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+  if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  HIPCHK(hipEventRecord(h->ev0, h->strm()));
   lbm_status st = enqueue_steps(h, nsteps);
   if (st != LBM_OK) return st;
-  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipEventRecord(h->ev1, h->strm()));
   HIPCHK(hipEventSynchronize(h->ev1));
   float t = 0;
   HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
@@ -330,7 +345,7 @@ lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms) {
 lbm_status lbm_sync(lbm_handle h) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   return LBM_OK;
 }
 
@@ -344,11 +359,11 @@ lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, dou
   double *dst[4] = {rho, ux, uy, uz};
   double *dev[4];
   for (int k = 0; k < 4; ++k) dev[k] = dst[k] ? h->fields + (size_t)k * h->N : nullptr;
-  HIPCHK(launch_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->stream));
+  HIPCHK(launch_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
   for (int k = 0; k < 4; ++k)
     if (dst[k])
-      HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   return LBM_OK;
 }
 
@@ -357,8 +372,9 @@ size_t lbm_state_bytes(lbm_handle h) { return h ? h->pop_bytes() : 0; }
 lbm_status lbm_get_state(lbm_handle h, void *dst) {
   if (!h || !dst) return fail(LBM_EINVAL, "NULL argument");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipMemcpyAsync(dst, h->f[h->cur], h->pop_bytes(), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
+  HIPCHK(hipMemcpy2DAsync(dst, row, h->f[h->cur], pitch, row, (size_t)h->Q, hipMemcpyDeviceToHost, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   return LBM_OK;
 }
 
@@ -366,9 +382,10 @@ lbm_status lbm_set_state(lbm_handle h, const void *src) {
   if (!h || !src) return fail(LBM_EINVAL, "NULL argument");
   HIPCHK(hipSetDevice(h->device));
   // Both buffers: in PHYSICAL mode solid nodes are never rewritten by a step.
+  const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
   for (int b = 0; b < 2; ++b)
-    HIPCHK(hipMemcpyAsync(h->f[b], src, h->pop_bytes(), hipMemcpyHostToDevice, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy2DAsync(h->f[b], pitch, src, row, row, (size_t)h->Q, hipMemcpyHostToDevice, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   return LBM_OK;
 }
 
@@ -415,10 +432,10 @@ lbm_status lbm_set_populations(lbm_handle h, const double *f) {
 lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(launch_monitors(h->cfg, h->params(h->cur), h->partials, h->red, h->stream));
+  HIPCHK(launch_monitors(h->cfg, h->params(h->cur), h->partials, h->red, h->strm()));
   double out[3];
-  HIPCHK(hipMemcpyAsync(out, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpyAsync(out, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
   if (mass) *mass = out[0];
   if (kinetic_energy) *kinetic_energy = out[1];
   if (nonfinite) *nonfinite = (int64_t)out[2];
@@ -426,3 +443,28 @@ lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int6
 }
 
 }  // extern "C"
+
+namespace lbm {
+lbm_status set_error(lbm_status s, const std::string &m) { return fail(s, m); }
+
+int64_t population_stride(int64_t N, int esize) {
+  static const bool pad = [] {
+    const char *e = std::getenv("LBM_POP_PAD");
+    return !(e && e[0] == '0');
+  }();
+  if (!pad || N * esize < (1 << 20)) return N;  // small arrays live in cache anyway
+  const int64_t bytes = ((N * esize + 4095) / 4096) * 4096 + 4096 + 256;
+  return bytes / esize;
+}
+
+int device_cu_count(int device) {
+  static int cached[64] = {0};
+  if (device < 0 || device >= 64) return 256;
+  if (!cached[device]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    cached[device] = n;
+  }
+  return cached[device];
+}
+}  // namespace lbm

@@ -84,8 +84,7 @@ struct NodePos {
   bool valid;
 };
 
-__device__ __forceinline__ NodePos node_of_thread(const StepParams &p) {
-  const int tile = xcd_tile((int)blockIdx.x, p.ntiles);
+__device__ __forceinline__ NodePos node_of_tile(const StepParams &p, int tile) {
   const int bx = tile % p.gx, t2 = tile / p.gx, by = t2 % p.gy, bz = t2 / p.gy;
   NodePos q;
   q.x = bx * TILE_X + (int)(threadIdx.x & (TILE_X - 1));
@@ -94,6 +93,32 @@ __device__ __forceinline__ NodePos node_of_thread(const StepParams &p) {
   q.valid = q.x < p.nx && q.y < p.ny;
   q.n = (uint32_t)q.x + (uint32_t)p.nx * ((uint32_t)q.y + (uint32_t)p.ny * (uint32_t)q.z);
   return q;
+}
+
+__device__ __forceinline__ NodePos node_of_thread(const StepParams &p) {
+  return node_of_tile(p, xcd_tile((int)blockIdx.x, p.ntiles));
+}
+
+// Batched launches: a persistent workgroup walks one contiguous chunk of the batch's tile range.
+template <class F>
+__device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
+  const int nb = (int)gridDim.x;
+  const int blk = xcd_tile((int)blockIdx.x, nb);
+  const int chunk = (b.total_tiles + nb - 1) / nb;
+  const int t0 = blk * chunk;
+  const int t1 = t0 + chunk < b.total_tiles ? t0 + chunk : b.total_tiles;
+  if (t0 >= t1) return;
+  int lo = 0, hi = b.nmembers - 1;  // member holding t0
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.prefix[mid] <= t0) lo = mid;
+    else hi = mid - 1;
+  }
+  int m = lo;
+  for (int t = t0; t < t1; ++t) {
+    while (t >= b.prefix[m + 1]) ++m;
+    f(b.members[m], t - b.prefix[m]);
+  }
 }
 
 // ---- pulls -------------------------------------------------------------------------------
@@ -114,7 +139,7 @@ __device__ __forceinline__ void pull_periodic(const StepParams &p, const NodePos
     const int sy = cy == 1 ? ym : (cy == -1 ? yp : q.y);
     const int sz = cz == 1 ? zm : (cz == -1 ? zp : q.z);
     const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
-    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, src);
+    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, src);
   });
 }
 
@@ -191,7 +216,7 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
       const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
       const uint8_t ts = p.type[src];
       if (ts & NODE_SOLID) {
-        C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.N, q.n);
+        C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.stride, q.n);
         if (ts & NODE_MOVING) {
           C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
           if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
@@ -217,7 +242,7 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
       C g[Q], r, vx, vy, vz;
       static_for<Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        g[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.N, q.n - 1);
+        g[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n - 1);
       });
       moments<Q>(g, r, vx, vy, vz);
       uy = vy;
@@ -414,7 +439,7 @@ __device__ __forceinline__ void store_all(const StepParams &p, uint32_t n, const
   T *fout = (T *)p.fout;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    CD::template st<i>(fout + (size_t)i * (size_t)p.N, n, out[i]);
+    CD::template st<i>(fout + (size_t)i * (size_t)p.stride, n, out[i]);
   });
 }
 

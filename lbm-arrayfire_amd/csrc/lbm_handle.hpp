@@ -1,0 +1,58 @@
+// lbm_handle.hpp — the lattice handle behind the C ABI's lbm_handle (shared by the ABI, batch and
+// server implementations).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lbm_capi.h"
+#include "lbm_internal.hpp"
+
+struct lbm_batch_ctx;
+
+struct lbm_ctx {
+  lbm_desc d{};
+  int device = 0;
+  int Q = 9;
+  int64_t N = 0;
+  int64_t stride = 0;  // elements between population arrays
+  int esize = 4;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *f[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint8_t *types = nullptr;
+  double *partials = nullptr;
+  double *red = nullptr;
+  double *fields = nullptr;  // 4N doubles, allocated on first use
+  float *mrt_tab = nullptr;  // MFMA MRT per-lane operands
+  int64_t steps = 0;
+  lbm_batch_ctx *batch = nullptr;     // set while the lattice is a batch member
+  hipStream_t borrowed = nullptr;     // the batch's stream while a member: all work goes there
+  hipGraphExec_t graph = nullptr;     // two captured steps (cur -> other -> cur)
+  int graph_cur = -1;                 // buffer index the graph starts from
+  std::vector<uint8_t> host_types;
+  lbm::Config cfg{};
+  lbm::StepParams base{};
+
+  hipStream_t strm() const { return borrowed ? borrowed : stream; }
+  size_t pop_bytes() const { return (size_t)N * (size_t)Q * (size_t)esize; }       // packed
+  size_t buf_bytes() const { return (size_t)stride * (size_t)Q * (size_t)esize; }   // allocated
+
+  lbm::StepParams params(int from) const {
+    lbm::StepParams p = base;
+    p.fin = f[from];
+    p.fout = f[from ^ 1];
+    return p;
+  }
+};
+
+
+namespace lbm {
+// Sets the thread-local last-error message and returns s.
+lbm_status set_error(lbm_status s, const std::string &m);
+// Number of compute units of the handle's device (cached).
+int device_cu_count(int device);
+}  // namespace lbm

@@ -18,7 +18,8 @@ struct StepParams {
   int32_t inlet_profile;
   int32_t meq_mode;
   int32_t pad_;
-  int64_t N;
+  int64_t N;                // nodes
+  int64_t stride;           // elements between consecutive population arrays (>= N)
   const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
   double omega;
   double u_wall[3];
@@ -37,6 +38,12 @@ constexpr int TILE_X = 64, TILE_Y = 4, BLOCK_THREADS = 256;
 // Per-lane constants of the MFMA MRT kernel: M (2x8), s.E (2x4), 1 - s (2x4), M^-1 (2x8).
 constexpr int MFMA_TAB = 48;
 
+// Population arrays are laid out with a stride of round_up(N * s, 4 KiB) + 4 KiB + 256 B bytes, so
+// that arrays of power-of-two sized lattices do not start at power-of-two distances (which would
+// send the 2Q streams of a node update to the same memory channels).  Arrays under 1 MiB are not
+// padded.  LBM_POP_PAD=0 in the environment disables the padding.
+int64_t population_stride(int64_t N, int esize);
+
 inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t &gz, int32_t &nt) {
   gx = (nx + TILE_X - 1) / TILE_X;
   gy = (ny + TILE_Y - 1) / TILE_Y;
@@ -47,6 +54,17 @@ inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t 
 hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s);
 hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s);
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s);
+// Batched launches: member m of a batch owns tiles [prefix[m], prefix[m+1]) of the batch's tile
+// range.
+struct BatchArgs {
+  const StepParams *members;  // one parameter block per member (fin/fout of this step)
+  const int32_t *prefix;      // nmembers + 1 tile offsets
+  int32_t nmembers;
+  int32_t total_tiles;
+};
+hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
+hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
+hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 // True when the step of this configuration runs on the MFMA MRT kernel.
 inline bool uses_mfma(const Config &c) {
   return c.collision != CO_BGK && c.storage != ST_F64 && (c.engine == 0 || c.engine == 2);

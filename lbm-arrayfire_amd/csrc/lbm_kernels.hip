@@ -8,9 +8,9 @@ namespace lbm {
 // One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
 // solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
 template <int Q, int ST, int BC, int CO>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
+__device__ __forceinline__ void step_tile(const StepParams &p, int tile) {
   using C = typename Codec<ST, Q>::C;
-  const NodePos q = node_of_thread(p);
+  const NodePos q = node_of_tile(p, tile);
   if (!q.valid) return;
   const uint8_t t = p.type[q.n];
   C f[Q], out[Q], rho, ux, uy, uz;
@@ -32,6 +32,17 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
     collide<Q, CO>(p, f, rho, ux, uy, uz, out);
   }
   store_all<Q, ST>(p, q.n, out);
+}
+
+template <int Q, int ST, int BC, int CO>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
+  step_tile<Q, ST, BC, CO>(p, xcd_tile((int)blockIdx.x, p.ntiles));
+}
+
+// Persistent batched step: every member of one configuration group, one launch.
+template <int Q, int ST, int BC, int CO>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
+  for_batch_tiles(b, [&](const StepParams &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
 }
 
 // rho, u the reference holds after a step: moments of the pulled populations with the overrides
@@ -125,7 +136,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_monitors(StepParams p, double
     bool finite = true;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      const double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.N, q.n);
+      const double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
       finite = finite && __builtin_isfinite(v);
       r += v;
       jx += (double)Lat<Q>::cx[i] * v;
@@ -220,6 +231,15 @@ hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t 
   });
 }
 
+hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
+  return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
+    hipLaunchKernelGGL((k_step_batch<decltype(Qc)::value, decltype(STc)::value, decltype(BCc)::value,
+                                     decltype(COc)::value>),
+                       dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+    return hipGetLastError();
+  });
+}
+
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s) {
   return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
@@ -271,5 +291,10 @@ namespace lbm {
 hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s) {
   if (uses_mfma(c)) return launch_step_mrt_mfma(c, p, s);
   return launch_step_scalar(c, p, s);
+}
+
+hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
+  if (uses_mfma(c)) return launch_step_batch_mrt_mfma(c, b, grid, s);
+  return launch_step_batch_scalar(c, b, grid, s);
 }
 }  // namespace lbm

@@ -32,17 +32,16 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 }
 
 template <int ST, int BC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
+__device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, float (*lds)[64 * MT_STRIDE]) {
   using CD = Codec<ST, 27>;
   using C = typename CD::C;
   static_assert(sizeof(C) == 4, "MFMA path computes in FP32");
-  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
   float *L = lds[wv];
 
   // ---- phase 1: node per lane -----------------------------------------------------------
-  const NodePos q = node_of_thread(p);
+  const NodePos q = node_of_tile(p, tile);
   const uint8_t t = q.valid ? p.type[q.n] : (uint8_t)NODE_SOLID;
   const bool active = q.valid && !(BC == BC_PHYS && (t & NODE_SOLID));
   {
@@ -110,12 +109,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
   __syncthreads();
 
   // ---- phase 3: node per lane again, store ----------------------------------------------
-  if (!active) return;
   C out[27];
   {
     const float *src = L + lane * MT_STRIDE;
     static_for<27>([&](auto I) { out[decltype(I)::value] = src[decltype(I)::value]; });
   }
+  __syncthreads();  // the LDS rows are free for the next tile of a persistent workgroup
+  if (!active) return;
   if constexpr (BC == BC_REF) {
     if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK, as the scalar kernel
       C f[27];
@@ -127,6 +127,35 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
     }
   }
   store_all<27, ST>(p, q.n, out);
+}
+
+template <int ST, int BC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE];
+  mrt_mfma_tile<ST, BC>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
+}
+
+template <int ST, int BC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch_mrt_mfma(BatchArgs b) {
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE];
+  for_batch_tiles(b, [&](const StepParams &p, int tile) { mrt_mfma_tile<ST, BC>(p, tile, lds); });
+}
+
+hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
+  auto go = [&](auto STc) -> hipError_t {
+    constexpr int ST = decltype(STc)::value;
+    if (c.bc == BC_REF)
+      hipLaunchKernelGGL((k_step_batch_mrt_mfma<ST, BC_REF>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+    else
+      hipLaunchKernelGGL((k_step_batch_mrt_mfma<ST, BC_PHYS>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+    return hipGetLastError();
+  };
+  switch (c.storage) {
+    case ST_F32: return go(IC<ST_F32>{});
+    case ST_F16S: return go(IC<ST_F16S>{});
+    case ST_BF16S: return go(IC<ST_BF16S>{});
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s) {

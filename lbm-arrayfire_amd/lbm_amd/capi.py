@@ -32,6 +32,11 @@ EXPORTS = [
     "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
     "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_get_fields", "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
+    "lbm_batch_create", "lbm_batch_add", "lbm_batch_remove", "lbm_batch_step", "lbm_batch_step_timed",
+    "lbm_batch_sync", "lbm_batch_tune", "lbm_batch_size", "lbm_batch_destroy",
+    "lbm_server_create", "lbm_server_submit", "lbm_server_tick", "lbm_server_poll", "lbm_server_take",
+    "lbm_server_cancel", "lbm_server_set_check_every", "lbm_server_used_bytes", "lbm_server_clock",
+    "lbm_server_destroy",
 ]
 
 
@@ -50,6 +55,20 @@ class LbmDesc(ctypes.Structure):
         ("s", ctypes.c_double * 27), ("u_wall", ctypes.c_double * 3), ("u_in", ctypes.c_double),
         ("rho_bc", ctypes.c_double),
     ]
+
+
+class JobSpec(ctypes.Structure):
+    _fields_ = [("desc", LbmDesc), ("node_type", ctypes.POINTER(ctypes.c_uint8)),
+                ("rho0", ctypes.c_double), ("u0", ctypes.c_double * 3), ("seed", ctypes.c_uint64),
+                ("noise_amp", ctypes.c_double), ("nsteps", ctypes.c_int64),
+                ("want_fields", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("job_id", ctypes.c_int64), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+                ("mass", ctypes.c_double), ("kinetic_energy", ctypes.c_double),
+                ("nodes", ctypes.c_int64), ("steps", ctypes.c_int64)]
 
 
 def make_desc(cfg):
@@ -112,6 +131,28 @@ def load(path=None):
     L.lbm_get_state.argtypes = [H, ctypes.c_void_p]
     L.lbm_set_state.argtypes = [H, ctypes.c_void_p]
     L.lbm_monitors.argtypes = [H, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_batch_create.argtypes = [ctypes.POINTER(H)]
+    L.lbm_batch_add.argtypes = [H, H]
+    L.lbm_batch_remove.argtypes = [H, H]
+    L.lbm_batch_step.argtypes = [H, ctypes.c_int64]
+    L.lbm_batch_step_timed.argtypes = [H, ctypes.c_int64, dp]
+    L.lbm_batch_sync.argtypes = [H]
+    L.lbm_batch_tune.argtypes = [H, ctypes.c_int32]
+    L.lbm_batch_size.argtypes = [H]
+    L.lbm_batch_size.restype = ctypes.c_int32
+    L.lbm_batch_destroy.argtypes = [H]
+    L.lbm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(H)]
+    L.lbm_server_submit.argtypes = [H, ctypes.POINTER(JobSpec), ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_server_tick.argtypes = [H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.lbm_server_poll.argtypes = [H, ctypes.POINTER(JobResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.lbm_server_take.argtypes = [H, ctypes.c_int64, dp, dp, dp, dp]
+    L.lbm_server_cancel.argtypes = [H, ctypes.c_int64]
+    L.lbm_server_set_check_every.argtypes = [H, ctypes.c_int32]
+    L.lbm_server_used_bytes.argtypes = [H]
+    L.lbm_server_used_bytes.restype = ctypes.c_size_t
+    L.lbm_server_clock.argtypes = [H]
+    L.lbm_server_clock.restype = ctypes.c_double
+    L.lbm_server_destroy.argtypes = [H]
     _lib = L
     return L
 
@@ -222,3 +263,133 @@ class Lattice:
         if st not in (0, 4):
             check(st)
         return m.value, k.value, n.value
+
+
+class Batch:
+    """Lattices stepped together, one persistent launch per configuration group per step."""
+
+    def __init__(self, lattices=()):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.lbm_batch_create(ctypes.byref(self.h)))
+        self.members = []
+        for lat in lattices:
+            self.add(lat)
+
+    def add(self, lat):
+        check(self.lib.lbm_batch_add(self.h, lat.h))
+        self.members.append(lat)
+
+    def remove(self, lat):
+        check(self.lib.lbm_batch_remove(self.h, lat.h))
+        self.members.remove(lat)
+
+    def step(self, n=1):
+        check(self.lib.lbm_batch_step(self.h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        check(self.lib.lbm_batch_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(self.lib.lbm_batch_sync(self.h))
+
+    def tune(self, trials=5):
+        check(self.lib.lbm_batch_tune(self.h, int(trials)))
+
+    def __len__(self):
+        return self.lib.lbm_batch_size(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching job server (lbm_server_*)."""
+
+    def __init__(self, budget_bytes, steps_per_tick=100, check_every=16):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        check(self.lib.lbm_server_create(int(budget_bytes), int(steps_per_tick), ctypes.byref(self.h)))
+        check(self.lib.lbm_server_set_check_every(self.h, int(check_every)))
+        self._keep = {}
+
+    def submit(self, cfg, node_type, nsteps, rho0=1.0, u0=(0.0, 0.0, 0.0), seed=0, noise_amp=0.0,
+               want_fields=False):
+        types = np.ascontiguousarray(node_type, dtype=np.uint8).reshape(-1)
+        spec = JobSpec()
+        spec.desc = make_desc(cfg)
+        spec.node_type = types.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        spec.rho0 = rho0
+        for k in range(3):
+            spec.u0[k] = u0[k]
+        spec.seed = seed
+        spec.noise_amp = noise_amp
+        spec.nsteps = int(nsteps)
+        spec.want_fields = int(bool(want_fields))
+        jid = ctypes.c_int64()
+        check(self.lib.lbm_server_submit(self.h, ctypes.byref(spec), ctypes.byref(jid)))
+        self._keep[jid.value] = int(np.prod([cfg["nx"], cfg["ny"], cfg.get("nz", 1)]))
+        return jid.value
+
+    def tick(self):
+        a, q = ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.lbm_server_tick(self.h, ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def poll(self, cap=4096):
+        buf = (JobResult * cap)()
+        n = ctypes.c_int32()
+        check(self.lib.lbm_server_poll(self.h, buf, cap, ctypes.byref(n)))
+        return [dict(job_id=r.job_id, status=r.status, submit_s=r.submit_s, admit_s=r.admit_s,
+                     finish_s=r.finish_s, mass=r.mass, kinetic_energy=r.kinetic_energy, nodes=r.nodes,
+                     steps=r.steps) for r in buf[:n.value]]
+
+    def take(self, job_id):
+        N = self._keep[job_id]
+        out = [np.zeros(N) for _ in range(4)]
+        check(self.lib.lbm_server_take(self.h, int(job_id), *[_dp(a) for a in out]))
+        return out
+
+    def cancel(self, job_id):
+        check(self.lib.lbm_server_cancel(self.h, int(job_id)))
+
+    @property
+    def used_bytes(self):
+        return self.lib.lbm_server_used_bytes(self.h)
+
+    def clock(self):
+        return self.lib.lbm_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

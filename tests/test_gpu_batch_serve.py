@@ -1,0 +1,189 @@
+"""Batched launches and the continuous-batching server: every lattice/job ends with exactly the
+populations it reaches when stepped alone (same per-node arithmetic), the server keeps resident
+jobs within its HBM budget, and a diverging job is retired alone."""
+import numpy as np
+import pytest
+
+from lbm_amd import Batch, Lattice, LbmError, Server, cases, geometry as geo
+from lbm_amd.capi import F16S, F32, F64, MRT_SCALAR
+
+pytestmark = pytest.mark.gpu
+
+
+def member_specs():
+    """(cfg, types, init) of a mixed set: several kernel configuration groups, uneven sizes."""
+    specs = []
+    cfg, t = cases.d2q9_lid_reference(64, storage=F32)
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d2q9_lid_reference(32, storage=F32)
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d2q9_karman(128, 48, 32, 24, 5, re=30.0, storage=F16S)
+    specs.append((cfg, t, ("eq", (0.1, 0, 0), 5, 1e-4)))
+    cfg, t = cases.d3q27_mrt_lid_gs(24, re=100.0, storage=F16S)  # MFMA engine
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F32)
+    specs.append((cfg, t, ("eq", (0.05, 0, 0), 9, 1e-4)))
+    cfg, t = cases.d3q27_mrt_lid_reference(12, storage=F64, engine=MRT_SCALAR)
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d3q27_bgk_lid_reference(20, u_lid=0.05, re=50.0, storage=F32)
+    cfg = dict(cfg, nx=70)  # 70 x 20 x 20: ragged tile row
+    t = geo.lid_cavity_3d_reference(70, 20, 20)
+    specs.append((cfg, t, ("ref",)))
+    return specs
+
+
+def make(spec):
+    cfg, t, init = spec
+    lat = Lattice(cfg, t)
+    if init[0] == "ref":
+        lat.init_reference(1.0)
+    else:
+        lat.init_equilibrium(1.0, init[1], init[2], init[3])
+    return lat
+
+
+def test_batch_matches_individual_stepping():
+    specs = member_specs()
+    alone = []
+    for sp in specs:
+        with make(sp) as lat:
+            lat.step(17)
+            lat.step(10)
+            alone.append(lat.state())
+    lats = [make(sp) for sp in specs]
+    with Batch(lats) as b:
+        assert len(b) == len(lats)
+        b.step(7)
+        b.tune(2)  # timing launches must leave every state as it is
+        b.step(19)
+        b.step(1)
+        got = [lat.state() for lat in lats]
+        steps = [lat.steps_done for lat in lats]
+    for k, (a, g) in enumerate(zip(alone, got)):
+        assert steps[k] == 27
+        assert np.array_equal(a, g), f"member {k} differs"
+    for lat in lats:
+        lat.close()
+
+
+def test_batch_membership_changes():
+    specs = member_specs()
+    ref = []
+    for sp in specs[:3]:
+        with make(sp) as lat:
+            lat.step(30)
+            ref.append(lat.state())
+    a, b_, c = (make(sp) for sp in specs[:3])
+    a.step(11)  # a joins at step 11, c joins late, b leaves early
+    with Batch([b_]) as bt:
+        bt.step(11)
+        bt.add(a)
+        bt.add(c)
+        bt.step(5)
+        bt.remove(b_)
+        b_.step(14)
+        bt.step(14)
+        c_missing = 30 - c.steps_done
+        bt.remove(c)
+        c.step(c_missing)
+        assert np.array_equal(a.state(), ref[0])
+    assert np.array_equal(b_.state(), ref[1])
+    assert np.array_equal(c.state(), ref[2])
+    with pytest.raises(LbmError):
+        with Batch([a]):
+            a.step(1)  # a member is stepped through its batch only
+    for lat in (a, b_, c):
+        lat.close()
+
+
+def job_list():
+    jobs = []
+    rng = np.random.default_rng(5)
+    for k in range(12):
+        if k % 2 == 0:
+            n = int(rng.integers(24, 96))
+            cfg, t = cases.d2q9_lid_reference(n, re=float(rng.uniform(20, 100)), storage=F16S)
+        else:
+            n = int(rng.integers(12, 28))
+            cfg, t = cases.d3q27_mrt_lid_gs(n, re=float(rng.uniform(20, 100)), storage=F16S)
+        jobs.append((cfg, t, int(rng.integers(20, 400)), int(rng.integers(1 << 30))))
+    return jobs
+
+
+def test_server_alone_equals_mix_and_budget():
+    jobs = job_list()
+    alone = []
+    for cfg, t, steps, seed in jobs:
+        with Lattice(cfg, t) as lat:
+            lat.init_equilibrium(1.0, (0, 0, 0), seed, 1e-4)
+            lat.step(steps)
+            alone.append(lat.fields())
+    from lbm_amd.capi import footprint_bytes
+    fps = [footprint_bytes(cfg) for cfg, _, _, _ in jobs]
+    budget = 3 * max(fps)  # forces queueing and admission as memory frees up
+    with Server(budget, steps_per_tick=37) as srv:
+        ids = []
+        for i, (cfg, t, steps, seed) in enumerate(jobs[:6]):
+            ids.append(srv.submit(cfg, t, steps, 1.0, (0, 0, 0), seed, 1e-4, want_fields=True))
+        results = []
+        max_used = 0
+        late = 6
+        while len(results) < len(jobs):
+            active, queued = srv.tick()
+            max_used = max(max_used, srv.used_bytes)
+            if late < len(jobs):  # later arrivals join while others are resident
+                cfg, t, steps, seed = jobs[late]
+                ids.append(srv.submit(cfg, t, steps, 1.0, (0, 0, 0), seed, 1e-4, want_fields=True))
+                late += 1
+            results += srv.poll()
+        assert max_used <= budget
+        by_id = {r["job_id"]: r for r in results}
+        for k, jid in enumerate(ids):
+            r = by_id[jid]
+            assert r["status"] == 0 and r["steps"] == jobs[k][2]
+            assert r["submit_s"] <= r["admit_s"] <= r["finish_s"]
+            got = srv.take(jid)
+            for a, b in zip(got, alone[k]):
+                assert np.array_equal(a, b), f"job {k}"
+
+
+def test_server_isolates_a_diverging_job():
+    good_cfg, good_t = cases.d2q9_lid_reference(48, storage=F32)
+    bad_cfg = dict(good_cfg, omega=1.99, u_wall=(0.4, 0.0, 0.0))
+    with Lattice(good_cfg, good_t) as lat:
+        lat.init_equilibrium(1.0, (0, 0, 0), 1, 0.0)
+        lat.step(3000)
+        want = lat.fields()
+    with Server(1 << 30, steps_per_tick=100, check_every=2) as srv:
+        g = srv.submit(good_cfg, good_t, 3000, seed=1, want_fields=True)
+        b = srv.submit(bad_cfg, good_t, 3000, seed=2, noise_amp=2.0)
+        done = []
+        while len(done) < 2:
+            srv.tick()
+            done += srv.poll()
+        st = {r["job_id"]: r for r in done}
+        assert st[b]["status"] == 4  # LBM_EUNSTABLE, retired early
+        assert st[b]["steps"] < 3000
+        assert st[g]["status"] == 0 and st[g]["steps"] == 3000
+        for a, w in zip(srv.take(g), want):
+            assert np.array_equal(a, w)
+
+
+def test_server_rejects_and_cancels():
+    cfg, t = cases.d2q9_lid_reference(64, storage=F32)
+    with Server(1 << 16, steps_per_tick=10) as srv:  # 64 KiB: too small for the job
+        with pytest.raises(LbmError):
+            srv.submit(cfg, t, 10)
+    with Server(1 << 30, steps_per_tick=10) as srv:
+        a = srv.submit(cfg, t, 100)
+        b = srv.submit(cfg, t, 100)
+        srv.tick()
+        srv.cancel(a)
+        assert srv.used_bytes > 0
+        while True:
+            active, queued = srv.tick()
+            if active == 0 and queued == 0:
+                break
+        ids = [r["job_id"] for r in srv.poll()]
+        assert ids == [b]
+        assert srv.used_bytes == 0
