@@ -26,13 +26,19 @@ HBM_PEAK_GBS = 8000.0
 
 def workloads():
     from lbm_amd import cases
-    from lbm_amd.capi import BF16S, F16S, F32, MRT_GS
+    from lbm_amd.capi import BF16S, F16S, F32, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
     W = {
         "d2q9_bgk_karman_1024x256_fp32": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
         "d2q9_bgk_karman_1024x256_f16s": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F16S),
         "d3q27_mrt_gs_lid_256_fp32": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32),
         "d3q27_mrt_gs_lid_256_f16s": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S),
+        "d3q27_mrt_gs_lid_256_fp32_mfma2": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_MFMA_F32),
+        "d3q27_mrt_gs_lid_256_fp32_fused": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_MFMA_FUSED),
+        "d3q27_mrt_gs_lid_256_fp32_scalar": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_SCALAR),
+        "d3q27_mrt_gs_lid_256_f16s_mfma2": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_MFMA_F32),
+        "d3q27_mrt_gs_lid_256_f16s_fused": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_MFMA_FUSED),
+        "d3q27_mrt_gs_lid_256_f16s_scalar": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_SCALAR),
         "d3q27_bgk_sphere_512x256x256_fp32": lambda: cases.d3q27_sphere_channel(storage=F32),
         "d3q27_bgk_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S),
         "d3q27_bgk_sphere_512x256x256_bf16s": lambda: cases.d3q27_sphere_channel(storage=BF16S),

@@ -67,10 +67,12 @@ typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
 /* How the MRT moment transforms are evaluated. */
 typedef enum {
-  LBM_MRT_AUTO = 0,         /* MFMA_F32 for FP32-arithmetic storages, SCALAR for F64 */
+  LBM_MRT_AUTO = 0,         /* an MFMA engine for FP32-arithmetic storages, SCALAR for F64 */
   LBM_MRT_SCALAR = 1,       /* per-node unrolled FMA code with compile-time matrices */
-  LBM_MRT_MFMA_F32 = 2,     /* 27x27 transforms on v_mfma_f32_16x16x4_f32 over 64-node tiles */
-  LBM_MRT_MFMA_BF16X3 = 3   /* transforms on bf16 MFMA, population operand split in 3 */
+  LBM_MRT_MFMA_F32 = 2,     /* m = M f and f = M^-1 m* on v_mfma_f32_16x16x4_f32, 64-node tiles */
+  LBM_MRT_MFMA_BF16X3 = 3,  /* reserved: transforms on bf16 MFMA (not available in this build) */
+  LBM_MRT_MFMA_FUSED = 4    /* whole collision as one FP32 MFMA contraction with the operator
+                               [M^-1 (I - S) M | M^-1 S E] precomputed on the host */
 } lbm_mrt_engine;
 
 /* Equilibrium moments for MRT_RAW: REFERENCE = the reference's hand-written table

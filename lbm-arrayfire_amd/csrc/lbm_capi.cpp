@@ -75,10 +75,10 @@ std::string check_desc(const lbm_desc *d) {
   if (d->collision != LBM_BGK && d->lattice != 27) return "MRT is implemented for D3Q27 only";
   if (d->storage < 0 || d->storage > 3) return "unknown storage format";
   if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
-  if (d->mrt_engine < 0 || d->mrt_engine > 3) return "unknown MRT engine";
+  if (d->mrt_engine < 0 || d->mrt_engine > 4) return "unknown MRT engine";
   if (d->mrt_engine == LBM_MRT_MFMA_BF16X3) return "the BF16x3 MFMA engine is not available in this build";
-  if (d->mrt_engine == LBM_MRT_MFMA_F32 && d->storage == LBM_F64)
-    return "the FP32 MFMA engine needs FP32 arithmetic (storage F32, F16S or BF16S)";
+  if ((d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED) && d->storage == LBM_F64)
+    return "the FP32 MFMA engines need FP32 arithmetic (storage F32, F16S or BF16S)";
   if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
   if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
   if (!std::isfinite(d->omega)) return "omega is not finite";
@@ -130,7 +130,9 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   h->N = (int64_t)d->nx * d->ny * d->nz;
   h->esize = storage_bytes(d->storage);
   h->stride = population_stride(h->N, h->esize);
-  h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, d->mrt_engine};
+  int engine = d->mrt_engine;
+  if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
+  h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
 
   // Node types; PHYSICAL mode marks the non-solid nodes whose pull needs the boundary path.
   h->host_types.assign(node_type, node_type + h->N);
@@ -192,7 +194,7 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   p.type = h->types;
   if (d->collision != LBM_BGK) {
     std::vector<float> tab(64 * MFMA_TAB);
-    build_mfma_table(d->collision, d->meq_mode, d->s, tab.data());
+    build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
     if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
       return cleanup(LBM_ENOMEM, "MRT table allocation failed");
     if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
@@ -281,7 +283,7 @@ static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
     h->steps++;
     return LBM_OK;
   };
-  if (nsteps >= 4 && !h->graph) {
+  if (nsteps >= 4 && !h->graph && h->use_graph) {
     hipStream_t cs = h->strm();
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

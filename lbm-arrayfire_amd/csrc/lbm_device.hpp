@@ -4,10 +4,12 @@
 // the reference's af::shift, cpp/d2q9_bgk_lid.cpp:18-28), applies the boundary rules, takes the
 // moments, collides and writes Q populations: 2*Q*s bytes of HBM traffic per node update.
 //
-// Arithmetic follows the reference's operation order (cpp/d2q9_bgk_lid.cpp:131-153,
+// FP64 arithmetic follows the reference's operation order (cpp/d2q9_bgk_lid.cpp:131-153,
 // rust/examples/d3q27_mrt_lid.rs:311-387) term by term, with every product and sum rounded
-// separately (-ffp-contract=off), so the FP64 kernels reproduce the CPU restatement bit for bit.
-// Only the FP32 MRT moment transforms use explicit FMAs.
+// separately (-ffp-contract=off).  FP32 arithmetic uses explicit FMAs in the equilibrium, the
+// BGK relaxation and the MRT moment transforms (the contraction the reference's FP32 GPU builds
+// get from their JIT compilers).  The CPU oracle follows the same sequences, so both precisions
+// reproduce it bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -47,14 +49,17 @@ template <int Q> struct Codec<ST_F32, Q> {
 // power-of-two scale keeps the small deviations of the low-weight D3Q27 directions out of the
 // half-precision subnormal range, so every stored value carries 11 significant bits.
 constexpr float F16S_SCALE = 32768.0f, F16S_INV_SCALE = 1.0f / 32768.0f;
+// The scale is a power of two, so h * 2^-15 + w_i as one FMA rounds exactly like the separate
+// product and sum, and so does v * 2^15 - w_i * 2^15 against (v - w_i) * 2^15.
 template <int Q> struct Codec<ST_F16S, Q> {
   using T = uint16_t;
   using C = float;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
-    return (float)__builtin_bit_cast(_Float16, p[k]) * F16S_INV_SCALE + (float)Lat<Q>::w[I];
+    return __builtin_fmaf((float)__builtin_bit_cast(_Float16, p[k]), F16S_INV_SCALE, (float)Lat<Q>::w[I]);
   }
   template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) {
-    p[k] = __builtin_bit_cast(uint16_t, (_Float16)((v - (float)Lat<Q>::w[I]) * F16S_SCALE));
+    constexpr float ws = -(float)Lat<Q>::w[I] * F16S_SCALE;
+    p[k] = __builtin_bit_cast(uint16_t, (_Float16)__builtin_fmaf(v, F16S_SCALE, ws));
   }
 };
 // Shifted bfloat16 storage: the stored value is f - w_i rounded to nearest even.
@@ -308,20 +313,35 @@ __device__ __forceinline__ C cdotu(C ux, C uy, C uz) {
   return e;
 }
 
+// f_eq,i = w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u).  FP64 evaluates the reference's expression
+// term by term (cpp/d2q9_bgk_lid.cpp:149-151); FP32 uses the multiply-add form
+// w_i rho (A + eu (3 + 4.5 eu)) with A = 1 - 1.5 u.u, three FMAs per direction.  usq_of returns u.u
+// in FP64 and A in FP32.
 template <int Q, int i, class C>
 __device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq) {
   const C eu = cdotu<Q, i>(ux, uy, uz);
-  return ((C)Lat<Q>::w[i] * rho) * ((((C)1 + (C)3 * eu) + (C)4.5 * (eu * eu)) - (C)1.5 * usq);
+  if constexpr (sizeof(C) == 4) {
+    const C t = __builtin_fmaf(4.5f, eu, 3.0f);
+    return ((C)Lat<Q>::w[i] * rho) * __builtin_fmaf(eu, t, usq);
+  } else {
+    return ((C)Lat<Q>::w[i] * rho) * ((((C)1 + (C)3 * eu) + (C)4.5 * (eu * eu)) - (C)1.5 * usq);
+  }
 }
 
 template <int Q, class C>
 __device__ __forceinline__ C usq_of(C ux, C uy, C uz) {
-  C u2 = ux * ux + uy * uy;
-  if constexpr (Lat<Q>::D == 3) u2 = u2 + uz * uz;
-  return u2;
+  if constexpr (sizeof(C) == 4) {
+    C u2 = __builtin_fmaf(uy, uy, ux * ux);
+    if constexpr (Lat<Q>::D == 3) u2 = __builtin_fmaf(uz, uz, u2);
+    return __builtin_fmaf(-1.5f, u2, 1.0f);
+  } else {
+    C u2 = ux * ux + uy * uy;
+    if constexpr (Lat<Q>::D == 3) u2 = u2 + uz * uz;
+    return u2;
+  }
 }
 
-// F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153)
+// F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153); FP32: f + omega (f_eq - f).
 template <int Q, class C>
 __device__ __forceinline__ void collide_bgk(const StepParams &p, const C *f, C rho, C ux, C uy, C uz,
                                             C *out) {
@@ -329,7 +349,8 @@ __device__ __forceinline__ void collide_bgk(const StepParams &p, const C *f, C r
   const C usq = usq_of<Q>(ux, uy, uz);
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    out[i] = omega * feq_i<Q, i>(rho, ux, uy, uz, usq) + om1 * f[i];
+    if constexpr (sizeof(C) == 4) out[i] = __builtin_fmaf(omega, feq_i<Q, i>(rho, ux, uy, uz, usq) - f[i], f[i]);
+    else out[i] = omega * feq_i<Q, i>(rho, ux, uy, uz, usq) + om1 * f[i];
   });
 }
 
@@ -412,7 +433,7 @@ __device__ __forceinline__ void collide_mrt_scalar(const StepParams &p, const C 
     meq_reference<C>(rho, ux, uy, uz, meq);
   } else {
     C fe[27];
-    const C usq = ux * ux + uy * uy + uz * uz;
+    const C usq = usq_of<27>(ux, uy, uz);
     static_for<27>([&](auto I) {
       constexpr int i = decltype(I)::value;
       fe[i] = feq_i<27, i>(rho, ux, uy, uz, usq);

@@ -33,6 +33,7 @@ struct lbm_ctx {
   hipStream_t borrowed = nullptr;     // the batch's stream while a member: all work goes there
   hipGraphExec_t graph = nullptr;     // two captured steps (cur -> other -> cur)
   int graph_cur = -1;                 // buffer index the graph starts from
+  bool use_graph = false;             // direct launches measure faster for single lattices
   std::vector<uint8_t> host_types;
   lbm::Config cfg{};
   lbm::StepParams base{};

@@ -66,11 +66,16 @@ hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipS
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 // True when the step of this configuration runs on the MFMA MRT kernel.
+// Engine AUTO resolves to MRT_ENGINE_AUTO_FP32 for FP32-arithmetic storages, scalar for FP64.
+enum { MRT_ENGINE_AUTO = 0, MRT_ENGINE_SCALAR = 1, MRT_ENGINE_MFMA = 2, MRT_ENGINE_BF16X3 = 3, MRT_ENGINE_FUSED = 4,
+       MRT_ENGINE_AUTO_FP32 = MRT_ENGINE_MFMA };
 inline bool uses_mfma(const Config &c) {
-  return c.collision != CO_BGK && c.storage != ST_F64 && (c.engine == 0 || c.engine == 2);
+  return c.collision != CO_BGK && c.storage != ST_F64 &&
+         (c.engine == MRT_ENGINE_MFMA || c.engine == MRT_ENGINE_FUSED);
 }
-// Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice.
-void build_mfma_table(int collision, int meq_mode, const double *s, float *tab);
+// Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice (two-stage
+// transform layout, or the fused collision operator when fused is set).
+void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, float *tab);
 // Reference-style fields (rho, u after the boundary overrides) of the state p.fin.
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);

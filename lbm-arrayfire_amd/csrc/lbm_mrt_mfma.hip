@@ -19,20 +19,29 @@
 //            (full-way bounce-back at solid nodes in REFERENCE mode).
 // Per-lane A operands (M, s.E, M^-1 entries and the 1 - s scales, 48 floats) come from a table
 // the host builds for the lattice's basis and rates (build_mfma_table).
+//
+// The fused engine folds the three stages into one operator on the host,
+//   f' = M^-1 (I - S) M f + M^-1 S E phi = [C | D] [f; phi],
+// and runs it as a single K = 48 contraction per tile (24 MFMA instead of 40).
 #include "lbm_device.hpp"
 
 namespace lbm {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MT_STRIDE = 44;  // floats per node row: f[0..26], 0, phi[0..12] at 28..40, pad
+// LDS row of one node.  Two-stage engine: f[0..26], 0, phi[0..12] at 28..40, pad to 44.
+// Fused engine: f[0..26], phi[0..12] at 27..39, zeros to 52 (K = 48 slots, 12 per lane group).
+template <bool FUSED> constexpr int mt_stride() { return FUSED ? 52 : 44; }
+template <bool FUSED> constexpr int phi_at() { return FUSED ? 27 : 28; }
+constexpr int MT_STRIDE_MAX = 52;
 
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int ST, int BC>
-__device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, float (*lds)[64 * MT_STRIDE]) {
+template <int ST, int BC, bool FUSED>
+__device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, float (*lds)[64 * MT_STRIDE_MAX]) {
+  constexpr int MT_STRIDE = mt_stride<FUSED>();
   using CD = Codec<ST, 27>;
   using C = typename CD::C;
   static_assert(sizeof(C) == 4, "MFMA path computes in FP32");
@@ -56,17 +65,21 @@ __device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, flo
     } else {
       static_for<27>([&](auto I) { f[decltype(I)::value] = 0; });
     }
-    const float row[MT_STRIDE] = {
-        f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], f[9], f[10], f[11], f[12], f[13],
-        f[14], f[15], f[16], f[17], f[18], f[19], f[20], f[21], f[22], f[23], f[24], f[25], f[26], 0.f,
-        rho, rho * ux, rho * uy, rho * uz, rho * ux * ux, rho * uy * uy, rho * uz * uz,
-        rho * ux * uy, rho * ux * uz, rho * uy * uz, ux * ux, uy * uy, uz * uz, 0.f, 0.f, 0.f};
+    float row[MT_STRIDE];
+    static_for<27>([&](auto I) { row[decltype(I)::value] = f[decltype(I)::value]; });
+    constexpr int P = phi_at<FUSED>();
+    const float phi[13] = {rho, rho * ux, rho * uy, rho * uz, rho * ux * ux, rho * uy * uy, rho * uz * uz,
+                           rho * ux * uy, rho * ux * uz, rho * uy * uz, ux * ux, uy * uy, uz * uz};
+#pragma unroll
+    for (int k = 27; k < MT_STRIDE; ++k) row[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) row[P + k] = phi[k];
     f32x4 *dst = (f32x4 *)(L + lane * MT_STRIDE);
 #pragma unroll
     for (int k = 0; k < MT_STRIDE / 4; ++k)
       dst[k] = f32x4{row[4 * k], row[4 * k + 1], row[4 * k + 2], row[4 * k + 3]};
   }
-  // lane constants: AF[2][8], AP[2][4], SC[2][4], AI[2][8]
+  // lane constants: two-stage AF[2][8], AP[2][4], SC[2][4], AI[2][8]; fused AC[2][12]
   f32x4 tab[12];
   {
     const f32x4 *src = (const f32x4 *)(p.mrt_tab + lane * MFMA_TAB);
@@ -76,6 +89,25 @@ __device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, flo
   __syncthreads();
 
   // ---- phase 2: 4 tiles of 16 nodes on MFMA ----------------------------------------------
+  if constexpr (FUSED) {
+    // f' = [M^-1 (I - S) M | M^-1 S E] [f; phi]: one K = 48 contraction, 2 row tiles x 12 steps
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float *row = L + (16 * j + c) * MT_STRIDE;
+      const f32x4 b0 = *(const f32x4 *)(row + 12 * g);
+      const f32x4 b1 = *(const f32x4 *)(row + 12 * g + 4);
+      const f32x4 b2 = *(const f32x4 *)(row + 12 * g + 8);
+      f32x4 o0 = {0, 0, 0, 0}, o1 = {0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < 12; ++s) {
+        const float b = s < 4 ? b0[s & 3] : (s < 8 ? b1[s & 3] : b2[s & 3]);
+        o0 = mfma4(tab[s >> 2][s & 3], b, o0);        // AC[0][s]
+        o1 = mfma4(tab[3 + (s >> 2)][s & 3], b, o1);  // AC[1][s]
+      }
+      *(f32x4 *)(row + 4 * g) = o0;
+      *(f32x4 *)(row + 16 + 4 * g) = o1;
+    }
+  } else {
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     float *row = L + (16 * j + c) * MT_STRIDE;
@@ -106,6 +138,7 @@ __device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, flo
     *(f32x4 *)(row + 4 * g) = o0;       // f'[4g + r]
     *(f32x4 *)(row + 16 + 4 * g) = o1;  // f'[16 + 4g + r]
   }
+  }
   __syncthreads();
 
   // ---- phase 3: node per lane again, store ----------------------------------------------
@@ -129,50 +162,48 @@ __device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, flo
   store_all<27, ST>(p, q.n, out);
 }
 
-template <int ST, int BC>
+template <int ST, int BC, bool FUSED>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
-  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE];
-  mrt_mfma_tile<ST, BC>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE_MAX];
+  mrt_mfma_tile<ST, BC, FUSED>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
 }
 
-template <int ST, int BC>
+template <int ST, int BC, bool FUSED>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch_mrt_mfma(BatchArgs b) {
-  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE];
-  for_batch_tiles(b, [&](const StepParams &p, int tile) { mrt_mfma_tile<ST, BC>(p, tile, lds); });
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE_MAX];
+  for_batch_tiles(b, [&](const StepParams &p, int tile) { mrt_mfma_tile<ST, BC, FUSED>(p, tile, lds); });
+}
+
+template <class F>
+static hipError_t with_mfma_variant(const Config &c, F &&f) {
+  auto byST = [&](auto STc) -> hipError_t {
+    auto byBC = [&](auto BCc) -> hipError_t {
+      return c.engine == MRT_ENGINE_FUSED ? f(STc, BCc, std::true_type{}) : f(STc, BCc, std::false_type{});
+    };
+    return c.bc == BC_REF ? byBC(IC<BC_REF>{}) : byBC(IC<BC_PHYS>{});
+  };
+  switch (c.storage) {
+    case ST_F32: return byST(IC<ST_F32>{});
+    case ST_F16S: return byST(IC<ST_F16S>{});
+    case ST_BF16S: return byST(IC<ST_BF16S>{});
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
-  auto go = [&](auto STc) -> hipError_t {
-    constexpr int ST = decltype(STc)::value;
-    if (c.bc == BC_REF)
-      hipLaunchKernelGGL((k_step_batch_mrt_mfma<ST, BC_REF>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
-    else
-      hipLaunchKernelGGL((k_step_batch_mrt_mfma<ST, BC_PHYS>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
+    hipLaunchKernelGGL((k_step_batch_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),
+                       dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
     return hipGetLastError();
-  };
-  switch (c.storage) {
-    case ST_F32: return go(IC<ST_F32>{});
-    case ST_F16S: return go(IC<ST_F16S>{});
-    case ST_BF16S: return go(IC<ST_BF16S>{});
-  }
-  return hipErrorInvalidValue;
+  });
 }
 
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s) {
-  auto go = [&](auto STc) -> hipError_t {
-    constexpr int ST = decltype(STc)::value;
-    if (c.bc == BC_REF)
-      hipLaunchKernelGGL((k_step_mrt_mfma<ST, BC_REF>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
-    else
-      hipLaunchKernelGGL((k_step_mrt_mfma<ST, BC_PHYS>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
+  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
+    hipLaunchKernelGGL((k_step_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),
+                       dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
     return hipGetLastError();
-  };
-  switch (c.storage) {
-    case ST_F32: return go(IC<ST_F32>{});
-    case ST_F16S: return go(IC<ST_F16S>{});
-    case ST_BF16S: return go(IC<ST_BF16S>{});
-  }
-  return hipErrorInvalidValue;
+  });
 }
 
 }  // namespace lbm

@@ -15,6 +15,9 @@
 //   AI[R][t] = M^-1[16R + c][16(t/4) + 4g + t%4]  inverse, K slot = accumulator register t%4 of
 //                                                 row tile t/4, i.e. moment 16(t/4) + 4g + t%4
 // Entries outside the 27 x 27 (x 13) matrices are zero.
+//
+// Fused engine: lane l supplies AC[R][t] = [C | D][16R + c][12g + t] with
+// C = M^-1 diag(1 - s) M and D = M^-1 diag(s) E (K slots 0..26 populations, 27..39 monomials).
 #include <vector>
 
 #include "lbm_internal.hpp"
@@ -22,7 +25,7 @@
 
 namespace lbm {
 
-void build_mfma_table(int collision, int meq_mode, const double *s, float *tab) {
+void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, float *tab) {
   double M[27][27], Mi[27][27], E[27][13] = {};
   const bool raw = collision == CO_MRT_RAW;
   for (int k = 0; k < 27; ++k)
@@ -67,6 +70,33 @@ void build_mfma_table(int collision, int meq_mode, const double *s, float *tab) 
         for (int i = 0; i < 27; ++i) acc += M[k][i] * W[i][t];
         E[k][t] = acc;
       }
+  }
+  if (fused) {
+    // K slot k of the fused operator: population k (k < 27), monomial k - 27 (27 <= k < 40), zero.
+    double Cf[27][48] = {};
+    for (int i = 0; i < 27; ++i) {
+      for (int j = 0; j < 27; ++j) {
+        double acc = 0;
+        for (int k = 0; k < 27; ++k) acc += Mi[i][k] * (1.0 - s[k]) * M[k][j];
+        Cf[i][j] = acc;
+      }
+      for (int t = 0; t < 13; ++t) {
+        double acc = 0;
+        for (int k = 0; k < 27; ++k) acc += Mi[i][k] * s[k] * E[k][t];
+        Cf[i][27 + t] = acc;
+      }
+    }
+    for (int lane = 0; lane < 64; ++lane) {
+      const int g = lane >> 4, c = lane & 15;
+      float *T = tab + lane * MFMA_TAB;
+      for (int k = 0; k < MFMA_TAB; ++k) T[k] = 0.f;
+      for (int R = 0; R < 2; ++R)
+        for (int t = 0; t < 12; ++t) {
+          const int row = 16 * R + c;
+          T[R * 12 + t] = row < 27 ? (float)Cf[row][12 * g + t] : 0.f;
+        }
+    }
+    return;
   }
   for (int lane = 0; lane < 64; ++lane) {
     const int g = lane >> 4, c = lane & 15;

@@ -17,7 +17,7 @@ D2Q9, D3Q27 = 9, 27
 BGK, MRT_RAW, MRT_GS = 0, 1, 2
 F64, F32, F16S, BF16S = 0, 1, 2, 3
 BC_REFERENCE, BC_PHYSICAL = 0, 1
-MRT_AUTO, MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_BF16X3 = 0, 1, 2, 3
+MRT_AUTO, MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_BF16X3, MRT_MFMA_FUSED = 0, 1, 2, 3, 4
 MEQ_REFERENCE, MEQ_PROJECTED = 0, 1
 INLET_UNIFORM, INLET_PARABOLIC = 0, 1
 NODE_FLUID, NODE_SOLID, NODE_MOVING, NODE_INLET, NODE_OUTLET, NODE_RHO_FIX = 0, 1, 2, 4, 8, 16

@@ -25,6 +25,9 @@
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
 //
+// FP32 mode contracts the equilibrium and the BGK relaxation into FMAs (see feq()), matching the
+// kernels; FP64 mode keeps every operation separately rounded.
+//
 // Summation order inside a node is ascending population index; ArrayFire's reduction order is
 // not observable without ArrayFire, so FP32 bitwise parity with ArrayFire is unpinnable (FP64
 // differences stay at round-off).  Pinned in tests/test_oracle.py by closed forms, brute force
@@ -270,16 +273,39 @@ struct Sim {
     if (t & RHO_FIX) rho = (T)d.rho_bc;
   }
 
-  // BGK equilibrium + relaxation at one node (cpp/d2q9_bgk_lid.cpp:149-153 operation order).
+  // Equilibrium.  FP64: the reference's expression term by term (cpp/d2q9_bgk_lid.cpp:149-151).
+  // FP32: the contracted form w rho (A + eu (3 + 4.5 eu)), A = 1 - 1.5 u.u, as the reference's
+  // FP32 GPU builds get from their JIT compilers' multiply-add contraction.
+  T usq_term(T ux, T uy, T uz) const {
+    if constexpr (std::is_same<T, float>::value) {
+      T u2 = std::fma(uy, uy, ux * ux);
+      if (L.D == 3) u2 = std::fma(uz, uz, u2);
+      return std::fma(-1.5f, u2, 1.0f);
+    } else {
+      T u2 = ux * ux + uy * uy;
+      if (L.D == 3) u2 = u2 + uz * uz;
+      return u2;
+    }
+  }
+  T feq(int i, T rho, T ux, T uy, T uz, T usq) const {
+    T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
+    if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
+    if constexpr (std::is_same<T, float>::value) {
+      const T t = std::fma(4.5f, eu, 3.0f);
+      return (w[i] * rho) * std::fma(eu, t, usq);
+    } else {
+      return (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+    }
+  }
+
+  // BGK relaxation (cpp/d2q9_bgk_lid.cpp:153): omega feq + (1 - omega) f; FP32 f + omega (feq - f).
   void bgk(const T *S, T rho, T ux, T uy, T uz, T *out) const {
     const T omega = (T)d.omega, om1 = (T)1 - omega;
-    T usq = ux * ux + uy * uy;
-    if (L.D == 3) usq = usq + uz * uz;
+    const T usq = usq_term(ux, uy, uz);
     for (int i = 0; i < L.Q; ++i) {
-      T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
-      if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
-      const T feq = (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
-      out[i] = omega * feq + om1 * S[i];
+      const T fe = feq(i, rho, ux, uy, uz, usq);
+      if constexpr (std::is_same<T, float>::value) out[i] = std::fma(omega, fe - S[i], S[i]);
+      else out[i] = omega * fe + om1 * S[i];
     }
   }
 
@@ -312,11 +338,8 @@ struct Sim {
       meq_reference(rho, ux, uy, uz, meq);
     } else {  // projected: meq = M f_eq(rho, u)
       T feq[27];
-      const T usq = ux * ux + uy * uy + uz * uz;
-      for (int i = 0; i < 27; ++i) {
-        const T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy + (T)L.cz[i] * uz;
-        feq[i] = (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
-      }
+      const T usq = usq_term(ux, uy, uz);
+      for (int i = 0; i < 27; ++i) feq[i] = this->feq(i, rho, ux, uy, uz, usq);
       for (int k = 0; k < 27; ++k) {
         T acc = 0;
         for (int i = 0; i < 27; ++i) acc += feq[i] * M[k][i];
@@ -554,16 +577,9 @@ int init_fields_impl(const oracle_desc *d, const uint8_t *types, const double *r
   for (int64_t n = 0; n < sim.N; ++n) {
     const T r = rho ? (T)rho[n] : (T)1, vx = ux ? (T)ux[n] : (T)0, vy = uy ? (T)uy[n] : (T)0,
             vz = (uz && sim.L.D == 3) ? (T)uz[n] : (T)0;
-    T zero[27] = {0}, out[27];
-    // F = f_eq: the BGK relaxation with omega = 1 evaluates w*rho*(...) exactly as f_eq
-    T usq = vx * vx + vy * vy;
-    if (sim.L.D == 3) usq = usq + vz * vz;
-    for (int i = 0; i < Q; ++i) {
-      T eu = (T)sim.L.cx[i] * vx + (T)sim.L.cy[i] * vy;
-      if (sim.L.D == 3) eu = eu + (T)sim.L.cz[i] * vz;
-      out[i] = (sim.w[i] * r) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
-    }
-    (void)zero;
+    T out[27];
+    const T usq = sim.usq_term(vx, vy, vz);
+    for (int i = 0; i < Q; ++i) out[i] = sim.feq(i, r, vx, vy, vz, usq);
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
   sim.codec();
@@ -583,7 +599,8 @@ int init_reference_impl(const oracle_desc *d, const uint8_t *types, double rho0,
       for (int i = 0; i < 27; ++i) S[i] = (T)rho0 / (T)27;
       sim.mrt_collide(S, (T)rho0, 0, 0, 0, out);
     } else {
-      for (int i = 0; i < Q; ++i) out[i] = (sim.w[i] * (T)rho0) * ((((T)1 + (T)0) + (T)0) - (T)0);
+      const T usq = sim.usq_term((T)0, (T)0, (T)0);
+      for (int i = 0; i < Q; ++i) out[i] = sim.feq(i, (T)rho0, (T)0, (T)0, (T)0, usq);
     }
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
