@@ -159,6 +159,10 @@ int64_t lbm_steps_done(lbm_handle h);
  * applied (REFERENCE) or of the post-boundary populations (PHYSICAL).  N doubles each; any
  * pointer may be NULL.  Blocks. */
 lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, double *uz);
+/* The same fields at `count` selected nodes (sensors); each output holds count doubles, NULL ok.
+ * Blocks.  No reference counterpart (the reference reads whole fields for display). */
+lbm_status lbm_get_fields_at(lbm_handle h, const int64_t *nodes, int32_t count, double *rho, double *ux,
+                             double *uy, double *uz);
 /* Current post-collision populations as doubles (decoded for shifted storage), N*Q.  Blocks. */
 lbm_status lbm_get_populations(lbm_handle h, double *f);
 lbm_status lbm_set_populations(lbm_handle h, const double *f);

@@ -218,6 +218,7 @@ lbm_status lbm_destroy(lbm_handle h) {
   if (h->red) (void)hipFree(h->red);
   if (h->fields) (void)hipFree(h->fields);
   if (h->mrt_tab) (void)hipFree(h->mrt_tab);
+  if (h->probe) (void)hipFree(h->probe);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -366,6 +367,37 @@ lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, dou
     if (dst[k])
       HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
   HIPCHK(hipStreamSynchronize(h->strm()));
+  return LBM_OK;
+}
+
+lbm_status lbm_get_fields_at(lbm_handle h, const int64_t *nodes, int32_t count, double *rho, double *ux,
+                             double *uy, double *uz) {
+  if (!h || (!nodes && count > 0) || count < 0) return fail(LBM_EINVAL, "bad argument");
+  if (count == 0) return LBM_OK;
+  for (int32_t k = 0; k < count; ++k)
+    if (nodes[k] < 0 || nodes[k] >= h->N) return fail(LBM_EINVAL, "node index out of range");
+  HIPCHK(hipSetDevice(h->device));
+  lbm_status st = ensure_fields(h);
+  if (st != LBM_OK) return st;
+  const size_t need = (size_t)count * (sizeof(int64_t) + 4 * sizeof(double));
+  if (need > h->probe_bytes) {
+    if (h->probe) HIPCHK(hipFree(h->probe));
+    h->probe = nullptr;
+    HIPCHK(hipMalloc(&h->probe, need));
+    h->probe_bytes = need;
+  }
+  int64_t *dnodes = (int64_t *)h->probe;
+  double *dout = (double *)((char *)h->probe + (size_t)count * sizeof(int64_t));
+  HIPCHK(hipMemcpyAsync(dnodes, nodes, (size_t)count * sizeof(int64_t), hipMemcpyHostToDevice, h->strm()));
+  double *f = h->fields;
+  HIPCHK(launch_fields(h->cfg, h->params(h->cur), f, f + h->N, f + 2 * h->N, f + 3 * h->N, h->strm()));
+  HIPCHK(launch_gather_fields(f, h->N, dnodes, count, dout, h->strm()));
+  std::vector<double> host((size_t)4 * count);
+  HIPCHK(hipMemcpyAsync(host.data(), dout, host.size() * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
+  double *dst[4] = {rho, ux, uy, uz};
+  for (int a = 0; a < 4; ++a)
+    if (dst[a]) std::copy(host.begin() + (long)a * count, host.begin() + (long)(a + 1) * count, dst[a]);
   return LBM_OK;
 }
 

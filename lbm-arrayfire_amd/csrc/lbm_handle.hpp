@@ -28,6 +28,8 @@ struct lbm_ctx {
   double *red = nullptr;
   double *fields = nullptr;  // 4N doubles, allocated on first use
   float *mrt_tab = nullptr;  // MFMA MRT per-lane operands
+  void *probe = nullptr;     // sensor node list + gathered values
+  size_t probe_bytes = 0;
   int64_t steps = 0;
   lbm_batch_ctx *batch = nullptr;     // set while the lattice is a batch member
   hipStream_t borrowed = nullptr;     // the batch's stream while a member: all work goes there

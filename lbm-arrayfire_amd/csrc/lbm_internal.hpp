@@ -79,6 +79,9 @@ void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, 
 // Reference-style fields (rho, u after the boundary overrides) of the state p.fin.
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);
+// out[a * n + k] = fields[a * N + nodes[k]] for a = rho, ux, uy, uz.
+hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out,
+                                hipStream_t s);
 // f = f_eq(rho, u) into both buffers (fin and fout); field pointers may be null.
 hipError_t launch_init_fields(const Config &c, const StepParams &p, const double *rho, const double *ux,
                               const double *uy, const double *uz, hipStream_t s);

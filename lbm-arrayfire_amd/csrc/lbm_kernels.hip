@@ -189,6 +189,20 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_reduce_partials(const double 
   }
 }
 
+// Gathers rho/u of selected nodes from the four field arrays (sensors).
+__global__ void k_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t v = nodes[k];
+  for (int a = 0; a < 4; ++a) out[(size_t)a * n + k] = fields[(size_t)a * N + v];
+}
+
+hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_fields, dim3((n + 255) / 256), dim3(256), 0, s, fields, N, nodes, n, out);
+  return hipGetLastError();
+}
+
 // ---- dispatch ----------------------------------------------------------------------------
 template <class F>
 static hipError_t with_storage(const Config &c, F &&f) {

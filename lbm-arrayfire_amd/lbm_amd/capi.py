@@ -30,7 +30,8 @@ STORAGE_BYTES = {F64: 8, F32: 4, F16S: 2, BF16S: 2}
 EXPORTS = [
     "lbm_abi_version", "lbm_set_device", "lbm_last_error", "lbm_footprint_bytes", "lbm_create",
     "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
-    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_get_fields", "lbm_get_populations",
+    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_get_fields", "lbm_get_fields_at",
+    "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
     "lbm_batch_create", "lbm_batch_add", "lbm_batch_remove", "lbm_batch_step", "lbm_batch_step_timed",
     "lbm_batch_sync", "lbm_batch_tune", "lbm_batch_size", "lbm_batch_destroy",
@@ -124,6 +125,7 @@ def load(path=None):
     L.lbm_steps_done.argtypes = [H]
     L.lbm_steps_done.restype = ctypes.c_int64
     L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
+    L.lbm_get_fields_at.argtypes = [H, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, dp, dp, dp, dp]
     L.lbm_get_populations.argtypes = [H, dp]
     L.lbm_set_populations.argtypes = [H, dp]
     L.lbm_state_bytes.argtypes = [H]
@@ -236,6 +238,14 @@ class Lattice:
     def fields(self):
         out = [np.zeros(self.N) for _ in range(4)]
         check(self.lib.lbm_get_fields(self.h, *[_dp(a) for a in out]))
+        return out
+
+    def fields_at(self, nodes):
+        """rho, ux, uy, uz at the given node indices (x + nx*(y + ny*z))."""
+        idx = np.ascontiguousarray(nodes, dtype=np.int64).reshape(-1)
+        out = [np.zeros(idx.size) for _ in range(4)]
+        check(self.lib.lbm_get_fields_at(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                         int(idx.size), *[_dp(a) for a in out]))
         return out
 
     def populations(self):
