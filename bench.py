@@ -106,6 +106,25 @@ def cpu_baseline(name, target_s=10.0):
                       f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
 
 
+def serving_leg(njobs, scale):
+    """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
+    32^3-192^3, FP16-shifted storage) through the continuous-batching server, in real time and
+    saturated, against the same jobs run one at a time with the plain solver."""
+    from lbm_amd import serving
+
+    jobs = serving.make_mix(njobs, step_scale=scale)
+    budget = 64 << 30
+    rt = serving.serve(jobs, budget, realtime=True)
+    sat = serving.serve(jobs, budget, realtime=False)
+    seq = serving.sequential(jobs)
+    return {"jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
+            "storage": "F16S", "realtime": {k: round(v, 4) for k, v in rt.items()},
+            "saturated": {k: round(v, 4) for k, v in sat.items()},
+            "one_at_a_time": {k: round(v, 4) for k, v in seq.items()},
+            "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
+            "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +135,10 @@ def main():
                     help="comma list of extra workloads timed briefly, 'none', or 'default'")
     ap.add_argument("--also-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=120,
+                    help="jobs of the serving mix (0 skips the serving leg)")
+    ap.add_argument("--serve-scale", type=float, default=0.05,
+                    help="scale of the mix's 1k-20k steps per job")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "
@@ -135,6 +158,7 @@ def main():
                        "nonfinite": m["nonfinite"]}
         except Exception as e:  # noqa: BLE001 — report, keep the headline line
             also[k] = {"error": str(e)}
+    serving = serving_leg(args.serve_jobs, args.serve_scale) if args.serve_jobs > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     cfg = r["cfg"]
     out = {
@@ -164,6 +188,7 @@ def main():
         "wall_s_timed": round(r["wall_s"], 4),
         "nonfinite": r["nonfinite"],
         "also": also,
+        "serving": serving,
     }
     print(json.dumps(out))
 

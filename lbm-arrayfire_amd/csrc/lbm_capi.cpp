@@ -57,13 +57,6 @@ double dec_bf16(uint16_t b, double w) {
   return (double)__builtin_bit_cast(float, (uint32_t)b << 16) + (double)(float)w;
 }
 
-uint64_t splitmix64(uint64_t &s) {
-  uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
 std::string check_desc(const lbm_desc *d) {
   if (!d) return "descriptor is NULL";
   if (d->lattice != 9 && d->lattice != 27) return "lattice must be 9 (D2Q9) or 27 (D3Q27)";
@@ -254,16 +247,12 @@ lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, co
 lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], uint64_t seed,
                                 double noise_amp) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  const int D = h->Q == 9 ? 2 : 3;
-  std::vector<double> rho(h->N, rho0), u[3];
-  for (int a = 0; a < 3; ++a) u[a].assign(h->N, u0 ? u0[a] : 0.0);
-  uint64_t s = seed;
-  for (int64_t n = 0; n < h->N; ++n)
-    for (int a = 0; a < D; ++a) {
-      const double r = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
-      if (!(h->host_types[n] & NODE_SOLID)) u[a][n] += noise_amp * (2.0 * r - 1.0);
-    }
-  return lbm_init_fields(h, rho.data(), u[0].data(), u[1].data(), D == 3 ? u[2].data() : nullptr);
+  HIPCHK(hipSetDevice(h->device));
+  const double u[3] = {u0 ? u0[0] : 0.0, u0 ? u0[1] : 0.0, u0 ? u0[2] : 0.0};
+  HIPCHK(launch_init_equilibrium(h->cfg, h->params(h->cur), rho0, u, seed, noise_amp, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
+  h->steps = 0;
+  return LBM_OK;
 }
 
 lbm_status lbm_init_reference(lbm_handle h, double rho0) {

@@ -85,6 +85,9 @@ hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *
 // f = f_eq(rho, u) into both buffers (fin and fout); field pointers may be null.
 hipError_t launch_init_fields(const Config &c, const StepParams &p, const double *rho, const double *ux,
                               const double *uy, const double *uz, hipStream_t s);
+// f = f_eq(rho0, u0 + SplitMix64 noise at non-solid nodes), generated on the device.  Both buffers.
+hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double rho0, const double *u0,
+                                   uint64_t seed, double amp, hipStream_t s);
 // MRT: f = rho0/27 then one collision at rho0, u = 0; BGK: f_eq(rho0, 0). Both buffers.
 hipError_t launch_init_reference(const Config &c, const StepParams &p, double rho0, hipStream_t s);
 // Per-tile partial sums (mass, kinetic energy, nonfinite) then a final single-block reduction

@@ -69,6 +69,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fields(StepParams p, double *
   if (uz_o) uz_o[q.n] = (double)uz;
 }
 
+template <class F>
+static hipError_t with_storage_fwd(const Config &c, F &&f);
+
 // f = f_eq(rho, u) (cpp/d2q9_bgk_lid.cpp:100-107) into both buffers.
 template <int Q, int ST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_init_fields(StepParams p, const double *rho_i,
@@ -89,6 +92,52 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_fields(StepParams p, con
   StepParams p2 = p;
   p2.fout = const_cast<void *>(p.fin);
   store_all<Q, ST>(p2, q.n, out);
+}
+
+// f = f_eq(rho0, u0 + noise), the noise drawn on the device.  SplitMix64 is counter based: draw
+// k of the stream seeded with s is mix(s + (k + 1) * 0x9e3779b97f4a7c15), so node n, component a
+// (draw k = n * D + a) needs no sequential state.  The double arithmetic is the host contract's.
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+template <int Q, int ST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_equilibrium(StepParams p, double rho0, double u0x,
+                                                                    double u0y, double u0z, uint64_t seed,
+                                                                    double amp) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_thread(p);
+  if (!q.valid) return;
+  constexpr int D = Lat<Q>::D;
+  double u[3] = {u0x, u0y, u0z};
+  if (!(p.type[q.n] & NODE_SOLID))
+    for (int a = 0; a < D; ++a) {
+      const double r = (double)(splitmix64_at(seed, (uint64_t)q.n * D + a) >> 11) * 0x1.0p-53;
+      u[a] += amp * (2.0 * r - 1.0);
+    }
+  const C r = (C)rho0, vx = (C)u[0], vy = (C)u[1], vz = D == 3 ? (C)u[2] : (C)0;
+  const C usq = usq_of<Q>(vx, vy, vz);
+  C out[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    out[i] = feq_i<Q, i>(r, vx, vy, vz, usq);
+  });
+  store_all<Q, ST>(p, q.n, out);
+  StepParams p2 = p;
+  p2.fout = const_cast<void *>(p.fin);
+  store_all<Q, ST>(p2, q.n, out);
+}
+
+hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double rho0, const double *u0,
+                                   uint64_t seed, double amp, hipStream_t s) {
+  return with_storage_fwd(c, [&](auto Qc, auto STc) -> hipError_t {
+    hipLaunchKernelGGL((k_init_equilibrium<decltype(Qc)::value, decltype(STc)::value>), dim3(p.ntiles),
+                       dim3(BLOCK_THREADS), 0, s, p, rho0, u0[0], u0[1], u0[2], seed, amp);
+    return hipGetLastError();
+  });
 }
 
 // MRT: f = rho0/27 then one collision at rho = rho0, u = 0, no bounce-back
@@ -218,6 +267,11 @@ static hipError_t with_storage(const Config &c, F &&f) {
   if (c.Q == 9) return byQ(IC<9>{});
   if (c.Q == 27) return byQ(IC<27>{});
   return hipErrorInvalidValue;
+}
+
+template <class F>
+static hipError_t with_storage_fwd(const Config &c, F &&f) {
+  return with_storage(c, f);
 }
 
 template <class F>

@@ -1,0 +1,109 @@
+"""Serving-mix job generator and drivers for the continuous-batching server.
+
+The mix: jobs arrive as a Poisson process; half are D2Q9 BGK square lattices with side
+log-uniform in [64, 2048], half D3Q27 MRT (Gram-Schmidt basis) cubes with side log-uniform in
+[32, 192]; Re uniform in [50, 1000]; geometry is a lid cavity or a channel with a cylinder/sphere
+of radius 5-15 % of the height; steps uniform in [1000, 20000] times ``step_scale``; FP32
+arithmetic with FP16-shifted storage.
+
+Lattice-velocity policy (the reference fixes u and derives nu from Re): u = 0.05 for lids and
+inlets, nu = u L / Re, clamped to nu >= 1/150 (tau >= 0.52) so every job is stable; the Reynolds
+number actually run is recorded with the job.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import cases, geometry as geo
+from .capi import BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, INLET_UNIFORM, MRT_GS, Lattice, Server
+
+NU_MIN = 1.0 / 150.0
+U_LAT = 0.05
+
+
+def make_job(rng, step_scale=1.0):
+    d3 = rng.random() < 0.5
+    side = int(round(math.exp(rng.uniform(math.log(32), math.log(192))) if d3 else
+                     math.exp(rng.uniform(math.log(64), math.log(2048)))))
+    re = float(rng.uniform(50, 1000))
+    lid = rng.random() < 0.5
+    steps = max(1, int(round(int(rng.integers(1000, 20001)) * step_scale)))
+    L = side - 2
+    nu = max(U_LAT * L / re, NU_MIN)
+    re_run = U_LAT * L / nu
+    om = cases.omega_from_nu(nu)
+    if d3:
+        cfg = dict(lattice=D3Q27, nx=side, ny=side, nz=side, collision=MRT_GS, storage=F16S,
+                   bc_mode=BC_PHYSICAL, omega=om, s=cases.shear_mrt_rates(om))
+        if lid:
+            cfg["u_wall"] = (U_LAT, 0.0, 0.0)
+            types = geo.lid_cavity_3d_physical(side, side, side)
+        else:
+            r = max(1, int(round(rng.uniform(0.05, 0.15) * side)))
+            cfg.update(u_in=U_LAT, inlet_profile=INLET_UNIFORM)
+            types = geo.channel_3d_physical(side, side, side, side // 4, side // 2, side // 2, r)
+    else:
+        cfg = dict(lattice=D2Q9, nx=side, ny=side, nz=1, collision=BGK, storage=F16S,
+                   bc_mode=BC_PHYSICAL, omega=om)
+        if lid:
+            cfg["u_wall"] = (U_LAT, 0.0, 0.0)
+            types = geo.lid_cavity_2d_physical(side, side)
+        else:
+            r = max(1, int(round(rng.uniform(0.05, 0.15) * side)))
+            cfg.update(u_in=U_LAT, inlet_profile=INLET_UNIFORM)
+            types = geo.channel_2d_physical(side, side, side // 4, side // 2, r)
+    u0 = (0.0 if lid else U_LAT, 0.0, 0.0)
+    return dict(cfg=cfg, types=types.reshape(-1), steps=steps, re=re, re_run=re_run, u0=u0,
+                seed=int(rng.integers(1 << 62)), nodes=side ** (3 if d3 else 2))
+
+
+def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0):
+    rng = np.random.default_rng(seed)
+    jobs = [make_job(rng, step_scale) for _ in range(njobs)]
+    t = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    for j, a in zip(jobs, t):
+        j["arrival_s"] = float(a)
+    return jobs
+
+
+def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True):
+    """Runs the mix through the server.  realtime: submit each job at its arrival time (wall clock,
+    between ticks); otherwise submit everything at t = 0.  Returns per-job results and totals."""
+    res = []
+    with Server(budget_bytes, steps_per_tick=steps_per_tick, check_every=32) as srv:
+        t0 = time.perf_counter()
+        k = 0
+        ids = {}
+        while len(res) < len(jobs):
+            now = time.perf_counter() - t0
+            while k < len(jobs) and (not realtime or jobs[k]["arrival_s"] <= now):
+                j = jobs[k]
+                ids[srv.submit(j["cfg"], j["types"], j["steps"], 1.0, j["u0"], j["seed"], 1e-4)] = k
+                k += 1
+            active, queued = srv.tick()
+            res += srv.poll()
+            if active == 0 and queued == 0 and k < len(jobs):
+                time.sleep(max(0.0, min(0.002, jobs[k]["arrival_s"] - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+    # latency from the job's arrival (realtime) or its submission (everything at t = 0)
+    lat = np.array([r["finish_s"] - (jobs[ids[r["job_id"]]]["arrival_s"] if realtime else r["submit_s"])
+                    for r in res])
+    lups = sum(r["nodes"] * r["steps"] for r in res)
+    bad = sum(1 for r in res if r["status"] != 0)
+    return dict(wall_s=wall, jobs=len(res), failed=bad, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6)
+
+
+def sequential(jobs):
+    """The plain solver, one job at a time in arrival order (no idle time between jobs)."""
+    t0 = time.perf_counter()
+    lups = 0
+    for j in jobs:
+        with Lattice(j["cfg"], j["types"]) as lat:
+            lat.init_equilibrium(1.0, j["u0"], j["seed"], 1e-4)
+            lat.step(j["steps"])
+            lat.monitors()
+        lups += j["nodes"] * j["steps"]
+    wall = time.perf_counter() - t0
+    return dict(wall_s=wall, jobs=len(jobs), jobs_per_s=len(jobs) / wall, mlups=lups / wall / 1e6)
