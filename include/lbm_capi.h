@@ -61,8 +61,8 @@ typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_stora
  * bounce-back at solid nodes, Dirichlet overrides of rho/u before the equilibrium
  * (cpp/d2q9_bgk_lid.cpp:94-98,141-155; rust/examples/d2q9_bgk_channel.rs:190-201).
  * PHYSICAL: half-way (and moving-wall) bounce-back at links into solid nodes, non-equilibrium
- * bounce-back (Zou-He) velocity inlet, reference-pressure outlet with zero-gradient tangential
- * velocity (no reference counterpart). */
+ * bounce-back (Zou-He) velocity inlet, zero-gradient outlet at a reference density (no reference
+ * counterpart). */
 typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
 /* How the MRT moment transforms are evaluated. */
@@ -91,8 +91,8 @@ typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profil
                                   PHYSICAL (with SOLID): the wall moves with u_wall. */
 #define LBM_NODE_INLET 0x04u   /* REFERENCE: ux := inlet velocity override (channel :190).
                                   PHYSICAL: Zou-He velocity inlet on the x = 0 face. */
-#define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: outlet on the x = nx-1 face at density rho_bc, with
-                                  zero normal gradient of the tangential velocity. */
+#define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: zero-gradient outlet on the x = nx-1 face: unknowns
+                                  from f_eq(rho_bc, u) + f_neq of the upstream neighbour. */
 #define LBM_NODE_RHO_FIX 0x10u /* REFERENCE: rho := rho_bc after the solid override
                                   (channel set_row, rust/examples/d2q9_bgk_channel.rs:197-201). */
 

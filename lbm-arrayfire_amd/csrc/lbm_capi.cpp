@@ -36,9 +36,6 @@ lbm_status fail(lbm_status s, const std::string &m) {
   } while (0)
 
 double weight(int Q, int i) { return Q == 9 ? Lat<9>::w[i] : Lat<27>::w[i]; }
-int cxq(int Q, int i) { return Q == 9 ? Lat<9>::cx[i] : Lat<27>::cx[i]; }
-int cyq(int Q, int i) { return Q == 9 ? Lat<9>::cy[i] : Lat<27>::cy[i]; }
-int czq(int Q, int i) { return Q == 9 ? Lat<9>::cz[i] : Lat<27>::cz[i]; }
 
 // Host-side codecs for shifted storage: F16S stores (f - w_i) * 2^15 as IEEE half, BF16S stores
 // f - w_i as bfloat16, both rounded to nearest even.
@@ -127,30 +124,6 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
   h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
 
-  // Node types; PHYSICAL mode marks the non-solid nodes whose pull needs the boundary path.
-  h->host_types.assign(node_type, node_type + h->N);
-  for (auto &t : h->host_types) t &= 0x1f;
-  if (d->bc_mode == LBM_BC_PHYSICAL) {
-    const int nx = d->nx, ny = d->ny, nz = d->nz, Q = h->Q;
-    std::vector<uint8_t> slow(h->N, 0);
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-          const int64_t n = x + (int64_t)nx * (y + (int64_t)ny * z);
-          const uint8_t t = h->host_types[n];
-          if (t & NODE_SOLID) continue;
-          bool s = (t & (NODE_INLET | NODE_OUTLET)) != 0;
-          for (int i = 1; i < Q && !s; ++i) {
-            const int sx = ((x - cxq(Q, i)) % nx + nx) % nx, sy = ((y - cyq(Q, i)) % ny + ny) % ny,
-                      sz = ((z - czq(Q, i)) % nz + nz) % nz;
-            if (h->host_types[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) s = true;
-          }
-          slow[n] = s;
-        }
-    for (int64_t n = 0; n < h->N; ++n)
-      if (slow[n]) h->host_types[n] |= NODE_SLOW;
-  }
-
   StepParams &p = h->base;
   p.nx = d->nx;
   p.ny = d->ny;
@@ -182,8 +155,17 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
     return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
-  if (hipMemcpy(h->types, h->host_types.data(), (size_t)h->N, hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(LBM_EDEVICE, "node-type upload failed");
+  // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
+  // nodes whose pull needs the boundary path (NODE_SLOW).
+  {
+    uint8_t *raw = nullptr;
+    if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
+    hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(raw);
+    if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
+  }
   p.type = h->types;
   if (d->collision != LBM_BGK) {
     std::vector<float> tab(64 * MFMA_TAB);

@@ -164,6 +164,10 @@ template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz);
 template <int Q, int i, class C>
 __device__ __forceinline__ C cdotu(C ux, C uy, C uz);
+template <int Q, int i, class C>
+__device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq);
+template <int Q, class C>
+__device__ __forceinline__ C usq_of(C ux, C uy, C uz);
 
 // Non-equilibrium bounce-back closure of the unknowns (c_x == SGN) of an x-face node with
 // prescribed rho, u; transverse momentum corrected through N_a = sum_{c_x=0} f c_a - (2/3) rho u_a
@@ -193,8 +197,8 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 
 // PHYSICAL-mode pull: half-way bounce-back on links whose source is solid (plus the moving-wall
 // momentum term 6 w_i rho_w c_i.u_w); inlet nodes close their c_x = +1 unknowns for the
-// prescribed velocity; outlet nodes close their c_x = -1 unknowns for rho = rho_bc, the normal
-// velocity the known populations carry, and the upstream neighbour's tangential velocity.
+// prescribed velocity; outlet nodes set their c_x = -1 unknowns to f_eq(rho_bc, u_nb) plus the
+// non-equilibrium part of the upstream neighbour (zero normal gradient of u and f_neq).
 template <int Q, int ST>
 __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos &q, uint8_t t,
                                              typename Codec<ST, Q>::C *f) {
@@ -242,25 +246,25 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
     const C rho = (s0 + (C)2 * sm) / ((C)1 - uin);
     nebb_close<Q, 1>(f, rho, uin, (C)0, (C)0);
   } else if (t & NODE_OUTLET) {
-    C uy = 0, uz = 0;
-    if (q.x > 0 && !(p.type[q.n - 1] & NODE_SOLID)) {
-      C g[Q], r, vx, vy, vz;
+    // zero-gradient outlet at rho_bc: f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)),
+    // g = the upstream neighbour's post-collision populations
+    C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
+    const bool nb = q.x > 0 && !(p.type[q.n - 1] & NODE_SOLID);
+    if (nb) {
       static_for<Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         g[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n - 1);
       });
-      moments<Q>(g, r, vx, vy, vz);
-      uy = vy;
-      uz = vz;
+      moments<Q>(g, rn, vx, vy, vz);
     }
-    C s0 = 0, sp = 0;
+    const C usq = usq_of<Q>(vx, vy, vz);
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
-      if constexpr (Lat<Q>::cx[i] == 1) sp += f[i];
+      if constexpr (Lat<Q>::cx[i] == -1) {
+        const C fe = feq_i<Q, i>(rw, vx, vy, vz, usq);
+        f[i] = nb ? fe + (g[i] - feq_i<Q, i>(rn, vx, vy, vz, usq)) : fe;
+      }
     });
-    const C ux = (s0 + (C)2 * sp) / rw - (C)1;
-    nebb_close<Q, -1>(f, rw, ux, uy, uz);
   }
 }
 

@@ -36,7 +36,6 @@ struct lbm_ctx {
   hipGraphExec_t graph = nullptr;     // two captured steps (cur -> other -> cur)
   int graph_cur = -1;                 // buffer index the graph starts from
   bool use_graph = false;             // direct launches measure faster for single lattices
-  std::vector<uint8_t> host_types;
   lbm::Config cfg{};
   lbm::StepParams base{};
 

@@ -79,6 +79,10 @@ void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, 
 // Reference-style fields (rho, u after the boundary overrides) of the state p.fin.
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);
+// out = in & 0x1f, plus NODE_SLOW on non-solid nodes that are inlet/outlet nodes or pull from a
+// solid node (PHYSICAL mode).
+hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
+                                hipStream_t s);
 // out[a * n + k] = fields[a * N + nodes[k]] for a = rho, ux, uy, uz.
 hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out,
                                 hipStream_t s);

@@ -238,6 +238,37 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_reduce_partials(const double 
   }
 }
 
+template <int Q>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_prepare_types(int bc_mode, int nx, int ny, int nz,
+                                                                 const uint8_t *in, uint8_t *out) {
+  const int64_t n = (int64_t)blockIdx.x * BLOCK_THREADS + threadIdx.x;
+  if (n >= (int64_t)nx * ny * nz) return;
+  uint8_t t = in[n] & 0x1f;
+  if (bc_mode == BC_PHYS && !(t & NODE_SOLID)) {
+    const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
+    bool slow = (t & (NODE_INLET | NODE_OUTLET)) != 0;
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      int sx = x - Lat<Q>::cx[i], sy = y - Lat<Q>::cy[i], sz = z - Lat<Q>::cz[i];
+      sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+      sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+      sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+      if (in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) slow = true;
+    });
+    if (slow) t |= NODE_SLOW;
+  }
+  out[n] = t;
+}
+
+hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
+                                hipStream_t s) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  const dim3 grid((unsigned)((N + BLOCK_THREADS - 1) / BLOCK_THREADS));
+  if (Q == 9) hipLaunchKernelGGL(k_prepare_types<9>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
+  else hipLaunchKernelGGL(k_prepare_types<27>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
+  return hipGetLastError();
+}
+
 // Gathers rho/u of selected nodes from the four field arrays (sensors).
 __global__ void k_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -52,7 +52,7 @@ def d2q9_channel_reference(nx=300, ny=100, u_max=0.1, re=220.0, storage=F32):
 
 
 def d2q9_karman(nx=1024, ny=256, cx=256, cy=128, r=20, u_max=0.1, re=150.0, storage=F32):
-    """Channel with a cylinder: Zou-He parabolic inlet, reference-pressure outlet
+    """Channel with a cylinder: Zou-He parabolic inlet, zero-gradient outlet
     (zero-gradient tangential velocity), no-slip walls.
     Re = u_max * D / nu with D = 2r."""
     nu = u_max * 2 * r / re
@@ -89,7 +89,7 @@ def d3q27_bgk_lid_reference(n=64, u_lid=0.1, re=None, nu=None, storage=F32):
 
 def d3q27_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
                          storage=F32, collision=BGK):
-    """Duct with a sphere: uniform Zou-He inlet, reference-pressure outlet; Re = u * D / nu, D = 2r."""
+    """Duct with a sphere: uniform Zou-He inlet, zero-gradient outlet; Re = u * D / nu, D = 2r."""
     nu = u_in * 2 * r / re
     om = omega_from_nu(nu)
     cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,

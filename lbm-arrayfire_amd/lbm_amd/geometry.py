@@ -72,7 +72,7 @@ def lid_cavity_3d_physical(nx, ny, nz):
 
 def channel_2d_physical(nx, ny, cx, cy, r):
     """Channel with a circular obstacle (x-cx)^2+(y-cy)^2 <= r^2: walls y = 0 and y = ny-1, Zou-He
-    velocity inlet on x = 0, reference-pressure outlet on x = nx-1."""
+    velocity inlet on x = 0, zero-gradient outlet on x = nx-1."""
     x, y, _ = _grid(nx, ny)
     t = np.zeros((1, ny, nx), np.uint8)
     solid = ((x - cx) ** 2 + (y - cy) ** 2 <= r * r) | (y == 0) | (y == ny - 1)
@@ -84,7 +84,7 @@ def channel_2d_physical(nx, ny, cx, cy, r):
 
 def channel_3d_physical(nx, ny, nz, cx, cy, cz, r):
     """Duct with a spherical obstacle: walls y = 0, ny-1 and z = 0, nz-1, Zou-He inlet on x = 0,
-    reference-pressure outlet on x = nx-1."""
+    zero-gradient outlet on x = nx-1."""
     x, y, z = _grid(nx, ny, nz)
     t = np.zeros((nz, ny, nx), np.uint8)
     solid = ((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= r * r) | (y == 0) | (y == ny - 1) | \

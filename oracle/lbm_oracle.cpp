@@ -20,8 +20,8 @@
 //  * f32 index rounding the FP32 builds index through float arrays (cpp/d2q9_bgk_lid.cpp:86,
 //                       rust/examples/d2q9_bgk_lid.rs:97); f32_index_compat reproduces it.
 //  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
-//                       (non-equilibrium bounce-back) velocity inlet, reference-pressure outlet
-//                       with zero normal gradient of the tangential velocity.
+//                       (non-equilibrium bounce-back) velocity inlet, zero-gradient outlet at a
+//                       reference density (extrapolated velocity and non-equilibrium part).
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
 //
@@ -456,23 +456,21 @@ struct Sim {
       }
       const T rho = (s0 + (T)2 * sm) / ((T)1 - uin);
       nebb(f, 1, rho, uin, (T)0, (T)0);
-    } else if (t & OUTLET) {  // reference-pressure outlet, unknowns c_x = -1
-      T uy = 0, uz = 0;
-      if (x > 0 && !(type[n - 1] & SOLID)) {  // tangential velocity of the upstream neighbour
-        T g[27], r, vx, vy, vz;
+    } else if (t & OUTLET) {  // zero-gradient outlet at rho_bc, unknowns c_x = -1
+      // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
+      // post-collision populations (zero normal gradient of u and of the non-equilibrium part)
+      T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
+      const bool nb = x > 0 && !(type[n - 1] & SOLID);
+      if (nb) {
         for (int i = 0; i < L.Q; ++i) g[i] = Fin[(size_t)(n - 1 + N * i)];
-        moments(g, r, vx, vy, vz);
-        uy = vy;
-        uz = vz;
+        moments(g, rn, vx, vy, vz);
       }
-      T s0 = 0, sp = 0;
-      for (int i = 0; i < L.Q; ++i) {
-        if (L.cx[i] == 0) s0 += f[i];
-        if (L.cx[i] == 1) sp += f[i];
-      }
-      const T rho = rw;
-      const T ux = (s0 + (T)2 * sp) / rho - (T)1;
-      nebb(f, -1, rho, ux, uy, uz);
+      const T usq = usq_term(vx, vy, vz);
+      for (int i = 0; i < L.Q; ++i)
+        if (L.cx[i] == -1) {
+          const T fe = feq(i, rw, vx, vy, vz, usq);
+          f[i] = nb ? fe + (g[i] - feq(i, rn, vx, vy, vz, usq)) : fe;
+        }
     }
   }
 

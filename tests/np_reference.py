@@ -131,9 +131,9 @@ def step_reference(F, types, cfg):
 
 
 def step_physical(F, types, cfg):
-    """One PHYSICAL-mode step: half-way / moving-wall bounce-back, velocity inlet and
-    reference-pressure outlet closed by non-equilibrium bounce-back with transverse correction;
-    the outlet's tangential velocity is the upstream neighbour's (zero normal gradient)."""
+    """One PHYSICAL-mode step: half-way / moving-wall bounce-back, velocity inlet closed by
+    non-equilibrium bounce-back with transverse correction, zero-gradient outlet at rho_bc
+    (unknowns = f_eq(rho_bc, u_nb) + non-equilibrium part of the upstream neighbour)."""
     Q = cfg["lattice"]
     c, w, opp = lattice(Q)
     solid = (types & SOLID) != 0
@@ -167,16 +167,19 @@ def step_physical(F, types, cfg):
         rho_in = (sum(f[i] for i in zero) + 2 * sum(f[i] for i in neg)) / (1 - uin)
         close(f, inl, 1, rho_in, np.stack([uin, 0 * uin, 0 * uin]))
     if outlet.any():
-        # tangential velocity of the upstream neighbour (x - 1), previous post-collision state
+        # upstream neighbour (x - 1): previous post-collision populations and their moments
         Fm = np.roll(F, shift=1, axis=3)
-        um = np.einsum("ia,i...->a...", c.astype(float), Fm) / Fm.sum(0)
-        nb_solid = np.roll(solid, shift=1, axis=2)
-        pos = [i for i in range(Q) if c[i][0] == 1]
-        rho_out = np.full(types.shape, rw)
-        ux = (sum(f[i] for i in zero) + 2 * sum(f[i] for i in pos)) / rho_out - 1
-        uy = np.where(nb_solid, 0.0, um[1])
-        uz = np.where(nb_solid, 0.0, um[2])
-        close(f, outlet, -1, rho_out, np.stack([ux, uy, uz]))
+        rm = Fm.sum(0)
+        um = np.einsum("ia,i...->a...", c.astype(float), Fm) / rm
+        nb_ok = ~np.roll(solid, shift=1, axis=2)
+        nb_ok[..., 0] = False
+        um = np.where(nb_ok[None], um, 0.0)
+        fe = feq(np.full(types.shape, rw), um, c, w)
+        fn = feq(rm, um, c, w)
+        for i in range(Q):
+            if c[i][0] == -1:
+                new = np.where(nb_ok, fe[i] + (Fm[i] - fn[i]), fe[i])
+                f[i] = np.where(outlet, new, f[i])
     rho = f.sum(0)
     u = np.einsum("ia,i...->a...", c.astype(float), f) / np.where(solid, 1.0, rho)
     out = collide(f, rho, u, cfg, c, w)

@@ -307,6 +307,18 @@ def test_shifted_storage_roundtrip_bound(storage):
     assert err.max() > 0
 
 
+def test_outlet_keeps_bgk_duct_stable_at_low_viscosity():
+    """D3Q27 BGK duct from rest at tau = 0.536 with a uniform inlet: the outlet closure must not
+    seed the instability that an NEBB closure with the neighbour's tangential velocity did."""
+    from lbm_amd.capi import F32
+    nx, ny, nz = 64, 24, 24
+    cfg, _ = cases.d3q27_sphere_channel(nx, ny, nz, (16, 12, 12), 3, u_in=0.05, re=25.0, storage=F32)
+    assert abs(cfg["omega"] - 1.8656716) < 1e-6
+    types = geo.channel_3d_physical(nx, ny, nz, -100, -100, -100, 1).reshape(-1)
+    f = orc.run(cfg, types, orc.init_fields(cfg, types), 600)
+    assert np.isfinite(f).all()
+
+
 def test_zou_he_inlet_recovers_prescribed_velocity():
     cfg, types = cases.d2q9_karman(32, 14, 10, 7, 2, storage=F64)
     cfg = dict(cfg)
@@ -326,10 +338,10 @@ def test_zou_he_inlet_recovers_prescribed_velocity():
 
 
 def test_poiseuille_profile_develops():
-    """Empty channel, parabolic velocity inlet, reference-pressure outlet, half-way walls: the
-    flow becomes steady with a parabolic cross-section (walls at y = 0.5 and ny - 1.5), the mass
-    flux sum_y rho u_x is the same at every section, and the density falls to rho_bc at the
-    outlet."""
+    """Empty channel, parabolic velocity inlet, zero-gradient outlet at rho_bc, half-way walls:
+    the flow becomes steady with a parabolic cross-section (walls at y = 0.5 and ny - 1.5), the
+    mass flux sum_y rho u_x is the same at every section, and the density falls to about rho_bc
+    at the outlet."""
     nx, ny = 60, 12
     cfg = dict(lattice=9, nx=nx, ny=ny, nz=1, collision=BGK, storage=F64, bc_mode=BC_PHYSICAL,
                omega=1.0, u_in=0.04, inlet_profile=1)
@@ -348,7 +360,8 @@ def test_poiseuille_profile_develops():
     assert np.abs(flux[1:nx // 2] / flux[1] - 1).max() < 1e-4
     assert np.abs(flux[1:] / flux[1] - 1).max() < 5e-3  # odd-even ripple next to the outlet
     assert abs(flux[1] / inflow - 1) < 2e-2
-    assert np.abs(rho[1:-1, nx - 1] - 1.0).max() < 1e-12
+    # the outlet pins the unknown populations to rho_bc = 1; the known ones carry the rest
+    assert np.abs(rho[1:-1, nx - 1] - 1.0).max() < 1e-2
 
 
 def test_f32_index_compat_only_changes_large_lattices():
