@@ -2,7 +2,7 @@
 """Throughput benchmark of the fused LBM time step on one MI355X.
 
 Default workload: D2Q9 BGK channel with a cylinder (Karman street), 1024x256 nodes, FP32,
-cylinder r = 20 at (256,128), parabolic Zou-He inlet u_max = 0.1, reference-pressure outlet,
+cylinder r = 20 at (256,128), parabolic Zou-He inlet u_max = 0.1, zero-gradient outlet,
 no-slip walls, Re = 150, 1e-4 seeded noise (seed 0x5EED0002).  A "step" is one lattice time
 step over the whole lattice; MLUPS counts every node (solid included), 1e6 lattice updates/s.
 
@@ -106,6 +106,10 @@ def cpu_baseline(name, target_s=10.0):
                       f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
 
 
+def _rounded(d):
+    return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
+
+
 def serving_leg(njobs, scale):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage) through the continuous-batching server, in real time and
@@ -118,11 +122,23 @@ def serving_leg(njobs, scale):
     sat = serving.serve(jobs, budget, realtime=False)
     seq = serving.sequential(jobs)
     return {"jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
-            "storage": "F16S", "realtime": {k: round(v, 4) for k, v in rt.items()},
-            "saturated": {k: round(v, 4) for k, v in sat.items()},
-            "one_at_a_time": {k: round(v, 4) for k, v in seq.items()},
+            "storage": "F16S", "realtime": _rounded(rt), "saturated": _rounded(sat),
+            "one_at_a_time": _rounded(seq),
             "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
             "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+
+
+def copy_calibration(mib):
+    import torch
+
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
 
 
 def main():
@@ -139,7 +155,14 @@ def main():
                     help="jobs of the serving mix (0 skips the serving leg)")
     ap.add_argument("--serve-scale", type=float, default=0.05,
                     help="scale of the mix's 1k-20k steps per job")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain streaming copies of MIB MiB (out = -in, float32): the "
+                         "known-size kernel whose FETCH_SIZE / WRITE_SIZE counters scale the step "
+                         "kernel's counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "
               "multi-GPU path in this build.", file=sys.stderr)

@@ -67,7 +67,7 @@ typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
 /* How the MRT moment transforms are evaluated. */
 typedef enum {
-  LBM_MRT_AUTO = 0,         /* an MFMA engine for FP32-arithmetic storages, SCALAR for F64 */
+  LBM_MRT_AUTO = 0,         /* SCALAR: the fastest measured engine on gfx950 for every storage */
   LBM_MRT_SCALAR = 1,       /* per-node unrolled FMA code with compile-time matrices */
   LBM_MRT_MFMA_F32 = 2,     /* m = M f and f = M^-1 m* on v_mfma_f32_16x16x4_f32, 64-node tiles */
   LBM_MRT_MFMA_BF16X3 = 3,  /* reserved: transforms on bf16 MFMA (not available in this build) */
@@ -91,8 +91,9 @@ typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profil
                                   PHYSICAL (with SOLID): the wall moves with u_wall. */
 #define LBM_NODE_INLET 0x04u   /* REFERENCE: ux := inlet velocity override (channel :190).
                                   PHYSICAL: Zou-He velocity inlet on the x = 0 face. */
-#define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: zero-gradient outlet on the x = nx-1 face: unknowns
-                                  from f_eq(rho_bc, u) + f_neq of the upstream neighbour. */
+#define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: zero-gradient outlet on the x = nx-1 face: every
+                                  population = f_eq(rho_bc, u) + f_neq of the upstream neighbour
+                                  (u = the neighbour's velocity). */
 #define LBM_NODE_RHO_FIX 0x10u /* REFERENCE: rho := rho_bc after the solid override
                                   (channel set_row, rust/examples/d2q9_bgk_channel.rs:197-201). */
 
@@ -238,6 +239,21 @@ lbm_status lbm_server_cancel(lbm_server s, int64_t job_id);
 lbm_status lbm_server_set_check_every(lbm_server s, int32_t ticks);
 size_t lbm_server_used_bytes(lbm_server s);
 double lbm_server_clock(lbm_server s);
+
+/* Where a server's time went (host wall-clock seconds, cumulative since creation). */
+typedef struct {
+  int64_t ticks;            /* ticks that launched steps */
+  int64_t steps;            /* batch steps launched (sum of tick lengths) */
+  int64_t admitted, retired, unstable;
+  int64_t batch_rebuilds;   /* membership changes that rewrote the batch tables */
+  int64_t graph_captures;   /* step-graph captures */
+  double admit_s;           /* creating and initialising admitted jobs */
+  double retire_s;          /* final monitors, kept fields, releasing finished jobs */
+  double launch_s;          /* enqueueing tick steps (table rewrites and captures included) */
+  double wait_s;            /* waiting for the device at the end of ticks */
+  double check_s;           /* periodic non-finite checks of resident jobs */
+} lbm_server_stats;
+lbm_status lbm_server_get_stats(lbm_server s, lbm_server_stats *out);
 lbm_status lbm_server_destroy(lbm_server s);
 
 #ifdef __cplusplus

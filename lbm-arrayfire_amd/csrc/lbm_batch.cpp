@@ -3,22 +3,25 @@
 // Members are grouped by kernel configuration (lattice, storage, boundary mode, collision, MRT
 // engine).  One batch step is one persistent launch per group: the group's tiles are laid end to
 // end (member m owns tiles [prefix[m], prefix[m+1])) and each workgroup walks one contiguous chunk
-// (k_step_batch).  Parameter blocks for both buffer orientations live on the device, so a step
-// needs no host->device traffic; two steps are captured in a hipGraph and replayed.  The grid size
-// of each group is chosen by timing candidate sizes on the group itself (lbm_batch_tune): a trial
-// launch reads every member's current buffer and writes only its other buffer, which the next
-// real step overwrites, so tuning leaves the members' states untouched.
+// (k_step_batch).  Each group keeps one device table {header, prefix, parameter blocks for both
+// buffer orientations} with room for more members than it holds; a membership change rewrites the
+// table with one asynchronous copy from pinned staging memory, on the batch's stream.  The kernels
+// take only the table's address and read everything else from its header, so the hipGraph of two
+// captured steps stays valid across membership changes: it is re-captured only when the set of
+// groups changes, a table is reallocated (grown) or a grid size changes.  The grid size of each group is chosen by timing candidate sizes
+// on the group itself (lbm_batch_tune): a trial launch reads every member's current buffer and
+// writes only its other buffer, which the next real step overwrites, so tuning leaves the members'
+// states untouched.
 //
 // While a lattice is a member all of its work goes to the batch's stream, which keeps member
-// reads, writes and batch steps in one order.
+// reads, writes and batch steps in one order.  Joining and leaving are ordered against the
+// lattice's own stream with events, not host synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
-#include <map>
+#include <cstring>
 #include <new>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "lbm_handle.hpp"
@@ -27,18 +30,37 @@ using namespace lbm;
 
 namespace {
 
+
+// Device table of a group: BatchTable header | prefix[cap + 1] | params[0][cap] | params[1][cap].
+// The header holds the device addresses of the other parts, so launches depend only on d_tab.
 struct Group {
   Config cfg{};
   std::vector<lbm_ctx *> members;
-  StepParams *d_params[2] = {nullptr, nullptr};  // [phase][member]
-  int32_t *d_prefix = nullptr;
+  int cap = 0;                 // members the table has room for
+  size_t prefix_off = 0, par_off[2] = {0, 0}, bytes = 0;
+  char *d_tab = nullptr;       // device table
+  char *h_tab = nullptr;       // pinned staging copy
+  hipEvent_t uploaded = nullptr;
   int32_t total_tiles = 0;
   int grid = 0;
+  int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
+  int resident = 0;            // workgroups of this group's kernel that fit on one CU
+
+  BatchArgs args(int phase) const { return BatchArgs{(const BatchTable *)d_tab, phase}; }
+  void release() {
+    if (uploaded) (void)hipEventSynchronize(uploaded);
+    if (d_tab) (void)hipFree(d_tab);
+    if (h_tab) (void)hipHostFree(h_tab);
+    if (uploaded) (void)hipEventDestroy(uploaded);
+    d_tab = h_tab = nullptr;
+    uploaded = nullptr;
+    cap = 0;
+  }
 };
 
 bool same_cfg(const Config &a, const Config &b) {
   return a.Q == b.Q && a.storage == b.storage && a.bc == b.bc && a.collision == b.collision &&
-         uses_mfma(a) == uses_mfma(b);
+         uses_mfma(a) == uses_mfma(b) && (!uses_mfma(a) || a.engine == b.engine);
 }
 
 }  // namespace
@@ -46,22 +68,16 @@ bool same_cfg(const Config &a, const Config &b) {
 struct lbm_batch_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, order = nullptr;
   std::vector<lbm_ctx *> members;
   std::vector<Group> groups;
-  std::vector<int> tuned_grid;  // per group, 0 = default
   bool dirty = true;
-  int phase = 0;  // member m reads f[cur_at_build(m) ^ phase]
+  int phase = 0;  // the next launch uses parameter block [phase]
   hipGraphExec_t graph = nullptr;
   int64_t steps = 0;
+  lbm::BatchCounters counters{};
 
-  void free_groups() {
-    for (auto &g : groups) {
-      for (int k = 0; k < 2; ++k)
-        if (g.d_params[k]) (void)hipFree(g.d_params[k]);
-      if (g.d_prefix) (void)hipFree(g.d_prefix);
-    }
-    groups.clear();
+  void drop_graph() {
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
   }
@@ -75,15 +91,31 @@ namespace {
     if (e_ != hipSuccess) return set_error(LBM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-int default_grid(const Group &g, int cus) {
-  // enough resident workgroups to cover the chip several times over, never more than tiles
-  return std::max(1, std::min(g.total_tiles, 8 * cus));
+// Makes room for n members.  A reallocated table invalidates captured launches (*grown).
+lbm_status reserve(Group &g, int n, bool *grown) {
+  *grown = false;
+  if (n <= g.cap) return LBM_OK;
+  *grown = true;
+  g.release();
+  const int cap = std::max({n, 2 * g.cap, 8});
+  g.prefix_off = (sizeof(BatchTable) + 63) & ~(size_t)63;
+  const size_t pre_end = g.prefix_off + (size_t)(cap + 1) * sizeof(int32_t);
+  g.par_off[0] = (pre_end + 255) & ~(size_t)255;
+  g.par_off[1] = g.par_off[0] + (size_t)cap * sizeof(StepParams);
+  g.bytes = g.par_off[1] + (size_t)cap * sizeof(StepParams);
+  BCHK(hipMalloc((void **)&g.d_tab, g.bytes));
+  BCHK(hipHostMalloc((void **)&g.h_tab, g.bytes, hipHostMallocDefault));
+  std::memset(g.h_tab, 0, g.bytes);
+  BCHK(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming));
+  g.cap = cap;
+  return LBM_OK;
 }
 
+// Regroups the members and rewrites the group tables.  Members' cur is current (the batch phase
+// is folded in by enqueue), so the tables restart at phase 0.
 lbm_status rebuild(lbm_batch_ctx *b) {
-  // members' own state is current: fold the batch phase into their cur before regrouping
-  b->free_groups();
-  b->phase = 0;
+  bool recapture = false;
+  for (auto &g : b->groups) g.members.clear();
   for (lbm_ctx *m : b->members) {
     Group *grp = nullptr;
     for (auto &g : b->groups)
@@ -92,41 +124,63 @@ lbm_status rebuild(lbm_batch_ctx *b) {
       b->groups.emplace_back();
       grp = &b->groups.back();
       grp->cfg = m->cfg;
+      recapture = true;
     }
     grp->members.push_back(m);
   }
+  for (size_t gi = 0; gi < b->groups.size();) {
+    if (b->groups[gi].members.empty()) {
+      b->groups[gi].release();
+      b->groups.erase(b->groups.begin() + (long)gi);
+      recapture = true;
+    } else {
+      ++gi;
+    }
+  }
   const int cus = device_cu_count(b->device);
-  b->tuned_grid.resize(b->groups.size(), 0);
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    Group &g = b->groups[gi];
-    const size_t n = g.members.size();
-    std::vector<int32_t> prefix(n + 1, 0);
-    std::vector<StepParams> par[2];
-    for (int k = 0; k < 2; ++k) par[k].resize(n);
-    for (size_t i = 0; i < n; ++i) {
-      prefix[i + 1] = prefix[i] + g.members[i]->base.ntiles;
-      par[0][i] = g.members[i]->params(g.members[i]->cur);
-      par[1][i] = g.members[i]->params(g.members[i]->cur ^ 1);
+  for (auto &g : b->groups) {
+    const int n = (int)g.members.size();
+    bool grown = false;
+    lbm_status st = reserve(g, n, &grown);
+    if (st != LBM_OK) return st;
+    if (grown) recapture = true;
+    BCHK(hipEventSynchronize(g.uploaded));  // the previous upload has left the staging copy
+    BatchTable *hdr = (BatchTable *)g.h_tab;
+    int32_t *prefix = (int32_t *)(g.h_tab + g.prefix_off);
+    StepParams *par0 = (StepParams *)(g.h_tab + g.par_off[0]);
+    StepParams *par1 = (StepParams *)(g.h_tab + g.par_off[1]);
+    prefix[0] = 0;
+    for (int i = 0; i < n; ++i) {
+      prefix[i + 1] = prefix[i] + g.members[(size_t)i]->base.ntiles;
+      par0[i] = g.members[(size_t)i]->params(g.members[(size_t)i]->cur);
+      par1[i] = g.members[(size_t)i]->params(g.members[(size_t)i]->cur ^ 1);
     }
     g.total_tiles = prefix[n];
-    for (int k = 0; k < 2; ++k) {
-      BCHK(hipMalloc((void **)&g.d_params[k], n * sizeof(StepParams)));
-      BCHK(hipMemcpyAsync(g.d_params[k], par[k].data(), n * sizeof(StepParams), hipMemcpyHostToDevice, b->stream));
-    }
-    BCHK(hipMalloc((void **)&g.d_prefix, (n + 1) * sizeof(int32_t)));
-    BCHK(hipMemcpyAsync(g.d_prefix, prefix.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
-    g.grid = b->tuned_grid[gi] > 0 ? std::min(b->tuned_grid[gi], g.total_tiles) : default_grid(g, cus);
+    hdr->nmembers = n;
+    hdr->total_tiles = g.total_tiles;
+    hdr->pad_[0] = hdr->pad_[1] = 0;
+    hdr->prefix = (const int32_t *)(g.d_tab + g.prefix_off);
+    hdr->members[0] = (const StepParams *)(g.d_tab + g.par_off[0]);
+    hdr->members[1] = (const StepParams *)(g.d_tab + g.par_off[1]);
+    // the whole table (unused rows too) keeps the copy one contiguous transfer
+    BCHK(hipMemcpyAsync(g.d_tab, g.h_tab, g.bytes, hipMemcpyHostToDevice, b->stream));
+    BCHK(hipEventRecord(g.uploaded, b->stream));
+    if (g.resident == 0) BCHK(batch_resident_per_cu(g.cfg, &g.resident));
+    // default: one resident wave of workgroups; more would run the excess chunks as a second
+    // partial wave and stretch the launch
+    const int grid = g.tuned > 0 ? g.tuned : g.resident * cus;
+    if (grid != g.grid) recapture = true;
+    g.grid = grid;
   }
-  BCHK(hipStreamSynchronize(b->stream));
+  if (recapture) b->drop_graph();
+  b->phase = 0;
   b->dirty = false;
+  ++b->counters.rebuilds;
   return LBM_OK;
 }
 
 lbm_status launch_phase(lbm_batch_ctx *b, int phase, hipStream_t s) {
-  for (auto &g : b->groups) {
-    BatchArgs a{g.d_params[phase], g.d_prefix, (int32_t)g.members.size(), g.total_tiles};
-    BCHK(launch_step_batch(g.cfg, a, g.grid, s));
-  }
+  for (auto &g : b->groups) BCHK(launch_step_batch(g.cfg, g.args(phase), g.grid, s));
   return LBM_OK;
 }
 
@@ -147,6 +201,7 @@ lbm_status ensure_graph(lbm_batch_ctx *b) {
     b->graph = nullptr;
     return set_error(LBM_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
+  ++b->counters.captures;
   return LBM_OK;
 }
 
@@ -157,15 +212,15 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
   }
   if (b->groups.empty() || nsteps == 0) return LBM_OK;
   int64_t left = nsteps;
-  if (left >= 2) {
-    lbm_status st = ensure_graph(b);
-    if (st != LBM_OK) return st;
-  }
-  if (b->phase == 1 && left > 0) {
+  if (b->phase == 1) {
     lbm_status st = launch_phase(b, 1, b->stream);
     if (st != LBM_OK) return st;
     b->phase = 0;
     --left;
+  }
+  if (left >= 2) {
+    lbm_status st = ensure_graph(b);
+    if (st != LBM_OK) return st;
   }
   for (; left >= 2; left -= 2) BCHK(hipGraphLaunch(b->graph, b->stream));
   if (left == 1) {
@@ -184,6 +239,10 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
 
 }  // namespace
 
+namespace lbm {
+BatchCounters batch_counters(lbm_batch b) { return b ? b->counters : BatchCounters{}; }
+}  // namespace lbm
+
 extern "C" {
 
 lbm_status lbm_batch_create(lbm_batch *out) {
@@ -192,7 +251,8 @@ lbm_status lbm_batch_create(lbm_batch *out) {
   lbm_batch_ctx *b = new (std::nothrow) lbm_batch_ctx();
   if (!b) return set_error(LBM_ENOMEM, "host allocation failed");
   if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&b->order, hipEventDisableTiming) != hipSuccess) {
     lbm_batch_destroy(b);
     return set_error(LBM_EDEVICE, "batch stream/event creation failed");
   }
@@ -204,12 +264,9 @@ lbm_status lbm_batch_add(lbm_batch b, lbm_handle h) {
   if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
   if (h->batch) return set_error(LBM_EINVAL, "the lattice already belongs to a batch");
   if (h->device != b->device) return set_error(LBM_EINVAL, "the lattice lives on another device");
-  BCHK(hipStreamSynchronize(h->stream));
-  BCHK(hipStreamSynchronize(b->stream));
-  if (b->phase == 1) {  // fold the pending orientation into every current member first
-    b->phase = 0;
-    b->dirty = true;
-  }
+  // the batch's stream runs after everything already queued on the lattice's own stream
+  BCHK(hipEventRecord(b->order, h->stream));
+  BCHK(hipStreamWaitEvent(b->stream, b->order, 0));
   h->batch = b;
   h->borrowed = b->stream;
   b->members.push_back(h);
@@ -221,11 +278,12 @@ lbm_status lbm_batch_remove(lbm_batch b, lbm_handle h) {
   if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
   auto it = std::find(b->members.begin(), b->members.end(), h);
   if (it == b->members.end()) return set_error(LBM_EINVAL, "the lattice is not a member of this batch");
-  BCHK(hipStreamSynchronize(b->stream));
+  // the lattice's own stream runs after the batch work queued so far
+  BCHK(hipEventRecord(b->order, b->stream));
+  BCHK(hipStreamWaitEvent(h->stream, b->order, 0));
   b->members.erase(it);
   h->batch = nullptr;
   h->borrowed = nullptr;
-  b->phase = 0;
   b->dirty = true;
   return LBM_OK;
 }
@@ -241,6 +299,10 @@ lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms) {
   if (nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
   if (b->dirty) {
     lbm_status st = rebuild(b);
+    if (st != LBM_OK) return st;
+  }
+  if (nsteps >= 2 && b->phase == 0) {  // capture outside the timed region
+    lbm_status st = ensure_graph(b);
     if (st != LBM_OK) return st;
   }
   BCHK(hipEventRecord(b->ev0, b->stream));
@@ -268,14 +330,14 @@ lbm_status lbm_batch_tune(lbm_batch b, int32_t trials) {
   }
   const int cus = device_cu_count(b->device);
   const int reps = trials > 0 ? trials : 5;
-  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-    Group &g = b->groups[gi];
+  for (auto &g : b->groups) {
     int best = g.grid;
     float best_ms = 1e30f;
     for (int f : {1, 2, 4, 8, 16, 0}) {  // 0 = one workgroup per tile
       const int grid = f ? std::min(g.total_tiles, f * cus) : g.total_tiles;
+      if (grid < 1) continue;
       // trial launches read the current buffers and write only the other ones (b->phase)
-      BatchArgs a{g.d_params[b->phase], g.d_prefix, (int32_t)g.members.size(), g.total_tiles};
+      const BatchArgs a = g.args(b->phase);
       BCHK(launch_step_batch(g.cfg, a, grid, b->stream));  // warm-up
       BCHK(hipEventRecord(b->ev0, b->stream));
       for (int r = 0; r < reps; ++r) BCHK(launch_step_batch(g.cfg, a, grid, b->stream));
@@ -288,11 +350,10 @@ lbm_status lbm_batch_tune(lbm_batch b, int32_t trials) {
         best = grid;
       }
     }
+    g.tuned = best;
     g.grid = best;
-    b->tuned_grid[gi] = best;
   }
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  b->graph = nullptr;
+  b->drop_graph();
   return LBM_OK;
 }
 
@@ -305,9 +366,11 @@ lbm_status lbm_batch_destroy(lbm_batch b) {
     m->batch = nullptr;
     m->borrowed = nullptr;
   }
-  b->free_groups();
+  b->drop_graph();
+  for (auto &g : b->groups) g.release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->order) (void)hipEventDestroy(b->order);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return LBM_OK;

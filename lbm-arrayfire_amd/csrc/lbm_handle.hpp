@@ -57,4 +57,9 @@ namespace lbm {
 lbm_status set_error(lbm_status s, const std::string &m);
 // Number of compute units of the handle's device (cached).
 int device_cu_count(int device);
+// Work counters of a batch: table rewrites and step-graph captures.
+struct BatchCounters {
+  int64_t rebuilds = 0, captures = 0;
+};
+BatchCounters batch_counters(lbm_batch b);
 }  // namespace lbm

@@ -55,20 +55,32 @@ hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s);
 hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s);
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s);
 // Batched launches: member m of a batch owns tiles [prefix[m], prefix[m+1]) of the batch's tile
-// range.
-struct BatchArgs {
-  const StepParams *members;  // one parameter block per member (fin/fout of this step)
-  const int32_t *prefix;      // nmembers + 1 tile offsets
+// range.  Everything a launch needs about the members is read from a device-resident table whose
+// header locates the rest, so a launch captured in a graph stays valid when the host rewrites the
+// table (members joining or leaving) between replays.
+struct BatchTable {
   int32_t nmembers;
   int32_t total_tiles;
+  int32_t pad_[2];
+  const int32_t *prefix;         // nmembers + 1 tile offsets
+  const StepParams *members[2];  // parameter blocks of both buffer orientations
+};
+struct BatchArgs {
+  const BatchTable *tab;
+  int32_t phase;  // which orientation this launch steps
 };
 hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
+// Workgroups of the batched step kernel of c that fit on one compute unit at once.
+hipError_t batch_resident_per_cu(const Config &c, int *per_cu);
+hipError_t batch_resident_scalar(const Config &c, int *per_cu);
+hipError_t batch_resident_mrt_mfma(const Config &c, int *per_cu);
 // True when the step of this configuration runs on the MFMA MRT kernel.
-// Engine AUTO resolves to MRT_ENGINE_AUTO_FP32 for FP32-arithmetic storages, scalar for FP64.
+// Engine AUTO resolves to MRT_ENGINE_AUTO_FP32 for FP32-arithmetic storages (the sparse integer
+// scalar transform: faster on gfx950 than either MFMA engine), scalar for FP64.
 enum { MRT_ENGINE_AUTO = 0, MRT_ENGINE_SCALAR = 1, MRT_ENGINE_MFMA = 2, MRT_ENGINE_BF16X3 = 3, MRT_ENGINE_FUSED = 4,
-       MRT_ENGINE_AUTO_FP32 = MRT_ENGINE_MFMA };
+       MRT_ENGINE_AUTO_FP32 = MRT_ENGINE_SCALAR };
 inline bool uses_mfma(const Config &c) {
   return c.collision != CO_BGK && c.storage != ST_F64 &&
          (c.engine == MRT_ENGINE_MFMA || c.engine == MRT_ENGINE_FUSED);

@@ -53,6 +53,7 @@ struct lbm_server_ctx {
   std::vector<lbm_job_result> finished;
   std::map<int64_t, Fields> kept;
   lbm_batch batch = nullptr;
+  lbm_server_stats stats{};
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now() const {
@@ -65,6 +66,7 @@ namespace {
 constexpr int kMaxPassOver = 64;
 
 lbm_status retire(lbm_server_ctx *s, Job *j, bool check_fields) {
+  const double t_in = s->now();
   lbm_job_result r{};
   r.job_id = j->id;
   r.submit_s = j->submit_s;
@@ -92,11 +94,16 @@ lbm_status retire(lbm_server_ctx *s, Job *j, bool check_fields) {
   s->used -= j->footprint;
   r.finish_s = s->now();
   s->finished.push_back(r);
+  ++s->stats.retired;
+  if (r.status == LBM_EUNSTABLE) ++s->stats.unstable;
+  s->stats.retire_s += r.finish_s - t_in;
   delete j;
   return LBM_OK;
 }
 
 lbm_status admit(lbm_server_ctx *s) {
+  const double t_in = s->now();
+  lbm_status st = LBM_OK;
   for (auto it = s->queue.begin(); it != s->queue.end();) {
     Job *j = *it;
     if (s->used + j->footprint > s->budget) {
@@ -104,8 +111,11 @@ lbm_status admit(lbm_server_ctx *s) {
       ++it;
       continue;
     }
-    lbm_status st = lbm_create(&j->spec.desc, j->types.data(), &j->h);
-    if (st == LBM_ENOMEM) break;  // the device is fuller than the budget says: wait
+    st = lbm_create(&j->spec.desc, j->types.data(), &j->h);
+    if (st == LBM_ENOMEM) {  // the device is fuller than the budget says: wait
+      st = LBM_OK;
+      break;
+    }
     if (st != LBM_OK) {  // bad job: report and drop it
       lbm_job_result r{};
       r.job_id = j->id;
@@ -115,20 +125,24 @@ lbm_status admit(lbm_server_ctx *s) {
       s->finished.push_back(r);
       delete j;
       it = s->queue.erase(it);
+      st = LBM_OK;
       continue;
     }
     st = lbm_init_equilibrium(j->h, j->spec.rho0, j->spec.u0, j->spec.seed, j->spec.noise_amp);
     if (st == LBM_OK) st = lbm_batch_add(s->batch, j->h);
     if (st != LBM_OK) {
       lbm_destroy(j->h);
-      return st;
+      j->h = nullptr;
+      break;
     }
     s->used += j->footprint;
     j->admit_s = s->now();
     s->active.push_back(j);
+    ++s->stats.admitted;
     it = s->queue.erase(it);
   }
-  return LBM_OK;
+  s->stats.admit_s += s->now() - t_in;
+  return st;
 }
 
 }  // namespace
@@ -189,10 +203,17 @@ lbm_status lbm_server_tick(lbm_server s, int32_t *active, int32_t *queued) {
   if (!s->active.empty()) {
     int64_t k = s->steps_per_tick;
     for (Job *j : s->active) k = std::min<int64_t>(k, j->spec.nsteps - j->done);
+    const double t_launch = s->now();
     st = lbm_batch_step(s->batch, k);
     if (st != LBM_OK) return st;
+    const double t_wait = s->now();
     st = lbm_batch_sync(s->batch);
     if (st != LBM_OK) return st;
+    const double t_done = s->now();
+    s->stats.launch_s += t_wait - t_launch;
+    s->stats.wait_s += t_done - t_wait;
+    ++s->stats.ticks;
+    s->stats.steps += k;
     ++s->ticks;
     const bool check = s->check_every > 0 && s->ticks % s->check_every == 0;
     for (size_t i = 0; i < s->active.size();) {
@@ -200,9 +221,11 @@ lbm_status lbm_server_tick(lbm_server s, int32_t *active, int32_t *queued) {
       j->done += k;
       bool bad = false;
       if (check && j->done < j->spec.nsteps) {
+        const double t_check = s->now();
         int64_t nf = 0;
         lbm_monitors(j->h, nullptr, nullptr, &nf);
         bad = nf > 0;
+        s->stats.check_s += s->now() - t_check;
       }
       if (j->done >= j->spec.nsteps || bad) {
         s->active.erase(s->active.begin() + (long)i);
@@ -270,6 +293,15 @@ lbm_status lbm_server_set_check_every(lbm_server s, int32_t ticks) {
 }
 
 size_t lbm_server_used_bytes(lbm_server s) { return s ? s->used : 0; }
+
+lbm_status lbm_server_get_stats(lbm_server s, lbm_server_stats *out) {
+  if (!s || !out) return set_error(LBM_EINVAL, "NULL argument");
+  *out = s->stats;
+  const BatchCounters c = batch_counters(s->batch);
+  out->batch_rebuilds = c.rebuilds;
+  out->graph_captures = c.captures;
+  return LBM_OK;
+}
 
 double lbm_server_clock(lbm_server s) { return s ? s->now() : -1.0; }
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "lbm_batch_sync", "lbm_batch_tune", "lbm_batch_size", "lbm_batch_destroy",
     "lbm_server_create", "lbm_server_submit", "lbm_server_tick", "lbm_server_poll", "lbm_server_take",
     "lbm_server_cancel", "lbm_server_set_check_every", "lbm_server_used_bytes", "lbm_server_clock",
-    "lbm_server_destroy",
+    "lbm_server_get_stats", "lbm_server_destroy",
 ]
 
 
@@ -70,6 +70,14 @@ class JobResult(ctypes.Structure):
                 ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double),
                 ("mass", ctypes.c_double), ("kinetic_energy", ctypes.c_double),
                 ("nodes", ctypes.c_int64), ("steps", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_int64), ("steps", ctypes.c_int64), ("admitted", ctypes.c_int64),
+                ("retired", ctypes.c_int64), ("unstable", ctypes.c_int64),
+                ("batch_rebuilds", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
+                ("admit_s", ctypes.c_double), ("retire_s", ctypes.c_double), ("launch_s", ctypes.c_double),
+                ("wait_s", ctypes.c_double), ("check_s", ctypes.c_double)]
 
 
 def make_desc(cfg):
@@ -154,6 +162,7 @@ def load(path=None):
     L.lbm_server_used_bytes.restype = ctypes.c_size_t
     L.lbm_server_clock.argtypes = [H]
     L.lbm_server_clock.restype = ctypes.c_double
+    L.lbm_server_get_stats.argtypes = [H, ctypes.POINTER(ServerStats)]
     L.lbm_server_destroy.argtypes = [H]
     _lib = L
     return L
@@ -383,6 +392,11 @@ class Server:
     @property
     def used_bytes(self):
         return self.lib.lbm_server_used_bytes(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        check(self.lib.lbm_server_get_stats(self.h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
     def clock(self):
         return self.lib.lbm_server_clock(self.h)

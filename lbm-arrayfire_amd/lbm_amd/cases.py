@@ -2,6 +2,8 @@
 
 omega follows the reference's unit convention nu = (tau - 1/2)/3, omega = 1/tau
 (cpp/d2q9_bgk_lid.cpp:45-49)."""
+import numpy as np
+
 from . import geometry as g
 from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q27, F32, INLET_PARABOLIC,
                    INLET_UNIFORM, MEQ_REFERENCE, MRT_AUTO, MRT_GS, MRT_RAW)
@@ -52,13 +54,29 @@ def d2q9_channel_reference(nx=300, ny=100, u_max=0.1, re=220.0, storage=F32):
 
 
 def d2q9_karman(nx=1024, ny=256, cx=256, cy=128, r=20, u_max=0.1, re=150.0, storage=F32):
-    """Channel with a cylinder: Zou-He parabolic inlet, zero-gradient outlet
-    (zero-gradient tangential velocity), no-slip walls.
+    """Channel with a cylinder: Zou-He parabolic inlet, zero-gradient outlet at rho_bc = 1,
+    no-slip walls.
     Re = u_max * D / nu with D = 2r."""
     nu = u_max * 2 * r / re
     cfg = dict(lattice=D2Q9, nx=nx, ny=ny, nz=1, collision=BGK, storage=storage, bc_mode=BC_PHYSICAL,
                omega=omega_from_nu(nu), u_in=u_max, inlet_profile=INLET_PARABOLIC)
     return cfg, g.channel_2d_physical(nx, ny, cx, cy, r)
+
+
+def poiseuille_start(nx, ny, u_max=0.1, seed=0x5EED0002, amp=1e-4):
+    """rho, ux, uy of the developed channel flow between the half-way walls at y = 0.5 and
+    ny - 1.5 (parabola of peak u_max), plus seeded uniform noise of amplitude amp on ux and uy.
+    Starting a channel from this state instead of rest avoids the strong pressure pulse of an
+    impulsive start, whose longitudinal acoustic mode rings between the velocity inlet and the
+    pressure outlet for tens of thousands of steps."""
+    H = ny - 2
+    y = np.arange(ny) - 0.5
+    prof = 4.0 * u_max * y * (H - y) / H ** 2
+    prof[0] = prof[-1] = 0.0
+    rng = np.random.default_rng(seed)
+    ux = np.repeat(prof[:, None], nx, 1) + amp * (2 * rng.random((ny, nx)) - 1)
+    uy = amp * (2 * rng.random((ny, nx)) - 1)
+    return np.ones(nx * ny), ux.reshape(-1), uy.reshape(-1)
 
 
 def d3q27_mrt_lid_reference(n=64, u_lid=0.1, storage=F32, engine=MRT_AUTO):

@@ -86,24 +86,35 @@ def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True):
             if active == 0 and queued == 0 and k < len(jobs):
                 time.sleep(max(0.0, min(0.002, jobs[k]["arrival_s"] - (time.perf_counter() - t0))))
         wall = time.perf_counter() - t0
+        stats = srv.stats()
     # latency from the job's arrival (realtime) or its submission (everything at t = 0)
     lat = np.array([r["finish_s"] - (jobs[ids[r["job_id"]]]["arrival_s"] if realtime else r["submit_s"])
                     for r in res])
     lups = sum(r["nodes"] * r["steps"] for r in res)
     bad = sum(1 for r in res if r["status"] != 0)
     return dict(wall_s=wall, jobs=len(res), failed=bad, p50_s=float(np.percentile(lat, 50)),
-                p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6)
+                p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6,
+                server=stats)
 
 
 def sequential(jobs):
     """The plain solver, one job at a time in arrival order (no idle time between jobs)."""
     t0 = time.perf_counter()
     lups = 0
+    t_setup = t_run = t_end = 0.0
     for j in jobs:
+        a = time.perf_counter()
         with Lattice(j["cfg"], j["types"]) as lat:
             lat.init_equilibrium(1.0, j["u0"], j["seed"], 1e-4)
+            b = time.perf_counter()
             lat.step(j["steps"])
             lat.monitors()
+            c = time.perf_counter()
+        d = time.perf_counter()
+        t_setup += b - a
+        t_run += c - b
+        t_end += d - c
         lups += j["nodes"] * j["steps"]
     wall = time.perf_counter() - t0
-    return dict(wall_s=wall, jobs=len(jobs), jobs_per_s=len(jobs) / wall, mlups=lups / wall / 1e6)
+    return dict(wall_s=wall, jobs=len(jobs), jobs_per_s=len(jobs) / wall, mlups=lups / wall / 1e6,
+                setup_s=t_setup, run_s=t_run, release_s=t_end)

@@ -21,7 +21,7 @@
 //                       rust/examples/d2q9_bgk_lid.rs:97); f32_index_compat reproduces it.
 //  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
 //                       (non-equilibrium bounce-back) velocity inlet, zero-gradient outlet at a
-//                       reference density (extrapolated velocity and non-equilibrium part).
+//                       reference density (non-equilibrium extrapolation of every population).
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
 //
@@ -456,9 +456,11 @@ struct Sim {
       }
       const T rho = (s0 + (T)2 * sm) / ((T)1 - uin);
       nebb(f, 1, rho, uin, (T)0, (T)0);
-    } else if (t & OUTLET) {  // zero-gradient outlet at rho_bc, unknowns c_x = -1
-      // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
-      // post-collision populations (zero normal gradient of u and of the non-equilibrium part)
+    } else if (t & OUTLET) {
+      // Zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi, Chinese Physics
+      // 11 (2002) 366): every population of the node is f_eq(rho_bc, u_nb) + (g - f_eq(rho_nb,
+      // u_nb)), g = the upstream neighbour's populations as the pull reads them (its
+      // post-collision state of the previous step).  The node then holds rho_bc and u_nb exactly.
       T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
       const bool nb = x > 0 && !(type[n - 1] & SOLID);
       if (nb) {
@@ -466,11 +468,10 @@ struct Sim {
         moments(g, rn, vx, vy, vz);
       }
       const T usq = usq_term(vx, vy, vz);
-      for (int i = 0; i < L.Q; ++i)
-        if (L.cx[i] == -1) {
-          const T fe = feq(i, rw, vx, vy, vz, usq);
-          f[i] = nb ? fe + (g[i] - feq(i, rn, vx, vy, vz, usq)) : fe;
-        }
+      for (int i = 0; i < L.Q; ++i) {
+        const T fe = feq(i, rw, vx, vy, vz, usq);
+        f[i] = nb ? fe + (g[i] - feq(i, rn, vx, vy, vz, usq)) : fe;
+      }
     }
   }
 

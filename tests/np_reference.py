@@ -133,7 +133,7 @@ def step_reference(F, types, cfg):
 def step_physical(F, types, cfg):
     """One PHYSICAL-mode step: half-way / moving-wall bounce-back, velocity inlet closed by
     non-equilibrium bounce-back with transverse correction, zero-gradient outlet at rho_bc
-    (unknowns = f_eq(rho_bc, u_nb) + non-equilibrium part of the upstream neighbour)."""
+    (every population = f_eq(rho_bc, u_nb) + non-equilibrium part of the upstream neighbour)."""
     Q = cfg["lattice"]
     c, w, opp = lattice(Q)
     solid = (types & SOLID) != 0
@@ -177,9 +177,8 @@ def step_physical(F, types, cfg):
         fe = feq(np.full(types.shape, rw), um, c, w)
         fn = feq(rm, um, c, w)
         for i in range(Q):
-            if c[i][0] == -1:
-                new = np.where(nb_ok, fe[i] + (Fm[i] - fn[i]), fe[i])
-                f[i] = np.where(outlet, new, f[i])
+            new = np.where(nb_ok, fe[i] + (Fm[i] - fn[i]), fe[i])
+            f[i] = np.where(outlet, new, f[i])
     rho = f.sum(0)
     u = np.einsum("ia,i...->a...", c.astype(float), f) / np.where(solid, 1.0, rho)
     out = collide(f, rho, u, cfg, c, w)

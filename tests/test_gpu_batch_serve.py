@@ -96,6 +96,44 @@ def test_batch_membership_changes():
         lat.close()
 
 
+def test_batch_grows_past_its_table_after_capture():
+    """A group whose step graph was captured with room for 8 members grows to 17 and shrinks
+    again: every member still matches stepping alone."""
+    def lid(n):
+        cfg, t = cases.d2q9_lid_reference(n, re=30.0, storage=F32)
+        return cfg, t
+
+    sizes = [16 + 2 * k for k in range(17)]
+    first, late = sizes[:3], sizes[3:]
+    # steps each member takes: first three 6 + 7 + 5, late ones 7 + 5, the last four late ones 7
+    want = {n: (18 if n in first else (7 if n in late[-4:] else 12)) for n in sizes}
+    alone = {}
+    for n in sizes:
+        cfg, t = lid(n)
+        with Lattice(cfg, t) as lat:
+            lat.init_reference(1.0)
+            lat.step(want[n])
+            alone[n] = lat.state()
+    lats = {}
+    for n in sizes:
+        cfg, t = lid(n)
+        lats[n] = Lattice(cfg, t)
+        lats[n].init_reference(1.0)
+    with Batch([lats[n] for n in first]) as b:
+        b.step(6)  # captures the two-step graph
+        for n in late:
+            b.add(lats[n])
+        b.step(7)
+        for n in late[-4:]:
+            b.remove(lats[n])
+        b.step(5)
+        got = {n: lats[n].state() for n in sizes}
+    for n in sizes:
+        assert lats[n].steps_done == want[n]
+        assert np.array_equal(got[n], alone[n]), f"member {n} differs"
+        lats[n].close()
+
+
 def job_list():
     jobs = []
     rng = np.random.default_rng(5)

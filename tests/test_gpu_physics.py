@@ -3,10 +3,11 @@ stated against the FP64 run.
 
 * Lid-driven cavity, Re = 100: u/U on the vertical centre line against Ghia, Ghia & Shin,
   J. Comput. Phys. 48 (1982) 387, Table I.
-* Channel with a cylinder (1024 x 256, r = 20, Re = 150, parabolic inlet u_max = 0.1): the
-  vortex-shedding Strouhal number St = f D / u_max from a transverse-velocity sensor in the wake;
-  FP32 and FP16-shifted storage agree with FP64, and St lies in the range reported for confined
-  cylinders at this Reynolds number (loose bound).
+* Channel with a cylinder (1024 x 256, r = 20, Re = 150, parabolic inlet u_max = 0.1, started
+  from the developed channel profile): the vortex-shedding Strouhal number St = f D / u_max from
+  a transverse-velocity sensor in the wake; FP32 and FP16-shifted storage agree with FP64 within
+  1e-2, BF16-shifted within 5e-2, and St lies in the range reported for confined cylinders at
+  this Reynolds number (loose bound).  The shedding period must be regular (jitter < 5 %).
 """
 import numpy as np
 import pytest
@@ -51,7 +52,9 @@ def strouhal(storage, steps=60000, sample_every=20, start=30000):
     probe = np.array([600 + 1024 * 128, 700 + 1024 * 140], dtype=np.int64)
     series = []
     with Lattice(cfg, types) as lat:
-        lat.init_equilibrium(1.0, (0.0, 0.0, 0.0), 0x5EED0002, 1e-4)
+        # developed channel flow + 1e-4 seeded noise: an impulsive start from rest leaves an
+        # acoustic mode ringing between inlet and outlet that modulates the shedding period
+        lat.init_fields(*cases.poiseuille_start(1024, 256, 0.1, 0x5EED0002, 1e-4))
         lat.step(start)
         for _ in range((steps - start) // sample_every):
             lat.step(sample_every)
@@ -71,6 +74,7 @@ def test_cylinder_strouhal_across_precisions():
     st64, jit = strouhal(F64)
     print(f"Strouhal FP64 {st64:.4f} (period jitter {jit:.3f})")
     assert 0.17 <= st64 <= 0.26
+    assert jit < 0.05
     for storage, tol in ((F32, 0.01), (F16S, 0.01), (BF16S, 0.05)):
         st, _ = strouhal(storage)
         print(f"Strouhal storage {storage}: {st:.4f}, relative error vs FP64 {abs(st - st64) / st64:.2e}")

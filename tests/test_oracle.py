@@ -340,8 +340,8 @@ def test_zou_he_inlet_recovers_prescribed_velocity():
 def test_poiseuille_profile_develops():
     """Empty channel, parabolic velocity inlet, zero-gradient outlet at rho_bc, half-way walls:
     the flow becomes steady with a parabolic cross-section (walls at y = 0.5 and ny - 1.5), the
-    mass flux sum_y rho u_x is the same at every section, and the density falls to about rho_bc
-    at the outlet."""
+    mass flux sum_y rho u_x is the same at every section, and the outlet node holds rho_bc and
+    the velocity of its upstream neighbour."""
     nx, ny = 60, 12
     cfg = dict(lattice=9, nx=nx, ny=ny, nz=1, collision=BGK, storage=F64, bc_mode=BC_PHYSICAL,
                omega=1.0, u_in=0.04, inlet_profile=1)
@@ -360,8 +360,9 @@ def test_poiseuille_profile_develops():
     assert np.abs(flux[1:nx // 2] / flux[1] - 1).max() < 1e-4
     assert np.abs(flux[1:] / flux[1] - 1).max() < 5e-3  # odd-even ripple next to the outlet
     assert abs(flux[1] / inflow - 1) < 2e-2
-    # the outlet pins the unknown populations to rho_bc = 1; the known ones carry the rest
-    assert np.abs(rho[1:-1, nx - 1] - 1.0).max() < 1e-2
+    # the outlet node holds rho_bc and its upstream neighbour's velocity
+    assert np.abs(rho[1:-1, nx - 1] - 1.0).max() < 1e-14
+    assert np.abs(ux[1:-1, nx - 1] - ux[1:-1, nx - 2]).max() < 1e-2 * ux.max()
 
 
 def test_f32_index_compat_only_changes_large_lattices():
