@@ -6,10 +6,13 @@ cylinder r = 20 at (256,128), parabolic Zou-He inlet u_max = 0.1, zero-gradient 
 no-slip walls, Re = 150, 1e-4 seeded noise (seed 0x5EED0002).  A "step" is one lattice time
 step over the whole lattice; MLUPS counts every node (solid included), 1e6 lattice updates/s.
 
-Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes of the fused kernel
-(2 * Q * bytes-per-population per node update) over the per-launch device time measured with
-HIP events on the lattice's stream; ``cpu_baseline`` times the CPU oracle (a restatement of the
-reference step, OpenMP) on a bounded sample of the same workload on this host.
+Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes of the dominant kernel
+(2 * Q * bytes-per-population per node update, times the node updates one launch performs: one
+step per launch, or several for the D2Q9 multi-step kernel) over its average launch duration,
+measured with HIP events on the lattice's stream; ``traffic`` is the HBM bytes per launch that
+rocprofv3 counted for the same kernel (profiles/traffic.json).  ``cpu_baseline`` times the CPU
+oracle (a restatement of the reference step, OpenMP) on a bounded sample of the same workload on
+this host.
 """
 import argparse
 import json
@@ -31,6 +34,9 @@ def workloads():
     W = {
         "d2q9_bgk_karman_1024x256_fp32": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
         "d2q9_bgk_karman_1024x256_f16s": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F16S),
+        # the same lattices with one launch per step (the multi-step launches switched off)
+        "d2q9_bgk_karman_1024x256_fp32_1step": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
+        "d2q9_bgk_karman_1024x256_f16s_1step": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F16S),
         "d3q27_mrt_gs_lid_256_fp32": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32),
         "d3q27_mrt_gs_lid_256_f16s": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S),
         "d3q27_mrt_gs_lid_256_fp32_mfma2": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_MFMA_F32),
@@ -48,12 +54,24 @@ def workloads():
 
 
 def measure(name, steps, warmup):
-    import numpy as np  # noqa: F401
     from lbm_amd import Lattice
     from lbm_amd.capi import STORAGE_BYTES
 
     cfg, types = workloads()[name]()
-    with Lattice(cfg, types) as lat:
+    saved = os.environ.get("LBM_MULTISTEP")
+    if name.endswith("_1step"):
+        os.environ["LBM_MULTISTEP"] = "off"
+    try:
+        lat = Lattice(cfg, types)
+    finally:
+        if saved is None:
+            os.environ.pop("LBM_MULTISTEP", None)
+        else:
+            os.environ["LBM_MULTISTEP"] = saved
+    with lat:
+        k = lat.steps_per_launch
+        # whole launches only, so the timed region holds steps // k launches of one kernel
+        steps, warmup = max(k, steps - steps % k), warmup - warmup % k
         u0 = (cfg.get("u_in", 0.0), 0.0, 0.0)
         lat.init_equilibrium(1.0, u0, 0x5EED0002, 1e-4)
         lat.step(warmup)
@@ -64,12 +82,22 @@ def measure(name, steps, warmup):
         _, _, bad = lat.monitors()
     N = lat.N
     Q = cfg["lattice"]
-    bytes_per_launch = 2 * Q * STORAGE_BYTES[cfg["storage"]] * N
-    per_launch_s = ms * 1e-3 / steps
-    achieved = bytes_per_launch / per_launch_s / 1e9
-    return dict(cfg=cfg, N=N, Q=Q, steps=steps, ms=ms, wall_s=wall, mlups=N * steps / (ms * 1e3),
-                ms_per_step=ms / steps, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
+    # algorithmic bytes: every population read and written once per node update, k updates of
+    # every node per launch
+    bytes_per_launch = 2 * Q * STORAGE_BYTES[cfg["storage"]] * N * k
+    launch_s = ms * 1e-3 / (steps // k)
+    achieved = bytes_per_launch / launch_s / 1e9
+    return dict(cfg=cfg, N=N, Q=Q, steps=steps, warmup=warmup, ms=ms, wall_s=wall,
+                mlups=N * steps / (ms * 1e3), ms_per_step=ms / steps, steps_per_launch=k,
+                launch_us=launch_s * 1e6, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
                 nonfinite=int(bad))
+
+
+def kernel_name(cfg, k):
+    """The kernel the timed launches run (as rocprofv3 lists it, template arguments abridged)."""
+    if k > 1:
+        return f"lbm::k_step_multi<storage {cfg['storage']}, bc {cfg.get('bc_mode', 0)}, {k} steps per launch>"
+    return f"lbm::k_step<{cfg['lattice']}, {cfg['storage']}, {cfg.get('bc_mode', 0)}, {cfg.get('collision', 0)}>"
 
 
 def traffic_for(name):
@@ -176,7 +204,7 @@ def main():
         try:
             m = measure(k, args.also_steps, 20)
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
-                       "achieved_gbs": round(m["achieved_gbs"], 1),
+                       "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
                        "frac_hbm": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4), "steps": m["steps"],
                        "nonfinite": m["nonfinite"]}
         except Exception as e:  # noqa: BLE001 — report, keep the headline line
@@ -189,8 +217,8 @@ def main():
         "value": round(r["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": r["steps"],
+        "warmup": r["warmup"],
         "ms_per_step": round(r["ms_per_step"], 6),
         "higher_is_better": True,
         "scaling": "weak",
@@ -206,6 +234,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic_for(args.workload),
+                     "kernel": kernel_name(cfg, r["steps_per_launch"]),
+                     "steps_per_launch": r["steps_per_launch"], "avg_launch_us": round(r["launch_us"], 3),
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"]},
         "cpu_baseline": cpu,
         "wall_s_timed": round(r["wall_s"], 4),

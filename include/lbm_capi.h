@@ -146,8 +146,13 @@ lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], u
 lbm_status lbm_init_reference(lbm_handle h, double rho0);
 
 /* Enqueues nsteps fused stream+collide steps on the handle's stream and returns.  Replaces the
- * loop body cpp/d2q9_bgk_lid.cpp:124-155 / rust/examples/d3q27_mrt_lid.rs:304-389. */
+ * loop body cpp/d2q9_bgk_lid.cpp:124-155 / rust/examples/d3q27_mrt_lid.rs:304-389.  2D D2Q9 BGK
+ * lattices with FP32-arithmetic storage advance several steps per launch from an on-chip tile
+ * (bit-identical to single steps; LBM_MULTISTEP=off in the environment at lbm_create turns it
+ * off, LBM_MULTISTEP=<shape index> picks another launch shape). */
 lbm_status lbm_step(lbm_handle h, int64_t nsteps);
+/* Steps one launch of lbm_step advances (1 = one launch per step).  No reference counterpart. */
+int32_t lbm_steps_per_launch(lbm_handle h);
 /* As lbm_step, then waits; *ms = device time between HIP events recorded on the handle's
  * stream around the launches. */
 lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms);

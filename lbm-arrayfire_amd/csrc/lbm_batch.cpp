@@ -1,17 +1,20 @@
 // lbm_batch.cpp — many lattices stepped together.
 //
 // Members are grouped by kernel configuration (lattice, storage, boundary mode, collision, MRT
-// engine).  One batch step is one persistent launch per group: the group's tiles are laid end to
-// end (member m owns tiles [prefix[m], prefix[m+1])) and each workgroup walks one contiguous chunk
-// (k_step_batch).  Each group keeps one device table {header, prefix, parameter blocks for both
-// buffer orientations} with room for more members than it holds; a membership change rewrites the
-// table with one asynchronous copy from pinned staging memory, on the batch's stream.  The kernels
-// take only the table's address and read everything else from its header, so the hipGraph of two
-// captured steps stays valid across membership changes: it is re-captured only when the set of
-// groups changes, a table is reallocated (grown) or a grid size changes.  The grid size of each group is chosen by timing candidate sizes
-// on the group itself (lbm_batch_tune): a trial launch reads every member's current buffer and
-// writes only its other buffer, which the next real step overwrites, so tuning leaves the members'
-// states untouched.
+// engine).  One batch step is one launch per group over all of the group's tiles: the members'
+// tiles are laid end to end (member m owns tiles [prefix[m], prefix[m+1])) and each workgroup
+// walks one contiguous chunk of them (k_step_batch); by default the chunk is one tile and the
+// hardware deals workgroups out as compute units free up, and lbm_batch_tune may pick a
+// persistent grid of fewer, longer chunks instead, by timing candidate sizes on the group itself
+// (a trial launch reads every member's current buffer and writes only its other buffer, which
+// the next real step overwrites, so tuning leaves the members' states untouched).
+//
+// Each group keeps one device table {header, prefix, parameter blocks for both buffer
+// orientations} with room for more members than it holds; a membership change rewrites the table
+// with one asynchronous copy from pinned staging memory, on the batch's stream.  The kernels take
+// only the table's address and read everything else from its header.  Two steps are captured in a
+// hipGraph and replayed; it is re-captured when the set of groups changes, a table is
+// reallocated (grown) or a grid size changes.
 //
 // While a lattice is a member all of its work goes to the batch's stream, which keeps member
 // reads, writes and batch steps in one order.  Joining and leaving are ordered against the
@@ -44,7 +47,6 @@ struct Group {
   int32_t total_tiles = 0;
   int grid = 0;
   int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
-  int resident = 0;            // workgroups of this group's kernel that fit on one CU
 
   BatchArgs args(int phase) const { return BatchArgs{(const BatchTable *)d_tab, phase}; }
   void release() {
@@ -137,7 +139,6 @@ lbm_status rebuild(lbm_batch_ctx *b) {
       ++gi;
     }
   }
-  const int cus = device_cu_count(b->device);
   for (auto &g : b->groups) {
     const int n = (int)g.members.size();
     bool grown = false;
@@ -165,10 +166,10 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     // the whole table (unused rows too) keeps the copy one contiguous transfer
     BCHK(hipMemcpyAsync(g.d_tab, g.h_tab, g.bytes, hipMemcpyHostToDevice, b->stream));
     BCHK(hipEventRecord(g.uploaded, b->stream));
-    if (g.resident == 0) BCHK(batch_resident_per_cu(g.cfg, &g.resident));
-    // default: one resident wave of workgroups; more would run the excess chunks as a second
-    // partial wave and stretch the launch
-    const int grid = g.tuned > 0 ? g.tuned : g.resident * cus;
+    // default: one workgroup per tile, dispatched by the hardware as they free up (measured
+    // faster than statically chunked persistent grids, whose chunks differ in cost: partial
+    // tiles, wall nodes, different members); the tuner may pick a persistent grid instead
+    const int grid = g.tuned > 0 ? std::min(g.tuned, std::max(g.total_tiles, 1)) : std::max(g.total_tiles, 1);
     if (grid != g.grid) recapture = true;
     g.grid = grid;
   }

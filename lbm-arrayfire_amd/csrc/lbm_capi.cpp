@@ -139,6 +139,18 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   p.rho_bc = d->rho_bc;
   for (int k = 0; k < 27; ++k) p.s[k] = d->s[k];
 
+  // Multi-step launches for the lattices that qualify: shape 0 unless LBM_MULTISTEP names another
+  // shape index, or "off" (single steps only).
+  if (d->nz == 1) {
+    int v = 0;
+    if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
+    const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
+    if (k > 1) {
+      h->multi_variant = v;
+      h->multi_k = k;
+    }
+  }
+
   auto cleanup = [&](lbm_status s, const std::string &m) {
     lbm_destroy(h);
     return fail(s, m);
@@ -275,6 +287,12 @@ static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
     h->graph_cur = h->cur;
   }
   int64_t left = nsteps;
+  if (h->multi_k > 1)
+    for (; left >= h->multi_k; left -= h->multi_k) {
+      HIPCHK(launch_step_multi(h->cfg, h->multi_variant, h->params(h->cur), h->strm()));
+      h->cur ^= 1;
+      h->steps += h->multi_k;
+    }
   if (h->graph && left >= 2 && h->cur != h->graph_cur) {
     lbm_status st = direct();
     if (st != LBM_OK) return st;
@@ -324,6 +342,8 @@ lbm_status lbm_sync(lbm_handle h) {
 }
 
 int64_t lbm_steps_done(lbm_handle h) { return h ? h->steps : -1; }
+
+int32_t lbm_steps_per_launch(lbm_handle h) { return h ? (h->multi_k > 1 ? h->multi_k : 1) : -1; }
 
 lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, double *uz) {
   if (!h) return fail(LBM_EINVAL, "handle is NULL");

@@ -210,37 +210,29 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
   });
 }
 
-// PHYSICAL-mode pull: half-way bounce-back on links whose source is solid (plus the moving-wall
-// momentum term 6 w_i rho_w c_i.u_w); inlet nodes close their c_x = +1 unknowns for the
-// prescribed velocity; outlet nodes take f_eq(rho_bc, u_nb) plus the non-equilibrium part of the
-// upstream neighbour in every direction (zero normal gradient of u and f_neq).
-template <int Q, int ST>
-__device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos &q, uint8_t t,
-                                             typename Codec<ST, Q>::C *f) {
-  using CD = Codec<ST, Q>;
-  using T = typename CD::T;
-  using C = typename CD::C;
-  // The periodic pull is issued by every node before its type is known; nodes flagged SLOW then
-  // replace the links whose source is solid and close their inlet/outlet unknowns.
-  pull_periodic<Q, ST>(p, q, f);
-  if (!(t & NODE_SLOW)) return;
-  const T *fin = (const T *)p.fin;
+// PHYSICAL-mode boundary rules applied to the periodically pulled populations f of a SLOW node
+// at (x, y, z) of type t: half-way bounce-back on links whose source is solid (plus the
+// moving-wall momentum term 6 w_i rho_w c_i.u_w); inlet nodes close their c_x = +1 unknowns for
+// the prescribed velocity; outlet nodes take f_eq(rho_bc, u_nb) plus the non-equilibrium part of
+// the upstream neighbour in every direction (zero normal gradient of u and f_neq).  The state is
+// read through accessors, so the single-step kernels (global memory) and the multi-step kernel
+// (an LDS tile) share one statement of the rules:
+//   src_type(IC<i>)  type of the node this node pulls population i from (node - c_i)
+//   own(IC<j>)       population j of this node in the state being advanced
+//   west(IC<j>)      population j of the node at x - 1;  west_type()  its type
+template <int Q, class C, class SrcType, class Own, class West, class WestType>
+__device__ __forceinline__ void physical_fixups(const StepParams &p, int x, int y, int z, uint8_t t, C *f,
+                                                SrcType &&src_type, Own &&own, West &&west,
+                                                WestType &&west_type) {
   const C rw = (C)p.rho_bc;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
     constexpr int io = Lat<Q>::opp[i];
     if constexpr (i != 0) {
-      int sx = q.x - cx;
-      sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
-      int sy = q.y - cy;
-      sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
-      int sz = q.z - cz;
-      sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
-      const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
-      const uint8_t ts = p.type[src];
+      const uint8_t ts = src_type(I);
       if (ts & NODE_SOLID) {
-        C v = CD::template ld<io>(fin + (size_t)io * (size_t)p.stride, q.n);
+        C v = own(IC<io>{});
         if (ts & NODE_MOVING) {
           C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
           if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
@@ -251,7 +243,7 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
     }
   });
   if (t & NODE_INLET) {
-    const C uin = inlet_velocity<C>(p, Lat<Q>::D, q.y, q.z);
+    const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = 0, sm = 0;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -265,12 +257,9 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
     // post-collision populations; the node then holds rho_bc and u_nb exactly
     C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
-    const bool nb = q.x > 0 && !(p.type[q.n - 1] & NODE_SOLID);
+    const bool nb = x > 0 && !(west_type() & NODE_SOLID);
     if (nb) {
-      static_for<Q>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        g[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n - 1);
-      });
+      static_for<Q>([&](auto I) { g[decltype(I)::value] = west(I); });
       moments<Q>(g, rn, vx, vy, vz);
     }
     const C usq = usq_of<Q>(vx, vy, vz);
@@ -280,6 +269,40 @@ __device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos
       f[i] = nb ? fe + (g[i] - feq_i<Q, i>(rn, vx, vy, vz, usq)) : fe;
     });
   }
+}
+
+// PHYSICAL-mode pull from global memory.  The periodic pull is issued by every node before its
+// type is known; nodes flagged SLOW then apply the boundary rules.
+template <int Q, int ST>
+__device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos &q, uint8_t t,
+                                             typename Codec<ST, Q>::C *f) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  pull_periodic<Q, ST>(p, q, f);
+  if (!(t & NODE_SLOW)) return;
+  const T *fin = (const T *)p.fin;
+  physical_fixups<Q>(
+      p, q.x, q.y, q.z, t, f,
+      [&](auto I) -> uint8_t {
+        constexpr int i = decltype(I)::value;
+        constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+        int sx = q.x - cx;
+        sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
+        int sy = q.y - cy;
+        sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
+        int sz = q.z - cz;
+        sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
+        return p.type[(uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz)];
+      },
+      [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n);
+      },
+      [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n - 1);
+      },
+      [&]() -> uint8_t { return p.type[q.n - 1]; });
 }
 
 // ---- moments, overrides, collisions ------------------------------------------------------

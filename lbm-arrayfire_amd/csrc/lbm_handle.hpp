@@ -36,6 +36,8 @@ struct lbm_ctx {
   hipGraphExec_t graph = nullptr;     // two captured steps (cur -> other -> cur)
   int graph_cur = -1;                 // buffer index the graph starts from
   bool use_graph = false;             // direct launches measure faster for single lattices
+  int multi_variant = -1;             // multi-step launch shape (D2Q9 BGK 2D), -1 = single steps
+  int multi_k = 0;                    // steps per multi-step launch
   lbm::Config cfg{};
   lbm::StepParams base{};
 

@@ -72,10 +72,12 @@ struct BatchArgs {
 hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
-// Workgroups of the batched step kernel of c that fit on one compute unit at once.
-hipError_t batch_resident_per_cu(const Config &c, int *per_cu);
-hipError_t batch_resident_scalar(const Config &c, int *per_cu);
-hipError_t batch_resident_mrt_mfma(const Config &c, int *per_cu);
+// Several D2Q9 BGK steps per launch from an LDS tile (2D lattices, FP32-arithmetic storages).
+// multi_steps_per_launch = the steps one launch of the given shape advances, 0 if c or the shape
+// does not qualify; the launch reads p.fin and leaves the state K steps later in p.fout.
+int multi_variant_count();
+int multi_steps_per_launch(const Config &c, int variant);
+hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, hipStream_t s);
 // True when the step of this configuration runs on the MFMA MRT kernel.
 // Engine AUTO resolves to MRT_ENGINE_AUTO_FP32 for FP32-arithmetic storages (the sparse integer
 // scalar transform: faster on gfx950 than either MFMA engine), scalar for FP64.

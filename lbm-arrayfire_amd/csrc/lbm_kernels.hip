@@ -339,16 +339,6 @@ hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int gri
   });
 }
 
-hipError_t batch_resident_scalar(const Config &c, int *per_cu) {
-  return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        per_cu,
-        reinterpret_cast<const void *>(&k_step_batch<decltype(Qc)::value, decltype(STc)::value,
-                                                     decltype(BCc)::value, decltype(COc)::value>),
-        BLOCK_THREADS, 0);
-  });
-}
-
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s) {
   return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
@@ -405,12 +395,5 @@ hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s) {
 hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
   if (uses_mfma(c)) return launch_step_batch_mrt_mfma(c, b, grid, s);
   return launch_step_batch_scalar(c, b, grid, s);
-}
-
-hipError_t batch_resident_per_cu(const Config &c, int *per_cu) {
-  *per_cu = 0;
-  hipError_t e = uses_mfma(c) ? batch_resident_mrt_mfma(c, per_cu) : batch_resident_scalar(c, per_cu);
-  if (e == hipSuccess && *per_cu < 1) *per_cu = 1;
-  return e;
 }
 }  // namespace lbm

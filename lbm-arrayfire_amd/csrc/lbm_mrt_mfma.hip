@@ -198,16 +198,6 @@ hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int g
   });
 }
 
-hipError_t batch_resident_mrt_mfma(const Config &c, int *per_cu) {
-  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        per_cu,
-        reinterpret_cast<const void *>(
-            &k_step_batch_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),
-        BLOCK_THREADS, 0);
-  });
-}
-
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s) {
   return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
     hipLaunchKernelGGL((k_step_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),

@@ -1,0 +1,210 @@
+// lbm_multistep.hip — K D2Q9 BGK time steps per launch from an LDS tile (temporal blocking).
+//
+// A workgroup owns an interior tile of TX x TY nodes.  It loads the tile plus a halo of K nodes
+// on every side (periodic wrap, so every region position holds the state of one lattice node),
+// then advances the region K times in LDS: sub-step s recomputes the positions at least s away
+// from the region's edge, whose pulls (radius 1, the outlet's upstream neighbour included) all lie
+// in the window of sub-step s - 1.  After K sub-steps the interior is exact and is written out.
+// Each sub-step runs the per-node rules of the single-step kernel (the same pulls, boundary
+// fix-ups, moments and collision, on the same values), and every intermediate population is
+// rounded through the storage codec exactly as a store and a load would round it, so K sub-steps
+// reproduce K single steps bit for bit.
+//
+// HBM traffic per launch: the region once in (halo factor (TX + 2K)(TY + 2K) / (TX TY)) and the
+// interior once out, for K steps, against 2 Q s bytes per node for each single step.  The D2Q9
+// 1024 x 256 lattice (18.9 MB per step) is latency-bound as single steps; this kernel trades the
+// per-step launch and memory round trip for LDS traffic and redundant halo updates.
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+template <int ST, int TX_, int K, int TY, int NT, bool DB>
+struct MultiShape {
+  static constexpr int Q = 9, TX = TX_;
+  static constexpr int RX = TX + 2 * K, RY = TY + 2 * K, RN = RX * RY;
+  static constexpr int NPOS = (RN + NT - 1) / NT;  // region positions per thread
+  static constexpr int NBUF = DB ? 2 : 1;         // LDS copies of the region
+};
+
+// DB: two LDS copies of the region, sub-step s reads copy (s - 1) & 1 and writes copy s & 1 (one
+// barrier per sub-step); otherwise one copy, results staged in registers between two barriers.
+template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
+__global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int ntm) {
+  using S = MultiShape<ST, TX, K, TY, NT, DB>;
+  constexpr int Q = S::Q, RX = S::RX, RY = S::RY, RN = S::RN, NPOS = S::NPOS;
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  using C = typename CD::C;
+  __shared__ C lf[S::NBUF * Q * RN];
+  __shared__ uint8_t lt[RN];
+
+  const int tile = xcd_tile((int)blockIdx.x, ntm);
+  const int bx = tile % gxm, by = tile / gxm;
+  const int x0 = bx * S::TX - K, y0 = by * TY - K;  // lattice coordinates of region position (0, 0)
+  const T *fin = (const T *)p.fin;
+
+  // this thread's region positions and the lattice nodes they hold (periodic)
+  int ri[NPOS], rj[NPOS], gx[NPOS], gy[NPOS];
+#pragma unroll
+  for (int k = 0; k < NPOS; ++k) {
+    const int pos = (int)threadIdx.x + k * NT;
+    ri[k] = pos % RX;
+    rj[k] = pos / RX;
+    int x = (x0 + ri[k]) % p.nx, y = (y0 + rj[k]) % p.ny;
+    gx[k] = x < 0 ? x + p.nx : x;
+    gy[k] = y < 0 ? y + p.ny : y;
+    if (pos < RN) {
+      const uint32_t n = (uint32_t)gx[k] + (uint32_t)p.nx * (uint32_t)gy[k];
+      static_for<Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        lf[q * RN + pos] = CD::template ld<q>(fin + (size_t)q * (size_t)p.stride, n);
+      });
+      lt[pos] = p.type[n];
+    }
+  }
+  __syncthreads();
+
+  // One node update of region position pos from the LDS copy src into out; false when the node
+  // is not updated (PHYSICAL walls keep their populations).
+  auto update = [&](const C *src, int pos, int x, int y, C *out) -> bool {
+    const uint8_t t = lt[pos];
+    C f[Q], rho, ux, uy, uz;
+    static_for<Q>([&](auto I) {
+      constexpr int q = decltype(I)::value;
+      f[q] = src[q * RN + pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
+    });
+    if constexpr (BC == BC_REF) {
+      moments<Q>(f, rho, ux, uy, uz);
+      overrides<Q>(p, t, y, 0, rho, ux, uy, uz);
+      collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
+      if (t & NODE_SOLID) {  // full-way bounce-back: out[opp(j)] = f[j], j >= 1
+        static_for<Q - 1>([&](auto J) {
+          constexpr int q = decltype(J)::value + 1;
+          out[Lat<Q>::opp[q]] = f[q];
+        });
+      }
+      return true;
+    } else {
+      if (t & NODE_SOLID) return false;
+      if (t & NODE_SLOW)
+        physical_fixups<Q>(
+            p, x, y, 0, t, f,
+            [&](auto I) -> uint8_t {
+              constexpr int q = decltype(I)::value;
+              return lt[pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
+            },
+            [&](auto J) -> C { return src[decltype(J)::value * RN + pos]; },
+            [&](auto J) -> C { return src[decltype(J)::value * RN + pos - 1]; },
+            [&]() -> uint8_t { return lt[pos - 1]; });
+      moments<Q>(f, rho, ux, uy, uz);
+      collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
+      return true;
+    }
+  };
+  // the rounding a store and a reload apply
+  auto round_trip = [&](auto I, C v) -> C {
+    constexpr int q = decltype(I)::value;
+    T w;
+    CD::template st<q>(&w, 0, v);
+    return CD::template ld<q>(&w, 0);
+  };
+
+  T *fout = (T *)p.fout;
+  C res[DB ? 1 : NPOS][Q];
+  bool upd[NPOS];
+#pragma unroll
+  for (int s = 1; s <= K; ++s) {
+    const C *src = lf + (DB ? ((s - 1) & 1) * Q * RN : 0);
+    C *dst = lf + (DB ? (s & 1) * Q * RN : 0);
+#pragma unroll
+    for (int k = 0; k < NPOS; ++k) {
+      const int pos = (int)threadIdx.x + k * NT;
+      upd[k] = pos < RN && ri[k] >= s && ri[k] < RX - s && rj[k] >= s && rj[k] < RY - s;
+      if (!upd[k]) continue;
+      C *out = res[DB ? 0 : k];
+      upd[k] = update(src, pos, gx[k], gy[k], out);
+      if (!upd[k]) continue;
+      if (s == K) {  // sub-step K updates exactly the interior: write the part inside the lattice
+        const int x = x0 + ri[k], y = y0 + rj[k];
+        if (x < p.nx && y < p.ny) {
+          const uint32_t n = (uint32_t)x + (uint32_t)p.nx * (uint32_t)y;
+          static_for<Q>([&](auto I) {
+            constexpr int q = decltype(I)::value;
+            CD::template st<q>(fout + (size_t)q * (size_t)p.stride, n, out[q]);
+          });
+        }
+      } else if constexpr (DB) {
+        static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
+      }
+    }
+    if (s == K) break;
+    __syncthreads();  // DB: the copy written is complete; otherwise: every read of sub-step s is done
+    if constexpr (!DB) {
+#pragma unroll
+      for (int k = 0; k < NPOS; ++k) {
+        if (!upd[k]) continue;
+        const int pos = (int)threadIdx.x + k * NT;
+        static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, res[k][decltype(I)::value]); });
+      }
+      __syncthreads();
+    }
+  }
+}
+
+namespace {
+
+// Launch shapes: interior TX x TY, steps per launch K, threads NT, double-buffered LDS.
+struct MultiVariant {
+  int TX, K, TY, NT;
+  bool DB;
+};
+#define LBM_MULTI_SHAPES(X)                                                                         \
+  X(0, 64, 4, 16, 1024, true) X(1, 64, 5, 16, 1024, true) X(2, 64, 3, 16, 1024, true)              \
+  X(3, 64, 4, 12, 1024, true) X(4, 32, 4, 16, 512, true) X(5, 32, 4, 32, 1024, true)               \
+  X(6, 64, 4, 16, 1024, false) X(7, 64, 6, 16, 1024, true) X(8, 32, 6, 32, 1024, true)
+#define LBM_MULTI_ENTRY(i, tx, k, ty, nt, db) {tx, k, ty, nt, db},
+constexpr MultiVariant kMultiVariants[] = {LBM_MULTI_SHAPES(LBM_MULTI_ENTRY)};
+
+template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
+hipError_t launch_multi_shape(const StepParams &p, hipStream_t s) {
+  const int gxm = (p.nx + TX - 1) / TX, gym = (p.ny + TY - 1) / TY;
+  hipLaunchKernelGGL((k_step_multi<ST, BC, TX, K, TY, NT, DB>), dim3(gxm * gym), dim3(NT), 0, s, p, gxm,
+                     gxm * gym);
+  return hipGetLastError();
+}
+
+template <int ST, int BC>
+hipError_t launch_multi_variant(int v, const StepParams &p, hipStream_t s) {
+#define LBM_MULTI_CASE(i, tx, k, ty, nt, db) \
+  case i: return launch_multi_shape<ST, BC, tx, k, ty, nt, db>(p, s);
+  switch (v) { LBM_MULTI_SHAPES(LBM_MULTI_CASE) }
+#undef LBM_MULTI_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int multi_variant_count() { return (int)(sizeof(kMultiVariants) / sizeof(kMultiVariants[0])); }
+
+int multi_steps_per_launch(const Config &c, int variant) {
+  if (c.Q != 9 || c.collision != CO_BGK || c.storage == ST_F64) return 0;
+  if (variant < 0 || variant >= multi_variant_count()) return 0;
+  return kMultiVariants[variant].K;
+}
+
+hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, hipStream_t s) {
+  if (multi_steps_per_launch(c, variant) == 0 || p.nz != 1) return hipErrorInvalidValue;
+  auto byBC = [&](auto STc) -> hipError_t {
+    constexpr int ST = decltype(STc)::value;
+    return c.bc == BC_REF ? launch_multi_variant<ST, BC_REF>(variant, p, s)
+                          : launch_multi_variant<ST, BC_PHYS>(variant, p, s);
+  };
+  switch (c.storage) {
+    case ST_F32: return byBC(IC<ST_F32>{});
+    case ST_F16S: return byBC(IC<ST_F16S>{});
+    case ST_BF16S: return byBC(IC<ST_BF16S>{});
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace lbm

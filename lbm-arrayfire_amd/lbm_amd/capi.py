@@ -30,7 +30,8 @@ STORAGE_BYTES = {F64: 8, F32: 4, F16S: 2, BF16S: 2}
 EXPORTS = [
     "lbm_abi_version", "lbm_set_device", "lbm_last_error", "lbm_footprint_bytes", "lbm_create",
     "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
-    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_get_fields", "lbm_get_fields_at",
+    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_steps_per_launch", "lbm_get_fields",
+    "lbm_get_fields_at",
     "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
     "lbm_batch_create", "lbm_batch_add", "lbm_batch_remove", "lbm_batch_step", "lbm_batch_step_timed",
@@ -132,6 +133,8 @@ def load(path=None):
     L.lbm_sync.argtypes = [H]
     L.lbm_steps_done.argtypes = [H]
     L.lbm_steps_done.restype = ctypes.c_int64
+    L.lbm_steps_per_launch.argtypes = [H]
+    L.lbm_steps_per_launch.restype = ctypes.c_int32
     L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
     L.lbm_get_fields_at.argtypes = [H, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, dp, dp, dp, dp]
     L.lbm_get_populations.argtypes = [H, dp]
@@ -243,6 +246,10 @@ class Lattice:
     @property
     def steps_done(self):
         return self.lib.lbm_steps_done(self.h)
+
+    @property
+    def steps_per_launch(self):
+        return self.lib.lbm_steps_per_launch(self.h)
 
     def fields(self):
         out = [np.zeros(self.N) for _ in range(4)]
