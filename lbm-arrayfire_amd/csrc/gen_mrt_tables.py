@@ -7,6 +7,12 @@
 * MGS  — the same rows Gram-Schmidt orthogonalised in row order (unweighted inner product over
   the 27 velocities), each row scaled to primitive integers.  MGS_NORM2[k] = |row k|^2, so
   MGS^-1 = MGS^T diag(1 / MGS_NORM2).
+* T_GS, TINV_GS — the Gram-Schmidt rows in terms of the raw ones: MGS = T_GS MRAW (T_GS is
+  integral and lower triangular), TINV_GS = T_GS^-1, so MGS^-1 = MRAW^-1 TINV_GS.
+* MEQ_RAW, MEQ_GS — the moments of the second-order equilibrium in closed form:
+  (M f_eq)_k = rho * sum_j MEQ[k][j] phi_j with phi = (1, ux, uy, uz, ux^2, uy^2, uz^2, ux uy,
+  ux uz, uy uz), from f_eq,i = w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u); exact rationals
+  printed as doubles.
 
 Exact rational arithmetic (fractions.Fraction); run from this directory:
     python3 gen_mrt_tables.py > mrt_tables.h
@@ -83,6 +89,37 @@ def primitive(row):
     return ints
 
 
+W = {0: Fraction(8, 27), 1: Fraction(2, 27), 2: Fraction(1, 54), 3: Fraction(1, 216)}
+
+
+def meq_coefficients(M):
+    """MEQ[k][j]: coefficient of phi_j in (M f_eq)_k / rho."""
+    out = []
+    for row in M:
+        co = [Fraction(0)] * 10
+        for i in range(27):
+            c = (CX[i], CY[i], CZ[i])
+            w = W[abs(c[0]) + abs(c[1]) + abs(c[2])] * row[i]
+            co[0] += w
+            for a in range(3):
+                co[1 + a] += 3 * w * c[a]
+                co[4 + a] += w * (Fraction(9, 2) * c[a] * c[a] - Fraction(3, 2))
+            co[7] += 9 * w * c[0] * c[1]
+            co[8] += 9 * w * c[0] * c[2]
+            co[9] += 9 * w * c[1] * c[2]
+        out.append(co)
+    return out
+
+
+def fmt_rational(name, mat):
+    lines = [f"constexpr double {name}[27][{len(mat[0])}] = {{"]
+    for row in mat:
+        lines.append("    {" + ", ".join(f"{float(v)!r}" if v.denominator == 1 else
+                                         f"{v.numerator}.0 / {v.denominator}.0" for v in row) + "},")
+    lines.append("};")
+    return "\n".join(lines)
+
+
 def fmt(name, ctype, mat):
     lines = [f"constexpr {ctype} {name}[27][27] = {{"]
     for row in mat:
@@ -111,6 +148,15 @@ def main():
     out.write(fmt("MRAW_INV24", "int", [[int(v) for v in row] for row in Mi24]) + "\n\n")
     out.write(fmt("MGS", "int", Gi) + "\n\n")
     out.write("constexpr long long MGS_NORM2[27] = {" + ", ".join(str(v) for v in norm2) + "};\n\n")
+    G = [[Fraction(v) for v in row] for row in Gi]
+    T = [[sum(G[k][i] * Mi[i][j] for i in range(27)) for j in range(27)] for k in range(27)]
+    assert all(v.denominator == 1 for row in T for v in row), "T_GS is not integral"
+    assert all(T[k][j] == 0 for k in range(27) for j in range(k + 1, 27)), "T_GS is not lower triangular"
+    Ti = inverse(T)
+    out.write(fmt("T_GS", "int", [[int(v) for v in row] for row in T]) + "\n\n")
+    out.write(fmt_rational("TINV_GS", Ti) + "\n\n")
+    out.write(fmt_rational("MEQ_RAW", meq_coefficients(M)) + "\n\n")
+    out.write(fmt_rational("MEQ_GS", meq_coefficients([[Fraction(v) for v in row] for row in Gi])) + "\n\n")
     out.write("}  // namespace lbm\n")
 
 

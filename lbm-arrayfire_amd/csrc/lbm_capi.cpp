@@ -137,7 +137,10 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
   p.u_in = d->u_in;
   p.rho_bc = d->rho_bc;
-  for (int k = 0; k < 27; ++k) p.s[k] = d->s[k];
+  for (int k = 0; k < 27; ++k) {
+    p.s[k] = d->s[k];
+    p.s24[k] = (float)(d->s[k] / 24.0);
+  }
 
   // Multi-step launches for the lattices that qualify: shape 0 unless LBM_MULTISTEP names another
   // shape index, or "off" (single steps only).

@@ -225,23 +225,24 @@ __device__ __forceinline__ void physical_fixups(const StepParams &p, int x, int 
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type) {
   const C rw = (C)p.rho_bc;
-  static_for<Q>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
-    constexpr int io = Lat<Q>::opp[i];
-    if constexpr (i != 0) {
-      const uint8_t ts = src_type(I);
-      if (ts & NODE_SOLID) {
-        C v = own(IC<io>{});
-        if (ts & NODE_MOVING) {
-          C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
-          if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
-          v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+  if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+      constexpr int io = Lat<Q>::opp[i];
+      if constexpr (i != 0) {
+        const uint8_t ts = src_type(I);
+        if (ts & NODE_SOLID) {
+          C v = own(IC<io>{});
+          if (ts & NODE_MOVING) {
+            C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
+            if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
+            v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+          }
+          f[i] = v;
         }
-        f[i] = v;
       }
-    }
-  });
+    });
   if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = 0, sm = 0;
@@ -464,10 +465,91 @@ __device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
   meq[26] = v2 + rc6;
 }
 
+// FP32 MRT through the raw moment matrix, with the same relaxation as below written as
+// out = f - M^-1 S (M f - meq):
+//   d = MRAW f - meq_raw (closed-form equilibrium moments, or the reference table),
+//   GS: d <- T_GS d (the Gram-Schmidt moments), r = (s / 24) d, GS: r <- T_GS^-1 r,
+//   out = f - (24 MRAW^-1) r.
+// MRAW and 24 MRAW^-1 are integral (339 and 226 non-zeros, mostly +-1) and T_GS, T_GS^-1 hold
+// 64 and 63, so a node costs about 820 operations against 1640 for M_GS f, M_GS f_eq and M_GS^-1
+// directly.  In exact arithmetic both forms are the same map.
+template <int CO>
+__device__ __forceinline__ void collide_mrt_fp32(const StepParams &p, const float *f, float rho, float ux,
+                                                 float uy, float uz, float *out) {
+  float d[27];
+  if (CO == CO_MRT_RAW && p.meq_mode == 0) {
+    meq_reference<float>(rho, ux, uy, uz, d);
+  } else {
+    const float phi[10] = {1.0f, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+    static_for<27>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      float acc = 0.0f;
+      static_for<10>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr float c = (float)MEQ_RAW[k][j];
+        if constexpr (c != 0.0f) acc = __builtin_fmaf(c, phi[j], acc);
+      });
+      d[k] = rho * acc;
+    });
+  }
+  static_for<27>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    float acc = 0.0f;
+    static_for<27>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      acc = add_int_term<float, MRAW[k][i]>(acc, f[i]);
+    });
+    d[k] = acc - d[k];
+  });
+  float r[27];
+  if constexpr (CO == CO_MRT_GS) {
+    float g[27];
+    static_for<27>([&](auto K) {  // Gram-Schmidt moments of the deviation, relaxed
+      constexpr int k = decltype(K)::value;
+      float acc = 0.0f;
+      static_for<k + 1>([&](auto J) {  // T_GS is lower triangular
+        constexpr int j = decltype(J)::value;
+        acc = add_int_term<float, T_GS[k][j]>(acc, d[j]);
+      });
+      g[k] = p.s24[k] * acc;
+    });
+    static_for<27>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      float acc = 0.0f;
+      static_for<27>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr float c = (float)TINV_GS[k][j];
+        if constexpr (c == 1.0f) acc = acc + g[j];
+        else if constexpr (c == -1.0f) acc = acc - g[j];
+        else if constexpr (c != 0.0f) acc = __builtin_fmaf(c, g[j], acc);
+      });
+      r[k] = acc;
+    });
+  } else {
+    static_for<27>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      r[k] = p.s24[k] * d[k];
+    });
+  }
+  static_for<27>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    float acc = 0.0f;
+    static_for<27>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      acc = add_int_term<float, MRAW_INV24[i][k]>(acc, r[k]);
+    });
+    out[i] = f[i] - acc;
+  });
+}
+
 // m = M f; m* = m - s (m - meq); out = M^-1 m*   (rust/examples/d3q27_mrt_lid.rs:336,385-387)
 template <int CO, class C>
 __device__ __forceinline__ void collide_mrt_scalar(const StepParams &p, const C *f, C rho, C ux, C uy,
                                                    C uz, C *out) {
+  if constexpr (sizeof(C) == 4) {
+    collide_mrt_fp32<CO>(p, f, rho, ux, uy, uz, out);
+    return;
+  }
   C m[27], meq[27];
   mrt_forward<C, CO>(f, m);
   if (CO == CO_MRT_RAW && p.meq_mode == 0) {

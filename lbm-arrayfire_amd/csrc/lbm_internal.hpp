@@ -26,6 +26,7 @@ struct StepParams {
   double u_in;
   double rho_bc;
   double s[27];
+  float s24[27];            // FP32 MRT: s_k / 24 (the scale of the integral inverse 24 M^-1)
 };
 
 struct Config {
@@ -94,7 +95,7 @@ void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, 
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);
 // out = in & 0x1f, plus NODE_SLOW on non-solid nodes that are inlet/outlet nodes or pull from a
-// solid node (PHYSICAL mode).
+// solid node, and NODE_WALLNB on the latter (PHYSICAL mode).
 hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
                                 hipStream_t s);
 // out[a * n + k] = fields[a * N + nodes[k]] for a = rho, ux, uy, uz.

@@ -246,16 +246,17 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_prepare_types(int bc_mode, in
   uint8_t t = in[n] & 0x1f;
   if (bc_mode == BC_PHYS && !(t & NODE_SOLID)) {
     const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
-    bool slow = (t & (NODE_INLET | NODE_OUTLET)) != 0;
+    bool wall = false;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       int sx = x - Lat<Q>::cx[i], sy = y - Lat<Q>::cy[i], sz = z - Lat<Q>::cz[i];
       sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
       sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
       sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
-      if (in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) slow = true;
+      if (in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) wall = true;
     });
-    if (slow) t |= NODE_SLOW;
+    if (wall) t |= NODE_SLOW | NODE_WALLNB;
+    if (t & (NODE_INLET | NODE_OUTLET)) t |= NODE_SLOW;
   }
   out[n] = t;
 }

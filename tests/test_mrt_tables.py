@@ -1,0 +1,87 @@
+"""The generated D3Q27 moment tables (lbm-arrayfire_amd/csrc/mrt_tables.h) that the FP32 MRT
+collision uses: the header is what the generator writes, the closed-form equilibrium moments equal
+M f_eq computed numerically, T_GS maps the raw rows onto the Gram-Schmidt rows, and the FP32 form
+out = f - (24 MRAW^-1) T_GS^-1 (s/24) T_GS (MRAW f - meq) equals M^-1 (m - S (m - meq))."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "lbm-arrayfire_amd", "csrc")
+sys.path.insert(0, CSRC)
+
+import gen_mrt_tables as gen  # noqa: E402
+from np_reference import lattice  # noqa: E402
+
+
+def tables():
+    from fractions import Fraction
+    M = gen.raw_rows()
+    Mi = gen.inverse(M)
+    G = [[Fraction(v) for v in gen.primitive(r)] for r in gen.gram_schmidt(M)]
+    T = [[sum(G[k][i] * Mi[i][j] for i in range(27)) for j in range(27)] for k in range(27)]
+    f = lambda A: np.array([[float(v) for v in r] for r in A])  # noqa: E731
+    return f(M), f(Mi), f(G), f(T), gen.meq_coefficients(M), gen.meq_coefficients(G)
+
+
+def feq(rho, u, c, w):
+    cu = c @ u
+    return w * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+
+
+def phi(u):
+    ux, uy, uz = u
+    return np.array([1, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz])
+
+
+def test_header_is_the_generator_output():
+    out = subprocess.run([sys.executable, os.path.join(CSRC, "gen_mrt_tables.py")], check=True,
+                         capture_output=True, text=True).stdout
+    assert out == open(os.path.join(CSRC, "mrt_tables.h")).read()
+
+
+def test_closed_form_equilibrium_moments():
+    M, _, G, _, eraw, egs = tables()
+    c, w, _ = lattice(27)
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        rho, u = 1 + 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal(3)
+        fe = feq(rho, u, c.astype(float), w)
+        for A, E in ((M, eraw), (G, egs)):
+            closed = rho * np.array([[float(v) for v in row] for row in E]) @ phi(u)
+            assert np.allclose(closed, A @ fe, rtol=0, atol=1e-14)
+
+
+def test_gs_rows_from_raw_rows():
+    M, _, G, T, _, _ = tables()
+    assert np.array_equal(T @ M, G)
+    assert np.all(np.triu(T, 1) == 0)
+    Ti = np.linalg.inv(T)
+    assert np.allclose(Ti @ T, np.eye(27), atol=1e-13)
+
+
+def test_fp32_form_equals_reference_relaxation():
+    """Both bases, random populations and rates, projected equilibrium, in float64."""
+    M, Mi, G, T, eraw, _ = tables()
+    c, w, _ = lattice(27)
+    rng = np.random.default_rng(4)
+    Ti = np.linalg.inv(T)
+    E = np.array([[float(v) for v in row] for row in eraw])
+    for basis in ("raw", "gs"):
+        A = M if basis == "raw" else G
+        for _ in range(10):
+            f = w * (1 + 0.05 * rng.standard_normal(27))
+            s = rng.uniform(0.5, 1.8, 27)
+            rho = f.sum()
+            u = (c.T.astype(float) @ f) / rho
+            meq = A @ feq(rho, u, c.astype(float), w)
+            ref = np.linalg.solve(A, A @ f - s * (A @ f - meq))
+            d = M @ f - rho * (E @ phi(u))
+            if basis == "gs":
+                d = Ti @ ((s / 24) * (T @ d))
+            else:
+                d = (s / 24) * d
+            got = f - (24 * Mi) @ d
+            assert np.allclose(got, ref, rtol=0, atol=1e-14)
