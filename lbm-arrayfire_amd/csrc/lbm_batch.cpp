@@ -34,17 +34,21 @@ using namespace lbm;
 namespace {
 
 
-// Device table of a group: BatchTable header | prefix[cap + 1] | params[0][cap] | params[1][cap].
-// The header holds the device addresses of the other parts, so launches depend only on d_tab.
+// Device table of a group: BatchTable header | prefix[cap + 1] | prefix_multi[cap + 1] |
+// params[0][cap] | params[1][cap].  The header holds the device addresses of the other parts, so
+// launches depend only on d_tab.
 struct Group {
   Config cfg{};
   std::vector<lbm_ctx *> members;
   int cap = 0;                 // members the table has room for
-  size_t prefix_off = 0, par_off[2] = {0, 0}, bytes = 0;
+  size_t prefix_off = 0, pmulti_off = 0, par_off[2] = {0, 0}, bytes = 0;
   char *d_tab = nullptr;       // device table
   char *h_tab = nullptr;       // pinned staging copy
   hipEvent_t uploaded = nullptr;
   int32_t total_tiles = 0;
+  int32_t total_multi = 0;     // multi-step tiles (multi_k > 0)
+  int multi_k = 0;             // steps per multi-step launch, 0 = single steps only
+  int phase = 0;               // multi-step groups: the next launch uses parameter block [phase]
   int grid = 0;
   int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
 
@@ -101,7 +105,8 @@ lbm_status reserve(Group &g, int n, bool *grown) {
   g.release();
   const int cap = std::max({n, 2 * g.cap, 8});
   g.prefix_off = (sizeof(BatchTable) + 63) & ~(size_t)63;
-  const size_t pre_end = g.prefix_off + (size_t)(cap + 1) * sizeof(int32_t);
+  g.pmulti_off = g.prefix_off + (size_t)(cap + 1) * sizeof(int32_t);
+  const size_t pre_end = g.pmulti_off + (size_t)(cap + 1) * sizeof(int32_t);
   g.par_off[0] = (pre_end + 255) & ~(size_t)255;
   g.par_off[1] = g.par_off[0] + (size_t)cap * sizeof(StepParams);
   g.bytes = g.par_off[1] + (size_t)cap * sizeof(StepParams);
@@ -148,19 +153,30 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     BCHK(hipEventSynchronize(g.uploaded));  // the previous upload has left the staging copy
     BatchTable *hdr = (BatchTable *)g.h_tab;
     int32_t *prefix = (int32_t *)(g.h_tab + g.prefix_off);
+    int32_t *pmulti = (int32_t *)(g.h_tab + g.pmulti_off);
     StepParams *par0 = (StepParams *)(g.h_tab + g.par_off[0]);
     StepParams *par1 = (StepParams *)(g.h_tab + g.par_off[1]);
-    prefix[0] = 0;
+    // multi-step launches when every member uses the default multi-step shape
+    g.multi_k = multi_steps_per_launch(g.cfg, 0);
+    for (lbm_ctx *m : g.members)
+      if (m->multi_variant != 0 || m->d.nz != 1) g.multi_k = 0;
+    prefix[0] = pmulti[0] = 0;
     for (int i = 0; i < n; ++i) {
-      prefix[i + 1] = prefix[i] + g.members[(size_t)i]->base.ntiles;
-      par0[i] = g.members[(size_t)i]->params(g.members[(size_t)i]->cur);
-      par1[i] = g.members[(size_t)i]->params(g.members[(size_t)i]->cur ^ 1);
+      const lbm_ctx *m = g.members[(size_t)i];
+      prefix[i + 1] = prefix[i] + m->base.ntiles;
+      pmulti[i + 1] = pmulti[i] + (g.multi_k ? multi_tiles(0, m->d.nx, m->d.ny) : 0);
+      par0[i] = m->params(m->cur);
+      par1[i] = m->params(m->cur ^ 1);
     }
     g.total_tiles = prefix[n];
+    g.total_multi = pmulti[n];
+    g.phase = 0;
     hdr->nmembers = n;
     hdr->total_tiles = g.total_tiles;
-    hdr->pad_[0] = hdr->pad_[1] = 0;
+    hdr->total_multi = g.total_multi;
+    hdr->pad_ = 0;
     hdr->prefix = (const int32_t *)(g.d_tab + g.prefix_off);
+    hdr->prefix_multi = (const int32_t *)(g.d_tab + g.pmulti_off);
     hdr->members[0] = (const StepParams *)(g.d_tab + g.par_off[0]);
     hdr->members[1] = (const StepParams *)(g.d_tab + g.par_off[1]);
     // the whole table (unused rows too) keeps the copy one contiguous transfer
@@ -180,9 +196,17 @@ lbm_status rebuild(lbm_batch_ctx *b) {
   return LBM_OK;
 }
 
+// One step of every single-step group (multi-step groups advance in enqueue on their own).
 lbm_status launch_phase(lbm_batch_ctx *b, int phase, hipStream_t s) {
-  for (auto &g : b->groups) BCHK(launch_step_batch(g.cfg, g.args(phase), g.grid, s));
+  for (auto &g : b->groups)
+    if (!g.multi_k) BCHK(launch_step_batch(g.cfg, g.args(phase), g.grid, s));
   return LBM_OK;
+}
+
+bool has_single_groups(const lbm_batch_ctx *b) {
+  for (const auto &g : b->groups)
+    if (!g.multi_k) return true;
+  return false;
 }
 
 lbm_status ensure_graph(lbm_batch_ctx *b) {
@@ -212,6 +236,25 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
     if (st != LBM_OK) return st;
   }
   if (b->groups.empty() || nsteps == 0) return LBM_OK;
+  // multi-step groups: K steps per launch, then single steps; each launch flips the buffers once
+  for (auto &g : b->groups) {
+    if (!g.multi_k) continue;
+    int64_t gl = nsteps, flips = 0;
+    for (; gl >= g.multi_k; gl -= g.multi_k, ++flips) {
+      BCHK(launch_step_multi_batch(g.cfg, g.args(g.phase), std::max(g.total_multi, 1), b->stream));
+      g.phase ^= 1;
+    }
+    for (; gl > 0; --gl, ++flips) {
+      BCHK(launch_step_batch(g.cfg, g.args(g.phase), g.grid, b->stream));
+      g.phase ^= 1;
+    }
+    for (lbm_ctx *m : g.members) {
+      m->cur ^= (int)(flips & 1);
+      m->steps += nsteps;
+    }
+  }
+  b->steps += nsteps;
+  if (!has_single_groups(b)) return LBM_OK;
   int64_t left = nsteps;
   if (b->phase == 1) {
     lbm_status st = launch_phase(b, 1, b->stream);
@@ -229,11 +272,13 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
     if (st != LBM_OK) return st;
     b->phase = 1;
   }
-  b->steps += nsteps;
   const int flip = (int)(nsteps & 1);
-  for (lbm_ctx *m : b->members) {
-    m->cur ^= flip;
-    m->steps += nsteps;
+  for (auto &g : b->groups) {
+    if (g.multi_k) continue;
+    for (lbm_ctx *m : g.members) {
+      m->cur ^= flip;
+      m->steps += nsteps;
+    }
   }
   return LBM_OK;
 }
@@ -302,7 +347,7 @@ lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms) {
     lbm_status st = rebuild(b);
     if (st != LBM_OK) return st;
   }
-  if (nsteps >= 2 && b->phase == 0) {  // capture outside the timed region
+  if (nsteps >= 2 && b->phase == 0 && has_single_groups(b)) {  // capture outside the timed region
     lbm_status st = ensure_graph(b);
     if (st != LBM_OK) return st;
   }
@@ -337,8 +382,8 @@ lbm_status lbm_batch_tune(lbm_batch b, int32_t trials) {
     for (int f : {1, 2, 4, 8, 16, 0}) {  // 0 = one workgroup per tile
       const int grid = f ? std::min(g.total_tiles, f * cus) : g.total_tiles;
       if (grid < 1) continue;
-      // trial launches read the current buffers and write only the other ones (b->phase)
-      const BatchArgs a = g.args(b->phase);
+      // trial launches read the current buffers and write only the other ones
+      const BatchArgs a = g.args(g.multi_k ? g.phase : b->phase);
       BCHK(launch_step_batch(g.cfg, a, grid, b->stream));  // warm-up
       BCHK(hipEventRecord(b->ev0, b->stream));
       for (int r = 0; r < reps; ++r) BCHK(launch_step_batch(g.cfg, a, grid, b->stream));

@@ -89,7 +89,8 @@ struct NodePos {
   bool valid;
 };
 
-__device__ __forceinline__ NodePos node_of_tile(const StepParams &p, int tile) {
+template <class P>
+__device__ __forceinline__ NodePos node_of_tile(const P &p, int tile) {
   const int bx = tile % p.gx, t2 = tile / p.gx, by = t2 % p.gy, bz = t2 / p.gy;
   NodePos q;
   q.x = bx * TILE_X + (int)(threadIdx.x & (TILE_X - 1));
@@ -100,26 +101,30 @@ __device__ __forceinline__ NodePos node_of_tile(const StepParams &p, int tile) {
   return q;
 }
 
-__device__ __forceinline__ NodePos node_of_thread(const StepParams &p) {
+template <class P>
+__device__ __forceinline__ NodePos node_of_thread(const P &p) {
   return node_of_tile(p, xcd_tile((int)blockIdx.x, p.ntiles));
 }
 
 // Batched launches: a persistent workgroup walks one contiguous chunk of the batch's tile range.
 // The table is read through the constant address space: it does not change while a launch runs
 // (the host rewrites it between launches, in stream order), so every read of it is a uniform
-// scalar load and a member's parameter block lands in SGPRs exactly like a kernel argument.
-// Through generic pointers the parameters landed in VGPRs instead (D3Q27 MRT: 165 VGPRs against
-// 81 for the single-lattice kernel, half the occupancy).
+// scalar load.  The step functions take the member's parameter block by constant-space reference
+// (they are templates on the parameter type), so, as with a kernel argument, the compiler loads
+// each field where it is used and may load it again instead of holding it.  Through generic
+// pointers the parameters landed in VGPRs (D3Q27 MRT: 165 VGPRs against 81 for the
+// single-lattice kernel); copied into a local they spilled SGPRs into VGPR lanes (101 against 69).
 #if defined(__HIP_DEVICE_COMPILE__)
 template <class T> using const_as = const __attribute__((address_space(4))) T *;
 #else  // the host pass only parses device code; it has no constant address space
 template <class T> using const_as = const T *;
 #endif
-template <class F>
+// MULTI walks the multi-step tiles (prefix_multi) instead of the single-step ones.
+template <bool MULTI = false, class F>
 __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
   const_as<BatchTable> tab = (const_as<BatchTable>)b.tab;
-  const int nmembers = tab->nmembers, total = tab->total_tiles;
-  const_as<int32_t> prefix = (const_as<int32_t>)tab->prefix;
+  const int nmembers = tab->nmembers, total = MULTI ? tab->total_multi : tab->total_tiles;
+  const_as<int32_t> prefix = (const_as<int32_t>)(MULTI ? tab->prefix_multi : tab->prefix);
   const_as<StepParams> members = (const_as<StepParams>)tab->members[b.phase];
   const int nb = (int)gridDim.x;
   const int blk = xcd_tile((int)blockIdx.x, nb);
@@ -136,15 +141,14 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
   int m = lo;
   for (int t = t0; t < t1; ++t) {
     while (t >= prefix[m + 1]) ++m;
-    const StepParams p = members[m];
-    f(p, t - prefix[m]);
+    f(members[m], t - prefix[m]);  // a constant-space reference: fields are re-loaded as needed
   }
 }
 
 // ---- pulls -------------------------------------------------------------------------------
 // Periodic pull: f_i(x) = F_i(x - c_i mod dims).
-template <int Q, int ST>
-__device__ __forceinline__ void pull_periodic(const StepParams &p, const NodePos &q,
+template <int Q, int ST, class P>
+__device__ __forceinline__ void pull_periodic(const P &p, const NodePos &q,
                                               typename Codec<ST, Q>::C *f) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
@@ -163,8 +167,8 @@ __device__ __forceinline__ void pull_periodic(const StepParams &p, const NodePos
   });
 }
 
-template <class C>
-__device__ __forceinline__ C inlet_velocity(const StepParams &p, int D, int y, int z) {
+template <class C, class P>
+__device__ __forceinline__ C inlet_velocity(const P &p, int D, int y, int z) {
   if (p.inlet_profile == 0) return (C)p.u_in;
   const C H = (C)(p.ny - 2), eta = (C)y - (C)0.5;
   C u = (((C)4 * (C)p.u_in) * eta * (H - eta)) / (H * H);
@@ -220,8 +224,8 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   src_type(IC<i>)  type of the node this node pulls population i from (node - c_i)
 //   own(IC<j>)       population j of this node in the state being advanced
 //   west(IC<j>)      population j of the node at x - 1;  west_type()  its type
-template <int Q, class C, class SrcType, class Own, class West, class WestType>
-__device__ __forceinline__ void physical_fixups(const StepParams &p, int x, int y, int z, uint8_t t, C *f,
+template <int Q, class C, class SrcType, class Own, class West, class WestType, class P>
+__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type) {
   const C rw = (C)p.rho_bc;
@@ -274,8 +278,8 @@ __device__ __forceinline__ void physical_fixups(const StepParams &p, int x, int 
 
 // PHYSICAL-mode pull from global memory.  The periodic pull is issued by every node before its
 // type is known; nodes flagged SLOW then apply the boundary rules.
-template <int Q, int ST>
-__device__ __forceinline__ void pull_physical(const StepParams &p, const NodePos &q, uint8_t t,
+template <int Q, int ST, class P>
+__device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint8_t t,
                                              typename Codec<ST, Q>::C *f) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
@@ -338,8 +342,8 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
 // REFERENCE Dirichlet overrides, in the reference's order: lid velocity, inlet x-velocity,
 // solid rho = u = 0, fixed-density columns (cpp/d2q9_bgk_lid.cpp:141-146,
 // rust/examples/d2q9_bgk_channel.rs:190-201, rust/examples/d3q27_mrt_lid.rs:323-330).
-template <int Q, class C>
-__device__ __forceinline__ void overrides(const StepParams &p, uint8_t t, int y, int z, C &rho, C &ux,
+template <int Q, class C, class P>
+__device__ __forceinline__ void overrides(const P &p, uint8_t t, int y, int z, C &rho, C &ux,
                                           C &uy, C &uz) {
   if (t & NODE_MOVING) { ux = (C)p.u_wall[0]; uy = (C)p.u_wall[1]; uz = (C)p.u_wall[2]; }
   if (t & NODE_INLET) ux = inlet_velocity<C>(p, Lat<Q>::D, y, z);
@@ -384,8 +388,8 @@ __device__ __forceinline__ C usq_of(C ux, C uy, C uz) {
 }
 
 // F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153); FP32: f + omega (f_eq - f).
-template <int Q, class C>
-__device__ __forceinline__ void collide_bgk(const StepParams &p, const C *f, C rho, C ux, C uy, C uz,
+template <int Q, class C, class P>
+__device__ __forceinline__ void collide_bgk(const P &p, const C *f, C rho, C ux, C uy, C uz,
                                             C *out) {
   const C omega = (C)p.omega, om1 = (C)1 - omega;
   const C usq = usq_of<Q>(ux, uy, uz);
@@ -473,8 +477,8 @@ __device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
 // MRAW and 24 MRAW^-1 are integral (339 and 226 non-zeros, mostly +-1) and T_GS, T_GS^-1 hold
 // 64 and 63, so a node costs about 820 operations against 1640 for M_GS f, M_GS f_eq and M_GS^-1
 // directly.  In exact arithmetic both forms are the same map.
-template <int CO>
-__device__ __forceinline__ void collide_mrt_fp32(const StepParams &p, const float *f, float rho, float ux,
+template <int CO, class P>
+__device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, float rho, float ux,
                                                  float uy, float uz, float *out) {
   float d[27];
   if (CO == CO_MRT_RAW && p.meq_mode == 0) {
@@ -543,8 +547,8 @@ __device__ __forceinline__ void collide_mrt_fp32(const StepParams &p, const floa
 }
 
 // m = M f; m* = m - s (m - meq); out = M^-1 m*   (rust/examples/d3q27_mrt_lid.rs:336,385-387)
-template <int CO, class C>
-__device__ __forceinline__ void collide_mrt_scalar(const StepParams &p, const C *f, C rho, C ux, C uy,
+template <int CO, class C, class P>
+__device__ __forceinline__ void collide_mrt_scalar(const P &p, const C *f, C rho, C ux, C uy,
                                                    C uz, C *out) {
   if constexpr (sizeof(C) == 4) {
     collide_mrt_fp32<CO>(p, f, rho, ux, uy, uz, out);
@@ -570,14 +574,14 @@ __device__ __forceinline__ void collide_mrt_scalar(const StepParams &p, const C 
   mrt_inverse<C, CO>(m, out);
 }
 
-template <int Q, int CO, class C>
-__device__ __forceinline__ void collide(const StepParams &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
+template <int Q, int CO, class C, class P>
+__device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
   if constexpr (CO == CO_BGK) collide_bgk<Q>(p, f, rho, ux, uy, uz, out);
   else collide_mrt_scalar<CO>(p, f, rho, ux, uy, uz, out);
 }
 
-template <int Q, int ST>
-__device__ __forceinline__ void store_all(const StepParams &p, uint32_t n, const typename Codec<ST, Q>::C *out) {
+template <int Q, int ST, class P>
+__device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename Codec<ST, Q>::C *out) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   T *fout = (T *)p.fout;

@@ -62,8 +62,10 @@ hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_
 struct BatchTable {
   int32_t nmembers;
   int32_t total_tiles;
-  int32_t pad_[2];
+  int32_t total_multi;           // multi-step tiles (D2Q9 BGK groups), 0 otherwise
+  int32_t pad_;
   const int32_t *prefix;         // nmembers + 1 tile offsets
+  const int32_t *prefix_multi;   // nmembers + 1 multi-step tile offsets (or null)
   const StepParams *members[2];  // parameter blocks of both buffer orientations
 };
 struct BatchArgs {
@@ -79,6 +81,10 @@ hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int g
 int multi_variant_count();
 int multi_steps_per_launch(const Config &c, int variant);
 hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, hipStream_t s);
+// Multi-step tiles of a nx x ny lattice in the given shape; batched multi-step launches (shape 0,
+// one workgroup per multi-step tile of the group by default).
+int multi_tiles(int variant, int nx, int ny);
+hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 // True when the step of this configuration runs on the MFMA MRT kernel.
 // Engine AUTO resolves to MRT_ENGINE_AUTO_FP32 for FP32-arithmetic storages (the sparse integer
 // scalar transform: faster on gfx950 than either MFMA engine), scalar for FP64.

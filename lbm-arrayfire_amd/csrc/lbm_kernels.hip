@@ -7,8 +7,8 @@ namespace lbm {
 
 // One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
 // solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
-template <int Q, int ST, int BC, int CO>
-__device__ __forceinline__ void step_tile(const StepParams &p, int tile) {
+template <int Q, int ST, int BC, int CO, class P>
+__device__ __forceinline__ void step_tile(const P &p, int tile) {
   using C = typename Codec<ST, Q>::C;
   const NodePos q = node_of_tile(p, tile);
   if (!q.valid) return;
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
 // Persistent batched step: every member of one configuration group, one launch.
 template <int Q, int ST, int BC, int CO>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
-  for_batch_tiles(b, [&](const StepParams &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
+  for_batch_tiles(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
 }
 
 // rho, u the reference holds after a step: moments of the pulled populations with the overrides

@@ -39,8 +39,8 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int ST, int BC, bool FUSED>
-__device__ __forceinline__ void mrt_mfma_tile(const StepParams &p, int tile, float (*lds)[64 * MT_STRIDE_MAX]) {
+template <int ST, int BC, bool FUSED, class Params>
+__device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (*lds)[64 * MT_STRIDE_MAX]) {
   constexpr int MT_STRIDE = mt_stride<FUSED>();
   using CD = Codec<ST, 27>;
   using C = typename CD::C;
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
 template <int ST, int BC, bool FUSED>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch_mrt_mfma(BatchArgs b) {
   __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE_MAX];
-  for_batch_tiles(b, [&](const StepParams &p, int tile) { mrt_mfma_tile<ST, BC, FUSED>(p, tile, lds); });
+  for_batch_tiles(b, [&](const auto &p, int tile) { mrt_mfma_tile<ST, BC, FUSED>(p, tile, lds); });
 }
 
 template <class F>

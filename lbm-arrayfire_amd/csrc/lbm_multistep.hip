@@ -26,19 +26,19 @@ struct MultiShape {
   static constexpr int NBUF = DB ? 2 : 1;         // LDS copies of the region
 };
 
-// DB: two LDS copies of the region, sub-step s reads copy (s - 1) & 1 and writes copy s & 1 (one
-// barrier per sub-step); otherwise one copy, results staged in registers between two barriers.
-template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
-__global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int ntm) {
+// One tile of one lattice, K steps.  DB: two LDS copies of the region, sub-step s reads copy
+// (s - 1) & 1 and writes copy s & 1 (one barrier per sub-step); otherwise one copy, results staged
+// in registers between two barriers.  lf / lt are the workgroup's LDS arrays (MultiShape sizes).
+template <int ST, int BC, int TX, int K, int TY, int NT, bool DB, class P>
+__device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
+                                           typename Codec<ST, 9>::C *lf, uint8_t *lt) {
   using S = MultiShape<ST, TX, K, TY, NT, DB>;
   constexpr int Q = S::Q, RX = S::RX, RY = S::RY, RN = S::RN, NPOS = S::NPOS;
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   using C = typename CD::C;
-  __shared__ C lf[S::NBUF * Q * RN];
-  __shared__ uint8_t lt[RN];
 
-  const int tile = xcd_tile((int)blockIdx.x, ntm);
+  __syncthreads();  // a workgroup that walks several tiles: the previous tile's reads are done
   const int bx = tile % gxm, by = tile / gxm;
   const int x0 = bx * S::TX - K, y0 = by * TY - K;  // lattice coordinates of region position (0, 0)
   const T *fin = (const T *)p.fin;
@@ -151,6 +151,29 @@ __global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int nt
   }
 }
 
+template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
+__global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int ntm) {
+  using S = MultiShape<ST, TX, K, TY, NT, DB>;
+  __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
+  __shared__ uint8_t lt[S::RN];
+  multi_tile<ST, BC, TX, K, TY, NT, DB>(p, gxm, xcd_tile((int)blockIdx.x, ntm), lf, lt);
+}
+
+// Batched: every member of one D2Q9 BGK group, K steps, in the default launch shape.  Member m
+// owns multi-step tiles [prefix_multi[m], prefix_multi[m+1]).
+constexpr int BATCH_TX = 64, BATCH_K = 4, BATCH_TY = 16, BATCH_NT = 1024;
+constexpr bool BATCH_DB = true;
+template <int ST, int BC>
+__global__ void __launch_bounds__(BATCH_NT) k_step_multi_batch(BatchArgs b) {
+  using S = MultiShape<ST, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>;
+  __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
+  __shared__ uint8_t lt[S::RN];
+  for_batch_tiles<true>(b, [&](const auto &p, int tile) {
+    multi_tile<ST, BC, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>(p, (p.nx + BATCH_TX - 1) / BATCH_TX, tile,
+                                                                       lf, lt);
+  });
+}
+
 namespace {
 
 // Launch shapes: interior TX x TY, steps per launch K, threads NT, double-buffered LDS.
@@ -185,6 +208,30 @@ hipError_t launch_multi_variant(int v, const StepParams &p, hipStream_t s) {
 }  // namespace
 
 int multi_variant_count() { return (int)(sizeof(kMultiVariants) / sizeof(kMultiVariants[0])); }
+
+int multi_tiles(int variant, int nx, int ny) {
+  if (variant < 0 || variant >= multi_variant_count()) return 0;
+  const MultiVariant &v = kMultiVariants[variant];
+  return ((nx + v.TX - 1) / v.TX) * ((ny + v.TY - 1) / v.TY);
+}
+
+hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
+  if (multi_steps_per_launch(c, 0) != BATCH_K) return hipErrorInvalidValue;
+  auto byBC = [&](auto STc) -> hipError_t {
+    constexpr int ST = decltype(STc)::value;
+    if (c.bc == BC_REF)
+      hipLaunchKernelGGL((k_step_multi_batch<ST, BC_REF>), dim3(grid), dim3(BATCH_NT), 0, s, b);
+    else
+      hipLaunchKernelGGL((k_step_multi_batch<ST, BC_PHYS>), dim3(grid), dim3(BATCH_NT), 0, s, b);
+    return hipGetLastError();
+  };
+  switch (c.storage) {
+    case ST_F32: return byBC(IC<ST_F32>{});
+    case ST_F16S: return byBC(IC<ST_F16S>{});
+    case ST_BF16S: return byBC(IC<ST_BF16S>{});
+  }
+  return hipErrorInvalidValue;
+}
 
 int multi_steps_per_launch(const Config &c, int variant) {
   if (c.Q != 9 || c.collision != CO_BGK || c.storage == ST_F64) return 0;
