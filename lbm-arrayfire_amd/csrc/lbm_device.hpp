@@ -311,7 +311,8 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
 }
 
 // ---- moments, overrides, collisions ------------------------------------------------------
-// rho = sum_i f_i; u = (sum_i c_i f_i) / rho, ascending i (cpp/d2q9_bgk_lid.cpp:131-138).
+// rho = sum_i f_i; u = (sum_i c_i f_i) / rho, ascending i (cpp/d2q9_bgk_lid.cpp:131-138); FP32
+// multiplies by the correctly rounded 1/rho instead of dividing three times.
 template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz) {
   C r = 0, jx = 0, jy = 0, jz = 0;
@@ -334,9 +335,16 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
     });
   }
   rho = r;
-  ux = jx / r;
-  uy = jy / r;
-  uz = (Lat<Q>::D == 3) ? jz / r : (C)0;
+  if constexpr (sizeof(C) == 4) {  // FP32: one IEEE division, then products
+    const C inv = (C)1 / r;
+    ux = jx * inv;
+    uy = jy * inv;
+    uz = (Lat<Q>::D == 3) ? jz * inv : (C)0;
+  } else {
+    ux = jx / r;
+    uy = jy / r;
+    uz = (Lat<Q>::D == 3) ? jz / r : (C)0;
+  }
 }
 
 // REFERENCE Dirichlet overrides, in the reference's order: lid velocity, inlet x-velocity,

@@ -182,9 +182,9 @@ struct MultiVariant {
   bool DB;
 };
 #define LBM_MULTI_SHAPES(X)                                                                         \
-  X(0, 64, 4, 16, 1024, true) X(1, 64, 5, 16, 1024, true) X(2, 64, 3, 16, 1024, true)              \
-  X(3, 64, 4, 12, 1024, true) X(4, 32, 4, 16, 512, true) X(5, 32, 4, 32, 1024, true)               \
-  X(6, 64, 4, 16, 1024, false) X(7, 64, 6, 16, 1024, true) X(8, 32, 6, 32, 1024, true)
+  X(0, 64, 4, 16, 1024, true) X(1, 64, 3, 16, 1024, true) X(2, 32, 4, 32, 1024, true)              \
+  X(3, 64, 4, 16, 1024, false) X(4, 64, 4, 8, 1024, false) X(5, 64, 4, 8, 512, false)              \
+  X(6, 32, 4, 16, 1024, false) X(7, 32, 4, 16, 512, false) X(8, 64, 2, 8, 1024, true)
 #define LBM_MULTI_ENTRY(i, tx, k, ty, nt, db) {tx, k, ty, nt, db},
 constexpr MultiVariant kMultiVariants[] = {LBM_MULTI_SHAPES(LBM_MULTI_ENTRY)};
 

@@ -25,8 +25,9 @@
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
 //
-// FP32 mode contracts the equilibrium and the BGK relaxation into FMAs (see feq()), matching the
-// kernels; FP64 mode keeps every operation separately rounded.
+// FP32 mode contracts the equilibrium and the BGK relaxation into FMAs (see feq()) and takes
+// u = j * (1/rho), matching the kernels; FP64 mode keeps every operation separately rounded and
+// divides as the reference does.
 //
 // Summation order inside a node is ascending population index; ArrayFire's reduction order is
 // not observable without ArrayFire, so FP32 bitwise parity with ArrayFire is unpinnable (FP64
@@ -361,7 +362,13 @@ struct Sim {
     for (int i = 0; i < L.Q; ++i) jy += (T)L.cy[i] * S[i];
     if (L.D == 3)
       for (int i = 0; i < L.Q; ++i) jz += (T)L.cz[i] * S[i];
-    rho = r; ux = jx / r; uy = jy / r; uz = (L.D == 3) ? jz / r : (T)0;
+    rho = r;
+    if (sizeof(T) == 4) {  // FP32 as the kernels: the correctly rounded 1/rho, then products
+      const T inv = (T)1 / r;
+      ux = jx * inv; uy = jy * inv; uz = (L.D == 3) ? jz * inv : (T)0;
+    } else {
+      ux = jx / r; uy = jy / r; uz = (L.D == 3) ? jz / r : (T)0;
+    }
   }
 
   // ---- REFERENCE step, multi-pass as the reference's ArrayFire program ----
