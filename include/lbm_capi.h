@@ -180,6 +180,13 @@ lbm_status lbm_set_state(lbm_handle h, const void *src);
  * mass = sum_n sum_i f, kinetic_energy = sum_n |sum_i c_i f|^2 / (2 sum_i f) over non-solid
  * nodes, nonfinite = count of non-finite populations.  Any pointer may be NULL.  Blocks. */
 lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite);
+/* Force on the solid nodes inside box = {x0, x1, y0, y1, z0, z1} (half-open node ranges; NULL =
+ * every solid node) by momentum exchange over the links from non-solid nodes into them:
+ * F = sum c_i (2 f_i - 6 w_i rho_bc c_i.u_wall), the second term only for moving walls, f = the
+ * post-collision populations of the current state, FP64 accumulation.  Drag coefficient of a
+ * body in a uniform stream: C_d = 2 F_x / (rho u^2 A).  No reference counterpart (the reference
+ * reports no forces).  Blocks. */
+lbm_status lbm_force(lbm_handle h, const int32_t box[6], double force[3]);
 
 /* ---- batching: many lattices, one persistent launch per configuration per step -------- */
 /* No reference counterpart (each reference program steps one lattice).  Members keep their own

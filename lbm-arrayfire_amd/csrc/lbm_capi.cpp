@@ -470,6 +470,18 @@ lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int6
   return out[2] > 0 ? fail(LBM_EUNSTABLE, "non-finite populations") : LBM_OK;
 }
 
+lbm_status lbm_force(lbm_handle h, const int32_t box[6], double force[3]) {
+  if (!h || !force) return fail(LBM_EINVAL, "NULL argument");
+  int b[6] = {0, h->d.nx, 0, h->d.ny, 0, h->d.nz};
+  if (box)
+    for (int k = 0; k < 6; ++k) b[k] = box[k];
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(launch_force(h->cfg, h->params(h->cur), b, h->partials, h->red, h->strm()));
+  HIPCHK(hipMemcpyAsync(force, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+  HIPCHK(hipStreamSynchronize(h->strm()));
+  return LBM_OK;
+}
+
 }  // extern "C"
 
 namespace lbm {

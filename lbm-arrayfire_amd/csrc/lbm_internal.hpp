@@ -115,6 +115,10 @@ hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double 
                                    uint64_t seed, double amp, hipStream_t s);
 // MRT: f = rho0/27 then one collision at rho0, u = 0; BGK: f_eq(rho0, 0). Both buffers.
 hipError_t launch_init_reference(const Config &c, const StepParams &p, double rho0, hipStream_t s);
+// Momentum-exchange force on the solid nodes inside box[6] = {x0, x1, y0, y1, z0, z1} (half-open)
+// into out[3]; partials as for launch_monitors.
+hipError_t launch_force(const Config &c, const StepParams &p, const int *box, double *partials, double *out,
+                        hipStream_t s);
 // Per-tile partial sums (mass, kinetic energy, nonfinite) then a final single-block reduction
 // into out[3].  partials must hold 3 * p.ntiles doubles.
 hipError_t launch_monitors(const Config &c, const StepParams &p, double *partials, double *out,

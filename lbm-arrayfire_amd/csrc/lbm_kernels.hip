@@ -215,6 +215,57 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_monitors(StepParams p, double
   }
 }
 
+// Force on the solid nodes inside box (x0, x1, y0, y1, z0, z1, half-open) by momentum exchange:
+// every link from a non-solid node x into such a solid node x + c_i carries c_i (f_i(x) + the
+// population it returns), i.e. c_i (2 f_i(x) - 6 w_i rho_bc c_i.u_wall) (second term for moving
+// walls), f = the post-collision populations the state holds.  FP64 partial sums per tile.
+template <int Q, int ST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_force(StepParams p, int x0, int x1, int y0, int y1, int z0,
+                                                         int z1, double *partials) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  __shared__ double red[3][BLOCK_THREADS / 64];
+  const NodePos q = node_of_thread(p);
+  double fx = 0, fy = 0, fz = 0;
+  if (q.valid && !(p.type[q.n] & NODE_SOLID)) {
+    const T *fin = (const T *)p.fin;
+    const double rw = (double)p.rho_bc;
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+      if constexpr (i != 0) {
+        int sx = q.x + cx, sy = q.y + cy, sz = q.z + cz;
+        sx = sx < 0 ? sx + p.nx : (sx >= p.nx ? sx - p.nx : sx);
+        sy = sy < 0 ? sy + p.ny : (sy >= p.ny ? sy - p.ny : sy);
+        sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
+        const uint8_t ts = p.type[(uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz)];
+        if ((ts & NODE_SOLID) && sx >= x0 && sx < x1 && sy >= y0 && sy < y1 && sz >= z0 && sz < z1) {
+          double m = 2.0 * (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+          if (ts & NODE_MOVING)
+            m -= 6.0 * Lat<Q>::w[i] * rw * (cx * p.u_wall[0] + cy * p.u_wall[1] + cz * p.u_wall[2]);
+          fx += cx * m;
+          fy += cy * m;
+          fz += cz * m;
+        }
+      }
+    });
+  }
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wv] = fx; red[1][wv] = fy; red[2][wv] = fz; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0;
+    for (int k = 0; k < BLOCK_THREADS / 64; ++k) { a += red[0][k]; b += red[1][k]; c += red[2][k]; }
+    const size_t t = blockIdx.x;
+    partials[3 * t + 0] = a;
+    partials[3 * t + 1] = b;
+    partials[3 * t + 2] = c;
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK_THREADS) k_reduce_partials(const double *partials, int n, double *out) {
   __shared__ double red[3][BLOCK_THREADS / 64];
   double a = 0, b = 0, c = 0;
@@ -371,6 +422,18 @@ hipError_t launch_init_reference(const Config &c, const StepParams &p, double rh
                        dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p, rho0);
     return hipGetLastError();
   });
+}
+
+hipError_t launch_force(const Config &c, const StepParams &p, const int *box, double *partials, double *out,
+                        hipStream_t s) {
+  hipError_t e = with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    hipLaunchKernelGGL((k_force<decltype(Qc)::value, decltype(STc)::value>), dim3(p.ntiles), dim3(BLOCK_THREADS),
+                       0, s, p, box[0], box[1], box[2], box[3], box[4], box[5], partials);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK_THREADS), 0, s, partials, p.ntiles, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_monitors(const Config &c, const StepParams &p, double *partials, double *out,

@@ -34,6 +34,7 @@ EXPORTS = [
     "lbm_get_fields_at",
     "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
+    "lbm_force",
     "lbm_batch_create", "lbm_batch_add", "lbm_batch_remove", "lbm_batch_step", "lbm_batch_step_timed",
     "lbm_batch_sync", "lbm_batch_tune", "lbm_batch_size", "lbm_batch_destroy",
     "lbm_server_create", "lbm_server_submit", "lbm_server_tick", "lbm_server_poll", "lbm_server_take",
@@ -144,6 +145,7 @@ def load(path=None):
     L.lbm_get_state.argtypes = [H, ctypes.c_void_p]
     L.lbm_set_state.argtypes = [H, ctypes.c_void_p]
     L.lbm_monitors.argtypes = [H, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_force.argtypes = [H, ctypes.POINTER(ctypes.c_int32), dp]
     L.lbm_batch_create.argtypes = [ctypes.POINTER(H)]
     L.lbm_batch_add.argtypes = [H, H]
     L.lbm_batch_remove.argtypes = [H, H]
@@ -289,6 +291,14 @@ class Lattice:
         if st not in (0, 4):
             check(st)
         return m.value, k.value, n.value
+
+    def force(self, box=None):
+        """Momentum-exchange force on the solid nodes inside box = (x0, x1, y0, y1, z0, z1)
+        (half-open; None = all solid nodes)."""
+        out = np.zeros(3)
+        b = None if box is None else (ctypes.c_int32 * 6)(*[int(v) for v in box])
+        check(self.lib.lbm_force(self.h, b, _dp(out)))
+        return out
 
 
 class Batch:

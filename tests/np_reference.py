@@ -198,3 +198,29 @@ def init_feq(types_shape, cfg, rho=None, u=None):
     rho = np.ones(types_shape) if rho is None else rho
     u = np.zeros((3,) + types_shape) if u is None else u
     return feq(rho, u, c, w)
+
+
+def force(F, types, cfg, box=None):
+    """Momentum-exchange force on the solid nodes inside box = (x0, x1, y0, y1, z0, z1)
+    (half-open): sum over links from non-solid nodes x into such solid nodes x + c_i of
+    c_i (2 f_i(x) - 6 w_i rho_bc c_i.u_wall), the last term for moving walls only."""
+    Q = cfg["lattice"]
+    c, w, _ = lattice(Q)
+    nz, ny, nx = types.shape
+    solid = (types & SOLID) != 0
+    moving = (types & MOVING) != 0
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    if box is None:
+        box = (0, nx, 0, ny, 0, nz)
+    inbox = (x >= box[0]) & (x < box[1]) & (y >= box[2]) & (y < box[3]) & (z >= box[4]) & (z < box[5])
+    target = solid & inbox
+    uw = np.asarray(cfg.get("u_wall", (0, 0, 0)), dtype=float)
+    rw = cfg.get("rho_bc", 1.0)
+    out = np.zeros(3)
+    for i in range(1, Q):
+        shift = (-c[i][2], -c[i][1], -c[i][0])  # value at x + c_i, seen from x
+        hit = np.roll(target, shift=shift, axis=(0, 1, 2)) & ~solid
+        mov = np.roll(moving, shift=shift, axis=(0, 1, 2))
+        m = 2.0 * F[i] - np.where(mov, 6.0 * w[i] * rw * float(c[i] @ uw), 0.0)
+        out += c[i].astype(float) * m[hit].sum()
+    return out

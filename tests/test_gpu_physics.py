@@ -79,3 +79,28 @@ def test_cylinder_strouhal_across_precisions():
         st, _ = strouhal(storage)
         print(f"Strouhal storage {storage}: {st:.4f}, relative error vs FP64 {abs(st - st64) / st64:.2e}")
         assert abs(st - st64) <= tol * st64
+
+
+def sphere_drag(storage):
+    """Drag coefficient C_d = 2 F_x / (rho u^2 pi r^2) of a sphere (r = 6) in a 128 x 48 x 48 duct,
+    uniform inlet u = 0.05, Re = 30, after 4000 steps from a uniform start."""
+    r, u = 6, 0.05
+    cfg, types = cases.d3q27_sphere_channel(128, 48, 48, (32, 24, 24), r, u_in=u, re=30.0, storage=storage)
+    with Lattice(cfg, types) as lat:
+        lat.init_equilibrium(1.0, (u, 0.0, 0.0), 0x5EED0004, 1e-4)
+        lat.step(4000)
+        F = lat.force((32 - r - 1, 32 + r + 2, 24 - r - 1, 24 + r + 2, 24 - r - 1, 24 + r + 2))
+        _, _, bad = lat.monitors()
+    assert bad == 0
+    return 2.0 * F[0] / (u * u * np.pi * r * r)
+
+
+def test_sphere_drag_across_precisions():
+    """The reduced-size sphere case: drag coefficient of each storage format against FP64."""
+    cd64 = sphere_drag(F64)
+    print(f"sphere C_d FP64 {cd64:.5f}")
+    assert 1.0 < cd64 < 20.0  # confined sphere (blockage 25 %) at Re 30: loose physical range
+    for storage, tol in ((F32, 1e-4), (F16S, 1e-2), (BF16S, 5e-2)):
+        cd = sphere_drag(storage)
+        print(f"sphere C_d storage {storage}: {cd:.5f}, relative error vs FP64 {abs(cd - cd64) / cd64:.2e}")
+        assert abs(cd - cd64) <= tol * cd64

@@ -372,3 +372,19 @@ def test_f32_index_compat_only_changes_large_lattices():
     a = orc.run(cfg, types, f, 3, f32_index_compat=False)
     b = orc.run(cfg, types, f, 3, f32_index_compat=True)
     assert np.array_equal(a, b)
+
+
+def test_force_restatement_closed_form():
+    """An isolated solid node in a uniform equilibrium flow: the links into it carry
+    2 sum_{i>0} c_i f_eq,i = 2 rho u."""
+    from np_reference import force, init_feq
+    for Q, shape in ((9, (1, 7, 9)), (27, (5, 6, 7))):
+        types = np.zeros(shape, np.uint8)
+        types[shape[0] // 2, shape[1] // 2, shape[2] // 2] = 1
+        u = np.array([0.03, -0.02, 0.01 if Q == 27 else 0.0])
+        rho = 1.1
+        F = init_feq(shape, dict(lattice=Q), np.full(shape, rho), np.broadcast_to(u[:, None, None, None], (3,) + shape))
+        got = force(F, types, dict(lattice=Q))
+        assert np.allclose(got, 2 * rho * u, rtol=0, atol=1e-15)
+        # nothing inside an empty box
+        assert np.all(force(F, types, dict(lattice=Q), box=(0, 1, 0, 1, 0, 1)) == 0)
