@@ -53,7 +53,7 @@ def workloads():
     return W
 
 
-def measure(name, steps, warmup):
+def measure(name, steps, warmup, repeats=0):
     from lbm_amd import Lattice
     from lbm_amd.capi import STORAGE_BYTES
 
@@ -68,6 +68,7 @@ def measure(name, steps, warmup):
             os.environ.pop("LBM_MULTISTEP", None)
         else:
             os.environ["LBM_MULTISTEP"] = saved
+    N_lat = cfg["nx"] * cfg["ny"] * cfg.get("nz", 1)
     with lat:
         k = lat.steps_per_launch
         # whole launches only, so the timed region holds steps // k launches of one kernel
@@ -79,6 +80,8 @@ def measure(name, steps, warmup):
         t0 = time.perf_counter()
         ms = lat.step_timed(steps)
         wall = time.perf_counter() - t0
+        # the same timed region repeated, for the spread (value is the first region above)
+        rep = [N_lat * steps / (lat.step_timed(steps) * 1e3) for _ in range(repeats)]
         _, _, bad = lat.monitors()
     N = lat.N
     Q = cfg["lattice"]
@@ -90,7 +93,7 @@ def measure(name, steps, warmup):
     return dict(cfg=cfg, N=N, Q=Q, steps=steps, warmup=warmup, ms=ms, wall_s=wall,
                 mlups=N * steps / (ms * 1e3), ms_per_step=ms / steps, steps_per_launch=k,
                 launch_us=launch_s * 1e6, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
-                nonfinite=int(bad))
+                repeat_mlups=rep, nonfinite=int(bad))
 
 
 def kernel_name(cfg, k):
@@ -129,9 +132,18 @@ def cpu_baseline(name, target_s=10.0):
                 break
     except OSError:
         pass
-    return {"value": N * n / (t * 1e6), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{name}: {n} steps of the CPU oracle (restated reference step, same boundary "
-                      f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
+    out = {"value": N * n / (t * 1e6), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"{name}: {n} steps of the CPU oracle (restated reference step, same boundary "
+                     f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
+    # the same step on one core, and in FP64 on all cores (about 3 s each)
+    n1 = max(2, int(3.0 / max(t / n * threads, 1e-6)))
+    t1 = orc.time_steps(cfg, types, n1, threads=1)
+    out["single_core_mlups"] = N * n1 / (t1 * 1e6)
+    cfg64 = dict(cfg, storage=0)
+    n64 = max(2, int(3.0 / max(2 * t / n, 1e-6)))
+    t64 = orc.time_steps(cfg64, types, n64, threads=threads)
+    out["fp64_mlups"] = N * n64 / (t64 * 1e6)
+    return out
 
 
 def _rounded(d):
@@ -154,6 +166,26 @@ def serving_leg(njobs, scale):
             "one_at_a_time": _rounded(seq),
             "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
             "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+
+
+def copy_bandwidth(mib=1024, reps=10):
+    """GB/s of a plain streaming copy (read + write) of mib MiB: the attainable HBM rate."""
+    import torch
+
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.neg(a, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * n * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
+MRT_FLOP_PER_LUP = 2 * (2 * 27 * 27)  # the two dense 27 x 27 transforms per node update
 
 
 def copy_calibration(mib):
@@ -196,7 +228,8 @@ def main():
               "multi-GPU path in this build.", file=sys.stderr)
         sys.exit(2)
 
-    r = measure(args.workload, args.steps, args.warmup)
+    r = measure(args.workload, args.steps, args.warmup, repeats=4)
+    copy_gbs = copy_bandwidth()
     also = {}
     names = [] if args.also == "none" else (
         [k for k in workloads() if k != args.workload] if args.also == "default" else args.also.split(","))
@@ -205,6 +238,9 @@ def main():
             m = measure(k, args.also_steps, 20)
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
                        "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
+                       **({"algorithmic_mrt_tflops": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 1e12, 2),
+                           "frac_fp32_matrix_peak": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 157.3e12, 4)}
+                          if m["cfg"].get("collision", 0) else {}),
                        "frac_hbm": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4), "steps": m["steps"],
                        "nonfinite": m["nonfinite"]}
         except Exception as e:  # noqa: BLE001 — report, keep the headline line
@@ -236,7 +272,10 @@ def main():
                      "traffic": traffic_for(args.workload),
                      "kernel": kernel_name(cfg, r["steps_per_launch"]),
                      "steps_per_launch": r["steps_per_launch"], "avg_launch_us": round(r["launch_us"], 3),
-                     "algorithmic_bytes_per_launch": r["bytes_per_launch"]},
+                     "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+                     "attainable_copy_gbs": round(copy_gbs, 1)},
+        "repeats": {"mlups": [round(r["mlups"], 1)] + [round(v, 1) for v in r["repeat_mlups"]],
+                    "median": round(sorted([r["mlups"]] + r["repeat_mlups"])[2], 1)},
         "cpu_baseline": cpu,
         "wall_s_timed": round(r["wall_s"], 4),
         "nonfinite": r["nonfinite"],
