@@ -29,16 +29,30 @@ struct MultiShape {
 // One tile of one lattice, K steps.  DB: two LDS copies of the region, sub-step s reads copy
 // (s - 1) & 1 and writes copy s & 1 (one barrier per sub-step); otherwise one copy, results staged
 // in registers between two barriers.  lf / lt are the workgroup's LDS arrays (MultiShape sizes).
+//
+// PHYSICAL boundaries with two LDS copies: the region's boundary (SLOW) nodes are listed at load
+// time and each sub-step updates the other nodes first, with no boundary code, then the listed
+// ones, one per lane.  A wavefront that held one inlet or outlet lane would otherwise run the whole
+// boundary path for all 64 lanes, and with one tile per compute unit the tiles at the inlet and
+// outlet (every wavefront of theirs holds such a lane) set the launch time.
+template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
+constexpr bool multi_uses_slow_list() { return DB && BC == BC_PHYS; }
+
 template <int ST, int BC, int TX, int K, int TY, int NT, bool DB, class P>
-__device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
-                                           typename Codec<ST, 9>::C *lf, uint8_t *lt) {
+__device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typename Codec<ST, 9>::C *lf,
+                                           uint8_t *lt, int *slow_list, int *nslow) {
   using S = MultiShape<ST, TX, K, TY, NT, DB>;
   constexpr int Q = S::Q, RX = S::RX, RY = S::RY, RN = S::RN, NPOS = S::NPOS;
+  constexpr bool LIST = multi_uses_slow_list<ST, BC, TX, K, TY, NT, DB>();
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   using C = typename CD::C;
 
   __syncthreads();  // a workgroup that walks several tiles: the previous tile's reads are done
+  if constexpr (LIST) {
+    if (threadIdx.x == 0) *nslow = 0;
+    __syncthreads();
+  }
   const int bx = tile % gxm, by = tile / gxm;
   const int x0 = bx * S::TX - K, y0 = by * TY - K;  // lattice coordinates of region position (0, 0)
   const T *fin = (const T *)p.fin;
@@ -59,14 +73,17 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
         constexpr int q = decltype(I)::value;
         lf[q * RN + pos] = CD::template ld<q>(fin + (size_t)q * (size_t)p.stride, n);
       });
-      lt[pos] = p.type[n];
+      const uint8_t t = p.type[n];
+      lt[pos] = t;
+      if constexpr (LIST)
+        if (t & NODE_SLOW) slow_list[atomicAdd(nslow, 1)] = pos;
     }
   }
   __syncthreads();
 
   // One node update of region position pos from the LDS copy src into out; false when the node
   // is not updated (PHYSICAL walls keep their populations).
-  auto update = [&](const C *src, int pos, int x, int y, C *out) -> bool {
+  auto update = [&](const C *src, int pos, int x, int y, C *out, auto with_fixups) -> bool {
     const uint8_t t = lt[pos];
     C f[Q], rho, ux, uy, uz;
     static_for<Q>([&](auto I) {
@@ -86,7 +103,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
       return true;
     } else {
       if (t & NODE_SOLID) return false;
-      if (t & NODE_SLOW)
+      if (decltype(with_fixups)::value && (t & NODE_SLOW))
         physical_fixups<Q>(
             p, x, y, 0, t, f,
             [&](auto I) -> uint8_t {
@@ -112,6 +129,18 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
   T *fout = (T *)p.fout;
   C res[DB ? 1 : NPOS][Q];
   bool upd[NPOS];
+  const int ns = LIST ? *nslow : 0;
+  // sub-step K updates exactly the interior: write the part inside the lattice
+  auto write_out = [&](int i, int j, const C *out) {
+    const int x = x0 + i, y = y0 + j;
+    if (x < p.nx && y < p.ny) {
+      const uint32_t n = (uint32_t)x + (uint32_t)p.nx * (uint32_t)y;
+      static_for<Q>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        CD::template st<q>(fout + (size_t)q * (size_t)p.stride, n, out[q]);
+      });
+    }
+  };
 #pragma unroll
   for (int s = 1; s <= K; ++s) {
     const C *src = lf + (DB ? ((s - 1) & 1) * Q * RN : 0);
@@ -120,21 +149,31 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
     for (int k = 0; k < NPOS; ++k) {
       const int pos = (int)threadIdx.x + k * NT;
       upd[k] = pos < RN && ri[k] >= s && ri[k] < RX - s && rj[k] >= s && rj[k] < RY - s;
+      if (LIST && upd[k] && (lt[pos] & NODE_SLOW)) upd[k] = false;  // the slow pass below
       if (!upd[k]) continue;
       C *out = res[DB ? 0 : k];
-      upd[k] = update(src, pos, gx[k], gy[k], out);
+      upd[k] = update(src, pos, gx[k], gy[k], out, std::integral_constant<bool, !LIST>{});
       if (!upd[k]) continue;
-      if (s == K) {  // sub-step K updates exactly the interior: write the part inside the lattice
-        const int x = x0 + ri[k], y = y0 + rj[k];
-        if (x < p.nx && y < p.ny) {
-          const uint32_t n = (uint32_t)x + (uint32_t)p.nx * (uint32_t)y;
-          static_for<Q>([&](auto I) {
-            constexpr int q = decltype(I)::value;
-            CD::template st<q>(fout + (size_t)q * (size_t)p.stride, n, out[q]);
-          });
-        }
+      if (s == K) {
+        write_out(ri[k], rj[k], out);
       } else if constexpr (DB) {
         static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
+      }
+    }
+    if constexpr (LIST) {
+      // dealt from thread 0 up (dealing from the last thread, which holds one position fewer
+      // when the region does not fill NPOS per thread, measured 10 % slower)
+      for (int e = (int)threadIdx.x; e < ns; e += NT) {
+        const int pos = slow_list[e];
+        const int i = pos % RX, j = pos / RX;
+        if (i < s || i >= RX - s || j < s || j >= RY - s) continue;
+        int x = (x0 + i) % p.nx, y = (y0 + j) % p.ny;
+        x = x < 0 ? x + p.nx : x;
+        y = y < 0 ? y + p.ny : y;
+        C out[Q];
+        update(src, pos, x, y, out, std::true_type{});  // SLOW nodes are never solid
+        if (s == K) write_out(i, j, out);
+        else static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
       }
     }
     if (s == K) break;
@@ -154,9 +193,12 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile,
 template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
 __global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int ntm) {
   using S = MultiShape<ST, TX, K, TY, NT, DB>;
+  constexpr bool LIST = multi_uses_slow_list<ST, BC, TX, K, TY, NT, DB>();
   __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
   __shared__ uint8_t lt[S::RN];
-  multi_tile<ST, BC, TX, K, TY, NT, DB>(p, gxm, xcd_tile((int)blockIdx.x, ntm), lf, lt);
+  __shared__ int slow_list[LIST ? S::RN : 1];
+  __shared__ int nslow;
+  multi_tile<ST, BC, TX, K, TY, NT, DB>(p, gxm, xcd_tile((int)blockIdx.x, ntm), lf, lt, slow_list, &nslow);
 }
 
 // Batched: every member of one D2Q9 BGK group, K steps, in the default launch shape.  Member m
@@ -166,11 +208,14 @@ constexpr bool BATCH_DB = true;
 template <int ST, int BC>
 __global__ void __launch_bounds__(BATCH_NT) k_step_multi_batch(BatchArgs b) {
   using S = MultiShape<ST, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>;
+  constexpr bool LIST = multi_uses_slow_list<ST, BC, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>();
   __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
   __shared__ uint8_t lt[S::RN];
+  __shared__ int slow_list[LIST ? S::RN : 1];
+  __shared__ int nslow;
   for_batch_tiles<true>(b, [&](const auto &p, int tile) {
     multi_tile<ST, BC, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>(p, (p.nx + BATCH_TX - 1) / BATCH_TX, tile,
-                                                                       lf, lt);
+                                                                       lf, lt, slow_list, &nslow);
   });
 }
 
