@@ -260,7 +260,8 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   } else if (t & NODE_OUTLET) {
     // zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi 2002), all i:
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
-    // post-collision populations; the node then holds rho_bc and u_nb exactly
+    // post-collision populations; the node then holds rho_bc and u_nb exactly.  f_eq is linear in
+    // rho, so this is evaluated as g_i + (rho_bc - rho_nb) f_eq,i(1, u_nb).
     C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
     const bool nb = x > 0 && !(west_type() & NODE_SOLID);
     if (nb) {
@@ -268,10 +269,16 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
       moments<Q>(g, rn, vx, vy, vz);
     }
     const C usq = usq_of<Q>(vx, vy, vz);
+    const C dr = rw - rn;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      const C fe = feq_i<Q, i>(rw, vx, vy, vz, usq);
-      f[i] = nb ? fe + (g[i] - feq_i<Q, i>(rn, vx, vy, vz, usq)) : fe;
+      if (nb) {
+        const C e = feq_i<Q, i>((C)1, vx, vy, vz, usq);
+        if constexpr (sizeof(C) == 4) f[i] = __builtin_fmaf(dr, e, g[i]);
+        else f[i] = g[i] + dr * e;
+      } else {
+        f[i] = feq_i<Q, i>(rw, vx, vy, vz, usq);
+      }
     });
   }
 }

@@ -468,6 +468,7 @@ struct Sim {
       // 11 (2002) 366): every population of the node is f_eq(rho_bc, u_nb) + (g - f_eq(rho_nb,
       // u_nb)), g = the upstream neighbour's populations as the pull reads them (its
       // post-collision state of the previous step).  The node then holds rho_bc and u_nb exactly.
+      // f_eq is linear in rho: evaluated as g + (rho_bc - rho_nb) f_eq(1, u_nb), FP32 as one FMA.
       T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
       const bool nb = x > 0 && !(type[n - 1] & SOLID);
       if (nb) {
@@ -475,9 +476,15 @@ struct Sim {
         moments(g, rn, vx, vy, vz);
       }
       const T usq = usq_term(vx, vy, vz);
+      const T dr = rw - rn;
       for (int i = 0; i < L.Q; ++i) {
-        const T fe = feq(i, rw, vx, vy, vz, usq);
-        f[i] = nb ? fe + (g[i] - feq(i, rn, vx, vy, vz, usq)) : fe;
+        if (nb) {
+          const T e = feq(i, (T)1, vx, vy, vz, usq);
+          if constexpr (std::is_same<T, float>::value) f[i] = std::fma(dr, e, g[i]);
+          else f[i] = g[i] + dr * e;
+        } else {
+          f[i] = feq(i, rw, vx, vy, vz, usq);
+        }
       }
     }
   }

@@ -176,7 +176,7 @@ def step_physical(F, types, cfg):
         um = np.where(nb_ok[None], um, 0.0)
         fe = feq(np.full(types.shape, rw), um, c, w)
         fn = feq(rm, um, c, w)
-        for i in range(Q):
+        for i in range(Q):  # f_eq(rho_bc, u) + (g - f_eq(rho_nb, u)), the definition
             new = np.where(nb_ok, fe[i] + (Fm[i] - fn[i]), fe[i])
             f[i] = np.where(outlet, new, f[i])
     rho = f.sum(0)
