@@ -179,6 +179,16 @@ __device__ __forceinline__ C inlet_velocity(const P &p, int D, int y, int z) {
   return u;
 }
 
+// c u for a lattice component c in {-1, 0, 1}: +-u, and for c = 0 an exact -0 (which the
+// compiler folds out of the sums below; 0 u would cost a multiply and add the same value up to the
+// sign of a zero).
+template <int c, class C>
+__device__ __forceinline__ C c_times(C u) {
+  if constexpr (c == 0) return (C)-0.0;
+  else if constexpr (c == 1) return u;
+  else return -u;
+}
+
 template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz);
 template <int Q, int i, class C>
@@ -193,12 +203,12 @@ __device__ __forceinline__ C usq_of(C ux, C uy, C uz);
 // (reduces to Zou-He's formulas on D2Q9).
 template <int Q, int SGN, class C>
 __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
-  C ny_ = 0, nz_ = 0;
+  C ny_ = (C)-0.0, nz_ = (C)-0.0;  // sums from -0 over the nonzero c_a terms, as in moments()
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     if constexpr (Lat<Q>::cx[i] == 0) {
-      ny_ += (C)Lat<Q>::cy[i] * f[i];
-      nz_ += (C)Lat<Q>::cz[i] * f[i];
+      if constexpr (Lat<Q>::cy[i] != 0) ny_ = ny_ + c_times<Lat<Q>::cy[i]>(f[i]);
+      if constexpr (Lat<Q>::cz[i] != 0) nz_ = nz_ + c_times<Lat<Q>::cz[i]>(f[i]);
     }
   });
   ny_ = ny_ - (((C)2 / (C)3) * rho) * uy;
@@ -209,7 +219,7 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
       constexpr int io = Lat<Q>::opp[i];
       const C cu = cdotu<Q, i>(ux, uy, uz);
       f[i] = (f[io] + (((C)6 * (C)Lat<Q>::w[i]) * rho) * cu) -
-             ((C)18 * (C)Lat<Q>::w[i]) * ((C)Lat<Q>::cy[i] * ny_ + (C)Lat<Q>::cz[i] * nz_);
+             ((C)18 * (C)Lat<Q>::w[i]) * (c_times<Lat<Q>::cy[i]>(ny_) + c_times<Lat<Q>::cz[i]>(nz_));
     }
   });
 }
@@ -319,10 +329,12 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
 
 // ---- moments, overrides, collisions ------------------------------------------------------
 // rho = sum_i f_i; u = (sum_i c_i f_i) / rho, ascending i (cpp/d2q9_bgk_lid.cpp:131-138); FP32
-// multiplies by the correctly rounded 1/rho instead of dividing three times.
+// multiplies by the correctly rounded 1/rho instead of dividing three times.  The sums start from
+// -0, which adds exactly (-0 + x == x for every x), so the compiler drops the first addition; the
+// c_i = 0 terms are left out (0 f_i adds an exact zero).  Both change at most the sign of a zero.
 template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz) {
-  C r = 0, jx = 0, jy = 0, jz = 0;
+  C r = (C)-0.0, jx = (C)-0.0, jy = (C)-0.0, jz = (C)-0.0;
   static_for<Q>([&](auto I) { r = r + f[decltype(I)::value]; });
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
@@ -369,8 +381,8 @@ __device__ __forceinline__ void overrides(const P &p, uint8_t t, int y, int z, C
 // c_i . u evaluated as the reference does: (cx ux + cy uy) + cz uz with exact +-1/0 products.
 template <int Q, int i, class C>
 __device__ __forceinline__ C cdotu(C ux, C uy, C uz) {
-  C e = (C)Lat<Q>::cx[i] * ux + (C)Lat<Q>::cy[i] * uy;
-  if constexpr (Lat<Q>::D == 3) e = e + (C)Lat<Q>::cz[i] * uz;
+  C e = c_times<Lat<Q>::cx[i]>(ux) + c_times<Lat<Q>::cy[i]>(uy);
+  if constexpr (Lat<Q>::D == 3) e = e + c_times<Lat<Q>::cz[i]>(uz);
   return e;
 }
 
@@ -381,7 +393,9 @@ __device__ __forceinline__ C cdotu(C ux, C uy, C uz) {
 template <int Q, int i, class C>
 __device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq) {
   const C eu = cdotu<Q, i>(ux, uy, uz);
-  if constexpr (sizeof(C) == 4) {
+  if constexpr (sizeof(C) == 4 && Lat<Q>::cx[i] == 0 && Lat<Q>::cy[i] == 0 && Lat<Q>::cz[i] == 0) {
+    return ((C)Lat<Q>::w[i] * rho) * usq;  // eu = 0: the FMAs below return A exactly
+  } else if constexpr (sizeof(C) == 4) {
     const C t = __builtin_fmaf(4.5f, eu, 3.0f);
     return ((C)Lat<Q>::w[i] * rho) * __builtin_fmaf(eu, t, usq);
   } else {

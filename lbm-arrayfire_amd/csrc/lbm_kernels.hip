@@ -149,9 +149,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, 
   if (!q.valid) return;
   C out[Q];
   if constexpr (CO == CO_BGK) {
+    const C usq = usq_of<Q>((C)0, (C)0, (C)0);  // the u-term of f_eq's storage type (FP32: A = 1)
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      out[i] = feq_i<Q, i>((C)rho0, (C)0, (C)0, (C)0, (C)0);
+      out[i] = feq_i<Q, i>((C)rho0, (C)0, (C)0, (C)0, usq);
     });
   } else {
     C f[Q];

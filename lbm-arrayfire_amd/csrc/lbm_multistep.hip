@@ -40,7 +40,7 @@ constexpr bool multi_uses_slow_list() { return DB && BC == BC_PHYS; }
 
 template <int ST, int BC, int TX, int K, int TY, int NT, bool DB, class P>
 __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typename Codec<ST, 9>::C *lf,
-                                           uint8_t *lt, int *slow_list, int *nslow) {
+                                           uint8_t *lt, uint32_t *lxy, int *slow_list, int *nslow) {
   using S = MultiShape<ST, TX, K, TY, NT, DB>;
   constexpr int Q = S::Q, RX = S::RX, RY = S::RY, RN = S::RN, NPOS = S::NPOS;
   constexpr bool LIST = multi_uses_slow_list<ST, BC, TX, K, TY, NT, DB>();
@@ -57,24 +57,23 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   const int x0 = bx * S::TX - K, y0 = by * TY - K;  // lattice coordinates of region position (0, 0)
   const T *fin = (const T *)p.fin;
 
-  // this thread's region positions and the lattice nodes they hold (periodic)
-  int ri[NPOS], rj[NPOS], gx[NPOS], gy[NPOS];
+  // load: every region position, its node type and its lattice coordinates (periodic, packed
+  // x | y << 16; the lattice sides are below 65536)
 #pragma unroll
   for (int k = 0; k < NPOS; ++k) {
     const int pos = (int)threadIdx.x + k * NT;
-    ri[k] = pos % RX;
-    rj[k] = pos / RX;
-    int x = (x0 + ri[k]) % p.nx, y = (y0 + rj[k]) % p.ny;
-    gx[k] = x < 0 ? x + p.nx : x;
-    gy[k] = y < 0 ? y + p.ny : y;
     if (pos < RN) {
-      const uint32_t n = (uint32_t)gx[k] + (uint32_t)p.nx * (uint32_t)gy[k];
+      int x = (x0 + pos % RX) % p.nx, y = (y0 + pos / RX) % p.ny;
+      x = x < 0 ? x + p.nx : x;
+      y = y < 0 ? y + p.ny : y;
+      const uint32_t n = (uint32_t)x + (uint32_t)p.nx * (uint32_t)y;
       static_for<Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
         lf[q * RN + pos] = CD::template ld<q>(fin + (size_t)q * (size_t)p.stride, n);
       });
       const uint8_t t = p.type[n];
       lt[pos] = t;
+      lxy[pos] = (uint32_t)x | ((uint32_t)y << 16);
       if constexpr (LIST)
         if (t & NODE_SLOW) slow_list[atomicAdd(nslow, 1)] = pos;
     }
@@ -83,7 +82,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
 
   // One node update of region position pos from the LDS copy src into out; false when the node
   // is not updated (PHYSICAL walls keep their populations).
-  auto update = [&](const C *src, int pos, int x, int y, C *out, auto with_fixups) -> bool {
+  auto update = [&](const C *src, int pos, C *out, auto with_fixups) -> bool {
     const uint8_t t = lt[pos];
     C f[Q], rho, ux, uy, uz;
     static_for<Q>([&](auto I) {
@@ -92,7 +91,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     });
     if constexpr (BC == BC_REF) {
       moments<Q>(f, rho, ux, uy, uz);
-      overrides<Q>(p, t, y, 0, rho, ux, uy, uz);
+      overrides<Q>(p, t, (t & NODE_INLET) ? (int)(lxy[pos] >> 16) : 0, 0, rho, ux, uy, uz);
       collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
       if (t & NODE_SOLID) {  // full-way bounce-back: out[opp(j)] = f[j], j >= 1
         static_for<Q - 1>([&](auto J) {
@@ -103,9 +102,10 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       return true;
     } else {
       if (t & NODE_SOLID) return false;
-      if (decltype(with_fixups)::value && (t & NODE_SLOW))
+      if (decltype(with_fixups)::value && (t & NODE_SLOW)) {
+        const uint32_t xy = lxy[pos];
         physical_fixups<Q>(
-            p, x, y, 0, t, f,
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
               return lt[pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
@@ -113,6 +113,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos]; },
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos - 1]; },
             [&]() -> uint8_t { return lt[pos - 1]; });
+      }
       moments<Q>(f, rho, ux, uy, uz);
       collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
       return true;
@@ -125,10 +126,11 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     CD::template st<q>(&w, 0, v);
     return CD::template ld<q>(&w, 0);
   };
+  auto keep = [&](C *dst, int pos, const C *out) {
+    static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
+  };
 
   T *fout = (T *)p.fout;
-  C res[DB ? 1 : NPOS][Q];
-  bool upd[NPOS];
   const int ns = LIST ? *nslow : 0;
   // sub-step K updates exactly the interior: write the part inside the lattice
   auto write_out = [&](int i, int j, const C *out) {
@@ -141,53 +143,57 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       });
     }
   };
-#pragma unroll
-  for (int s = 1; s <= K; ++s) {
+  // Sub-step s updates the window [s, RX - s) x [s, RY - s), its positions dealt to consecutive
+  // threads (row-major), so a sub-step runs ceil(window / 64) wavefronts and the last one writes
+  // whole tile rows.
+  static_for<K>([&](auto SI) {
+    constexpr int s = decltype(SI)::value + 1;
+    constexpr int WX = RX - 2 * s, W = WX * (RY - 2 * s), NW = (W + NT - 1) / NT;
     const C *src = lf + (DB ? ((s - 1) & 1) * Q * RN : 0);
     C *dst = lf + (DB ? (s & 1) * Q * RN : 0);
-#pragma unroll
-    for (int k = 0; k < NPOS; ++k) {
-      const int pos = (int)threadIdx.x + k * NT;
-      upd[k] = pos < RN && ri[k] >= s && ri[k] < RX - s && rj[k] >= s && rj[k] < RY - s;
-      if (LIST && upd[k] && (lt[pos] & NODE_SLOW)) upd[k] = false;  // the slow pass below
-      if (!upd[k]) continue;
-      C *out = res[DB ? 0 : k];
-      upd[k] = update(src, pos, gx[k], gy[k], out, std::integral_constant<bool, !LIST>{});
-      if (!upd[k]) continue;
-      if (s == K) {
-        write_out(ri[k], rj[k], out);
-      } else if constexpr (DB) {
-        static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
-      }
-    }
     if constexpr (LIST) {
-      // dealt from thread 0 up (dealing from the last thread, which holds one position fewer
-      // when the region does not fill NPOS per thread, measured 10 % slower)
+      // boundary nodes first, dealt from thread 0 up: their long dependent chains then run on the
+      // oldest wavefronts while the others update the plain nodes (positions of one sub-step are
+      // independent)
       for (int e = (int)threadIdx.x; e < ns; e += NT) {
         const int pos = slow_list[e];
         const int i = pos % RX, j = pos / RX;
         if (i < s || i >= RX - s || j < s || j >= RY - s) continue;
-        int x = (x0 + i) % p.nx, y = (y0 + j) % p.ny;
-        x = x < 0 ? x + p.nx : x;
-        y = y < 0 ? y + p.ny : y;
         C out[Q];
-        update(src, pos, x, y, out, std::true_type{});  // SLOW nodes are never solid
-        if (s == K) write_out(i, j, out);
-        else static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
+        update(src, pos, out, std::true_type{});  // SLOW nodes are never solid
+        if constexpr (s == K) write_out(i, j, out);
+        else keep(dst, pos, out);
       }
     }
-    if (s == K) break;
-    __syncthreads();  // DB: the copy written is complete; otherwise: every read of sub-step s is done
-    if constexpr (!DB) {
-#pragma unroll
-      for (int k = 0; k < NPOS; ++k) {
-        if (!upd[k]) continue;
-        const int pos = (int)threadIdx.x + k * NT;
-        static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, res[k][decltype(I)::value]); });
+    C res[DB ? 1 : NW][Q];
+    int wpos[NW];
+    bool upd[NW];
+    static_for<NW>([&](auto KI) {
+      constexpr int k = decltype(KI)::value;
+      const unsigned e = threadIdx.x + (unsigned)(k * NT);
+      const int i = s + (int)(e % (unsigned)WX), j = s + (int)(e / (unsigned)WX);
+      const int pos = i + j * RX;
+      wpos[k] = pos;
+      upd[k] = (W % NT == 0 || k < NW - 1 || e < (unsigned)W);
+      if (LIST && upd[k] && (lt[pos] & NODE_SLOW)) upd[k] = false;  // done in the pass above
+      if (!upd[k]) return;
+      C *out = res[DB ? 0 : k];
+      upd[k] = update(src, pos, out, std::integral_constant<bool, !LIST>{});
+      if (!upd[k]) return;
+      if constexpr (s == K) write_out(i, j, out);
+      else if constexpr (DB) keep(dst, pos, out);
+    });
+    if constexpr (s < K) {
+      __syncthreads();  // DB: the copy written is complete; otherwise: every read of sub-step s is done
+      if constexpr (!DB) {
+        static_for<NW>([&](auto KI) {
+          constexpr int k = decltype(KI)::value;
+          if (upd[k]) keep(dst, wpos[k], res[k]);
+        });
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
+  });
 }
 
 template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
@@ -196,9 +202,10 @@ __global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int nt
   constexpr bool LIST = multi_uses_slow_list<ST, BC, TX, K, TY, NT, DB>();
   __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
   __shared__ uint8_t lt[S::RN];
+  __shared__ uint32_t lxy[S::RN];
   __shared__ int slow_list[LIST ? S::RN : 1];
   __shared__ int nslow;
-  multi_tile<ST, BC, TX, K, TY, NT, DB>(p, gxm, xcd_tile((int)blockIdx.x, ntm), lf, lt, slow_list, &nslow);
+  multi_tile<ST, BC, TX, K, TY, NT, DB>(p, gxm, xcd_tile((int)blockIdx.x, ntm), lf, lt, lxy, slow_list, &nslow);
 }
 
 // Batched: every member of one D2Q9 BGK group, K steps, in the default launch shape.  Member m
@@ -211,11 +218,12 @@ __global__ void __launch_bounds__(BATCH_NT) k_step_multi_batch(BatchArgs b) {
   constexpr bool LIST = multi_uses_slow_list<ST, BC, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>();
   __shared__ typename Codec<ST, 9>::C lf[S::NBUF * S::Q * S::RN];
   __shared__ uint8_t lt[S::RN];
+  __shared__ uint32_t lxy[S::RN];
   __shared__ int slow_list[LIST ? S::RN : 1];
   __shared__ int nslow;
   for_batch_tiles<true>(b, [&](const auto &p, int tile) {
     multi_tile<ST, BC, BATCH_TX, BATCH_K, BATCH_TY, BATCH_NT, BATCH_DB>(p, (p.nx + BATCH_TX - 1) / BATCH_TX, tile,
-                                                                       lf, lt, slow_list, &nslow);
+                                                                       lf, lt, lxy, slow_list, &nslow);
   });
 }
 
@@ -279,7 +287,7 @@ hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid
 }
 
 int multi_steps_per_launch(const Config &c, int variant) {
-  if (c.Q != 9 || c.collision != CO_BGK || c.storage == ST_F64) return 0;
+  if (c.Q != 9 || c.collision != CO_BGK || c.storage == ST_F64) return 0;  // and nx, ny < 65536
   if (variant < 0 || variant >= multi_variant_count()) return 0;
   return kMultiVariants[variant].K;
 }

@@ -207,6 +207,29 @@ def test_mrt_reference_init_matches_oracle():
     assert np.array_equal(g, orc.init_reference(cfg, types, 1.0))
 
 
+@pytest.mark.parametrize("storage", [F64, F32, F16S, BF16S])
+@pytest.mark.parametrize("q", [9, 27])
+def test_bgk_reference_init_matches_oracle(q, storage):
+    """f = f_eq(rho0, 0) = w_i rho0 in every storage (exact: the shifted codecs store 0 for it),
+    and a few steps from it stay on the oracle's trajectory."""
+    if q == 9:
+        cfg, types = cases.d2q9_lid_reference(24, storage=storage)
+    else:
+        cfg, types = cases.d3q27_bgk_lid_reference(12, u_lid=0.05, re=50.0, storage=storage)
+    types = types.reshape(-1)
+    want = orc.init_reference(cfg, types, 1.0)
+    with Lattice(cfg, types) as lat:
+        lat.init_reference(1.0)
+        got = lat.populations()
+        assert np.array_equal(got, want)
+        lat.step(5)
+        got = lat.populations()
+    ref = orc.run(cfg, types, want, 5)
+    assert np.all(np.isfinite(got))
+    tol = {F64: 1e-12, F32: 1e-6, F16S: 1e-3, BF16S: 1e-2}[storage]
+    assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
+
+
 MFMA_CASES = [
     ("d3q27_mrt_raw_lid_ref_20", lambda st, en: cases.d3q27_mrt_lid_reference(20, storage=st, engine=en), 60),
     ("d3q27_mrt_gs_lid_ref_20", lambda st, en: cases.d3q27_mrt_lid_gs(20, re=100.0, storage=st, engine=en), 60),
