@@ -158,6 +158,12 @@ int32_t lbm_steps_per_launch(lbm_handle h);
 lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms);
 /* Waits for the handle's stream.  Replaces af::sync() (cpp/d2q9_bgk_lid.cpp:118,182). */
 lbm_status lbm_sync(lbm_handle h);
+/* Device self-check, no reference counterpart: compares the FP32 kernels' reciprocal (hardware
+ * estimate + one Newton step, used for 1/rho) with IEEE division for every float bit pattern in
+ * [lo, hi) on the current device and stores the number of differing results in *mismatches.
+ * Zero over the normal floats with a normal reciprocal is the identity FP32 bit-exactness rests
+ * on. */
+lbm_status lbm_check_reciprocal(uint32_t lo, uint32_t hi, uint64_t *mismatches);
 int64_t lbm_steps_done(lbm_handle h);
 
 /* Macroscopic fields the reference holds after a step (DENSITY/UX/UY[/UZ] of

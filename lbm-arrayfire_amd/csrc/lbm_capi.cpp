@@ -142,6 +142,22 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
     p.s24[k] = (float)(d->s[k] / 24.0);
   }
 
+  // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an
+  // inlet and every node of column nx - 1 solid or an outlet (the inlet replaces the populations it
+  // pulls across the wrap, the outlet takes all of its populations from its west neighbour); y:
+  // rows 0 and ny - 1 are solid (nodes pull nothing from beyond a wall).  Multi-step tiles hold the
+  // halo positions across such a wrap instead of updating them.
+  if (d->bc_mode == LBM_BC_PHYSICAL && d->nz == 1) {
+    bool xcut = d->nx >= 2, ycut = d->ny >= 2;
+    for (int y = 0; y < d->ny && xcut; ++y) {
+      const uint8_t a = node_type[(size_t)y * d->nx], b = node_type[(size_t)y * d->nx + d->nx - 1];
+      xcut = (a & (LBM_NODE_SOLID | LBM_NODE_INLET)) && (b & (LBM_NODE_SOLID | LBM_NODE_OUTLET));
+    }
+    for (int x = 0; x < d->nx && ycut; ++x)
+      ycut = (node_type[x] & LBM_NODE_SOLID) && (node_type[(size_t)(d->ny - 1) * d->nx + x] & LBM_NODE_SOLID);
+    p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
+  }
+
   // Multi-step launches for the lattices that qualify: shape 0 unless LBM_MULTISTEP names another
   // shape index, or "off" (single steps only).
   if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
@@ -345,6 +361,21 @@ lbm_status lbm_sync(lbm_handle h) {
 }
 
 int64_t lbm_steps_done(lbm_handle h) { return h ? h->steps : -1; }
+
+lbm_status lbm_check_reciprocal(uint32_t lo, uint32_t hi, uint64_t *mismatches) {
+  if (!mismatches) return fail(LBM_EINVAL, "mismatches is NULL");
+  HIPCHK(hipSetDevice(g_device));
+  unsigned long long *d = nullptr;
+  HIPCHK(hipMalloc((void **)&d, sizeof(*d)));
+  unsigned long long n = 0;
+  hipError_t e = hipMemcpy(d, &n, sizeof(n), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_check_reciprocal(lo, hi, d, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&n, d, sizeof(n), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(LBM_EDEVICE, std::string("reciprocal check failed: ") + hipGetErrorString(e));
+  *mismatches = (uint64_t)n;
+  return LBM_OK;
+}
 
 int32_t lbm_steps_per_launch(lbm_handle h) { return h ? (h->multi_k > 1 ? h->multi_k : 1) : -1; }
 

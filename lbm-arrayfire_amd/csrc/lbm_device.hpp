@@ -234,32 +234,40 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   src_type(IC<i>)  type of the node this node pulls population i from (node - c_i)
 //   own(IC<j>)       population j of this node in the state being advanced
 //   west(IC<j>)      population j of the node at x - 1;  west_type()  its type
+// Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
+// 6 w_i rho_w c_i.u_w when the source is a moving wall.
+template <int Q, class C, class SrcType, class Own, class P>
+__device__ __forceinline__ void wall_links(const P &p, C *f, SrcType &&src_type, Own &&own) {
+  const C rw = (C)p.rho_bc;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+    constexpr int io = Lat<Q>::opp[i];
+    if constexpr (i != 0) {
+      const uint8_t ts = src_type(I);
+      if (ts & NODE_SOLID) {
+        C v = own(IC<io>{});
+        if (ts & NODE_MOVING) {
+          C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
+          if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
+          v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+        }
+        f[i] = v;
+      }
+    }
+  });
+}
+
 template <int Q, class C, class SrcType, class Own, class West, class WestType, class P>
 __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type) {
   const C rw = (C)p.rho_bc;
   if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
-    static_for<Q>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
-      constexpr int io = Lat<Q>::opp[i];
-      if constexpr (i != 0) {
-        const uint8_t ts = src_type(I);
-        if (ts & NODE_SOLID) {
-          C v = own(IC<io>{});
-          if (ts & NODE_MOVING) {
-            C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
-            if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
-            v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
-          }
-          f[i] = v;
-        }
-      }
-    });
+    wall_links<Q>(p, f, src_type, own);
   if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
-    C s0 = 0, sm = 0;
+    C s0 = (C)-0.0, sm = (C)-0.0;  // sums from -0, as in moments()
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
@@ -328,6 +336,17 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
 }
 
 // ---- moments, overrides, collisions ------------------------------------------------------
+// 1/x rounded to nearest for normal x with a normal reciprocal (2^-126 <= |x| < 2^125): the
+// hardware reciprocal (error below one ulp) refined by one Newton step y + y (1 - x y), both FMAs
+// exact in their products.  Equal bit for bit to the IEEE division 1.0f / x for every such x
+// (lbm_check_reciprocal runs the exhaustive comparison on the device); 3 instructions where the
+// division takes 10.  Outside that range (a density of 0, inf or NaN: a lattice that has already
+// blown up) it may return NaN where the division returns 0 or inf.
+__device__ __forceinline__ float recip_rn(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, y, 1.0f), y, y);
+}
+
 // rho = sum_i f_i; u = (sum_i c_i f_i) / rho, ascending i (cpp/d2q9_bgk_lid.cpp:131-138); FP32
 // multiplies by the correctly rounded 1/rho instead of dividing three times.  The sums start from
 // -0, which adds exactly (-0 + x == x for every x), so the compiler drops the first addition; the
@@ -354,8 +373,8 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
     });
   }
   rho = r;
-  if constexpr (sizeof(C) == 4) {  // FP32: one IEEE division, then products
-    const C inv = (C)1 / r;
+  if constexpr (sizeof(C) == 4) {  // FP32: one correctly rounded reciprocal, then products
+    const C inv = recip_rn(r);
     ux = jx * inv;
     uy = jy * inv;
     uz = (Lat<Q>::D == 3) ? jz * inv : (C)0;

@@ -17,7 +17,7 @@ struct StepParams {
   int32_t ntiles;
   int32_t inlet_profile;
   int32_t meq_mode;
-  int32_t pad_;
+  int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
   int64_t N;                // nodes
   int64_t stride;           // elements between consecutive population arrays (>= N)
   const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
@@ -119,6 +119,8 @@ hipError_t launch_init_reference(const Config &c, const StepParams &p, double rh
 // into out[3]; partials as for launch_monitors.
 hipError_t launch_force(const Config &c, const StepParams &p, const int *box, double *partials, double *out,
                         hipStream_t s);
+// Adds to *count the x in [lo, hi) (float bit patterns) whose recip_rn(x) differs from 1.0f / x.
+hipError_t launch_check_reciprocal(uint32_t lo, uint32_t hi, unsigned long long *count, hipStream_t s);
 // Per-tile partial sums (mass, kinetic energy, nonfinite) then a final single-block reduction
 // into out[3].  partials must hold 3 * p.ntiles doubles.
 hipError_t launch_monitors(const Config &c, const StepParams &p, double *partials, double *out,

@@ -1,6 +1,8 @@
 // lbm_kernels.hip — node-per-thread kernels: the fused step, field extraction, initial states
 // and the mass / kinetic-energy monitors, plus the host-side dispatch over
 // (lattice, storage, boundary mode, collision).
+#include <algorithm>
+
 #include "lbm_device.hpp"
 
 namespace lbm {
@@ -434,6 +436,25 @@ hipError_t launch_force(const Config &c, const StepParams &p, const int *box, do
   });
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK_THREADS), 0, s, partials, p.ntiles, out);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(BLOCK_THREADS) k_check_reciprocal(uint32_t lo, uint32_t hi,
+                                                                    unsigned long long *count) {
+  unsigned long long bad = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK_THREADS;
+  for (uint64_t b = (uint64_t)lo + blockIdx.x * BLOCK_THREADS + threadIdx.x; b < hi; b += stride) {
+    const float x = __uint_as_float((uint32_t)b);
+    bad += __float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x);
+  }
+  if (bad) atomicAdd(count, bad);
+}
+
+hipError_t launch_check_reciprocal(uint32_t lo, uint32_t hi, unsigned long long *count, hipStream_t s) {
+  if (hi <= lo) return hipSuccess;
+  const uint64_t n = (uint64_t)hi - lo, per_block = (uint64_t)BLOCK_THREADS * 64;
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + per_block - 1) / per_block, 65536);
+  hipLaunchKernelGGL(k_check_reciprocal, dim3(grid), dim3(BLOCK_THREADS), 0, s, lo, hi, count);
   return hipGetLastError();
 }
 

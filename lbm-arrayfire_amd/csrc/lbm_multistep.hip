@@ -10,6 +10,10 @@
 // rounded through the storage codec exactly as a store and a load would round it, so K sub-steps
 // reproduce K single steps bit for bit.
 //
+// PHYSICAL lattices whose periodic wrap carries nothing (x: inlet / solid column 0 and outlet /
+// solid column nx - 1; y: solid rows 0 and ny - 1) hold the halo positions beyond that wrap as
+// walls: their values never reach the interior, so updating them would be wasted work.
+//
 // HBM traffic per launch: the region once in (halo factor (TX + 2K)(TY + 2K) / (TX TY)) and the
 // interior once out, for K steps, against 2 Q s bytes per node for each single step.  The D2Q9
 // 1024 x 256 lattice (18.9 MB per step) is latency-bound as single steps; this kernel trades the
@@ -71,7 +75,12 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
         constexpr int q = decltype(I)::value;
         lf[q * RN + pos] = CD::template ld<q>(fin + (size_t)q * (size_t)p.stride, n);
       });
-      const uint8_t t = p.type[n];
+      uint8_t t = p.type[n];
+      if constexpr (BC == BC_PHYS) {  // held across a wrap that carries nothing (never updated)
+        const int ux = x0 + pos % RX, uy = y0 + pos / RX;  // unwrapped
+        if (((p.wrap_cut & 1) && (ux < 0 || ux >= p.nx)) || ((p.wrap_cut & 2) && (uy < 0 || uy >= p.ny)))
+          t = NODE_SOLID;
+      }
       lt[pos] = t;
       lxy[pos] = (uint32_t)x | ((uint32_t)y << 16);
       if constexpr (LIST)

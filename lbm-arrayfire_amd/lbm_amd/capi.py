@@ -30,7 +30,8 @@ STORAGE_BYTES = {F64: 8, F32: 4, F16S: 2, BF16S: 2}
 EXPORTS = [
     "lbm_abi_version", "lbm_set_device", "lbm_last_error", "lbm_footprint_bytes", "lbm_create",
     "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
-    "lbm_step_timed", "lbm_sync", "lbm_steps_done", "lbm_steps_per_launch", "lbm_get_fields",
+    "lbm_step_timed", "lbm_sync", "lbm_check_reciprocal", "lbm_steps_done", "lbm_steps_per_launch",
+    "lbm_get_fields",
     "lbm_get_fields_at",
     "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
@@ -134,6 +135,7 @@ def load(path=None):
     L.lbm_sync.argtypes = [H]
     L.lbm_steps_done.argtypes = [H]
     L.lbm_steps_done.restype = ctypes.c_int64
+    L.lbm_check_reciprocal.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
     L.lbm_steps_per_launch.argtypes = [H]
     L.lbm_steps_per_launch.restype = ctypes.c_int32
     L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
@@ -180,6 +182,13 @@ def _dp(a):
 def check(status):
     if status != 0:
         raise LbmError(status, load().lbm_last_error().decode())
+
+
+def check_reciprocal(lo, hi):
+    """Float bit patterns x in [lo, hi) whose FP32-kernel reciprocal differs from 1.0f / x."""
+    n = ctypes.c_uint64(0)
+    check(load().lbm_check_reciprocal(lo, hi, ctypes.byref(n)))
+    return n.value
 
 
 def footprint_bytes(cfg):

@@ -207,6 +207,14 @@ def test_mrt_reference_init_matches_oracle():
     assert np.array_equal(g, orc.init_reference(cfg, types, 1.0))
 
 
+@pytest.mark.parametrize("sign", [0, 0x80000000])
+def test_fp32_reciprocal_equals_division_exhaustively(sign):
+    """The FP32 kernels take 1/rho as the hardware reciprocal plus one Newton step; over every
+    normal float with a normal reciprocal (2^-126 <= |x| < 2^125) it equals IEEE 1.0f / x."""
+    from lbm_amd.capi import check_reciprocal
+    assert check_reciprocal(sign | 0x00800000, sign | 0x7E000000) == 0
+
+
 @pytest.mark.parametrize("storage", [F64, F32, F16S, BF16S])
 @pytest.mark.parametrize("q", [9, 27])
 def test_bgk_reference_init_matches_oracle(q, storage):
