@@ -141,6 +141,8 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
 
   T *fout = (T *)p.fout;
   const int ns = LIST ? *nslow : 0;
+  // rows dealt top-down when the held rows are at the region's end, so that they come first
+  const bool flip = BC == BC_PHYS && (p.wrap_cut & 2) && y0 + RY > p.ny;
   // sub-step K updates exactly the interior: write the part inside the lattice
   auto write_out = [&](int i, int j, const C *out) {
     const int x = x0 + i, y = y0 + j;
@@ -180,7 +182,8 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     static_for<NW>([&](auto KI) {
       constexpr int k = decltype(KI)::value;
       const unsigned e = threadIdx.x + (unsigned)(k * NT);
-      const int i = s + (int)(e % (unsigned)WX), j = s + (int)(e / (unsigned)WX);
+      const int i = s + (int)(e % (unsigned)WX);
+      const int j = flip ? RY - 1 - s - (int)(e / (unsigned)WX) : s + (int)(e / (unsigned)WX);
       const int pos = i + j * RX;
       wpos[k] = pos;
       upd[k] = (W % NT == 0 || k < NW - 1 || e < (unsigned)W);
