@@ -88,6 +88,29 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     }
   }
   __syncthreads();
+  const int ns = LIST ? *nslow : 0;
+  if constexpr (LIST) {
+    // Each thread annotates the list entries it updates in every sub-step (so no barrier): bits
+    // 0-10 the position, 11-18 solid pull sources i = 1..8 (bit 10 + i), 19-26 moving-wall ones
+    // (bit 18 + i), 27 the west neighbour is solid, 28 inlet, 29 outlet, 30 wall neighbour.
+    static_assert(RN <= 2048, "list entries hold the position in 11 bits");
+    for (int e = (int)threadIdx.x; e < ns; e += NT) {
+      const int pos = slow_list[e];
+      const uint8_t t = lt[pos];
+      const int i = pos % RX, j = pos / RX;
+      uint32_t w = (uint32_t)pos | ((t & NODE_INLET) ? 1u << 28 : 0u) | ((t & NODE_OUTLET) ? 1u << 29 : 0u) |
+                   ((t & NODE_WALLNB) ? 1u << 30 : 0u);
+      if (i >= 1 && i < RX - 1 && j >= 1 && j < RY - 1) {  // edge positions are never updated
+        static_for<Q - 1>([&](auto I) {
+          constexpr int q = decltype(I)::value + 1;
+          const uint8_t ts = lt[pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
+          if (ts & NODE_SOLID) w |= (1u << (10 + q)) | ((ts & NODE_MOVING) ? 1u << (18 + q) : 0u);
+        });
+        if (lt[pos - 1] & NODE_SOLID) w |= 1u << 27;
+      }
+      slow_list[e] = (int)w;
+    }
+  }
 
   // One node update of region position pos from the LDS copy src into out; false when the node
   // is not updated (PHYSICAL walls keep their populations).
@@ -140,7 +163,6 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   };
 
   T *fout = (T *)p.fout;
-  const int ns = LIST ? *nslow : 0;
   // rows dealt top-down when the held rows are at the region's end, so that they come first
   const bool flip = BC == BC_PHYS && (p.wrap_cut & 2) && y0 + RY > p.ny;
   // sub-step K updates exactly the interior: write the part inside the lattice
@@ -167,11 +189,31 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       // oldest wavefronts while the others update the plain nodes (positions of one sub-step are
       // independent)
       for (int e = (int)threadIdx.x; e < ns; e += NT) {
-        const int pos = slow_list[e];
+        const uint32_t w = (uint32_t)slow_list[e];
+        const int pos = (int)(w & 0x7ffu);
         const int i = pos % RX, j = pos / RX;
         if (i < s || i >= RX - s || j < s || j >= RY - s) continue;
-        C out[Q];
-        update(src, pos, out, std::true_type{});  // SLOW nodes are never solid
+        // SLOW nodes are never solid; every LDS read below is issued before the first use
+        C f[Q], own[Q], out[Q], rho, ux, uy, uz;
+        static_for<Q>([&](auto I) {
+          constexpr int q = decltype(I)::value;
+          f[q] = src[q * RN + pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
+          own[q] = src[q * RN + pos];
+        });
+        const uint32_t xy = lxy[pos];
+        const uint8_t t = (uint8_t)(((w >> 28) & 1u ? NODE_INLET : 0) | ((w >> 29) & 1u ? NODE_OUTLET : 0) |
+                                    ((w >> 30) & 1u ? NODE_WALLNB : 0));
+        physical_fixups<Q>(
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
+            [&](auto I) -> uint8_t {
+              constexpr int q = decltype(I)::value;
+              return (uint8_t)(((w >> (10 + q)) & 1u ? NODE_SOLID : 0) | ((w >> (18 + q)) & 1u ? NODE_MOVING : 0));
+            },
+            [&](auto J) -> C { return own[decltype(J)::value]; },
+            [&](auto J) -> C { return src[decltype(J)::value * RN + pos - 1]; },
+            [&]() -> uint8_t { return (w >> 27) & 1u ? NODE_SOLID : 0; });
+        moments<Q>(f, rho, ux, uy, uz);
+        collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
         if constexpr (s == K) write_out(i, j, out);
         else keep(dst, pos, out);
       }
