@@ -157,14 +157,14 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     StepParams *par0 = (StepParams *)(g.h_tab + g.par_off[0]);
     StepParams *par1 = (StepParams *)(g.h_tab + g.par_off[1]);
     // multi-step launches when every member uses the default multi-step shape
-    g.multi_k = multi_steps_per_launch(g.cfg, 0);
+    g.multi_k = multi_steps_per_launch(g.cfg, MULTI_DEFAULT);
     for (lbm_ctx *m : g.members)
-      if (m->multi_variant != 0 || m->d.nz != 1) g.multi_k = 0;
+      if (m->multi_variant != MULTI_DEFAULT || m->d.nz != 1) g.multi_k = 0;
     prefix[0] = pmulti[0] = 0;
     for (int i = 0; i < n; ++i) {
       const lbm_ctx *m = g.members[(size_t)i];
       prefix[i + 1] = prefix[i] + m->base.ntiles;
-      pmulti[i + 1] = pmulti[i] + (g.multi_k ? multi_tiles(0, m->d.nx, m->d.ny) : 0);
+      pmulti[i + 1] = pmulti[i] + (g.multi_k ? multi_tiles(MULTI_DEFAULT, m->d.nx, m->d.ny) : 0);
       par0[i] = m->params(m->cur);
       par1[i] = m->params(m->cur ^ 1);
     }

@@ -79,10 +79,11 @@ hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int g
 // multi_steps_per_launch = the steps one launch of the given shape advances, 0 if c or the shape
 // does not qualify; the launch reads p.fin and leaves the state K steps later in p.fout.
 int multi_variant_count();
+constexpr int MULTI_DEFAULT = 9;  // the default launch shape (64 x 16 tiles, 5 steps), also batched
 int multi_steps_per_launch(const Config &c, int variant);
 hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, hipStream_t s);
-// Multi-step tiles of a nx x ny lattice in the given shape; batched multi-step launches (shape 0,
-// one workgroup per multi-step tile of the group by default).
+// Multi-step tiles of a nx x ny lattice in the given shape; batched multi-step launches (shape
+// MULTI_DEFAULT, one workgroup per multi-step tile of the group by default).
 int multi_tiles(int variant, int nx, int ny);
 hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 // True when the step of this configuration runs on the MFMA MRT kernel.

@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(NT) k_step_multi(StepParams p, int gxm, int nt
 
 // Batched: every member of one D2Q9 BGK group, K steps, in the default launch shape.  Member m
 // owns multi-step tiles [prefix_multi[m], prefix_multi[m+1]).
-constexpr int BATCH_TX = 64, BATCH_K = 4, BATCH_TY = 16, BATCH_NT = 1024;
+constexpr int BATCH_TX = 64, BATCH_K = 5, BATCH_TY = 16, BATCH_NT = 1024;
 constexpr bool BATCH_DB = true;
 template <int ST, int BC>
 __global__ void __launch_bounds__(BATCH_NT) k_step_multi_batch(BatchArgs b) {
@@ -291,7 +291,8 @@ struct MultiVariant {
 #define LBM_MULTI_SHAPES(X)                                                                         \
   X(0, 64, 4, 16, 1024, true) X(1, 64, 3, 16, 1024, true) X(2, 32, 4, 32, 1024, true)              \
   X(3, 64, 4, 16, 1024, false) X(4, 64, 4, 8, 1024, false) X(5, 64, 4, 8, 512, false)              \
-  X(6, 32, 4, 16, 1024, false) X(7, 32, 4, 16, 512, false) X(8, 64, 2, 8, 1024, true)
+  X(6, 32, 4, 16, 1024, false) X(7, 32, 4, 16, 512, false) X(8, 64, 2, 8, 1024, true)               \
+  X(9, 64, 5, 16, 1024, true)
 #define LBM_MULTI_ENTRY(i, tx, k, ty, nt, db) {tx, k, ty, nt, db},
 constexpr MultiVariant kMultiVariants[] = {LBM_MULTI_SHAPES(LBM_MULTI_ENTRY)};
 
@@ -314,6 +315,11 @@ hipError_t launch_multi_variant(int v, const StepParams &p, hipStream_t s) {
 
 }  // namespace
 
+static_assert(kMultiVariants[MULTI_DEFAULT].TX == BATCH_TX && kMultiVariants[MULTI_DEFAULT].K == BATCH_K &&
+                  kMultiVariants[MULTI_DEFAULT].TY == BATCH_TY && kMultiVariants[MULTI_DEFAULT].NT == BATCH_NT &&
+                  kMultiVariants[MULTI_DEFAULT].DB == BATCH_DB,
+              "batched multi-step launches run the default shape");
+
 int multi_variant_count() { return (int)(sizeof(kMultiVariants) / sizeof(kMultiVariants[0])); }
 
 int multi_tiles(int variant, int nx, int ny) {
@@ -323,7 +329,7 @@ int multi_tiles(int variant, int nx, int ny) {
 }
 
 hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
-  if (multi_steps_per_launch(c, 0) != BATCH_K) return hipErrorInvalidValue;
+  if (multi_steps_per_launch(c, MULTI_DEFAULT) != BATCH_K) return hipErrorInvalidValue;
   auto byBC = [&](auto STc) -> hipError_t {
     constexpr int ST = decltype(STc)::value;
     if (c.bc == BC_REF)

@@ -12,8 +12,8 @@ from lbm_amd.capi import BF16S, F16S, F32
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = range(9)                       # launch shapes of lbm_multistep.hip
-SHAPE_K = [4, 3, 4, 4, 4, 4, 4, 4, 2]   # steps per launch of each shape
+SHAPES = range(10)                      # launch shapes of lbm_multistep.hip
+SHAPE_K = [4, 3, 4, 4, 4, 4, 4, 4, 2, 5]  # steps per launch of each shape
 
 
 def case_list():
