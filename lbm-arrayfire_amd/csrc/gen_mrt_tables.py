@@ -120,6 +120,27 @@ def fmt_rational(name, mat):
     return "\n".join(lines)
 
 
+def monomial_rows():
+    """P[a + 3b + 9c][i] = cx_i^a cy_i^b cz_i^c: the tensor-product moment basis."""
+    rows = []
+    for c in range(3):
+        for b in range(3):
+            for a in range(3):
+                rows.append([Fraction(CX[i] ** a * CY[i] ** b * CZ[i] ** c) for i in range(27)])
+    return rows
+
+
+def raw_in_monomials(M):
+    """A = M P^-1 (raw rows as integer combinations of monomial rows): per row the (monomial,
+    coefficient) terms, and P M^-1 (monomial moments from raw moments)."""
+    P = monomial_rows()
+    A = [[sum(M[k][i] * v for i, v in enumerate(col)) for col in zip(*inverse(P))] for k in range(27)]
+    assert all(v.denominator == 1 for row in A for v in row), "raw rows are not integral in monomials"
+    terms = [[(j, int(v)) for j, v in enumerate(row) if v != 0] for row in A]
+    Ainv = [[sum(P[u][i] * Mi_ij for i, Mi_ij in enumerate(col)) for col in zip(*inverse(M))] for u in range(27)]
+    return terms, Ainv
+
+
 def fmt(name, ctype, mat):
     lines = [f"constexpr {ctype} {name}[27][27] = {{"]
     for row in mat:
@@ -157,6 +178,17 @@ def main():
     out.write(fmt_rational("TINV_GS", Ti) + "\n\n")
     out.write(fmt_rational("MEQ_RAW", meq_coefficients(M)) + "\n\n")
     out.write(fmt_rational("MEQ_GS", meq_coefficients([[Fraction(v) for v in row] for row in Gi])) + "\n\n")
+    terms, Ainv = raw_in_monomials(M)
+    nt = max(len(t) for t in terms)
+    out.write("// Raw row k = sum_t MRT_RAW_COEF[k][t] * monomial MRT_RAW_MONO[k][t] (t < MRT_RAW_NT[k]),\n"
+              "// monomial a + 3b + 9c = sum_i cx^a cy^b cz^c f_i.\n")
+    out.write("constexpr int MRT_RAW_NT[27] = {" + ", ".join(str(len(t)) for t in terms) + "};\n")
+    out.write(f"constexpr int MRT_RAW_MONO[27][{nt}] = {{" + ", ".join(
+        "{" + ", ".join(str(j) for j, _ in t + [(0, 0)] * (nt - len(t))) + "}" for t in terms) + "};\n")
+    out.write(f"constexpr int MRT_RAW_COEF[27][{nt}] = {{" + ", ".join(
+        "{" + ", ".join(str(v) for _, v in t + [(0, 0)] * (nt - len(t))) + "}" for t in terms) + "};\n\n")
+    out.write("// Monomial moments from raw moments (P M^-1).\n")
+    out.write(fmt_rational("MRT_AINV", Ainv) + "\n\n")
     out.write("}  // namespace lbm\n")
 
 

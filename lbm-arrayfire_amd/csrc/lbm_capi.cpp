@@ -139,7 +139,7 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   p.rho_bc = d->rho_bc;
   for (int k = 0; k < 27; ++k) {
     p.s[k] = d->s[k];
-    p.s24[k] = (float)(d->s[k] / 24.0);
+    p.s32[k] = (float)d->s[k];
   }
 
   // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an

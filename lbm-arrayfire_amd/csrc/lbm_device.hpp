@@ -517,18 +517,83 @@ __device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
   meq[26] = v2 + rc6;
 }
 
-// FP32 MRT through the raw moment matrix, with the same relaxation as below written as
-// out = f - M^-1 S (M f - meq):
-//   d = MRAW f - meq_raw (closed-form equilibrium moments, or the reference table),
-//   GS: d <- T_GS d (the Gram-Schmidt moments), r = (s / 24) d, GS: r <- T_GS^-1 r,
-//   out = f - (24 MRAW^-1) r.
-// MRAW and 24 MRAW^-1 are integral (339 and 226 non-zeros, mostly +-1) and T_GS, T_GS^-1 hold
-// 64 and 63, so a node costs about 820 operations against 1640 for M_GS f, M_GS f_eq and M_GS^-1
-// directly.  In exact arithmetic both forms are the same map.
+// FP32 MRT, the relaxation below (out = f - M^-1 S (M f - meq), S acting in the collision's
+// moment basis) through the tensor-product structure of D3Q27:
+//   mu = P f, the monomial moments sum_i cx^a cy^b cz^c f_i (a, b, c in 0..2): three passes of the
+//      1D transform (f_-, f_0, f_+) -> (f_- + f_0 + f_+, f_+ - f_-, f_+ + f_-), 81 additions;
+//   d = A mu - meq_raw, the raw-row deviations (the raw rows are monomials except x^2+y^2+z^2,
+//      x^2-y^2, x^2-z^2; meq_raw the closed-form equilibrium moments or the reference table);
+//   r = T_GS^-1 diag(s) T_GS d (Gram-Schmidt basis; T_GS integral lower triangular with 64
+//      non-zeros, its inverse 63) or diag(s) d (raw basis), the relaxed raw-row deviations;
+//   mu' = P M^-1 r, the monomial moments of the correction (the identity but for the three
+//      mixed rows);
+//   out = f - P^-1 mu': three passes of (r0, r1, r2) -> (r2 - r1, r0 - r2, r2 + r1), whose result
+//      for direction c carries a factor 2^|c| taken out by the final FMA (|c| = non-zero
+//      components; the scalings are exact).
+// About 400 operations a node against 820 for the raw integer matrices or 1640 for M_GS, M_GS f_eq
+// and M_GS^-1 directly, with the 27 rates as the only run-time coefficients.  In exact arithmetic
+// all are the same map.
+namespace d27 {
+// direction of the D3Q27 table at grid slot (cx + 1) + 3 (cy + 1) + 9 (cz + 1)
+struct Grid {
+  int dir[27];
+};
+constexpr Grid make_grid() {
+  Grid g{};
+  for (int i = 0; i < 27; ++i) g.dir[(Lat<27>::cx[i] + 1) + 3 * (Lat<27>::cy[i] + 1) + 9 * (Lat<27>::cz[i] + 1)] = i;
+  return g;
+}
+constexpr Grid kGrid = make_grid();
+// slot o + k * step, k = 0..2, of the 9 lines along one axis (step 1, 3, 9)
+template <int STEP>
+__device__ __forceinline__ constexpr int line_base(int l) {
+  return STEP == 1 ? 3 * l : (STEP == 3 ? (l % 3) + 9 * (l / 3) : l);
+}
+template <int STEP>
+__device__ __forceinline__ void forward_pass(float *v) {
+  static_for<9>([&](auto L) {
+    constexpr int o = line_base<STEP>(decltype(L)::value);
+    const float fm = v[o], f0 = v[o + STEP], fp = v[o + 2 * STEP];
+    const float sp = fp + fm;
+    v[o] = sp + f0;
+    v[o + STEP] = fp - fm;
+    v[o + 2 * STEP] = sp;
+  });
+}
+template <int STEP>
+__device__ __forceinline__ void inverse_pass(float *v) {
+  static_for<9>([&](auto L) {
+    constexpr int o = line_base<STEP>(decltype(L)::value);
+    const float r0 = v[o], r1 = v[o + STEP], r2 = v[o + 2 * STEP];
+    v[o] = r2 - r1;
+    v[o + STEP] = r0 - r2;
+    v[o + 2 * STEP] = r2 + r1;
+  });
+}
+// the monomial basis reproduces MRAW: raw row k = sum_t coef_t monomial_t
+constexpr bool raw_rows_match() {
+  for (int k = 0; k < 27; ++k)
+    for (int i = 0; i < 27; ++i) {
+      int acc = 0;
+      for (int t = 0; t < MRT_RAW_NT[k]; ++t) {
+        const int j = MRT_RAW_MONO[k][t], a = j % 3, b = (j / 3) % 3, c = j / 9;
+        int m = 1;
+        for (int e = 0; e < a; ++e) m *= Lat<27>::cx[i];
+        for (int e = 0; e < b; ++e) m *= Lat<27>::cy[i];
+        for (int e = 0; e < c; ++e) m *= Lat<27>::cz[i];
+        acc += MRT_RAW_COEF[k][t] * m;
+      }
+      if (acc != MRAW[k][i]) return false;
+    }
+  return true;
+}
+static_assert(raw_rows_match(), "monomial tables disagree with MRAW");
+}  // namespace d27
+
 template <int CO, class P>
 __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, float rho, float ux,
                                                  float uy, float uz, float *out) {
-  float d[27];
+  float d[27];  // the equilibrium raw moments, then the deviations
   if (CO == CO_MRT_RAW && p.meq_mode == 0) {
     meq_reference<float>(rho, ux, uy, uz, d);
   } else {
@@ -544,31 +609,37 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
       d[k] = rho * acc;
     });
   }
+  float v[27];
+  static_for<27>([&](auto G) { v[decltype(G)::value] = f[d27::kGrid.dir[decltype(G)::value]]; });
+  d27::forward_pass<1>(v);
+  d27::forward_pass<3>(v);
+  d27::forward_pass<9>(v);
   static_for<27>([&](auto K) {
     constexpr int k = decltype(K)::value;
-    float acc = 0.0f;
-    static_for<27>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      acc = add_int_term<float, MRAW[k][i]>(acc, f[i]);
+    float raw = v[MRT_RAW_MONO[k][0]];
+    static_for<MRT_RAW_NT[k] - 1>([&](auto T) {
+      constexpr int t = decltype(T)::value + 1;
+      if constexpr (MRT_RAW_COEF[k][t] == 1) raw = raw + v[MRT_RAW_MONO[k][t]];
+      else raw = raw - v[MRT_RAW_MONO[k][t]];
     });
-    d[k] = acc - d[k];
+    d[k] = raw - d[k];
   });
   float r[27];
   if constexpr (CO == CO_MRT_GS) {
     float g[27];
     static_for<27>([&](auto K) {  // Gram-Schmidt moments of the deviation, relaxed
       constexpr int k = decltype(K)::value;
-      float acc = 0.0f;
+      float acc = -0.0f;  // adds exactly; the compiler drops it
       static_for<k + 1>([&](auto J) {  // T_GS is lower triangular
         constexpr int j = decltype(J)::value;
         acc = add_int_term<float, T_GS[k][j]>(acc, d[j]);
       });
-      g[k] = p.s24[k] * acc;
+      g[k] = p.s32[k] * acc;
     });
     static_for<27>([&](auto K) {
       constexpr int k = decltype(K)::value;
-      float acc = 0.0f;
-      static_for<27>([&](auto J) {
+      float acc = -0.0f;
+      static_for<k + 1>([&](auto J) {  // so is its inverse
         constexpr int j = decltype(J)::value;
         constexpr float c = (float)TINV_GS[k][j];
         if constexpr (c == 1.0f) acc = acc + g[j];
@@ -578,19 +649,29 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
       r[k] = acc;
     });
   } else {
+    static_for<27>([&](auto K) { r[decltype(K)::value] = p.s32[decltype(K)::value] * d[decltype(K)::value]; });
+  }
+  static_for<27>([&](auto U) {  // monomial moments of the correction
+    constexpr int u = decltype(U)::value;
+    float acc = -0.0f;
     static_for<27>([&](auto K) {
       constexpr int k = decltype(K)::value;
-      r[k] = p.s24[k] * d[k];
+      constexpr float c = (float)MRT_AINV[u][k];
+      if constexpr (c == 1.0f) acc = acc + r[k];
+      else if constexpr (c == -1.0f) acc = acc - r[k];
+      else if constexpr (c != 0.0f) acc = __builtin_fmaf(c, r[k], acc);
     });
-  }
+    v[u] = acc;
+  });
+  d27::inverse_pass<1>(v);
+  d27::inverse_pass<3>(v);
+  d27::inverse_pass<9>(v);
   static_for<27>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    float acc = 0.0f;
-    static_for<27>([&](auto K) {
-      constexpr int k = decltype(K)::value;
-      acc = add_int_term<float, MRAW_INV24[i][k]>(acc, r[k]);
-    });
-    out[i] = f[i] - acc;
+    constexpr int g = (Lat<27>::cx[i] + 1) + 3 * (Lat<27>::cy[i] + 1) + 9 * (Lat<27>::cz[i] + 1);
+    constexpr int nz = (Lat<27>::cx[i] != 0) + (Lat<27>::cy[i] != 0) + (Lat<27>::cz[i] != 0);
+    constexpr float w = nz == 0 ? 1.0f : (nz == 1 ? 0.5f : (nz == 2 ? 0.25f : 0.125f));
+    out[i] = __builtin_fmaf(-w, v[g], f[i]);
   });
 }
 

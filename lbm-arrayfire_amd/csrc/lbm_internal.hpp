@@ -26,7 +26,7 @@ struct StepParams {
   double u_in;
   double rho_bc;
   double s[27];
-  float s24[27];            // FP32 MRT: s_k / 24 (the scale of the integral inverse 24 M^-1)
+  float s32[27];            // FP32 MRT: the rates s rounded to FP32
 };
 
 struct Config {

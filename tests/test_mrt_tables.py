@@ -1,7 +1,8 @@
 """The generated D3Q27 moment tables (lbm-arrayfire_amd/csrc/mrt_tables.h) that the FP32 MRT
 collision uses: the header is what the generator writes, the closed-form equilibrium moments equal
 M f_eq computed numerically, T_GS maps the raw rows onto the Gram-Schmidt rows, and the FP32 form
-out = f - (24 MRAW^-1) T_GS^-1 (s/24) T_GS (MRAW f - meq) equals M^-1 (m - S (m - meq))."""
+(monomial moments by three 1D passes, raw rows from monomials, T_GS^-1 diag(s) T_GS, P M^-1,
+three inverse passes weighted 2^-|c|) equals M^-1 (m - S (m - meq))."""
 import os
 import subprocess
 import sys
@@ -84,4 +85,51 @@ def test_fp32_form_equals_reference_relaxation():
             else:
                 d = (s / 24) * d
             got = f - (24 * Mi) @ d
+            assert np.allclose(got, ref, rtol=0, atol=1e-14)
+
+
+def passes(v, inverse=False):
+    """The kernel's three 1D passes over the 3 x 3 x 3 grid (slot (cx+1) + 3(cy+1) + 9(cz+1))."""
+    v = np.array(v, dtype=float).reshape(3, 3, 3)  # [z][y][x]
+    for axis in (2, 1, 0):
+        a = np.moveaxis(v, axis, 0).copy()
+        if not inverse:
+            fm, f0, fp = a
+            out = np.stack([fp + fm + f0, fp - fm, fp + fm])
+        else:
+            r0, r1, r2 = a
+            out = np.stack([r2 - r1, r0 - r2, r2 + r1])
+        v = np.moveaxis(out, 0, axis)
+    return v.reshape(27)
+
+
+def test_tensor_product_fp32_form_equals_reference_relaxation():
+    """The factorized form the FP32 kernels evaluate, restated step by step in float64."""
+    M, Mi, G, T, eraw, _ = tables()
+    terms, Ainv = gen.raw_in_monomials(gen.raw_rows())
+    Ainv = np.array([[float(v) for v in row] for row in Ainv])
+    c, w, _ = lattice(27)
+    grid = [(c[i, 0] + 1) + 3 * (c[i, 1] + 1) + 9 * (c[i, 2] + 1) for i in range(27)]
+    nz = np.abs(c).sum(1)
+    rng = np.random.default_rng(5)
+    Ti = np.linalg.inv(T)
+    E = np.array([[float(v) for v in row] for row in eraw])
+    for basis in ("raw", "gs"):
+        A = M if basis == "raw" else G
+        for _ in range(10):
+            f = w * (1 + 0.05 * rng.standard_normal(27))
+            s = rng.uniform(0.5, 1.8, 27)
+            rho = f.sum()
+            u = (c.T.astype(float) @ f) / rho
+            meq = A @ feq(rho, u, c.astype(float), w)
+            ref = np.linalg.solve(A, A @ f - s * (A @ f - meq))
+            v = np.zeros(27)
+            v[grid] = f
+            mu = passes(v)
+            raw = np.array([sum(k * mu[j] for j, k in t) for t in terms])
+            assert np.allclose(raw, M @ f, rtol=0, atol=1e-14)
+            d = raw - rho * (E @ phi(u))
+            r = Ti @ (s * (T @ d)) if basis == "gs" else s * d
+            back = passes(Ainv @ r, inverse=True)
+            got = f - back[grid] * 0.5 ** nz
             assert np.allclose(got, ref, rtol=0, atol=1e-14)
