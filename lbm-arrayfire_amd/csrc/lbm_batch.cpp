@@ -52,7 +52,9 @@ struct Group {
   int grid = 0;
   int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
 
-  BatchArgs args(int phase) const { return BatchArgs{(const BatchTable *)d_tab, phase}; }
+  BatchArgs args(int phase, int launch_grid) const {
+    return BatchArgs{(const BatchTable *)d_tab, phase, launch_grid >= (multi_k ? total_multi : total_tiles) ? 1 : 0};
+  }
   void release() {
     if (uploaded) (void)hipEventSynchronize(uploaded);
     if (d_tab) (void)hipFree(d_tab);
@@ -199,7 +201,7 @@ lbm_status rebuild(lbm_batch_ctx *b) {
 // One step of every single-step group (multi-step groups advance in enqueue on their own).
 lbm_status launch_phase(lbm_batch_ctx *b, int phase, hipStream_t s) {
   for (auto &g : b->groups)
-    if (!g.multi_k) BCHK(launch_step_batch(g.cfg, g.args(phase), g.grid, s));
+    if (!g.multi_k) BCHK(launch_step_batch(g.cfg, g.args(phase, g.grid), g.grid, s));
   return LBM_OK;
 }
 
@@ -241,11 +243,12 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
     if (!g.multi_k) continue;
     int64_t gl = nsteps, flips = 0;
     for (; gl >= g.multi_k; gl -= g.multi_k, ++flips) {
-      BCHK(launch_step_multi_batch(g.cfg, g.args(g.phase), std::max(g.total_multi, 1), b->stream));
+      BCHK(launch_step_multi_batch(g.cfg, g.args(g.phase, std::max(g.total_multi, 1)), std::max(g.total_multi, 1),
+                                   b->stream));
       g.phase ^= 1;
     }
     for (; gl > 0; --gl, ++flips) {
-      BCHK(launch_step_batch(g.cfg, g.args(g.phase), g.grid, b->stream));
+      BCHK(launch_step_batch(g.cfg, g.args(g.phase, g.grid), g.grid, b->stream));
       g.phase ^= 1;
     }
     for (lbm_ctx *m : g.members) {
@@ -383,7 +386,7 @@ lbm_status lbm_batch_tune(lbm_batch b, int32_t trials) {
       const int grid = f ? std::min(g.total_tiles, f * cus) : g.total_tiles;
       if (grid < 1) continue;
       // trial launches read the current buffers and write only the other ones
-      const BatchArgs a = g.args(g.multi_k ? g.phase : b->phase);
+      const BatchArgs a = g.args(g.multi_k ? g.phase : b->phase, grid);
       BCHK(launch_step_batch(g.cfg, a, grid, b->stream));  // warm-up
       BCHK(hipEventRecord(b->ev0, b->stream));
       for (int r = 0; r < reps; ++r) BCHK(launch_step_batch(g.cfg, a, grid, b->stream));

@@ -119,8 +119,9 @@ template <class T> using const_as = const __attribute__((address_space(4))) T *;
 #else  // the host pass only parses device code; it has no constant address space
 template <class T> using const_as = const T *;
 #endif
-// MULTI walks the multi-step tiles (prefix_multi) instead of the single-step ones.
-template <bool MULTI = false, class F>
+// MULTI walks the multi-step tiles (prefix_multi) instead of the single-step ones.  ONE: the grid
+// has one workgroup per tile (no loop, so no state carried across tiles).
+template <bool MULTI = false, bool ONE = false, class F>
 __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
   const_as<BatchTable> tab = (const_as<BatchTable>)b.tab;
   const int nmembers = tab->nmembers, total = MULTI ? tab->total_multi : tab->total_tiles;
@@ -128,7 +129,7 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
   const_as<StepParams> members = (const_as<StepParams>)tab->members[b.phase];
   const int nb = (int)gridDim.x;
   const int blk = xcd_tile((int)blockIdx.x, nb);
-  const int chunk = (total + nb - 1) / nb;
+  const int chunk = ONE ? 1 : (total + nb - 1) / nb;
   const int t0 = blk * chunk;
   const int t1 = t0 + chunk < total ? t0 + chunk : total;
   if (t0 >= t1) return;
@@ -137,6 +138,10 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
     const int mid = (lo + hi + 1) >> 1;
     if (prefix[mid] <= t0) lo = mid;
     else hi = mid - 1;
+  }
+  if constexpr (ONE) {
+    f(members[lo], t0 - prefix[lo]);
+    return;
   }
   int m = lo;
   for (int t = t0; t < t1; ++t) {

@@ -70,7 +70,8 @@ struct BatchTable {
 };
 struct BatchArgs {
   const BatchTable *tab;
-  int32_t phase;  // which orientation this launch steps
+  int32_t phase;     // which orientation this launch steps
+  int32_t one_tile;  // the grid has one workgroup per tile (selects the loop-free kernel)
 };
 hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);

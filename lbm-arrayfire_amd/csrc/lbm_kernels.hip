@@ -42,9 +42,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
 }
 
 // Persistent batched step: every member of one configuration group, one launch.
-template <int Q, int ST, int BC, int CO>
+template <int Q, int ST, int BC, int CO, bool ONE>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
-  for_batch_tiles(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
+  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
 }
 
 // rho, u the reference holds after a step: moments of the pulled populations with the overrides
@@ -387,9 +387,12 @@ hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t 
 
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
   return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
-    hipLaunchKernelGGL((k_step_batch<decltype(Qc)::value, decltype(STc)::value, decltype(BCc)::value,
-                                     decltype(COc)::value>),
-                       dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+    constexpr int Q = decltype(Qc)::value, ST = decltype(STc)::value, BC = decltype(BCc)::value,
+                  CO = decltype(COc)::value;
+    if (b.one_tile)
+      hipLaunchKernelGGL((k_step_batch<Q, ST, BC, CO, true>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
+    else
+      hipLaunchKernelGGL((k_step_batch<Q, ST, BC, CO, false>), dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
     return hipGetLastError();
   });
 }
