@@ -182,6 +182,22 @@ def copy_bandwidth(mib=1024):
     return None
 
 
+def pull_ceilings():
+    """GB/s of the D3Q27 step's access pattern with no arithmetic (27 periodic pulls and stores per
+    node of a 256^3 lattice in the library's padded SoA layout; 4- and 2-byte populations) and of a
+    16-byte-per-lane copy of the same bytes: lib/pull_ceiling, built from csrc/pull_ceiling.hip."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lbm-arrayfire_amd", "lib", "pull_ceiling")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    for line in r.stdout.splitlines():
+        if line.startswith("{"):
+            return json.loads(line)
+    return None
+
+
 MRT_FLOP_PER_LUP = 2 * (2 * 27 * 27)  # the two dense 27 x 27 transforms per node update
 
 
@@ -233,15 +249,20 @@ def main():
         sys.exit(2)
 
     copy_gbs = copy_bandwidth()
-    r = measure(args.workload, args.steps, args.warmup, repeats=4)
-    also = {}
     names = [] if args.also == "none" else (
         [k for k in workloads() if k != args.workload] if args.also == "default" else args.also.split(","))
+    ceilings = pull_ceilings() if any(k.startswith("d3q27") for k in names + [args.workload]) else None
+    r = measure(args.workload, args.steps, args.warmup, repeats=4)
+    also = {}
     for k in names:
         try:
             m = measure(k, args.also_steps, 20)
+            ceil = None
+            if ceilings and m["cfg"]["lattice"] == 27 and m["cfg"]["storage"] in (1, 2, 3):
+                ceil = ceilings["pull_4byte_gbs" if m["cfg"]["storage"] == 1 else "pull_2byte_gbs"]
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
                        "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
+                       **({"frac_of_pull_ceiling": round(m["achieved_gbs"] / ceil, 4)} if ceil else {}),
                        **({"algorithmic_mrt_tflops": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 1e12, 2),
                            "frac_fp32_matrix_peak": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 157.3e12, 4)}
                           if m["cfg"].get("collision", 0) else {}),
@@ -284,6 +305,7 @@ def main():
         "wall_s_timed": round(r["wall_s"], 4),
         "nonfinite": r["nonfinite"],
         "also": also,
+        "ceilings_d3q27": ceilings,
         "serving": serving,
     }
     print(json.dumps(out))

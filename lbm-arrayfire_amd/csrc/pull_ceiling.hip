@@ -1,0 +1,104 @@
+// pull_ceiling.hip — the attainable rate of the D3Q27 step's memory access pattern, measured with
+// no arithmetic.  Every node of a 256^3 lattice pulls its 27 populations from x - c_i (periodic)
+// and stores them, in the library's padded SoA layout (population_stride), with 4-byte and 2-byte
+// populations; and a 16-byte-per-lane streaming copy moves the same number of bytes.  Prints one
+// JSON line of GB/s, counted as the step kernels count them (2 Q s bytes per node).  bench.py
+// reports these beside the D3Q27 kernels' rates: they bound what any step kernel in this layout
+// can reach on the device it runs on.
+#include <cstdio>
+
+#include "lbm_device.hpp"
+
+using namespace lbm;
+
+namespace {
+
+constexpr int N3 = 256, Q = 27;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_pull(const T *__restrict__ in, T *__restrict__ out, int64_t stride) {
+  const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+  const int z = (int)blockIdx.z;
+  const int xm = x == 0 ? N3 - 1 : x - 1, xp = x == N3 - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? N3 - 1 : y - 1, yp = y == N3 - 1 ? 0 : y + 1;
+  const int zm = z == 0 ? N3 - 1 : z - 1, zp = z == N3 - 1 ? 0 : z + 1;
+  T v[Q];
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+    const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
+    const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
+    const int sz = cz == 1 ? zm : (cz == -1 ? zp : z);
+    v[i] = in[(size_t)i * (size_t)stride + (size_t)sx + (size_t)N3 * ((size_t)sy + (size_t)N3 * (size_t)sz)];
+  });
+  const size_t n = (size_t)x + (size_t)N3 * ((size_t)y + (size_t)N3 * (size_t)z);
+  static_for<Q>([&](auto I) { out[(size_t)decltype(I)::value * (size_t)stride + n] = v[decltype(I)::value]; });
+}
+
+__global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ in, float4 *__restrict__ out, size_t n4) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256) out[k] = in[k];
+}
+
+bool ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  std::fprintf(stderr, "pull_ceiling: %s: %s\n", what, hipGetErrorString(e));
+  return false;
+}
+
+// Average seconds per launch over 20 launches after 3 warm-up launches, alternating buffers.
+template <class L>
+double time_launches(L &&launch) {
+  for (int w = 0; w < 3; ++w) launch(w & 1);
+  hipEvent_t e0, e1;
+  if (!ok(hipEventCreate(&e0), "event") || !ok(hipEventCreate(&e1), "event")) return -1;
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < 20; ++r) launch(r & 1);
+  (void)hipEventRecord(e1, 0);
+  if (!ok(hipEventSynchronize(e1), "launches")) return -1;
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms * 1e-3 / 20;
+}
+
+template <class T>
+double pull_gbs() {
+  const int64_t N = (int64_t)N3 * N3 * N3, stride = population_stride(N, (int)sizeof(T));
+  const size_t bytes = (size_t)stride * Q * sizeof(T);
+  T *a = nullptr, *b = nullptr;
+  if (!ok(hipMalloc(&a, bytes), "hipMalloc") || !ok(hipMalloc(&b, bytes), "hipMalloc")) return -1;
+  if (!ok(hipMemset(a, 0, bytes), "hipMemset") || !ok(hipMemset(b, 0, bytes), "hipMemset")) return -1;
+  const double t = time_launches([&](int odd) {
+    hipLaunchKernelGGL(k_pull<T>, dim3(N3 / 64, N3 / 4, N3), dim3(256), 0, 0, odd ? b : a, odd ? a : b, stride);
+  });
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return t > 0 ? 2.0 * (double)N * Q * sizeof(T) / t / 1e9 : -1;
+}
+
+double copy_gbs(size_t bytes) {
+  float4 *a = nullptr, *b = nullptr;
+  if (!ok(hipMalloc(&a, bytes), "hipMalloc") || !ok(hipMalloc(&b, bytes), "hipMalloc")) return -1;
+  if (!ok(hipMemset(a, 0, bytes), "hipMemset") || !ok(hipMemset(b, 0, bytes), "hipMemset")) return -1;
+  const size_t n4 = bytes / sizeof(float4);
+  const double t = time_launches([&](int odd) {
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, 0, odd ? b : a, odd ? a : b, n4);
+  });
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return t > 0 ? 2.0 * (double)bytes / t / 1e9 : -1;
+}
+
+}  // namespace
+
+int main() {
+  const double p4 = pull_gbs<float>(), p2 = pull_gbs<uint16_t>();
+  const size_t cb = (size_t)N3 * N3 * N3 * Q * sizeof(float);
+  const double c = copy_gbs(cb);
+  if (p4 < 0 || p2 < 0 || c < 0) return 1;
+  std::printf("{\"lattice\": \"256^3 D3Q27, padded SoA\", \"pull_4byte_gbs\": %.1f, \"pull_2byte_gbs\": %.1f, "
+              "\"copy_gbs\": %.1f, \"copy_bytes\": %zu}\n",
+              p4, p2, c, cb);
+  return 0;
+}
