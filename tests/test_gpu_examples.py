@@ -1,0 +1,49 @@
+"""The C++ host examples on the device: the D2Q9 lid program (the reference's cpp/d2q9_bgk_lid.cpp
+inputs and outputs) writes its headless window snapshots and MLUPS CSV, and the fields in its VTK
+snapshot equal the same lattice stepped through the library from Python, bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from lbm_amd import Lattice, output
+from lbm_amd.capi import BC_PHYSICAL, D2Q9, F32, NODE_MOVING, NODE_SOLID
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "lbm-arrayfire_amd", "lib", "d2q9_bgk_lid")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="examples not built (run build())")
+def test_d2q9_example_snapshots_match_library(tmp_path):
+    n, steps, re = 64, 300, 100.0
+    env = dict(os.environ, LBM_GPU_NAME="T")
+    r = subprocess.run([EXE, "0", str(n), str(steps), str(re), "-", "physical", "100"], cwd=tmp_path,
+                       env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for it in (100, 200, 300):
+        for suffix in (".ppm", "_vectors.csv", ".vtk"):
+            assert (tmp_path / f"lid_{it}{suffix}").exists()
+    rows = (tmp_path / f"T_d2q9_bgk_lid_mlups_{n}.csv").read_text().splitlines()
+    assert rows[0] == "Iterations,MLUPS,MLUPS_x10" and [int(x.split(",")[0]) for x in rows[1:]] == [100, 200, 300]
+    dims, rho, u = output.read_vtk(tmp_path / f"lid_{steps}.vtk")
+    assert dims == (n, n, 1)
+
+    u_lid = 0.05
+    nu = u_lid * n / re
+    types = np.zeros((n, n), np.uint8)
+    types[0, :] = types[-1, :] = NODE_SOLID
+    types[:, 0] = types[:, -1] = NODE_SOLID
+    types[-1, :] |= NODE_MOVING
+    cfg = dict(lattice=D2Q9, nx=n, ny=n, nz=1, storage=F32, bc_mode=BC_PHYSICAL,
+               omega=1.0 / (3.0 * nu + 0.5), u_wall=(u_lid, 0.0, 0.0), rho_bc=1.0)
+    with Lattice(cfg, types.reshape(-1)) as lat:
+        lat.init_reference(1.0)
+        lat.step(steps)
+        want = lat.fields()
+    assert np.array_equal(rho, want[0])
+    assert np.array_equal(u[:, 0], want[1]) and np.array_equal(u[:, 1], want[2])
+    img = (tmp_path / f"lid_{steps}.ppm").read_bytes()
+    assert img.startswith(f"P6\n{n} {n}\n255\n".encode())
