@@ -49,6 +49,10 @@ def workloads():
         "d3q27_bgk_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S),
         "d3q27_bgk_sphere_512x256x256_bf16s": lambda: cases.d3q27_sphere_channel(storage=BF16S),
         "d3q27_mrt_gs_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S, collision=MRT_GS),
+        "d3q19_bgk_sphere_512x256x256_fp32": lambda: cases.d3q19_sphere_channel(storage=F32),
+        "d3q19_bgk_sphere_512x256x256_f16s": lambda: cases.d3q19_sphere_channel(storage=F16S),
+        "d3q19_mrt_lid_256_fp32": lambda: cases.d3q19_lid(256, re=400.0, storage=F32, collision=MRT_GS),
+        "d3q19_mrt_lid_256_f16s": lambda: cases.d3q19_lid(256, re=400.0, storage=F16S, collision=MRT_GS),
     }
     return W
 
@@ -198,7 +202,9 @@ def pull_ceilings():
     return None
 
 
-MRT_FLOP_PER_LUP = 2 * (2 * 27 * 27)  # the two dense 27 x 27 transforms per node update
+def mrt_flop_per_lup(Q):
+    """The two dense Q x Q transforms (M f and M^-1 m*) per node update."""
+    return 2 * (2 * Q * Q)
 
 
 def copy_calibration(mib):
@@ -263,8 +269,8 @@ def main():
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
                        "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
                        **({"frac_of_pull_ceiling": round(m["achieved_gbs"] / ceil, 4)} if ceil else {}),
-                       **({"algorithmic_mrt_tflops": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 1e12, 2),
-                           "frac_fp32_matrix_peak": round(m["mlups"] * 1e6 * MRT_FLOP_PER_LUP / 157.3e12, 4)}
+                       **({"algorithmic_mrt_tflops": round(m["mlups"] * 1e6 * mrt_flop_per_lup(m["Q"]) / 1e12, 2),
+                           "frac_fp32_matrix_peak": round(m["mlups"] * 1e6 * mrt_flop_per_lup(m["Q"]) / 157.3e12, 4)}
                           if m["cfg"].get("collision", 0) else {}),
                        "frac_hbm": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4), "steps": m["steps"],
                        "nonfinite": m["nonfinite"]}

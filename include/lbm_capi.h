@@ -44,12 +44,16 @@ typedef enum {
 } lbm_status;
 
 /* Velocity sets (rows of the reference tables cpp/d2q9_bgk_lid.cpp:72-79 and
- * rust/examples/d3q27_mrt_lid.rs:98-104). */
-typedef enum { LBM_D2Q9 = 9, LBM_D3Q27 = 27 } lbm_lattice;
+ * rust/examples/d3q27_mrt_lid.rs:98-104).  D3Q19 (the next lattice the reference's work in
+ * progress cpp/d3q19_mrt_channel.cpp aims at) is D3Q27's rest, face and edge directions, in
+ * D3Q27's order, with weights 1/3, 1/18, 1/36. */
+typedef enum { LBM_D2Q9 = 9, LBM_D3Q19 = 19, LBM_D3Q27 = 27 } lbm_lattice;
 
 /* Collision operators. BGK = cpp/d2q9_bgk_lid.cpp:149-153.  MRT_RAW = the reference's raw
  * monomial moment matrix (rust/examples/d3q27_mrt_lid.rs:167-199).  MRT_GS = the same rows
- * Gram-Schmidt orthogonalised (not in the reference). */
+ * Gram-Schmidt orthogonalised (not in the reference); for D3Q19, the orthogonal basis of
+ * d'Humieres et al. (2002), rows rho, e, eps, jx, qx, jy, qy, jz, qz, 3pxx, 3pi_xx, pww, pi_ww,
+ * pxy, pyz, pxz, mx, my, mz, with projected equilibrium moments (s[0..18]).  D2Q9 is BGK only. */
 typedef enum { LBM_BGK = 0, LBM_MRT_RAW = 1, LBM_MRT_GS = 2 } lbm_collision;
 
 /* Population storage in HBM.  F16S stores the shifted value (f - w_i) * 2^15 as IEEE half,
@@ -98,7 +102,7 @@ typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profil
                                   (channel set_row, rust/examples/d2q9_bgk_channel.rs:197-201). */
 
 typedef struct {
-  int32_t lattice;        /* lbm_lattice: 9 or 27 */
+  int32_t lattice;        /* lbm_lattice: 9, 19 or 27 */
   int32_t nx, ny, nz;     /* nz must be 1 for D2Q9 */
   int32_t collision;      /* lbm_collision */
   int32_t storage;        /* lbm_storage */

@@ -120,6 +120,45 @@ def fmt_rational(name, mat):
     return "\n".join(lines)
 
 
+W19 = {0: Fraction(1, 3), 1: Fraction(1, 18), 2: Fraction(1, 36)}
+
+
+def d3q19_rows():
+    """The orthogonal D3Q19 moment basis of d'Humieres, Ginzburg, Krafczyk, Lallemand & Luo,
+    Phil. Trans. R. Soc. A 360 (2002) 437, over the first 19 directions of the D3Q27 table:
+    rho, e, eps, jx, qx, jy, qy, jz, qz, 3pxx, 3pi_xx, pww, pi_ww, pxy, pyz, pxz, mx, my, mz."""
+    rows = [[] for _ in range(19)]
+    for i in range(19):
+        x, y, z = CX[i], CY[i], CZ[i]
+        c2 = x * x + y * y + z * z
+        vals = [1, 19 * c2 - 30, Fraction(21 * c2 * c2 - 53 * c2 + 24, 2), x, (5 * c2 - 9) * x, y,
+                (5 * c2 - 9) * y, z, (5 * c2 - 9) * z, 3 * x * x - c2, (3 * c2 - 5) * (3 * x * x - c2),
+                y * y - z * z, (3 * c2 - 5) * (y * y - z * z), x * y, y * z, x * z, (y * y - z * z) * x,
+                (z * z - x * x) * y, (x * x - y * y) * z]
+        for k, v in enumerate(vals):
+            rows[k].append(Fraction(v))
+    return rows
+
+
+def meq_coefficients_q(M, n, weight):
+    """MEQ[k][j]: coefficient of phi_j in (M f_eq)_k / rho over the first n directions."""
+    out = []
+    for row in M:
+        co = [Fraction(0)] * 10
+        for i in range(n):
+            c = (CX[i], CY[i], CZ[i])
+            w = weight(i) * row[i]
+            co[0] += w
+            for a in range(3):
+                co[1 + a] += 3 * w * c[a]
+                co[4 + a] += w * (Fraction(9, 2) * c[a] * c[a] - Fraction(3, 2))
+            co[7] += 9 * w * c[0] * c[1]
+            co[8] += 9 * w * c[0] * c[2]
+            co[9] += 9 * w * c[1] * c[2]
+        out.append(co)
+    return out
+
+
 def monomial_rows():
     """P[a + 3b + 9c][i] = cx_i^a cy_i^b cz_i^c: the tensor-product moment basis."""
     rows = []
@@ -189,6 +228,19 @@ def main():
         "{" + ", ".join(str(v) for _, v in t + [(0, 0)] * (nt - len(t))) + "}" for t in terms) + "};\n\n")
     out.write("// Monomial moments from raw moments (P M^-1).\n")
     out.write(fmt_rational("MRT_AINV", Ainv) + "\n\n")
+    D = d3q19_rows()
+    assert all(v.denominator == 1 for row in D for v in row), "D3Q19 rows are not integral"
+    for a in range(19):
+        for b in range(a):
+            assert sum(x * y for x, y in zip(D[a], D[b])) == 0, "D3Q19 rows are not orthogonal"
+    w19 = lambda i: W19[abs(CX[i]) + abs(CY[i]) + abs(CZ[i])]  # noqa: E731
+    out.write("// D3Q19 MRT: the orthogonal basis of d'Humieres et al. (2002) over the first 19 directions;\n"
+              "// its inverse is M19^T diag(1 / M19_NORM2).  MEQ19: (M19 f_eq)_k / rho in phi.\n")
+    out.write("constexpr int M19[19][19] = {\n" + "\n".join(
+        "    {" + ", ".join(str(int(v)) for v in row) + "}," for row in D) + "\n};\n\n")
+    out.write("constexpr long long M19_NORM2[19] = {" + ", ".join(
+        str(int(sum(v * v for v in row))) for row in D) + "};\n\n")
+    out.write(fmt_rational("MEQ19", meq_coefficients_q(D, 19, w19)).replace("[27][10]", "[19][10]") + "\n\n")
     out.write("}  // namespace lbm\n")
 
 

@@ -35,7 +35,7 @@ lbm_status fail(lbm_status s, const std::string &m) {
     if (e_ != hipSuccess) return fail(LBM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-double weight(int Q, int i) { return Q == 9 ? Lat<9>::w[i] : Lat<27>::w[i]; }
+double weight(int Q, int i) { return Q == 9 ? Lat<9>::w[i] : (Q == 19 ? Lat<19>::w[i] : Lat<27>::w[i]); }
 
 // Host-side codecs for shifted storage: F16S stores (f - w_i) * 2^15 as IEEE half, BF16S stores
 // f - w_i as bfloat16, both rounded to nearest even.
@@ -56,25 +56,29 @@ double dec_bf16(uint16_t b, double w) {
 
 std::string check_desc(const lbm_desc *d) {
   if (!d) return "descriptor is NULL";
-  if (d->lattice != 9 && d->lattice != 27) return "lattice must be 9 (D2Q9) or 27 (D3Q27)";
+  if (d->lattice != 9 && d->lattice != 19 && d->lattice != 27)
+    return "lattice must be 9 (D2Q9), 19 (D3Q19) or 27 (D3Q27)";
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return "lattice dimensions must be >= 1";
   if (d->lattice == 9 && d->nz != 1) return "D2Q9 requires nz == 1";
   const uint64_t N = (uint64_t)d->nx * (uint64_t)d->ny * (uint64_t)d->nz;
   if (N >= (1ull << 32)) return "lattice has 2^32 or more nodes";
   if (d->collision < 0 || d->collision > 2) return "unknown collision operator";
-  if (d->collision != LBM_BGK && d->lattice != 27) return "MRT is implemented for D3Q27 only";
+  if (d->collision != LBM_BGK && d->lattice == 9) return "MRT is implemented for D3Q19 and D3Q27";
+  if (d->collision == LBM_MRT_RAW && d->lattice == 19) return "D3Q19 MRT uses the orthogonal basis (LBM_MRT_GS)";
   if (d->storage < 0 || d->storage > 3) return "unknown storage format";
   if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
   if (d->mrt_engine < 0 || d->mrt_engine > 4) return "unknown MRT engine";
   if (d->mrt_engine == LBM_MRT_MFMA_BF16X3) return "the BF16x3 MFMA engine is not available in this build";
   if ((d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED) && d->storage == LBM_F64)
     return "the FP32 MFMA engines need FP32 arithmetic (storage F32, F16S or BF16S)";
+  if ((d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED) && d->lattice != 27)
+    return "the MFMA engines are implemented for D3Q27";
   if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
   if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
   if (!std::isfinite(d->omega)) return "omega is not finite";
   for (int k = 0; k < 27; ++k)
     if (!std::isfinite(d->s[k])) return "MRT rate is not finite";
-  if (d->inlet_profile == LBM_INLET_PARABOLIC && (d->ny < 3 || (d->lattice == 27 && d->nz < 3)))
+  if (d->inlet_profile == LBM_INLET_PARABOLIC && (d->ny < 3 || (d->lattice != 9 && d->nz < 3)))
     return "parabolic inlet needs at least 3 nodes across the channel";
   return "";
 }
@@ -198,7 +202,7 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
     if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
   }
   p.type = h->types;
-  if (d->collision != LBM_BGK) {
+  if (d->collision != LBM_BGK && d->lattice == 27) {
     std::vector<float> tab(64 * MFMA_TAB);
     build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
     if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)

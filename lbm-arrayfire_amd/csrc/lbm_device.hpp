@@ -708,9 +708,52 @@ __device__ __forceinline__ void collide_mrt_scalar(const P &p, const C *f, C rho
   mrt_inverse<C, CO>(m, out);
 }
 
+// D3Q19 MRT in the orthogonal basis M19 (d'Humieres et al. 2002): m = M19 f, projected
+// equilibrium moments meq = M19 f_eq (closed form MEQ19), m* = m - s (m - meq),
+// out = M19^T diag(1 / |row|^2) m*.  FP32: out = f - M19^T ((s / |row|^2) (m - meq)), FMAs.
+template <class C, class P>
+__device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
+  const C phi[10] = {(C)1, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+  C d[19];
+  static_for<19>([&](auto K) {
+    constexpr int k = decltype(K)::value;
+    C meq = (C)-0.0;
+    static_for<10>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      constexpr C c = (C)MEQ19[k][j];
+      if constexpr (c != (C)0) {
+        if constexpr (sizeof(C) == 4) meq = __builtin_fmaf(c, phi[j], meq);
+        else meq = meq + c * phi[j];
+      }
+    });
+    C m = (C)-0.0;
+    static_for<19>([&](auto I) { m = add_int_term<C, M19[k][decltype(I)::value]>(m, f[decltype(I)::value]); });
+    meq = rho * meq;
+    if constexpr (sizeof(C) == 4) d[k] = (p.s32[k] * (1.0f / (float)M19_NORM2[k])) * (m - meq);
+    else d[k] = m - (m - meq) * (C)p.s[k];  // m*
+  });
+  static_for<19>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (sizeof(C) == 4) {
+      C acc = (C)-0.0;
+      static_for<19>([&](auto K) { acc = add_int_term<C, M19[decltype(K)::value][i]>(acc, d[decltype(K)::value]); });
+      out[i] = f[i] - acc;
+    } else {
+      C acc = (C)-0.0;
+      static_for<19>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr C c = (C)M19[k][i] / (C)M19_NORM2[k];
+        if constexpr (c != (C)0) acc = acc + d[k] * c;
+      });
+      out[i] = acc;
+    }
+  });
+}
+
 template <int Q, int CO, class C, class P>
 __device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
   if constexpr (CO == CO_BGK) collide_bgk<Q>(p, f, rho, ux, uy, uz, out);
+  else if constexpr (Q == 19) collide_mrt19(p, f, rho, ux, uy, uz, out);
   else collide_mrt_scalar<CO>(p, f, rho, ux, uy, uz, out);
 }
 

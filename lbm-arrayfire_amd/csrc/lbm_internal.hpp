@@ -93,7 +93,7 @@ hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid
 enum { MRT_ENGINE_AUTO = 0, MRT_ENGINE_SCALAR = 1, MRT_ENGINE_MFMA = 2, MRT_ENGINE_BF16X3 = 3, MRT_ENGINE_FUSED = 4,
        MRT_ENGINE_AUTO_FP32 = MRT_ENGINE_SCALAR };
 inline bool uses_mfma(const Config &c) {
-  return c.collision != CO_BGK && c.storage != ST_F64 &&
+  return c.Q == 27 && c.collision != CO_BGK && c.storage != ST_F64 &&
          (c.engine == MRT_ENGINE_MFMA || c.engine == MRT_ENGINE_FUSED);
 }
 // Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice (two-stage

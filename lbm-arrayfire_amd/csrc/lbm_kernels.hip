@@ -142,15 +142,15 @@ hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double 
   });
 }
 
-// MRT: f = rho0/27 then one collision at rho = rho0, u = 0, no bounce-back
-// (rust/examples/d3q27_mrt_lid.rs:228-287).  BGK: f = f_eq(rho0, 0).
+// MRT (D3Q27): f = rho0/27 then one collision at rho = rho0, u = 0, no bounce-back
+// (rust/examples/d3q27_mrt_lid.rs:228-287).  BGK, and D3Q19 MRT: f = f_eq(rho0, 0).
 template <int Q, int ST, int CO>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, double rho0) {
   using C = typename Codec<ST, Q>::C;
   const NodePos q = node_of_thread(p);
   if (!q.valid) return;
   C out[Q];
-  if constexpr (CO == CO_BGK) {
+  if constexpr (CO == CO_BGK || Q == 19) {
     const C usq = usq_of<Q>((C)0, (C)0, (C)0);  // the u-term of f_eq's storage type (FP32: A = 1)
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
@@ -320,6 +320,7 @@ hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, cons
   const int64_t N = (int64_t)nx * ny * nz;
   const dim3 grid((unsigned)((N + BLOCK_THREADS - 1) / BLOCK_THREADS));
   if (Q == 9) hipLaunchKernelGGL(k_prepare_types<9>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
+  else if (Q == 19) hipLaunchKernelGGL(k_prepare_types<19>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
   else hipLaunchKernelGGL(k_prepare_types<27>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
   return hipGetLastError();
 }
@@ -351,6 +352,7 @@ static hipError_t with_storage(const Config &c, F &&f) {
     return hipErrorInvalidValue;
   };
   if (c.Q == 9) return byQ(IC<9>{});
+  if (c.Q == 19) return byQ(IC<19>{});
   if (c.Q == 27) return byQ(IC<27>{});
   return hipErrorInvalidValue;
 }
@@ -368,6 +370,9 @@ static hipError_t with_full(const Config &c, F &&f) {
       if (c.collision == CO_BGK) return f(Qc, STc, BCc, IC<CO_BGK>{});
       if constexpr (Q == 27) {
         if (c.collision == CO_MRT_RAW) return f(Qc, STc, BCc, IC<CO_MRT_RAW>{});
+        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
+      }
+      if constexpr (Q == 19) {  // the orthogonal basis only
         if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
       }
       return hipErrorInvalidValue;

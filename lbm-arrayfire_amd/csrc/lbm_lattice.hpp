@@ -4,6 +4,8 @@
 //   D2Q9  cpp/d2q9_bgk_lid.cpp:72-84 (cx, cy, weights, nb_index_arr)
 //   D3Q27 rust/examples/d3q27_mrt_lid.rs:98-105 (ex, ey, ez, nbidx), weights
 //         cpp/d3q27_bgk_lid.cpp:86-89
+//   D3Q19 no reference table (cpp/d3q19_mrt_channel.cpp still uses the 27 velocities): the
+//         subset of D3Q27 without the corners, in D3Q27's order
 #pragma once
 #include <cstdint>
 
@@ -45,6 +47,20 @@ template <> struct Lat<27> {
                                    1. / 54., 1. / 54., 1. / 54., 1. / 54., 1. / 54., 1. / 54.,
                                    1. / 216., 1. / 216., 1. / 216., 1. / 216.,
                                    1. / 216., 1. / 216., 1. / 216., 1. / 216.};
+};
+
+// D3Q19: the rest, face and edge directions of D3Q27, in D3Q27's order (its first 19 entries;
+// opposites stay within them); weights 1/3, 1/18, 1/36.
+template <> struct Lat<19> {
+  static constexpr int D = 3;
+  static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+  static constexpr double w[19] = {1. / 3.,
+                                   1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18., 1. / 18.,
+                                   1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.,
+                                   1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
 };
 
 // Bytes of one stored population.

@@ -13,7 +13,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "liblbm_amd.so")
 
 # enums of lbm_capi.h
-D2Q9, D3Q27 = 9, 27
+D2Q9, D3Q19, D3Q27 = 9, 19, 27
 BGK, MRT_RAW, MRT_GS = 0, 1, 2
 F64, F32, F16S, BF16S = 0, 1, 2, 3
 BC_REFERENCE, BC_PHYSICAL = 0, 1

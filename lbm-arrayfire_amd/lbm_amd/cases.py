@@ -5,7 +5,7 @@ omega follows the reference's unit convention nu = (tau - 1/2)/3, omega = 1/tau
 import numpy as np
 
 from . import geometry as g
-from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q27, F32, INLET_PARABOLIC,
+from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q19, D3Q27, F32, INLET_PARABOLIC,
                    INLET_UNIFORM, MEQ_REFERENCE, MRT_AUTO, MRT_GS, MRT_RAW)
 
 
@@ -112,5 +112,36 @@ def d3q27_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u
     om = omega_from_nu(nu)
     cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
                bc_mode=BC_PHYSICAL, omega=om, s=shear_mrt_rates(om), u_in=u_in,
+               inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_3d_physical(nx, ny, nz, center[0], center[1], center[2], r)
+
+
+def dhumieres_rates(omega, s_e=1.19, s_eps=1.4, s_q=1.2, s_pi=1.4, s_m=1.98, conserved=1.0):
+    """D3Q19 rates in the row order of the orthogonal basis (rho, e, eps, jx, qx, jy, qy, jz, qz,
+    3pxx, 3pi_xx, pww, pi_ww, pxy, pyz, pxz, mx, my, mz): s_nu = omega on the five stress rows and
+    the values of d'Humieres et al. (2002) on the others (padded to 27 entries)."""
+    c = conserved
+    s = [c, s_e, s_eps, c, s_q, c, s_q, c, s_q, omega, s_pi, omega, s_pi, omega, omega, omega,
+         s_m, s_m, s_m]
+    return s + [1.0] * (27 - len(s))
+
+
+def d3q19_lid(n=64, u_lid=0.05, re=100.0, storage=F32, collision=BGK, bc_mode=BC_REFERENCE):
+    """D3Q19 lid cavity (BGK or MRT in the orthogonal basis), nu = u*n/Re."""
+    nu = u_lid * n / re
+    om = omega_from_nu(nu)
+    geo = g.lid_cavity_3d_reference(n, n, n) if bc_mode == BC_REFERENCE else g.lid_cavity_3d_physical(n, n, n)
+    cfg = dict(lattice=D3Q19, nx=n, ny=n, nz=n, collision=collision, storage=storage, bc_mode=bc_mode,
+               omega=om, s=dhumieres_rates(om), u_wall=(u_lid, 0.0, 0.0))
+    return cfg, geo
+
+
+def d3q19_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
+                         storage=F32, collision=BGK):
+    """D3Q19 duct with a sphere: uniform Zou-He inlet, zero-gradient outlet; Re = u * D / nu."""
+    nu = u_in * 2 * r / re
+    om = omega_from_nu(nu)
+    cfg = dict(lattice=D3Q19, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
+               bc_mode=BC_PHYSICAL, omega=om, s=dhumieres_rates(om), u_in=u_in,
                inlet_profile=INLET_UNIFORM)
     return cfg, g.channel_3d_physical(nx, ny, nz, center[0], center[1], center[2], r)

@@ -19,6 +19,10 @@
 //                       :228-287 (initial state rho0/27 + one collision)
 //  * f32 index rounding the FP32 builds index through float arrays (cpp/d2q9_bgk_lid.cpp:86,
 //                       rust/examples/d2q9_bgk_lid.rs:97); f32_index_compat reproduces it.
+//  * D3Q19              no reference counterpart (cpp/d3q19_mrt_channel.cpp is a work in progress
+//                       on 27 velocities): D3Q27's rest, face and edge directions, weights
+//                       1/3, 1/18, 1/36; MRT in the orthogonal basis of d'Humieres et al. (2002)
+//                       with projected equilibrium moments; initial state f_eq(rho0, 0)
 //  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
 //                       (non-equilibrium bounce-back) velocity inlet, zero-gradient outlet at a
 //                       reference density (non-equilibrium extrapolation of every population).
@@ -88,6 +92,12 @@ Lattice make_lattice(int Q) {
     const double t1 = 4. / 9., t2 = 1. / 9., t3 = 1. / 36.;
     const double w[9] = {t1, t2, t2, t2, t2, t3, t3, t3, t3};
     for (int i = 0; i < 9; ++i) L.w[i] = w[i];
+  } else if (Q == 19) {  // D3Q27 without its corners, in D3Q27's order
+    L.D = 3;
+    for (int i = 0; i < 19; ++i) { L.cx[i] = CX27[i]; L.cy[i] = CY27[i]; L.cz[i] = CZ27[i]; }
+    L.opp[0] = 0;
+    for (int j = 0; j < 18; ++j) L.opp[j + 1] = NBIDX27[j] + 1;
+    for (int i = 0; i < 19; ++i) L.w[i] = (i == 0) ? 1. / 3. : (i <= 6 ? 1. / 18. : 1. / 36.);
   } else {
     L.D = 3;
     for (int i = 0; i < 27; ++i) { L.cx[i] = CX27[i]; L.cy[i] = CY27[i]; L.cz[i] = CZ27[i]; }
@@ -178,27 +188,44 @@ void gs_moment_matrix(const Lattice &L, double M[27][27]) {
 }
 // Gauss-Jordan inverse with partial pivoting in long double (stands in for the reference's
 // LAPACK inverse, rust/examples/d3q27_mrt_lid.rs:196).
-bool invert27(const double A[27][27], double Ainv[27][27]) {
+// D3Q19 moment rows of d'Humieres, Ginzburg, Krafczyk, Lallemand & Luo, Phil. Trans. R. Soc. A
+// 360 (2002) 437: rho, e = 19c^2 - 30, eps = (21c^4 - 53c^2 + 24)/2, j_a = c_a,
+// q_a = (5c^2 - 9)c_a, 3p_xx = 3c_x^2 - c^2, 3pi_xx = (3c^2 - 5)(3c_x^2 - c^2), p_ww = c_y^2 - c_z^2,
+// pi_ww = (3c^2 - 5)(c_y^2 - c_z^2), p_xy, p_yz, p_xz = c_a c_b, m_x = (c_y^2 - c_z^2)c_x,
+// m_y = (c_z^2 - c_x^2)c_y, m_z = (c_x^2 - c_y^2)c_z (rows in that order, j and q interleaved).
+void d3q19_moment_matrix(const Lattice &L, double M[27][27]) {
+  for (int i = 0; i < 19; ++i) {
+    const double x = L.cx[i], y = L.cy[i], z = L.cz[i], c2 = x * x + y * y + z * z;
+    const double r[19] = {1, 19 * c2 - 30, (21 * c2 * c2 - 53 * c2 + 24) / 2, x, (5 * c2 - 9) * x, y,
+                          (5 * c2 - 9) * y, z, (5 * c2 - 9) * z, 3 * x * x - c2, (3 * c2 - 5) * (3 * x * x - c2),
+                          y * y - z * z, (3 * c2 - 5) * (y * y - z * z), x * y, y * z, x * z,
+                          (y * y - z * z) * x, (z * z - x * x) * y, (x * x - y * y) * z};
+    for (int k = 0; k < 19; ++k) M[k][i] = r[k];
+  }
+}
+
+// Gauss-Jordan inverse of the leading n x n block, partial pivoting, in long double.
+bool invert27(const double A[27][27], double Ainv[27][27], int n = 27) {
   long double a[27][54];
-  for (int r = 0; r < 27; ++r)
-    for (int c = 0; c < 54; ++c) a[r][c] = (c < 27) ? A[r][c] : (c - 27 == r ? 1.0L : 0.0L);
-  for (int c = 0; c < 27; ++c) {
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < 2 * n; ++c) a[r][c] = (c < n) ? A[r][c] : (c - n == r ? 1.0L : 0.0L);
+  for (int c = 0; c < n; ++c) {
     int p = c;
-    for (int r = c + 1; r < 27; ++r)
+    for (int r = c + 1; r < n; ++r)
       if (std::fabs((double)a[r][c]) > std::fabs((double)a[p][c])) p = r;
     if (std::fabs((double)a[p][c]) < 1e-12) return false;
     if (p != c)
-      for (int k = 0; k < 54; ++k) std::swap(a[p][k], a[c][k]);
+      for (int k = 0; k < 2 * n; ++k) std::swap(a[p][k], a[c][k]);
     const long double piv = a[c][c];
-    for (int k = 0; k < 54; ++k) a[c][k] /= piv;
-    for (int r = 0; r < 27; ++r) {
+    for (int k = 0; k < 2 * n; ++k) a[c][k] /= piv;
+    for (int r = 0; r < n; ++r) {
       if (r == c || a[r][c] == 0) continue;
       const long double f = a[r][c];
-      for (int k = 0; k < 54; ++k) a[r][k] -= f * a[c][k];
+      for (int k = 0; k < 2 * n; ++k) a[r][k] -= f * a[c][k];
     }
   }
-  for (int r = 0; r < 27; ++r)
-    for (int c = 0; c < 27; ++c) Ainv[r][c] = (double)a[r][c + 27];
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) Ainv[r][c] = (double)a[r][c + n];
   return true;
 }
 
@@ -226,12 +253,13 @@ struct Sim {
     if (d.collision != BGK) {
       mrt = true;
       double Mi[27][27];
-      if (d.collision == MRT_RAW) raw_moment_matrix(L, Md);
+      if (L.Q == 19) d3q19_moment_matrix(L, Md);
+      else if (d.collision == MRT_RAW) raw_moment_matrix(L, Md);
       else gs_moment_matrix(L, Md);
-      invert27(Md, Mi);
-      for (int k = 0; k < 27; ++k)
-        for (int i = 0; i < 27; ++i) { M[k][i] = (T)Md[k][i]; Minv[k][i] = (T)Mi[k][i]; }
-      for (int k = 0; k < 27; ++k) s[k] = (T)d.s[k];
+      invert27(Md, Mi, L.Q);
+      for (int k = 0; k < L.Q; ++k)
+        for (int i = 0; i < L.Q; ++i) { M[k][i] = (T)Md[k][i]; Minv[k][i] = (T)Mi[k][i]; }
+      for (int k = 0; k < L.Q; ++k) s[k] = (T)d.s[k];
     }
   }
 
@@ -329,28 +357,29 @@ struct Sim {
 
   // m = M S; m* = m - s (m - meq); out = M^-1 m*  (rust/examples/d3q27_mrt_lid.rs:336,385-387).
   void mrt_collide(const T *S, T rho, T ux, T uy, T uz, T *out) const {
+    const int Q = L.Q;
     T m[27], meq[27], mc[27];
-    for (int k = 0; k < 27; ++k) {
+    for (int k = 0; k < Q; ++k) {
       T acc = 0;
-      for (int i = 0; i < 27; ++i) acc += S[i] * M[k][i];
+      for (int i = 0; i < Q; ++i) acc += S[i] * M[k][i];
       m[k] = acc;
     }
-    if (d.collision == MRT_RAW && d.meq_mode == 0) {
+    if (Q == 27 && d.collision == MRT_RAW && d.meq_mode == 0) {
       meq_reference(rho, ux, uy, uz, meq);
     } else {  // projected: meq = M f_eq(rho, u)
       T feq[27];
       const T usq = usq_term(ux, uy, uz);
-      for (int i = 0; i < 27; ++i) feq[i] = this->feq(i, rho, ux, uy, uz, usq);
-      for (int k = 0; k < 27; ++k) {
+      for (int i = 0; i < Q; ++i) feq[i] = this->feq(i, rho, ux, uy, uz, usq);
+      for (int k = 0; k < Q; ++k) {
         T acc = 0;
-        for (int i = 0; i < 27; ++i) acc += feq[i] * M[k][i];
+        for (int i = 0; i < Q; ++i) acc += feq[i] * M[k][i];
         meq[k] = acc;
       }
     }
-    for (int k = 0; k < 27; ++k) mc[k] = m[k] - (m[k] - meq[k]) * s[k];
-    for (int i = 0; i < 27; ++i) {
+    for (int k = 0; k < Q; ++k) mc[k] = m[k] - (m[k] - meq[k]) * s[k];
+    for (int i = 0; i < Q; ++i) {
       T acc = 0;
-      for (int k = 0; k < 27; ++k) acc += mc[k] * Minv[i][k];
+      for (int k = 0; k < Q; ++k) acc += mc[k] * Minv[i][k];
       out[i] = acc;
     }
   }
@@ -564,10 +593,11 @@ struct Sim {
 
 bool desc_ok(const oracle_desc *d) {
   if (!d) return false;
-  if (d->lattice != 9 && d->lattice != 27) return false;
+  if (d->lattice != 9 && d->lattice != 19 && d->lattice != 27) return false;
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return false;
   if (d->lattice == 9 && d->nz != 1) return false;
-  if (d->collision != BGK && d->lattice != 27) return false;
+  if (d->collision != BGK && d->lattice == 9) return false;
+  if (d->collision == MRT_RAW && d->lattice == 19) return false;  // orthogonal basis only
   return true;
 }
 
@@ -607,7 +637,7 @@ int init_reference_impl(const oracle_desc *d, const uint8_t *types, double rho0,
   sim.F.assign((size_t)(sim.N * Q), (T)0);
   for (int64_t n = 0; n < sim.N; ++n) {
     T out[27];
-    if (sim.mrt) {
+    if (sim.mrt && Q == 27) {  // D3Q19 MRT starts from f_eq(rho0, 0), as BGK
       T S[27];
       for (int i = 0; i < 27; ++i) S[i] = (T)rho0 / (T)27;
       sim.mrt_collide(S, (T)rho0, 0, 0, 0, out);
@@ -696,7 +726,7 @@ int oracle_max_threads(void) {
 
 // Lattice tables: c (3*Q ints, x then y then z rows), w (Q), opp (Q).
 int oracle_tables(int Q, int32_t *c, double *w, int32_t *opp) {
-  if (Q != 9 && Q != 27) return 1;
+  if (Q != 9 && Q != 19 && Q != 27) return 1;
   Lattice L = make_lattice(Q);
   for (int i = 0; i < Q; ++i) {
     c[i] = L.cx[i]; c[Q + i] = L.cy[i]; c[2 * Q + i] = L.cz[i];
@@ -705,16 +735,19 @@ int oracle_tables(int Q, int32_t *c, double *w, int32_t *opp) {
   return 0;
 }
 
-// basis 1 = raw monomial (reference), 2 = Gram-Schmidt.  Row-major 27x27.
+// basis 1 = raw monomial (reference), 2 = Gram-Schmidt: row-major 27x27.  basis 3 = the D3Q19
+// orthogonal basis: row-major 19x19.
 int oracle_mrt_matrices(int basis, double *M, double *Minv) {
-  Lattice L = make_lattice(27);
+  const int n = basis == 3 ? 19 : 27;
+  Lattice L = make_lattice(n);
   double A[27][27], Ai[27][27];
   if (basis == 1) raw_moment_matrix(L, A);
   else if (basis == 2) gs_moment_matrix(L, A);
+  else if (basis == 3) d3q19_moment_matrix(L, A);
   else return 1;
-  if (!invert27(A, Ai)) return 2;
-  for (int k = 0; k < 27; ++k)
-    for (int i = 0; i < 27; ++i) { M[k * 27 + i] = A[k][i]; Minv[k * 27 + i] = Ai[k][i]; }
+  if (!invert27(A, Ai, n)) return 2;
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < n; ++i) { M[k * n + i] = A[k][i]; Minv[k * n + i] = Ai[k][i]; }
   return 0;
 }
 

@@ -162,11 +162,13 @@ def tables(Q):
 
 
 def mrt_matrices(basis):
+    """basis 1 raw, 2 Gram-Schmidt (27 x 27); 3 the D3Q19 orthogonal basis (19 x 19)."""
+    n = 19 if basis == 3 else 27
     M = np.zeros(729)
     Mi = np.zeros(729)
     rc = lib().oracle_mrt_matrices(int(basis), _dp(M), _dp(Mi))
     assert rc == 0, rc
-    return M.reshape(27, 27), Mi.reshape(27, 27)
+    return M[:n * n].reshape(n, n), Mi[:n * n].reshape(n, n)
 
 
 def f2h(x):

@@ -18,11 +18,17 @@ _ez = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1,
 D3Q27_C = np.array(list(zip(_ex, _ey, _ez)))
 D3Q27_W = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
 
+# D3Q19: rest, the six faces, the twelve edges (xy, xz, yz pairs), listed in D3Q27's order
+D3Q19_C = np.array([(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                    (1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1),
+                    (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1)])
+D3Q19_W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
 SOLID, MOVING, INLET, OUTLET, RHO_FIX = 1, 2, 4, 8, 16
 
 
 def lattice(Q):
-    c, w = (D2Q9_C, D2Q9_W) if Q == 9 else (D3Q27_C, D3Q27_W)
+    c, w = {9: (D2Q9_C, D2Q9_W), 19: (D3Q19_C, D3Q19_W), 27: (D3Q27_C, D3Q27_W)}[Q]
     # opposite direction found by search, not taken from a table
     opp = np.array([int(np.where((c == -ci).all(axis=1))[0][0]) for ci in c])
     return c, w, opp
@@ -69,6 +75,18 @@ def gs_moment_matrix():
     return np.array(G)
 
 
+def d3q19_moment_matrix():
+    """d'Humieres et al. (2002) D3Q19 rows: rho, e, eps, jx, qx, jy, qy, jz, qz, 3pxx, 3pi_xx,
+    pww, pi_ww, pxy, pyz, pxz, mx, my, mz."""
+    x, y, z = (D3Q19_C[:, k].astype(float) for k in range(3))
+    c2 = x * x + y * y + z * z
+    rows = [np.ones(19), 19 * c2 - 30, (21 * c2 ** 2 - 53 * c2 + 24) / 2, x, (5 * c2 - 9) * x, y,
+            (5 * c2 - 9) * y, z, (5 * c2 - 9) * z, 3 * x * x - c2, (3 * c2 - 5) * (3 * x * x - c2),
+            y * y - z * z, (3 * c2 - 5) * (y * y - z * z), x * y, y * z, x * z, (y * y - z * z) * x,
+            (z * z - x * x) * y, (x * x - y * y) * z]
+    return np.array(rows)
+
+
 def meq_reference(rho, u):
     ux, uy, uz = u
     cs2 = 1 / 3
@@ -94,14 +112,15 @@ def collide(S, rho, u, cfg, c, w):
     if coll == 0:
         om = cfg["omega"]
         return om * feq(rho, u, c, w) + (1 - om) * S
-    M = raw_moment_matrix() if coll == 1 else gs_moment_matrix()
+    Q = len(c)
+    M = d3q19_moment_matrix() if Q == 19 else (raw_moment_matrix() if coll == 1 else gs_moment_matrix())
     Minv = np.linalg.inv(M)
     m = np.einsum("ki,i...->k...", M, S)
-    if coll == 1 and cfg.get("meq_mode", 0) == 0:
+    if Q == 27 and coll == 1 and cfg.get("meq_mode", 0) == 0:
         me = meq_reference(rho, u)
     else:
         me = np.einsum("ki,i...->k...", M, feq(rho, u, c, w))
-    s = np.asarray(cfg["s"], dtype=float)[:, None, None, None]
+    s = np.asarray(cfg["s"], dtype=float)[:Q, None, None, None]
     return np.einsum("ik,k...->i...", Minv, m - s * (m - me))
 
 

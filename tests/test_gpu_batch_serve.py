@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from lbm_amd import Batch, Lattice, LbmError, Server, cases, geometry as geo
-from lbm_amd.capi import F16S, F32, F64, MRT_SCALAR
+from lbm_amd.capi import F16S, F32, F64, MRT_GS, MRT_SCALAR
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,8 @@ def member_specs():
     cfg = dict(cfg, nx=70)  # 70 x 20 x 20: ragged tile row
     t = geo.lid_cavity_3d_reference(70, 20, 20)
     specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d3q19_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F16S, collision=MRT_GS)
+    specs.append((cfg, t, ("eq", (0.05, 0, 0), 13, 1e-4)))
     return specs
 
 

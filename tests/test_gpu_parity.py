@@ -54,6 +54,12 @@ PARITY = [
     ("d3q27_sphere_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st), 100),
     ("d3q27_sphere_mrt_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(
         40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st, collision=MRT_GS), 60),
+    ("d3q19_bgk_lid_ref_20", lambda st: cases.d3q19_lid(20, re=100.0, storage=st), 100),
+    ("d3q19_mrt_lid_ref_20", lambda st: cases.d3q19_lid(20, re=100.0, storage=st, collision=MRT_GS), 60),
+    ("d3q19_sphere_phys_40x20x20", lambda st: cases.d3q19_sphere_channel(
+        40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st), 100),
+    ("d3q19_sphere_mrt_phys_40x20x20", lambda st: cases.d3q19_sphere_channel(
+        40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st, collision=MRT_GS), 60),
 ]
 
 
@@ -91,6 +97,8 @@ SHIFTED = [
     ("d3q27_mrt_gs_lid_ref_24_mfma", lambda st: cases.d3q27_mrt_lid_gs(24, re=100.0, storage=st), 600, 1e-3, 5e-3),
     ("d2q9_channel_phys_256x64", lambda st: cases.d2q9_karman(256, 64, 64, 32, 6, re=20.0, storage=st), 3000,
      2e-3, 5e-2),
+    ("d3q19_mrt_lid_ref_24", lambda st: cases.d3q19_lid(24, re=100.0, storage=st, collision=MRT_GS), 600,
+     1e-3, 5e-3),
 ]
 
 
@@ -216,12 +224,14 @@ def test_fp32_reciprocal_equals_division_exhaustively(sign):
 
 
 @pytest.mark.parametrize("storage", [F64, F32, F16S, BF16S])
-@pytest.mark.parametrize("q", [9, 27])
+@pytest.mark.parametrize("q", [9, 19, 27])
 def test_bgk_reference_init_matches_oracle(q, storage):
     """f = f_eq(rho0, 0) = w_i rho0 in every storage (exact: the shifted codecs store 0 for it),
     and a few steps from it stay on the oracle's trajectory."""
     if q == 9:
         cfg, types = cases.d2q9_lid_reference(24, storage=storage)
+    elif q == 19:
+        cfg, types = cases.d3q19_lid(12, u_lid=0.05, re=50.0, storage=storage)
     else:
         cfg, types = cases.d3q27_bgk_lid_reference(12, u_lid=0.05, re=50.0, storage=storage)
     types = types.reshape(-1)
@@ -272,7 +282,7 @@ def test_mrt_mfma_against_fp64(name, make, steps):
     assert max(errs.values()) <= 1e-5
 
 
-@pytest.mark.parametrize("Q", [9, 27])
+@pytest.mark.parametrize("Q", [9, 19, 27])
 def test_force_closed_form_on_device(Q):
     """An isolated solid node in a uniform equilibrium flow: F = 2 rho u exactly (FP64)."""
     shape = (1, 9, 11) if Q == 9 else (7, 8, 9)
@@ -280,7 +290,7 @@ def test_force_closed_form_on_device(Q):
     types = np.zeros(shape, np.uint8)
     types[nz // 2, ny // 2, nx // 2] = 1
     cfg = dict(lattice=Q, nx=nx, ny=ny, nz=nz, collision=0, storage=F64, bc_mode=1, omega=1.0)
-    u = (0.03, -0.02, 0.01 if Q == 27 else 0.0)
+    u = (0.03, -0.02, 0.01 if Q != 9 else 0.0)
     rho = 1.1
     n = nx * ny * nz
     with Lattice(cfg, types) as lat:

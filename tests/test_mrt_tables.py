@@ -133,3 +133,18 @@ def test_tensor_product_fp32_form_equals_reference_relaxation():
             back = passes(Ainv @ r, inverse=True)
             got = f - back[grid] * 0.5 ** nz
             assert np.allclose(got, ref, rtol=0, atol=1e-14)
+
+
+def test_d3q19_tables():
+    """M19 rows are the d'Humieres polynomials (integral, orthogonal, norms as generated) and MEQ19
+    is M19 f_eq / rho in closed form."""
+    D = np.array([[float(v) for v in row] for row in gen.d3q19_rows()])
+    assert np.allclose(D @ D.T, np.diag(np.diag(D @ D.T)))
+    c, w, _ = lattice(19)
+    E = np.array([[float(v) for v in row] for row in gen.meq_coefficients_q(
+        gen.d3q19_rows(), 19, lambda i: gen.W19[abs(gen.CX[i]) + abs(gen.CY[i]) + abs(gen.CZ[i])])])
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        rho, u = 1 + 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal(3)
+        fe = feq(rho, u, c.astype(float), w)
+        assert np.allclose(rho * E @ phi(u), D @ fe, rtol=0, atol=1e-14)

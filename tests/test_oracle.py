@@ -23,7 +23,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
-@pytest.mark.parametrize("Q", [9, 27])
+@pytest.mark.parametrize("Q", [9, 19, 27])
 def test_lattice_closed_forms(Q):
     c, w, opp = orc.tables(Q)
     cT = c.T.astype(float)
@@ -42,7 +42,7 @@ def test_lattice_closed_forms(Q):
     assert (cr.T == c).all() and np.allclose(wr, w, rtol=0, atol=0) and (oppr == opp).all()
 
 
-@pytest.mark.parametrize("Q,shape", [(9, (1, 4, 5)), (27, (3, 4, 3))])
+@pytest.mark.parametrize("Q,shape", [(9, (1, 4, 5)), (19, (3, 4, 3)), (27, (3, 4, 3))])
 def test_pure_streaming_is_pull_permutation(Q, shape):
     """omega = 0 in a fully periodic fluid box: the step is exactly the pull permutation
     f_i(x) <- f_i(x - c_i mod dims), checked slot by slot with distinct values."""
@@ -63,7 +63,7 @@ def test_pure_streaming_is_pull_permutation(Q, shape):
     assert out.sum() == F.sum()
 
 
-@pytest.mark.parametrize("Q", [9, 27])
+@pytest.mark.parametrize("Q", [9, 19, 27])
 def test_equilibrium_moments(Q):
     nx, ny, nz = (6, 5, 1) if Q == 9 else (4, 3, 3)
     N = nx * ny * nz
@@ -85,7 +85,7 @@ def test_equilibrium_moments(Q):
             assert np.allclose(Pab, rho * (u[a] * u[b] + (a == b) / 3), atol=1e-14)
 
 
-@pytest.mark.parametrize("Q", [9, 27])
+@pytest.mark.parametrize("Q", [9, 19, 27])
 def test_bgk_conserves_mass_and_momentum(Q):
     nx, ny, nz = (8, 6, 1) if Q == 9 else (5, 4, 4)
     N = nx * ny * nz
@@ -114,6 +114,10 @@ REF_CASES = {
                               geo.lid_cavity_3d_reference(5, 5, 5)),
     "d3q27_mrt_gs": lambda: (dict(cases.d3q27_mrt_lid_gs(5, re=10.0, storage=F64)[0]),
                              geo.lid_cavity_3d_reference(5, 5, 5)),
+    "d3q19_bgk_lid": lambda: (dict(cases.d3q19_lid(5, re=10.0, storage=F64)[0]),
+                              geo.lid_cavity_3d_reference(5, 5, 5)),
+    "d3q19_mrt_lid": lambda: (dict(cases.d3q19_lid(5, re=10.0, storage=F64, collision=MRT_GS)[0]),
+                              geo.lid_cavity_3d_reference(5, 5, 5)),
 }
 
 
@@ -133,7 +137,7 @@ def test_oracle_matches_numpy_restatement_reference_mode(name):
     u = np.zeros((3,) + shape)
     u[0] = 0.02 * rng.standard_normal(shape)
     u[1] = 0.02 * rng.standard_normal(shape)
-    if Q == 27:
+    if Q != 9:
         u[2] = 0.02 * rng.standard_normal(shape)
     rho = 1 + 0.01 * rng.standard_normal(shape)
     F0 = npr.init_feq(shape, cfg, rho, u)
@@ -152,6 +156,11 @@ PHYS_CASES = {
     "d3q27_sphere_mrt": lambda: (dict(cases.d3q27_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64,
                                                                   collision=MRT_GS)[0]),
                                  geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+    "d3q19_sphere": lambda: (dict(cases.d3q19_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64)[0]),
+                             geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+    "d3q19_sphere_mrt": lambda: (dict(cases.d3q19_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64,
+                                                                  collision=MRT_GS)[0]),
+                                 geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
 }
 
 
@@ -164,7 +173,7 @@ def test_oracle_matches_numpy_restatement_physical_mode(name):
     u = np.zeros((3,) + shape)
     u[0] = 0.05 + 0.01 * rng.standard_normal(shape)
     u[1] = 0.01 * rng.standard_normal(shape)
-    if cfg["lattice"] == 27:
+    if cfg["lattice"] != 9:
         u[2] = 0.01 * rng.standard_normal(shape)
     F0 = npr.init_feq(shape, cfg, None, u)
     f_or = orc.run(cfg, types.reshape(-1), F0.reshape(-1), 5)
@@ -172,9 +181,9 @@ def test_oracle_matches_numpy_restatement_physical_mode(name):
     assert rel_l2(f_or, f_np) < 1e-13, rel_l2(f_or, f_np)
 
 
-def _read_int_table(name):
+def _read_int_table(name, n=27):
     s = open(os.path.join(ROOT, "lbm-arrayfire_amd", "csrc", "mrt_tables.h")).read()
-    body = re.search(name + r"\[27\]\[27\] = \{(.*?)\};", s, re.S).group(1)
+    body = re.search(name + r"\[%d\]\[%d\] = \{(.*?)\};" % (n, n), s, re.S).group(1)
     return np.array([[int(v) for v in r.split(",") if v.strip()] for r in re.findall(r"\{([^{}]*)\}", body)])
 
 
@@ -388,3 +397,52 @@ def test_force_restatement_closed_form():
         assert np.allclose(got, 2 * rho * u, rtol=0, atol=1e-15)
         # nothing inside an empty box
         assert np.all(force(F, types, dict(lattice=Q), box=(0, 1, 0, 1, 0, 1)) == 0)
+
+
+def _d19_cfg(s, coll=MRT_GS):
+    return dict(lattice=19, nx=4, ny=3, nz=3, collision=coll, storage=F64, bc_mode=BC_REFERENCE, s=list(s))
+
+
+def test_d3q19_basis_known_answers():
+    """d'Humieres et al. (2002): rows mutually orthogonal with squared norms 19, 2394, 252, 10, 40
+    (j, q), 36, 72, 12, 24, 4, 8; e = -30/-11/8 and eps = 12/-4/1 on rest/face/edge directions."""
+    M, Mi = orc.mrt_matrices(3)
+    G = M @ M.T
+    assert np.allclose(G, np.diag(np.diag(G)), atol=1e-12)
+    assert np.allclose(np.diag(G), [19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8])
+    assert np.allclose(Mi, M.T / np.diag(G), atol=1e-15)
+    assert list(M[1, [0, 1, 7]]) == [-30, -11, 8] and list(M[2, [0, 1, 7]]) == [12, -4, 1]
+    assert np.array_equal(M, npr.d3q19_moment_matrix())
+    gen = _read_int_table("M19", 19)
+    assert np.array_equal(gen, M)
+
+
+def test_d3q19_mrt_zero_rates_is_pure_streaming():
+    cfg = _d19_cfg([0.0] * 27)
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(15)
+    f = 1 / 19 + 0.01 * rng.random(36 * 19)
+    assert rel_l2(orc.run(cfg, types, f, 2), orc.run(dict(cfg, collision=BGK, omega=0.0), types, f, 2)) < 1e-14
+
+
+def test_d3q19_mrt_uniform_rate_is_bgk():
+    om = 1.37
+    cfg = _d19_cfg([om] * 27)
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(16)
+    f = orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(36), *(0.03 * rng.standard_normal((3, 36))))
+    f = f + 1e-4 * rng.standard_normal(f.size)
+    assert rel_l2(orc.run(cfg, types, f, 3), orc.run(dict(cfg, collision=BGK, omega=om), types, f, 3)) < 1e-13
+
+
+def test_d3q19_mrt_conserves_mass_and_momentum():
+    cfg = _d19_cfg(cases.dhumieres_rates(1.6))
+    types = np.zeros(36, np.uint8)
+    rng = np.random.default_rng(17)
+    f = orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(36), *(0.03 * rng.standard_normal((3, 36))))
+    f = f + 1e-4 * rng.standard_normal(f.size)
+    g = orc.run(cfg, types, f, 5)
+    c, _, _ = orc.tables(19)
+    F0, F1 = f.reshape(19, -1), g.reshape(19, -1)
+    assert abs(F1.sum() - F0.sum()) < 1e-13
+    assert np.allclose(c.astype(float) @ F1.sum(1), c.astype(float) @ F0.sum(1), atol=1e-14)
