@@ -100,6 +100,31 @@ def measure(name, steps, warmup, repeats=0):
                 repeat_mlups=rep, nonfinite=int(bad))
 
 
+def shanchen_leg(steps=200, warmup=20, n=1024):
+    """Two-component Shan-Chen D2Q9 on an n x n periodic lattice (the reference's
+    cpp/d2q9_shanchen.cpp at a GPU-sized lattice), G = -1.2, omega = 1, rho = 1 +- 0.001 noise.
+    Algorithmic bytes per node update: both fluids' populations read and written once,
+    2 x 2 x 9 s."""
+    import numpy as np
+    from lbm_amd.capi import F32, F64, ShanChen
+
+    out = {}
+    rng = np.random.default_rng(0)
+    d = -0.001 * (1 - 2 * rng.random(n * n))
+    for storage, tag, s in ((F32, "fp32", 4), (F64, "fp64", 8)):
+        with ShanChen(n, n, 1.0, 1.0, -1.2, storage) as sc:
+            sc.init(1 + d, 1 - d)
+            sc.step(warmup)
+            ms = sc.step_timed(steps)
+            r1, r2 = sc.fields()[:2]
+        gbs = 2 * 2 * 9 * s * n * n * steps / (ms * 1e-3) / 1e9
+        out[f"d2q9_shanchen_{n}x{n}_{tag}"] = {
+            "mlups": round(n * n * steps / (ms * 1e3), 1), "ms_per_step": round(ms / steps, 5),
+            "kernels_per_step": 2, "achieved_gbs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "steps": steps, "nonfinite": int((~np.isfinite(r1)).sum() + (~np.isfinite(r2)).sum())}
+    return out
+
+
 def kernel_name(cfg, k):
     """The kernel the timed launches run (as rocprofv3 lists it, template arguments abridged)."""
     if k > 1:
@@ -277,6 +302,10 @@ def main():
         except Exception as e:  # noqa: BLE001 — report, keep the headline line
             also[k] = {"error": str(e)}
     serving = serving_leg(args.serve_jobs, args.serve_scale) if args.serve_jobs > 0 else None
+    try:
+        shanchen = None if args.also == "none" else shanchen_leg()
+    except Exception as e:  # noqa: BLE001 — report, keep the headline line
+        shanchen = {"error": str(e)}
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     cfg = r["cfg"]
     out = {
@@ -313,6 +342,7 @@ def main():
         "also": also,
         "ceilings_d3q27": ceilings,
         "serving": serving,
+        "shanchen": shanchen,
     }
     print(json.dumps(out))
 

@@ -278,6 +278,42 @@ typedef struct {
 lbm_status lbm_server_get_stats(lbm_server s, lbm_server_stats *out);
 lbm_status lbm_server_destroy(lbm_server s);
 
+/* ---- two-component Shan-Chen D2Q9 (cpp/d2q9_shanchen.cpp:39-181) --------------------------
+ * Two BGK populations f (fluid 1) and g (fluid 2) on a fully periodic nx x ny D2Q9 lattice,
+ * coupled through the common velocity u' = (omega1 j1 + omega2 j2) / (omega1 rho1 + omega2 rho2)
+ * and the pseudopotential interaction psi = rho: fluid s relaxes towards f_eq(rho_s, u_s) with
+ * u_s = u' - (G / omega_s) sum_{i>=1} w_i rho_t(x - c_i) c_i (t = the other fluid; densities after
+ * streaming).  This is the reference program with its work-in-progress defects corrected: the
+ * populations are streamed before the collision, each fluid's interaction uses the other fluid's
+ * density, the interaction sums start from zero every step, and u' weights the momenta j_s (the
+ * reference divides them by rho_s first).  Storage F64 or F32.  No reference counterpart for the
+ * handle itself: the reference runs the loop inline. */
+typedef struct lbm_sc_ctx *lbm_sc_handle;
+typedef struct {
+  int32_t nx, ny;    /* periodic D2Q9 lattice */
+  int32_t storage;   /* LBM_F64 or LBM_F32 */
+  int32_t pad_;
+  double omega1;     /* relaxation rate of fluid 1 */
+  double omega2;     /* relaxation rate of fluid 2 */
+  double g;          /* interaction amplitude G (the reference uses -1.2) */
+} lbm_sc_desc;
+lbm_status lbm_sc_create(const lbm_sc_desc *d, lbm_sc_handle *out);
+/* f = w_i rho1, g = w_i rho2 (both at rest), the reference's initial state (:76-79);
+ * rho1, rho2: N doubles each, x fastest. */
+lbm_status lbm_sc_init(lbm_sc_handle h, const double *rho1, const double *rho2);
+/* Enqueues nsteps steps (stream both fluids, densities, interaction, collision); asynchronous. */
+lbm_status lbm_sc_step(lbm_sc_handle h, int64_t nsteps);
+/* As lbm_sc_step, then waits; *ms = device time of the launches (HIP events). */
+lbm_status lbm_sc_step_timed(lbm_sc_handle h, int64_t nsteps, double *ms);
+/* Fields after the last step, N doubles each (any may be NULL): the fluids' densities and the
+ * mixture velocity (j1 + j2) / (rho1 + rho2) of the streamed populations.  Blocks. */
+lbm_status lbm_sc_get_fields(lbm_sc_handle h, double *rho1, double *rho2, double *ux, double *uy);
+/* Populations as doubles: 9 N of fluid 1 then 9 N of fluid 2, population i of node n at
+ * n + N i.  Blocks. */
+lbm_status lbm_sc_get_populations(lbm_sc_handle h, double *f);
+lbm_status lbm_sc_set_populations(lbm_sc_handle h, const double *f);
+lbm_status lbm_sc_destroy(lbm_sc_handle h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,390 @@
+// lbm_shanchen.hip — two-component Shan-Chen D2Q9 (cpp/d2q9_shanchen.cpp:39-181), with the
+// reference's work-in-progress defects corrected (streaming before the collision, each fluid's
+// interaction from the other fluid's density, interaction sums restarted every step, momenta in
+// the common velocity), and its C ABI (lbm_sc_*).
+//
+// Per step, two launches on the handle's stream:
+//   k_sc_density  pull both fluids (periodic), rho1 and rho2 of the streamed populations into a
+//                 density buffer (2 N values)
+//   k_sc_collide  pull both fluids again, moments, common velocity
+//                 u' = (w1 j1 + w2 j2) / (w1 rho1 + w2 rho2), interaction
+//                 u_s = u' - (G / omega_s) sum_{i>=1} w_i rho_t(x - c_i) c_i from the neighbours'
+//                 densities, BGK relaxation of each fluid towards f_eq(rho_s, u_s), store
+// HBM traffic per node update: 2 x 2 x 9 s for the populations (the algorithmic part), 2 x 9 s
+// more for the density pass's pulls and 4 s for the density buffer; the second pull of a node's
+// populations mostly hits L2 only when the lattice fits it.
+//
+// Arithmetic follows the single-fluid kernels: FP64 term by term, FP32 with the contracted
+// equilibrium, the relaxation fma(omega, f_eq - f, f) and the reciprocal of the common
+// velocity's denominator; sums run over ascending i.  The oracle (oracle_sc_run) evaluates the
+// same sequence.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lbm_device.hpp"
+#include "lbm_handle.hpp"
+
+namespace lbm {
+namespace {
+
+struct ScParams {
+  const void *fin;
+  void *fout;
+  void *rho;       // 2 N values: rho1 then rho2
+  int32_t nx, ny;
+  int64_t N, stride;  // populations of fluid 2 start at plane 9
+  double omega1, omega2, gw1, gw2;  // gw_s = G / omega_s
+};
+
+template <class T>
+__device__ __forceinline__ void sc_pull(const ScParams &p, int x, int y, int base, T *f) {
+  const T *in = (const T *)p.fin;
+  const int xm = x == 0 ? p.nx - 1 : x - 1, xp = x == p.nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
+  static_for<9>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int cx = Lat<9>::cx[i], cy = Lat<9>::cy[i];
+    const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
+    const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
+    f[i] = in[(size_t)(base + i) * (size_t)p.stride + (size_t)sx + (size_t)p.nx * (size_t)sy];
+  });
+}
+
+template <class T>
+__device__ __forceinline__ T sc_sum(const T *f) {
+  T r = (T)-0.0;
+  static_for<9>([&](auto I) { r = r + f[decltype(I)::value]; });
+  return r;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sc_density(ScParams p) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= p.N) return;
+  const int x = (int)(n % p.nx), y = (int)(n / p.nx);
+  T f[9], g[9];
+  sc_pull<T>(p, x, y, 0, f);
+  sc_pull<T>(p, x, y, 9, g);
+  T *rho = (T *)p.rho;
+  rho[n] = sc_sum(f);
+  rho[p.N + n] = sc_sum(g);
+}
+
+// sum_{i>=1} w_i rho(x - c_i) c_i of one fluid's density field (periodic), ascending i
+template <class T>
+__device__ __forceinline__ void sc_contrib(const ScParams &p, const T *rho, int x, int y, T &cx_out, T &cy_out) {
+  const int xm = x == 0 ? p.nx - 1 : x - 1, xp = x == p.nx - 1 ? 0 : x + 1;
+  const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
+  T ax = (T)-0.0, ay = (T)-0.0;
+  static_for<8>([&](auto I) {
+    constexpr int i = decltype(I)::value + 1;
+    constexpr int cx = Lat<9>::cx[i], cy = Lat<9>::cy[i];
+    const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
+    const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
+    const T t = (T)Lat<9>::w[i] * rho[(size_t)sx + (size_t)p.nx * (size_t)sy];
+    if constexpr (cx == 1) ax = ax + t;
+    if constexpr (cx == -1) ax = ax - t;
+    if constexpr (cy == 1) ay = ay + t;
+    if constexpr (cy == -1) ay = ay - t;
+  });
+  cx_out = ax;
+  cy_out = ay;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sc_collide(ScParams p) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= p.N) return;
+  const int x = (int)(n % p.nx), y = (int)(n / p.nx);
+  T f[9], g[9];
+  sc_pull<T>(p, x, y, 0, f);
+  sc_pull<T>(p, x, y, 9, g);
+  const T r1 = sc_sum(f), r2 = sc_sum(g);
+  T j1x = (T)-0.0, j1y = (T)-0.0, j2x = (T)-0.0, j2y = (T)-0.0;
+  static_for<9>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    j1x = j1x + c_times<Lat<9>::cx[i]>(f[i]);
+    j1y = j1y + c_times<Lat<9>::cy[i]>(f[i]);
+    j2x = j2x + c_times<Lat<9>::cx[i]>(g[i]);
+    j2y = j2y + c_times<Lat<9>::cy[i]>(g[i]);
+  });
+  const T w1 = (T)p.omega1, w2 = (T)p.omega2;
+  const T den = w1 * r1 + w2 * r2;
+  T upx, upy;
+  if constexpr (sizeof(T) == 4) {
+    const T inv = recip_rn(den);
+    upx = (w1 * j1x + w2 * j2x) * inv;
+    upy = (w1 * j1y + w2 * j2y) * inv;
+  } else {
+    upx = (w1 * j1x + w2 * j2x) / den;
+    upy = (w1 * j1y + w2 * j2y) / den;
+  }
+  const T *rho = (const T *)p.rho;
+  T c1x, c1y, c2x, c2y;
+  sc_contrib<T>(p, rho, x, y, c1x, c1y);
+  sc_contrib<T>(p, rho + p.N, x, y, c2x, c2y);
+  const T u1x = upx - (T)p.gw1 * c2x, u1y = upy - (T)p.gw1 * c2y;
+  const T u2x = upx - (T)p.gw2 * c1x, u2y = upy - (T)p.gw2 * c1y;
+  const T a1 = usq_of<9>(u1x, u1y, (T)0), a2 = usq_of<9>(u2x, u2y, (T)0);
+  T *out = (T *)p.fout;
+  const T om1 = (T)1 - w1, om2 = (T)1 - w2;
+  static_for<9>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    const T e1 = feq_i<9, i>(r1, u1x, u1y, (T)0, a1), e2 = feq_i<9, i>(r2, u2x, u2y, (T)0, a2);
+    T o1, o2;
+    if constexpr (sizeof(T) == 4) {
+      o1 = __builtin_fmaf(w1, e1 - f[i], f[i]);
+      o2 = __builtin_fmaf(w2, e2 - g[i], g[i]);
+    } else {
+      o1 = w1 * e1 + om1 * f[i];
+      o2 = w2 * e2 + om2 * g[i];
+    }
+    out[(size_t)i * (size_t)p.stride + (size_t)n] = o1;
+    out[(size_t)(9 + i) * (size_t)p.stride + (size_t)n] = o2;
+  });
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sc_fields(ScParams p, double *rho1, double *rho2, double *ux, double *uy) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= p.N) return;
+  const T *in = (const T *)p.fin;
+  T r1 = (T)-0.0, r2 = (T)-0.0, jx = (T)-0.0, jy = (T)-0.0;
+  static_for<9>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    const T a = in[(size_t)i * (size_t)p.stride + (size_t)n], b = in[(size_t)(9 + i) * (size_t)p.stride + (size_t)n];
+    r1 = r1 + a;
+    r2 = r2 + b;
+    jx = jx + c_times<Lat<9>::cx[i]>(a) + c_times<Lat<9>::cx[i]>(b);
+    jy = jy + c_times<Lat<9>::cy[i]>(a) + c_times<Lat<9>::cy[i]>(b);
+  });
+  rho1[n] = (double)r1;
+  rho2[n] = (double)r2;
+  ux[n] = (double)jx / ((double)r1 + (double)r2);
+  uy[n] = (double)jy / ((double)r1 + (double)r2);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sc_init(ScParams p, const double *rho1, const double *rho2) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= p.N) return;
+  T *a = (T *)p.fin, *b = (T *)p.fout;
+  static_for<9>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    const T w = (T)Lat<9>::w[i];
+    const T f = w * (T)rho1[n], g = w * (T)rho2[n];
+    a[(size_t)i * (size_t)p.stride + (size_t)n] = f;
+    a[(size_t)(9 + i) * (size_t)p.stride + (size_t)n] = g;
+    b[(size_t)i * (size_t)p.stride + (size_t)n] = f;
+    b[(size_t)(9 + i) * (size_t)p.stride + (size_t)n] = g;
+  });
+}
+
+}  // namespace
+}  // namespace lbm
+
+struct lbm_sc_ctx {
+  lbm_sc_desc d{};
+  int device = 0;
+  int64_t N = 0, stride = 0;
+  int esize = 8;
+  void *f[2] = {nullptr, nullptr};
+  void *rho = nullptr;
+  double *fields = nullptr;  // 4 N, lbm_sc_get_fields staging
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  lbm::ScParams params() const {
+    lbm::ScParams p{};
+    p.fin = f[cur];
+    p.fout = f[cur ^ 1];
+    p.rho = rho;
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.N = N;
+    p.stride = stride;
+    p.omega1 = d.omega1;
+    p.omega2 = d.omega2;
+    p.gw1 = d.g / d.omega1;
+    p.gw2 = d.g / d.omega2;
+    return p;
+  }
+  size_t buf_bytes() const { return (size_t)stride * 18 * (size_t)esize; }
+  unsigned grid() const { return (unsigned)((N + 255) / 256); }
+};
+
+namespace {
+
+lbm_status sc_fail(lbm_status s, const std::string &m) { return lbm::set_error(s, m); }
+
+#define SCCHK(call)                                                                               \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return sc_fail(LBM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class F>
+hipError_t sc_typed(const lbm_sc_ctx *h, F &&f) {
+  return h->d.storage == LBM_F64 ? f((double)0) : f((float)0);
+}
+
+lbm_status sc_steps(lbm_sc_handle h, int64_t nsteps) {
+  for (int64_t t = 0; t < nsteps; ++t) {
+    const lbm::ScParams p = h->params();
+    SCCHK(sc_typed(h, [&](auto z) {
+      using T = decltype(z);
+      hipLaunchKernelGGL(lbm::k_sc_density<T>, dim3(h->grid()), dim3(256), 0, h->stream, p);
+      hipLaunchKernelGGL(lbm::k_sc_collide<T>, dim3(h->grid()), dim3(256), 0, h->stream, p);
+      return hipGetLastError();
+    }));
+    h->cur ^= 1;
+  }
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lbm_status lbm_sc_create(const lbm_sc_desc *d, lbm_sc_handle *out) {
+  if (!out) return sc_fail(LBM_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!d) return sc_fail(LBM_EINVAL, "descriptor is NULL");
+  if (d->nx < 1 || d->ny < 1) return sc_fail(LBM_EINVAL, "lattice dimensions must be >= 1");
+  if ((uint64_t)d->nx * (uint64_t)d->ny >= (1ull << 31)) return sc_fail(LBM_EINVAL, "lattice too large");
+  if (d->storage != LBM_F64 && d->storage != LBM_F32) return sc_fail(LBM_EINVAL, "Shan-Chen storage must be F64 or F32");
+  if (!(d->omega1 > 0 && d->omega1 < 2 && d->omega2 > 0 && d->omega2 < 2))
+    return sc_fail(LBM_EINVAL, "relaxation rates must lie in (0, 2)");
+  if (!std::isfinite(d->g)) return sc_fail(LBM_EINVAL, "G is not finite");
+  lbm_sc_ctx *h = new (std::nothrow) lbm_sc_ctx();
+  if (!h) return sc_fail(LBM_ENOMEM, "host allocation failed");
+  h->d = *d;
+  h->N = (int64_t)d->nx * d->ny;
+  h->esize = d->storage == LBM_F64 ? 8 : 4;
+  h->stride = lbm::population_stride(h->N, h->esize);
+  auto cleanup = [&](lbm_status s, const std::string &m) {
+    lbm_sc_destroy(h);
+    return sc_fail(s, m);
+  };
+  if (hipGetDevice(&h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipGetDevice failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "hipEventCreate failed");
+  for (int b = 0; b < 2; ++b)
+    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess) return cleanup(LBM_ENOMEM, "population buffer allocation failed");
+  if (hipMalloc(&h->rho, 2 * (size_t)h->N * (size_t)h->esize) != hipSuccess ||
+      hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)) != hipSuccess)
+    return cleanup(LBM_ENOMEM, "density buffer allocation failed");
+  *out = h;
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_init(lbm_sc_handle h, const double *rho1, const double *rho2) {
+  if (!h || !rho1 || !rho2) return sc_fail(LBM_EINVAL, "handle or density is NULL");
+  SCCHK(hipSetDevice(h->device));
+  const size_t nb = (size_t)h->N * sizeof(double);
+  SCCHK(hipMemcpyAsync(h->fields, rho1, nb, hipMemcpyHostToDevice, h->stream));
+  SCCHK(hipMemcpyAsync(h->fields + h->N, rho2, nb, hipMemcpyHostToDevice, h->stream));
+  h->cur = 0;
+  const lbm::ScParams p = h->params();
+  SCCHK(sc_typed(h, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(lbm::k_sc_init<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, h->fields, h->fields + h->N);
+    return hipGetLastError();
+  }));
+  SCCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_step(lbm_sc_handle h, int64_t nsteps) {
+  if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
+  if (nsteps < 0) return sc_fail(LBM_EINVAL, "nsteps < 0");
+  SCCHK(hipSetDevice(h->device));
+  return sc_steps(h, nsteps);
+}
+
+lbm_status lbm_sc_step_timed(lbm_sc_handle h, int64_t nsteps, double *ms) {
+  if (!h || !ms) return sc_fail(LBM_EINVAL, "handle or ms is NULL");
+  SCCHK(hipSetDevice(h->device));
+  SCCHK(hipEventRecord(h->ev0, h->stream));
+  const lbm_status st = sc_steps(h, nsteps);
+  if (st != LBM_OK) return st;
+  SCCHK(hipEventRecord(h->ev1, h->stream));
+  SCCHK(hipEventSynchronize(h->ev1));
+  float t = 0;
+  SCCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  *ms = t;
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_get_fields(lbm_sc_handle h, double *rho1, double *rho2, double *ux, double *uy) {
+  if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
+  SCCHK(hipSetDevice(h->device));
+  lbm::ScParams p = h->params();
+  double *F = h->fields;
+  SCCHK(sc_typed(h, [&](auto z) {
+    using T = decltype(z);
+    hipLaunchKernelGGL(lbm::k_sc_fields<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, F, F + h->N, F + 2 * h->N,
+                       F + 3 * h->N);
+    return hipGetLastError();
+  }));
+  double *dst[4] = {rho1, rho2, ux, uy};
+  for (int k = 0; k < 4; ++k)
+    if (dst[k]) SCCHK(hipMemcpyAsync(dst[k], F + (size_t)k * h->N, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  SCCHK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_get_populations(lbm_sc_handle h, double *f) {
+  if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
+  SCCHK(hipSetDevice(h->device));
+  SCCHK(hipStreamSynchronize(h->stream));
+  std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize);
+  SCCHK(hipMemcpy(tmp.data(), h->f[h->cur], tmp.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 18; ++i)
+    for (int64_t n = 0; n < h->N; ++n) {
+      const size_t src = (size_t)i * (size_t)h->stride + (size_t)n;
+      f[(size_t)i * (size_t)h->N + (size_t)n] =
+          h->esize == 8 ? ((const double *)tmp.data())[src] : (double)((const float *)tmp.data())[src];
+    }
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_set_populations(lbm_sc_handle h, const double *f) {
+  if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
+  SCCHK(hipSetDevice(h->device));
+  SCCHK(hipStreamSynchronize(h->stream));
+  std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize, 0);
+  for (int i = 0; i < 18; ++i)
+    for (int64_t n = 0; n < h->N; ++n) {
+      const size_t dst = (size_t)i * (size_t)h->stride + (size_t)n;
+      const double v = f[(size_t)i * (size_t)h->N + (size_t)n];
+      if (h->esize == 8) ((double *)tmp.data())[dst] = v;
+      else ((float *)tmp.data())[dst] = (float)v;
+    }
+  SCCHK(hipMemcpy(h->f[h->cur], tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+lbm_status lbm_sc_destroy(lbm_sc_handle h) {
+  if (!h) return LBM_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (int b = 0; b < 2; ++b)
+    if (h->f[b]) (void)hipFree(h->f[b]);
+  if (h->rho) (void)hipFree(h->rho);
+  if (h->fields) (void)hipFree(h->fields);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return LBM_OK;
+}
+
+}  // extern "C"

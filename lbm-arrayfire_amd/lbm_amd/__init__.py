@@ -4,4 +4,4 @@ The compute path is the HIP library behind include/lbm_capi.h; this package only
 (ctypes), builds node-type masks and case descriptors, and drives batches and the job server.
 """
 from . import capi, cases, geometry  # noqa: F401
-from .capi import Batch, Lattice, LbmError, Server, load  # noqa: F401
+from .capi import Batch, Lattice, LbmError, Server, ShanChen, load  # noqa: F401

@@ -41,6 +41,8 @@ EXPORTS = [
     "lbm_server_create", "lbm_server_submit", "lbm_server_tick", "lbm_server_poll", "lbm_server_take",
     "lbm_server_cancel", "lbm_server_set_check_every", "lbm_server_used_bytes", "lbm_server_clock",
     "lbm_server_get_stats", "lbm_server_destroy",
+    "lbm_sc_create", "lbm_sc_init", "lbm_sc_step", "lbm_sc_step_timed", "lbm_sc_get_fields",
+    "lbm_sc_get_populations", "lbm_sc_set_populations", "lbm_sc_destroy",
 ]
 
 
@@ -81,6 +83,12 @@ class ServerStats(ctypes.Structure):
                 ("batch_rebuilds", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
                 ("admit_s", ctypes.c_double), ("retire_s", ctypes.c_double), ("launch_s", ctypes.c_double),
                 ("wait_s", ctypes.c_double), ("check_s", ctypes.c_double)]
+
+
+class ScDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("omega1", ctypes.c_double), ("omega2", ctypes.c_double),
+                ("g", ctypes.c_double)]
 
 
 def make_desc(cfg):
@@ -171,6 +179,14 @@ def load(path=None):
     L.lbm_server_clock.restype = ctypes.c_double
     L.lbm_server_get_stats.argtypes = [H, ctypes.POINTER(ServerStats)]
     L.lbm_server_destroy.argtypes = [H]
+    L.lbm_sc_create.argtypes = [ctypes.POINTER(ScDesc), ctypes.POINTER(H)]
+    L.lbm_sc_init.argtypes = [H, dp, dp]
+    L.lbm_sc_step.argtypes = [H, ctypes.c_int64]
+    L.lbm_sc_step_timed.argtypes = [H, ctypes.c_int64, dp]
+    L.lbm_sc_get_fields.argtypes = [H, dp, dp, dp, dp]
+    L.lbm_sc_get_populations.argtypes = [H, dp]
+    L.lbm_sc_set_populations.argtypes = [H, dp]
+    L.lbm_sc_destroy.argtypes = [H]
     _lib = L
     return L
 
@@ -430,6 +446,69 @@ class Server:
     def close(self):
         if self.h:
             self.lib.lbm_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ShanChen:
+    """Two-component Shan-Chen D2Q9 on a periodic lattice (lbm_sc_*), the reference's
+    cpp/d2q9_shanchen.cpp:39-181: fluids of relaxation rates omega1, omega2 coupled through
+    G sum_i w_i rho_other(x - c_i) c_i in each fluid's equilibrium velocity."""
+
+    def __init__(self, nx, ny, omega1=1.0, omega2=1.0, g=-1.2, storage=F64):
+        self.lib = load()
+        self.nx, self.ny, self.N = int(nx), int(ny), int(nx) * int(ny)
+        d = ScDesc(int(nx), int(ny), int(storage), 0, float(omega1), float(omega2), float(g))
+        self.h = ctypes.c_void_p()
+        check(self.lib.lbm_sc_create(ctypes.byref(d), ctypes.byref(self.h)))
+
+    def init(self, rho1, rho2):
+        """Both fluids at rest: f_i = w_i rho_s."""
+        r1 = np.ascontiguousarray(rho1, dtype=np.float64).reshape(-1)
+        r2 = np.ascontiguousarray(rho2, dtype=np.float64).reshape(-1)
+        assert r1.size == self.N and r2.size == self.N
+        check(self.lib.lbm_sc_init(self.h, _dp(r1), _dp(r2)))
+
+    def step(self, n=1):
+        check(self.lib.lbm_sc_step(self.h, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_double()
+        check(self.lib.lbm_sc_step_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def fields(self):
+        """(rho1, rho2, ux, uy) with u the barycentric velocity (j1 + j2) / (rho1 + rho2)."""
+        out = [np.zeros(self.N) for _ in range(4)]
+        check(self.lib.lbm_sc_get_fields(self.h, *[_dp(a) for a in out]))
+        return tuple(out)
+
+    def populations(self):
+        """18 N values: fluid 1's populations 0..8 then fluid 2's, population i of node n at
+        n + N i."""
+        f = np.zeros(18 * self.N)
+        check(self.lib.lbm_sc_get_populations(self.h, _dp(f)))
+        return f
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        assert f.size == 18 * self.N
+        check(self.lib.lbm_sc_set_populations(self.h, _dp(f)))
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_sc_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __enter__(self):

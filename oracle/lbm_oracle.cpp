@@ -702,9 +702,133 @@ double time_impl(const oracle_desc *d, const uint8_t *types, int64_t nsteps) {
 
 bool is_f64(const oracle_desc *d) { return d->storage == F64; }
 
+// ---- two-component Shan-Chen D2Q9 ----------------------------------------------------------
+// cpp/d2q9_shanchen.cpp:39-181 with its work-in-progress defects corrected: both fluids are pulled
+// (streamed) before the collision (the reference collides its unstreamed f and never applies the
+// streamed copy, :117-118,175-176), each fluid's interaction sum uses the other fluid's density
+// (the reference builds both from rho1, :145-149), the sums restart from zero every step (the
+// reference accumulates them across iterations, :84-89,145), and the common velocity weights the
+// momenta j_s = sum_i c_i f_i (the reference divides them by rho_s first, :128-131):
+//   u' = (omega1 j1 + omega2 j2) / (omega1 rho1 + omega2 rho2)                       (:133-135)
+//   u_1 = u' - (G / omega1) sum_{i>=1} w_i rho2(x - c_i) c_i, u_2 likewise with rho1  (:141-157)
+//   f_out = f - omega (f - f_eq(rho, u_s)) per fluid                                   (:161-172)
+// Evaluated node by node in the kernels' operation order (FP32: contracted equilibrium, relaxation
+// fma(omega, f_eq - f, f), u' through the correctly rounded reciprocal of its denominator).
+typedef struct {
+  int32_t nx, ny, storage, pad_;
+  double omega1, omega2, g;
+} oracle_sc_desc;
+
+template <class T>
+T sc_usq(T ux, T uy) {
+  if constexpr (std::is_same<T, float>::value) return std::fma(-1.5f, std::fma(uy, uy, ux * ux), 1.0f);
+  else return ux * ux + uy * uy;
+}
+
+template <class T>
+T sc_feq(const Lattice &L, int i, T rho, T ux, T uy, T usq) {
+  const T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
+  if constexpr (std::is_same<T, float>::value) {
+    const T t = std::fma(4.5f, eu, 3.0f);
+    return ((T)L.w[i] * rho) * std::fma(eu, t, usq);
+  } else {
+    return ((T)L.w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+  }
+}
+
+template <class T>
+void sc_run_impl(const oracle_sc_desc *d, double *f, int64_t nsteps) {
+  const Lattice L = make_lattice(9);
+  const int nx = d->nx, ny = d->ny;
+  const int64_t N = (int64_t)nx * ny;
+  std::vector<T> F((size_t)(18 * N)), S((size_t)(18 * N)), rho((size_t)(2 * N));
+  for (size_t k = 0; k < F.size(); ++k) F[k] = (T)f[k];
+  const T w1 = (T)d->omega1, w2 = (T)d->omega2, gw1 = (T)(d->g / d->omega1), gw2 = (T)(d->g / d->omega2);
+  auto wrap = [](int a, int n) { return ((a % n) + n) % n; };
+  for (int64_t t = 0; t < nsteps; ++t) {
+    // stream both fluids (pull), then the densities of the streamed populations
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const int64_t n = x + (int64_t)nx * y;
+        for (int i = 0; i < 9; ++i) {
+          const int64_t src = wrap(x - L.cx[i], nx) + (int64_t)nx * wrap(y - L.cy[i], ny);
+          S[(size_t)(i * N + n)] = F[(size_t)(i * N + src)];
+          S[(size_t)((9 + i) * N + n)] = F[(size_t)((9 + i) * N + src)];
+        }
+        T r1 = S[(size_t)n], r2 = S[(size_t)(9 * N + n)];
+        for (int i = 1; i < 9; ++i) { r1 = r1 + S[(size_t)(i * N + n)]; r2 = r2 + S[(size_t)((9 + i) * N + n)]; }
+        rho[(size_t)n] = r1;
+        rho[(size_t)(N + n)] = r2;
+      }
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const int64_t n = x + (int64_t)nx * y;
+        const T *sf = &S[0], *sg = &S[(size_t)(9 * N)];
+        const T r1 = rho[(size_t)n], r2 = rho[(size_t)(N + n)];
+        T j1x = -0.0, j1y = -0.0, j2x = -0.0, j2y = -0.0;
+        for (int i = 0; i < 9; ++i) {
+          const T a = sf[(size_t)(i * N + n)], b = sg[(size_t)(i * N + n)];
+          if (L.cx[i] == 1) { j1x = j1x + a; j2x = j2x + b; }
+          if (L.cx[i] == -1) { j1x = j1x - a; j2x = j2x - b; }
+          if (L.cy[i] == 1) { j1y = j1y + a; j2y = j2y + b; }
+          if (L.cy[i] == -1) { j1y = j1y - a; j2y = j2y - b; }
+        }
+        const T den = w1 * r1 + w2 * r2;
+        T upx, upy;
+        if constexpr (std::is_same<T, float>::value) {
+          const T inv = (T)1 / den;
+          upx = (w1 * j1x + w2 * j2x) * inv;
+          upy = (w1 * j1y + w2 * j2y) * inv;
+        } else {
+          upx = (w1 * j1x + w2 * j2x) / den;
+          upy = (w1 * j1y + w2 * j2y) / den;
+        }
+        T c[2][2];  // [fluid][axis]: sum_{i>=1} w_i rho_fluid(x - c_i) c_i
+        for (int s = 0; s < 2; ++s) {
+          T ax = -0.0, ay = -0.0;
+          for (int i = 1; i < 9; ++i) {
+            const int64_t src = wrap(x - L.cx[i], nx) + (int64_t)nx * wrap(y - L.cy[i], ny);
+            const T v = (T)L.w[i] * rho[(size_t)(s * N + src)];
+            if (L.cx[i] == 1) ax = ax + v;
+            if (L.cx[i] == -1) ax = ax - v;
+            if (L.cy[i] == 1) ay = ay + v;
+            if (L.cy[i] == -1) ay = ay - v;
+          }
+          c[s][0] = ax;
+          c[s][1] = ay;
+        }
+        const T u1x = upx - gw1 * c[1][0], u1y = upy - gw1 * c[1][1];
+        const T u2x = upx - gw2 * c[0][0], u2y = upy - gw2 * c[0][1];
+        const T a1 = sc_usq(u1x, u1y), a2 = sc_usq(u2x, u2y);
+        for (int i = 0; i < 9; ++i) {
+          const T fi = sf[(size_t)(i * N + n)], gi = sg[(size_t)(i * N + n)];
+          const T e1 = sc_feq(L, i, r1, u1x, u1y, a1), e2 = sc_feq(L, i, r2, u2x, u2y, a2);
+          if constexpr (std::is_same<T, float>::value) {
+            F[(size_t)(i * N + n)] = std::fma(w1, e1 - fi, fi);
+            F[(size_t)((9 + i) * N + n)] = std::fma(w2, e2 - gi, gi);
+          } else {
+            F[(size_t)(i * N + n)] = w1 * e1 + ((T)1 - w1) * fi;
+            F[(size_t)((9 + i) * N + n)] = w2 * e2 + ((T)1 - w2) * gi;
+          }
+        }
+      }
+  }
+  for (size_t k = 0; k < F.size(); ++k) f[k] = (double)F[k];
+}
+
 }  // namespace
 
 extern "C" {
+
+// 18 N populations (fluid 1 then fluid 2, population i of node n at n + N i), advanced nsteps.
+int oracle_sc_run(const oracle_sc_desc *d, double *f, int64_t nsteps) {
+  if (!d || d->nx < 1 || d->ny < 1) return 1;
+  if (d->storage == 0) sc_run_impl<double>(d, f, nsteps);
+  else if (d->storage == 1) sc_run_impl<float>(d, f, nsteps);
+  else return 1;
+  return 0;
+}
+
 
 int oracle_abi(void) { return 1; }
 

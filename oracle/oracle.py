@@ -25,6 +25,12 @@ class OracleDesc(ctypes.Structure):
     ]
 
 
+class ScDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("omega1", ctypes.c_double), ("omega2", ctypes.c_double),
+                ("g", ctypes.c_double)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -60,6 +66,7 @@ def lib():
         L.oracle_bf2f.restype = ctypes.c_float
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_max_threads.restype = ctypes.c_int
+        L.oracle_sc_run.argtypes = [ctypes.POINTER(ScDesc), dp, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -185,3 +192,13 @@ def f2bf(x):
 
 def bf2f(b):
     return lib().oracle_bf2f(int(b))
+
+
+def sc_run(cfg, f, nsteps):
+    """Two-component Shan-Chen D2Q9: 18 N populations advanced nsteps (cfg: nx, ny, storage,
+    omega1, omega2, g)."""
+    d = ScDesc(cfg["nx"], cfg["ny"], cfg.get("storage", 0), 0, cfg["omega1"], cfg["omega2"], cfg["g"])
+    out = np.array(f, dtype=np.float64, copy=True)
+    rc = lib().oracle_sc_run(ctypes.byref(d), _dp(out), int(nsteps))
+    assert rc == 0, rc
+    return out

@@ -243,3 +243,41 @@ def force(F, types, cfg, box=None):
         m = 2.0 * F[i] - np.where(mov, 6.0 * w[i] * rw * float(c[i] @ uw), 0.0)
         out += c[i].astype(float) * m[hit].sum()
     return out
+
+
+def shanchen_step(F, G, cfg):
+    """One step of two-component Shan-Chen D2Q9 (cpp/d2q9_shanchen.cpp:108-176) on periodic
+    (9, ny, nx) arrays F, G of the two fluids, with the reference's work-in-progress defects
+    corrected: both fluids are streamed before the collision, fluid s feels the other fluid's
+    density, the interaction sums restart every step and the common velocity weights momenta:
+        u' = (w1 j1 + w2 j2) / (w1 rho1 + w2 rho2)
+        u_1 = u' - (g / w1) sum_{i>=1} w_i rho2(x - c_i) c_i,  u_2 likewise with rho1
+        out = f - w (f - f_eq(rho, u_s))."""
+    c, w, _ = lattice(9)
+    w1, w2, g = cfg["omega1"], cfg["omega2"], cfg["g"]
+
+    def pull(A):
+        return np.stack([np.roll(A[i], shift=(c[i][1], c[i][0]), axis=(0, 1)) for i in range(9)])
+
+    F, G = pull(F), pull(G)
+    r1, r2 = F.sum(0), G.sum(0)
+    j1 = np.einsum("ia,iyx->ayx", c[:, :2].astype(float), F)
+    j2 = np.einsum("ia,iyx->ayx", c[:, :2].astype(float), G)
+    up = (w1 * j1 + w2 * j2) / (w1 * r1 + w2 * r2)
+
+    def contrib(r):
+        out = np.zeros((2,) + r.shape)
+        for i in range(1, 9):
+            sh = w[i] * np.roll(r, shift=(c[i][1], c[i][0]), axis=(0, 1))  # w_i rho(x - c_i)
+            out[0] += sh * c[i][0]
+            out[1] += sh * c[i][1]
+        return out
+
+    u1 = up - (g / w1) * contrib(r2)
+    u2 = up - (g / w2) * contrib(r1)
+
+    def eq(r, u):
+        u3 = np.concatenate([u, np.zeros((1,) + r.shape)])[:, None]
+        return feq(r[None], u3, c, w)[:, 0]
+
+    return F - w1 * (F - eq(r1, u1)), G - w2 * (G - eq(r2, u2))
