@@ -3,16 +3,14 @@
 // interaction from the other fluid's density, interaction sums restarted every step, momenta in
 // the common velocity), and its C ABI (lbm_sc_*).
 //
-// Per step, two launches on the handle's stream:
-//   k_sc_density  pull both fluids (periodic), rho1 and rho2 of the streamed populations into a
-//                 density buffer (2 N values)
-//   k_sc_collide  pull both fluids again, moments, common velocity
-//                 u' = (w1 j1 + w2 j2) / (w1 rho1 + w2 rho2), interaction
-//                 u_s = u' - (G / omega_s) sum_{i>=1} w_i rho_t(x - c_i) c_i from the neighbours'
-//                 densities, BGK relaxation of each fluid towards f_eq(rho_s, u_s), store
-// HBM traffic per node update: 2 x 2 x 9 s for the populations (the algorithmic part), 2 x 9 s
-// more for the density pass's pulls and 4 s for the density buffer; the second pull of a node's
-// populations mostly hits L2 only when the lattice fits it.
+// Per step, one launch on the handle's stream (k_sc_step): each workgroup pulls both fluids'
+// populations of a 64-column tile (periodic), keeps them in registers and puts both densities of
+// the tile and of a one-node ring around it in LDS; then, per node, the moments, the common
+// velocity u' = (w1 j1 + w2 j2) / (w1 rho1 + w2 rho2), the interaction
+// u_s = u' - (G / omega_s) sum_{i>=1} w_i rho_t(x - c_i) c_i from the neighbours' densities in LDS,
+// BGK relaxation of each fluid towards f_eq(rho_s, u_s), store.  HBM traffic per node update:
+// 2 x 2 x 9 s, both fluids' populations read and written once (the ring's pulls are the
+// neighbouring tiles' populations and mostly hit L2).
 //
 // Arithmetic follows the single-fluid kernels: FP64 term by term, FP32 with the contracted
 // equilibrium, the relaxation fma(omega, f_eq - f, f) and the reciprocal of the common
@@ -34,7 +32,6 @@ namespace {
 struct ScParams {
   const void *fin;
   void *fout;
-  void *rho;       // 2 N values: rho1 then rho2
   int32_t nx, ny;
   int64_t N, stride;  // populations of fluid 2 start at plane 9
   double omega1, omega2, gw1, gw2;  // gw_s = G / omega_s
@@ -61,92 +58,119 @@ __device__ __forceinline__ T sc_sum(const T *f) {
   return r;
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_sc_density(ScParams p) {
-  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (n >= p.N) return;
-  const int x = (int)(n % p.nx), y = (int)(n / p.nx);
-  T f[9], g[9];
-  sc_pull<T>(p, x, y, 0, f);
-  sc_pull<T>(p, x, y, 9, g);
-  T *rho = (T *)p.rho;
-  rho[n] = sc_sum(f);
-  rho[p.N + n] = sc_sum(g);
+// wrapped coordinate of a in [0, n) for a in [-n, 2n) and any n >= 1 (tiles reach 64 nodes past
+// the last row or column, which exceeds 2n only for n < 64: the general remainder then)
+__device__ __forceinline__ int sc_wrap(int a, int n) {
+  if (a < 0) a += n;
+  if (a >= n) a = (a < 2 * n) ? a - n : a % n;
+  return a;
 }
 
-// sum_{i>=1} w_i rho(x - c_i) c_i of one fluid's density field (periodic), ascending i
-template <class T>
-__device__ __forceinline__ void sc_contrib(const ScParams &p, const T *rho, int x, int y, T &cx_out, T &cy_out) {
-  const int xm = x == 0 ? p.nx - 1 : x - 1, xp = x == p.nx - 1 ? 0 : x + 1;
-  const int ym = y == 0 ? p.ny - 1 : y - 1, yp = y == p.ny - 1 ? 0 : y + 1;
-  T ax = (T)-0.0, ay = (T)-0.0;
-  static_for<8>([&](auto I) {
-    constexpr int i = decltype(I)::value + 1;
-    constexpr int cx = Lat<9>::cx[i], cy = Lat<9>::cy[i];
-    const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
-    const int sy = cy == 1 ? ym : (cy == -1 ? yp : y);
-    const T t = (T)Lat<9>::w[i] * rho[(size_t)sx + (size_t)p.nx * (size_t)sy];
-    if constexpr (cx == 1) ax = ax + t;
-    if constexpr (cx == -1) ax = ax - t;
-    if constexpr (cy == 1) ay = ay + t;
-    if constexpr (cy == -1) ay = ay - t;
-  });
-  cx_out = ax;
-  cy_out = ay;
-}
+// One step in one launch.  Workgroup = 256 threads on a SC_TX x (4 RPT) tile: thread t owns column
+// t & 63 and rows (t >> 6) + 4 r.  Phase 1 pulls both fluids' populations of the owned nodes into
+// registers and writes their densities to LDS; threads t < 2 (SC_TX + 2) + 2 TY also compute the
+// densities of one ring node around the tile (pulled and summed, nothing kept).  Phase 2 collides
+// the owned nodes with the 8 neighbour densities of each fluid read from LDS.  Tile slots past
+// the lattice's last row or column hold the densities of the wrapped coordinate, so the ring of
+// a partial tile is what its last valid nodes need.  Populations are read from HBM once per node
+// (ring pulls mostly hit L2: they are the neighbour tiles' nodes, on the same XCD by xcd_tile).
+constexpr int SC_TX = 64;
 
-template <class T>
-__global__ void __launch_bounds__(256) k_sc_collide(ScParams p) {
-  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (n >= p.N) return;
-  const int x = (int)(n % p.nx), y = (int)(n / p.nx);
-  T f[9], g[9];
-  sc_pull<T>(p, x, y, 0, f);
-  sc_pull<T>(p, x, y, 9, g);
-  const T r1 = sc_sum(f), r2 = sc_sum(g);
-  T j1x = (T)-0.0, j1y = (T)-0.0, j2x = (T)-0.0, j2y = (T)-0.0;
-  static_for<9>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    j1x = j1x + c_times<Lat<9>::cx[i]>(f[i]);
-    j1y = j1y + c_times<Lat<9>::cy[i]>(f[i]);
-    j2x = j2x + c_times<Lat<9>::cx[i]>(g[i]);
-    j2y = j2y + c_times<Lat<9>::cy[i]>(g[i]);
-  });
-  const T w1 = (T)p.omega1, w2 = (T)p.omega2;
-  const T den = w1 * r1 + w2 * r2;
-  T upx, upy;
-  if constexpr (sizeof(T) == 4) {
-    const T inv = recip_rn(den);
-    upx = (w1 * j1x + w2 * j2x) * inv;
-    upy = (w1 * j1y + w2 * j2y) * inv;
-  } else {
-    upx = (w1 * j1x + w2 * j2x) / den;
-    upy = (w1 * j1y + w2 * j2y) / den;
+template <class T, int RPT>
+__global__ void __launch_bounds__(256) k_sc_step(ScParams p, int tiles_x, int ntiles) {
+  constexpr int TY = 4 * RPT, LX = SC_TX + 2, LY = TY + 2;
+  __shared__ T lr[2][LY][LX];
+  const int tile = xcd_tile((int)blockIdx.x, ntiles);
+  const int x0 = (tile % tiles_x) * SC_TX, y0 = (tile / tiles_x) * TY;
+  const int t = (int)threadIdx.x, lx = t & 63, ly = t >> 6;
+  const int x = x0 + lx, gx = x < p.nx ? x : sc_wrap(x, p.nx);
+  T f[RPT][9], g[RPT][9];
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = y0 + ly + 4 * r, gy = y < p.ny ? y : sc_wrap(y, p.ny);
+    sc_pull<T>(p, gx, gy, 0, f[r]);
+    sc_pull<T>(p, gx, gy, 9, g[r]);
+    lr[0][ly + 4 * r + 1][lx + 1] = sc_sum(f[r]);
+    lr[1][ly + 4 * r + 1][lx + 1] = sc_sum(g[r]);
   }
-  const T *rho = (const T *)p.rho;
-  T c1x, c1y, c2x, c2y;
-  sc_contrib<T>(p, rho, x, y, c1x, c1y);
-  sc_contrib<T>(p, rho + p.N, x, y, c2x, c2y);
-  const T u1x = upx - (T)p.gw1 * c2x, u1y = upy - (T)p.gw1 * c2y;
-  const T u2x = upx - (T)p.gw2 * c1x, u2y = upy - (T)p.gw2 * c1y;
-  const T a1 = usq_of<9>(u1x, u1y, (T)0), a2 = usq_of<9>(u2x, u2y, (T)0);
+  if (t < 2 * LX + 2 * TY) {
+    int hx, hy;
+    if (t < LX) { hx = t; hy = 0; }
+    else if (t < 2 * LX) { hx = t - LX; hy = LY - 1; }
+    else if (t < 2 * LX + TY) { hx = 0; hy = t - 2 * LX + 1; }
+    else { hx = LX - 1; hy = t - 2 * LX - TY + 1; }
+    const int hgx = sc_wrap(x0 - 1 + hx, p.nx), hgy = sc_wrap(y0 - 1 + hy, p.ny);
+    T h[9];
+    sc_pull<T>(p, hgx, hgy, 0, h);
+    lr[0][hy][hx] = sc_sum(h);
+    sc_pull<T>(p, hgx, hgy, 9, h);
+    lr[1][hy][hx] = sc_sum(h);
+  }
+  __syncthreads();
+  if (x >= p.nx) return;
   T *out = (T *)p.fout;
+  const T w1 = (T)p.omega1, w2 = (T)p.omega2;
   const T om1 = (T)1 - w1, om2 = (T)1 - w2;
-  static_for<9>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    const T e1 = feq_i<9, i>(r1, u1x, u1y, (T)0, a1), e2 = feq_i<9, i>(r2, u2x, u2y, (T)0, a2);
-    T o1, o2;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    const int y = y0 + ly + 4 * r;
+    if (y >= p.ny) break;
+    const int cxl = lx + 1, cyl = ly + 4 * r + 1;
+    const T r1 = lr[0][cyl][cxl], r2 = lr[1][cyl][cxl];
+    T j1x = (T)-0.0, j1y = (T)-0.0, j2x = (T)-0.0, j2y = (T)-0.0;
+    static_for<9>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      j1x = j1x + c_times<Lat<9>::cx[i]>(f[r][i]);
+      j1y = j1y + c_times<Lat<9>::cy[i]>(f[r][i]);
+      j2x = j2x + c_times<Lat<9>::cx[i]>(g[r][i]);
+      j2y = j2y + c_times<Lat<9>::cy[i]>(g[r][i]);
+    });
+    const T den = w1 * r1 + w2 * r2;
+    T upx, upy;
     if constexpr (sizeof(T) == 4) {
-      o1 = __builtin_fmaf(w1, e1 - f[i], f[i]);
-      o2 = __builtin_fmaf(w2, e2 - g[i], g[i]);
+      const T inv = recip_rn(den);
+      upx = (w1 * j1x + w2 * j2x) * inv;
+      upy = (w1 * j1y + w2 * j2y) * inv;
     } else {
-      o1 = w1 * e1 + om1 * f[i];
-      o2 = w2 * e2 + om2 * g[i];
+      upx = (w1 * j1x + w2 * j2x) / den;
+      upy = (w1 * j1y + w2 * j2y) / den;
     }
-    out[(size_t)i * (size_t)p.stride + (size_t)n] = o1;
-    out[(size_t)(9 + i) * (size_t)p.stride + (size_t)n] = o2;
-  });
+    // sum_{i>=1} w_i rho_s(x - c_i) c_i of each fluid, ascending i
+    T c1x = (T)-0.0, c1y = (T)-0.0, c2x = (T)-0.0, c2y = (T)-0.0;
+    static_for<8>([&](auto I) {
+      constexpr int i = decltype(I)::value + 1;
+      constexpr int cx = Lat<9>::cx[i], cy = Lat<9>::cy[i];
+      const T a = (T)Lat<9>::w[i] * lr[0][cyl - cy][cxl - cx];
+      const T b = (T)Lat<9>::w[i] * lr[1][cyl - cy][cxl - cx];
+      if constexpr (cx == 1) { c1x = c1x + a; c2x = c2x + b; }
+      if constexpr (cx == -1) { c1x = c1x - a; c2x = c2x - b; }
+      if constexpr (cy == 1) { c1y = c1y + a; c2y = c2y + b; }
+      if constexpr (cy == -1) { c1y = c1y - a; c2y = c2y - b; }
+    });
+    const T u1x = upx - (T)p.gw1 * c2x, u1y = upy - (T)p.gw1 * c2y;
+    const T u2x = upx - (T)p.gw2 * c1x, u2y = upy - (T)p.gw2 * c1y;
+    const T a1 = usq_of<9>(u1x, u1y, (T)0), a2 = usq_of<9>(u2x, u2y, (T)0);
+    const size_t n = (size_t)x + (size_t)p.nx * (size_t)y;
+    static_for<9>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const T e1 = feq_i<9, i>(r1, u1x, u1y, (T)0, a1), e2 = feq_i<9, i>(r2, u2x, u2y, (T)0, a2);
+      T o1, o2;
+      if constexpr (sizeof(T) == 4) {
+        o1 = __builtin_fmaf(w1, e1 - f[r][i], f[r][i]);
+        o2 = __builtin_fmaf(w2, e2 - g[r][i], g[r][i]);
+      } else {
+        o1 = w1 * e1 + om1 * f[r][i];
+        o2 = w2 * e2 + om2 * g[r][i];
+      }
+      out[(size_t)i * (size_t)p.stride + n] = o1;
+      out[(size_t)(9 + i) * (size_t)p.stride + n] = o2;
+    });
+  }
 }
+
+// rows per thread: FP32 keeps 4 nodes' 18 populations in registers, FP64 2
+template <class T>
+constexpr int sc_rpt() { return sizeof(T) == 4 ? 4 : 2; }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_sc_fields(ScParams p, double *rho1, double *rho2, double *ux, double *uy) {
@@ -193,7 +217,6 @@ struct lbm_sc_ctx {
   int64_t N = 0, stride = 0;
   int esize = 8;
   void *f[2] = {nullptr, nullptr};
-  void *rho = nullptr;
   double *fields = nullptr;  // 4 N, lbm_sc_get_fields staging
   int cur = 0;
   hipStream_t stream = nullptr;
@@ -203,7 +226,6 @@ struct lbm_sc_ctx {
     lbm::ScParams p{};
     p.fin = f[cur];
     p.fout = f[cur ^ 1];
-    p.rho = rho;
     p.nx = d.nx;
     p.ny = d.ny;
     p.N = N;
@@ -216,6 +238,9 @@ struct lbm_sc_ctx {
   }
   size_t buf_bytes() const { return (size_t)stride * 18 * (size_t)esize; }
   unsigned grid() const { return (unsigned)((N + 255) / 256); }
+  // tiles of the one-launch step kernel: 64 x (4 rpt) nodes
+  int tiles_x() const { return (d.nx + lbm::SC_TX - 1) / lbm::SC_TX; }
+  int ntiles(int rpt) const { return tiles_x() * ((d.ny + 4 * rpt - 1) / (4 * rpt)); }
 };
 
 namespace {
@@ -238,8 +263,9 @@ lbm_status sc_steps(lbm_sc_handle h, int64_t nsteps) {
     const lbm::ScParams p = h->params();
     SCCHK(sc_typed(h, [&](auto z) {
       using T = decltype(z);
-      hipLaunchKernelGGL(lbm::k_sc_density<T>, dim3(h->grid()), dim3(256), 0, h->stream, p);
-      hipLaunchKernelGGL(lbm::k_sc_collide<T>, dim3(h->grid()), dim3(256), 0, h->stream, p);
+      constexpr int rpt = lbm::sc_rpt<T>();
+      const int nt = h->ntiles(rpt);
+      hipLaunchKernelGGL((lbm::k_sc_step<T, rpt>), dim3((unsigned)nt), dim3(256), 0, h->stream, p, h->tiles_x(), nt);
       return hipGetLastError();
     }));
     h->cur ^= 1;
@@ -278,9 +304,8 @@ lbm_status lbm_sc_create(const lbm_sc_desc *d, lbm_sc_handle *out) {
     return cleanup(LBM_EDEVICE, "hipEventCreate failed");
   for (int b = 0; b < 2; ++b)
     if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess) return cleanup(LBM_ENOMEM, "population buffer allocation failed");
-  if (hipMalloc(&h->rho, 2 * (size_t)h->N * (size_t)h->esize) != hipSuccess ||
-      hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)) != hipSuccess)
-    return cleanup(LBM_ENOMEM, "density buffer allocation failed");
+  if (hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)) != hipSuccess)
+    return cleanup(LBM_ENOMEM, "field buffer allocation failed");
   *out = h;
   return LBM_OK;
 }
@@ -378,7 +403,6 @@ lbm_status lbm_sc_destroy(lbm_sc_handle h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (int b = 0; b < 2; ++b)
     if (h->f[b]) (void)hipFree(h->f[b]);
-  if (h->rho) (void)hipFree(h->rho);
   if (h->fields) (void)hipFree(h->fields);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
