@@ -18,6 +18,7 @@
 // same sequence.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -66,8 +67,8 @@ __device__ __forceinline__ int sc_wrap(int a, int n) {
   return a;
 }
 
-// One step in one launch.  Workgroup = 256 threads on a SC_TX x (4 RPT) tile: thread t owns column
-// t & 63 and rows (t >> 6) + 4 r.  Phase 1 pulls both fluids' populations of the owned nodes into
+// One step in one launch.  Workgroup = NT threads on a SC_TX x (NT / 64 RPT) tile: thread t owns
+// column t & 63 and rows (t >> 6) + (NT / 64) r.  Phase 1 pulls both fluids' populations of the owned nodes into
 // registers and writes their densities to LDS; threads t < 2 (SC_TX + 2) + 2 TY also compute the
 // densities of one ring node around the tile (pulled and summed, nothing kept).  Phase 2 collides
 // the owned nodes with the 8 neighbour densities of each fluid read from LDS.  Tile slots past
@@ -76,9 +77,10 @@ __device__ __forceinline__ int sc_wrap(int a, int n) {
 // (ring pulls mostly hit L2: they are the neighbour tiles' nodes, on the same XCD by xcd_tile).
 constexpr int SC_TX = 64;
 
-template <class T, int RPT>
-__global__ void __launch_bounds__(256) k_sc_step(ScParams p, int tiles_x, int ntiles) {
-  constexpr int TY = 4 * RPT, LX = SC_TX + 2, LY = TY + 2;
+template <class T, int RPT, int NT>
+__global__ void __launch_bounds__(NT) k_sc_step(ScParams p, int tiles_x, int ntiles) {
+  constexpr int ROWS = NT / 64, TY = ROWS * RPT, LX = SC_TX + 2, LY = TY + 2;
+  static_assert(NT % 64 == 0 && 2 * LX + 2 * TY <= NT, "one ring node per thread at most");
   __shared__ T lr[2][LY][LX];
   const int tile = xcd_tile((int)blockIdx.x, ntiles);
   const int x0 = (tile % tiles_x) * SC_TX, y0 = (tile / tiles_x) * TY;
@@ -87,11 +89,11 @@ __global__ void __launch_bounds__(256) k_sc_step(ScParams p, int tiles_x, int nt
   T f[RPT][9], g[RPT][9];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int y = y0 + ly + 4 * r, gy = y < p.ny ? y : sc_wrap(y, p.ny);
+    const int y = y0 + ly + ROWS * r, gy = y < p.ny ? y : sc_wrap(y, p.ny);
     sc_pull<T>(p, gx, gy, 0, f[r]);
     sc_pull<T>(p, gx, gy, 9, g[r]);
-    lr[0][ly + 4 * r + 1][lx + 1] = sc_sum(f[r]);
-    lr[1][ly + 4 * r + 1][lx + 1] = sc_sum(g[r]);
+    lr[0][ly + ROWS * r + 1][lx + 1] = sc_sum(f[r]);
+    lr[1][ly + ROWS * r + 1][lx + 1] = sc_sum(g[r]);
   }
   if (t < 2 * LX + 2 * TY) {
     int hx, hy;
@@ -113,9 +115,9 @@ __global__ void __launch_bounds__(256) k_sc_step(ScParams p, int tiles_x, int nt
   const T om1 = (T)1 - w1, om2 = (T)1 - w2;
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    const int y = y0 + ly + 4 * r;
+    const int y = y0 + ly + ROWS * r;
     if (y >= p.ny) break;
-    const int cxl = lx + 1, cyl = ly + 4 * r + 1;
+    const int cxl = lx + 1, cyl = ly + ROWS * r + 1;
     const T r1 = lr[0][cyl][cxl], r2 = lr[1][cyl][cxl];
     T j1x = (T)-0.0, j1y = (T)-0.0, j2x = (T)-0.0, j2y = (T)-0.0;
     static_for<9>([&](auto I) {
@@ -168,9 +170,34 @@ __global__ void __launch_bounds__(256) k_sc_step(ScParams p, int tiles_x, int nt
   }
 }
 
-// rows per thread: FP32 keeps 4 nodes' 18 populations in registers, FP64 2
+// Tile shapes {rows per thread, threads}; LBM_SC_SHAPE=<index> picks one at create.  Default 0,
+// the fastest measured at 1024 x 1024: FP32 shape 3 (2 x 512: 64 x 16 tiles, 81 VGPRs, 5 waves per
+// SIMD), FP64 shape 8 (2 x 1024: 64 x 32 tiles, 123 VGPRs, 4 waves per SIMD).
+struct ScShape { int rpt, nt; };
+constexpr ScShape kScShapes[] = {{0, 0}, {2, 256}, {4, 256}, {2, 512}, {4, 512}, {8, 256}, {1, 512}, {1, 1024}, {2, 1024}};
+constexpr int kScNumShapes = (int)(sizeof(kScShapes) / sizeof(kScShapes[0]));
+
+template <class T, int RPT, int NT>
+hipError_t launch_sc_step(const ScParams &p, int tiles_x, int ty, int ny, hipStream_t s) {
+  const int nt = tiles_x * ((ny + ty - 1) / ty);
+  hipLaunchKernelGGL((k_sc_step<T, RPT, NT>), dim3((unsigned)nt), dim3(NT), 0, s, p, tiles_x, nt);
+  return hipGetLastError();
+}
+
 template <class T>
-constexpr int sc_rpt() { return sizeof(T) == 4 ? 4 : 2; }
+hipError_t launch_sc_shape(int shape, const ScParams &p, int tiles_x, int ny, hipStream_t s) {
+  if (shape == 0) shape = sizeof(T) == 4 ? 3 : 8;
+  switch (shape) {
+    case 1: return launch_sc_step<T, 2, 256>(p, tiles_x, 8, ny, s);
+    case 2: return launch_sc_step<T, 4, 256>(p, tiles_x, 16, ny, s);
+    case 3: return launch_sc_step<T, 2, 512>(p, tiles_x, 16, ny, s);
+    case 4: return launch_sc_step<T, 4, 512>(p, tiles_x, 32, ny, s);
+    case 5: return launch_sc_step<T, 8, 256>(p, tiles_x, 32, ny, s);
+    case 6: return launch_sc_step<T, 1, 512>(p, tiles_x, 8, ny, s);
+    case 7: return launch_sc_step<T, 1, 1024>(p, tiles_x, 16, ny, s);
+    default: return launch_sc_step<T, 2, 1024>(p, tiles_x, 32, ny, s);
+  }
+}
 
 template <class T>
 __global__ void __launch_bounds__(256) k_sc_fields(ScParams p, double *rho1, double *rho2, double *ux, double *uy) {
@@ -238,9 +265,8 @@ struct lbm_sc_ctx {
   }
   size_t buf_bytes() const { return (size_t)stride * 18 * (size_t)esize; }
   unsigned grid() const { return (unsigned)((N + 255) / 256); }
-  // tiles of the one-launch step kernel: 64 x (4 rpt) nodes
+  int shape = 0;  // index into kScShapes
   int tiles_x() const { return (d.nx + lbm::SC_TX - 1) / lbm::SC_TX; }
-  int ntiles(int rpt) const { return tiles_x() * ((d.ny + 4 * rpt - 1) / (4 * rpt)); }
 };
 
 namespace {
@@ -263,10 +289,7 @@ lbm_status sc_steps(lbm_sc_handle h, int64_t nsteps) {
     const lbm::ScParams p = h->params();
     SCCHK(sc_typed(h, [&](auto z) {
       using T = decltype(z);
-      constexpr int rpt = lbm::sc_rpt<T>();
-      const int nt = h->ntiles(rpt);
-      hipLaunchKernelGGL((lbm::k_sc_step<T, rpt>), dim3((unsigned)nt), dim3(256), 0, h->stream, p, h->tiles_x(), nt);
-      return hipGetLastError();
+      return lbm::launch_sc_shape<T>(h->shape, p, h->tiles_x(), h->d.ny, h->stream);
     }));
     h->cur ^= 1;
   }
@@ -293,6 +316,14 @@ lbm_status lbm_sc_create(const lbm_sc_desc *d, lbm_sc_handle *out) {
   h->N = (int64_t)d->nx * d->ny;
   h->esize = d->storage == LBM_F64 ? 8 : 4;
   h->stride = lbm::population_stride(h->N, h->esize);
+  if (const char *e = std::getenv("LBM_SC_SHAPE")) {
+    const int v = std::atoi(e);
+    if (v < 0 || v >= lbm::kScNumShapes) {
+      delete h;
+      return sc_fail(LBM_EINVAL, "LBM_SC_SHAPE out of range");
+    }
+    h->shape = v;
+  }
   auto cleanup = [&](lbm_status s, const std::string &m) {
     lbm_sc_destroy(h);
     return sc_fail(s, m);

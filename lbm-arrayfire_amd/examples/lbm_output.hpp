@@ -8,6 +8,9 @@
 //                       (the reference displays flip(transpose(uu), 0)), value normalize(|u|) =
 //                       |u| / (1.1 max|u|) + 0.1 over non-solid nodes, mapped through a spectrum
 //                       colour map (hue 270 deg at 0 down to 0 deg at 1); solid nodes black
+//   write_scalar_ppm    a scalar field (the two-fluid example's density of fluid 1,
+//                       cpp/d2q9_shanchen.cpp:181-187) as the same image, values scaled to their
+//                       range: (a - min) / (max - min), a constant field at 0.5
 //   write_vectors_csv   the sampled vector field: x,y,ux,uy at x, y = 0, s, 2s, ... (s = max(1, nx/30))
 //   write_vtk           rho and u of every node: legacy VTK, STRUCTURED_POINTS, big-endian doubles
 //   MlupsCsv            Iterations,MLUPS,MLUPS_x10 rows in the reference's file naming
@@ -80,6 +83,28 @@ inline bool write_speed_ppm(const char *path, int nx, int ny, const double *ux, 
   for (int r = 0; r < ny; ++r) {
     const int y = ny - 1 - r;
     for (int x = 0; x < nx; ++x) spectrum(s[(size_t)x + (size_t)nx * y], &row[(size_t)3 * x]);
+    std::fwrite(row.data(), 1, row.size(), fp);
+  }
+  return std::fclose(fp) == 0;
+}
+
+inline bool write_scalar_ppm(const char *path, int nx, int ny, const double *a) {
+  const size_t n = (size_t)nx * ny;
+  double lo = 0, hi = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (i == 0 || a[i] < lo) lo = a[i];
+    if (i == 0 || a[i] > hi) hi = a[i];
+  }
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return false;
+  std::fprintf(fp, "P6\n%d %d\n255\n", nx, ny);
+  std::vector<uint8_t> row((size_t)nx * 3);
+  for (int r = 0; r < ny; ++r) {
+    const int y = ny - 1 - r;
+    for (int x = 0; x < nx; ++x) {
+      const double v = a[(size_t)x + (size_t)nx * y];
+      spectrum(hi > lo ? (v - lo) / (hi - lo) : 0.5, &row[(size_t)3 * x]);
+    }
     std::fwrite(row.data(), 1, row.size(), fp);
   }
   return std::fclose(fp) == 0;

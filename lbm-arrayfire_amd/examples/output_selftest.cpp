@@ -21,6 +21,7 @@ int main(int argc, char **argv) {
   if (std::fread(t.data(), 1, n, stdin) != n) return 3;
   const double *rho = f.data(), *ux = f.data() + n, *uy = f.data() + 2 * n;
   bool ok = lbm_out::write_speed_ppm((dir + "/c.ppm").c_str(), nx, ny, ux, uy, t.data()) &&
+            lbm_out::write_scalar_ppm((dir + "/c_rho.ppm").c_str(), nx, ny, rho) &&
             lbm_out::write_vectors_csv((dir + "/c_vectors.csv").c_str(), nx, ny, ux, uy) &&
             lbm_out::write_vtk((dir + "/c.vtk").c_str(), nx, ny, 1, rho, ux, uy, nullptr);
   lbm_out::MlupsCsv csv;

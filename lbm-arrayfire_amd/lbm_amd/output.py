@@ -8,6 +8,9 @@ byte what the C++ examples' lbm-arrayfire_amd/examples/lbm_output.hpp writes:
 
 - ``write_speed_ppm``: binary PPM, rows top-down (y = ny - 1 first), values normalize(|u|) =
   |u| / (1.1 max|u|) + 0.1 over non-solid nodes through a spectrum colour map, solid nodes black;
+- ``write_scalar_ppm``: a scalar field (the two-fluid example's density of fluid 1,
+  cpp/d2q9_shanchen.cpp:181-187) as the same image, values scaled to (a - min) / (max - min)
+  (a constant field at 0.5);
 - ``write_vectors_csv``: x,y,ux,uy sampled at x, y = 0, s, 2s, ... with s = max(1, nx // 30);
 - ``write_vtk``: rho and u of every node, legacy VTK STRUCTURED_POINTS, big-endian doubles;
 - ``MlupsCsv``: Iterations,MLUPS,MLUPS_x10 rows in the reference's file naming, MLUPS with the
@@ -70,6 +73,20 @@ def write_speed_ppm(path, nx, ny, ux, uy, types=None):
     with open(path, "wb") as fp:
         fp.write(f"P6\n{nx} {ny}\n255\n".encode())
         fp.write(img.tobytes())
+
+
+def scalar_image(nx, ny, a):
+    """ny x nx x 3 bytes of a scalar field scaled to its range, row 0 at y = ny - 1."""
+    a = np.asarray(a, dtype=np.float64).reshape(ny, nx)
+    lo, hi = float(a.min()), float(a.max())
+    v = (a - lo) / (hi - lo) if hi > lo else np.full_like(a, 0.5)
+    return spectrum(v[::-1])
+
+
+def write_scalar_ppm(path, nx, ny, a):
+    with open(path, "wb") as fp:
+        fp.write(f"P6\n{nx} {ny}\n255\n".encode())
+        fp.write(scalar_image(nx, ny, a).tobytes())
 
 
 def vector_samples(nx, ny, ux, uy):

@@ -47,3 +47,43 @@ def test_d2q9_example_snapshots_match_library(tmp_path):
     assert np.array_equal(u[:, 0], want[1]) and np.array_equal(u[:, 1], want[2])
     img = (tmp_path / f"lid_{steps}.ppm").read_bytes()
     assert img.startswith(f"P6\n{n} {n}\n255\n".encode())
+
+
+def splitmix_uniform(seed, n):
+    """The example's noise: splitmix64(seed + i) >> 11, scaled to [0, 1)."""
+    z = (np.arange(n, dtype=np.uint64) + np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15))
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+SC_EXE = os.path.join(ROOT, "lbm-arrayfire_amd", "lib", "d2q9_shanchen")
+
+
+@pytest.mark.skipif(not os.path.exists(SC_EXE), reason="examples not built (run build())")
+def test_d2q9_shanchen_example_matches_library(tmp_path):
+    from lbm_amd.capi import ShanChen
+
+    n, steps, seed = 48, 300, 7
+    env = dict(os.environ, LBM_GPU_NAME="T")
+    r = subprocess.run([SC_EXE, "0", str(n), str(steps), "f32", "100", "out.bin", str(seed)], cwd=tmp_path,
+                       env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for it in (100, 200, 300):
+        assert (tmp_path / f"shanchen_{it}.ppm").exists() and (tmp_path / f"shanchen_{it}.vtk").exists()
+    rows = (tmp_path / f"T_d2q9_shanchen_mlups_{n}.csv").read_text().splitlines()
+    assert [int(x.split(",")[0]) for x in rows[1:]] == [100, 200, 300]
+    got = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(4, n * n)
+    d = -0.001 * (1.0 - 2.0 * splitmix_uniform(seed, n * n))
+    with ShanChen(n, n, 1.0, 1.0, -1.2, F32) as sc:
+        sc.init(1.0 + d, 1.0 - d)
+        sc.step(steps)
+        want = sc.fields()
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    dims, rho, u = output.read_vtk(tmp_path / f"shanchen_{steps}.vtk")
+    assert dims == (n, n, 1) and np.array_equal(rho, want[0]) and np.array_equal(u[:, 0], want[2])
+    img = (tmp_path / f"shanchen_{steps}.ppm").read_bytes()
+    assert img == f"P6\n{n} {n}\n255\n".encode() + output.scalar_image(n, n, want[0]).tobytes()

@@ -113,3 +113,19 @@ def test_conservation_at_bench_size(storage):
     j0, j = (C.T @ (f0[:9] + f0[9:])).sum(1), (C.T @ (f[:9] + f[9:])).sum(1)
     assert np.abs(j - j0).max() <= (1e-8 if storage == F64 else 0.05)
 
+
+
+@pytest.mark.parametrize("shape", range(1, 9))
+@pytest.mark.parametrize("storage", [F64, F32])
+def test_every_tile_shape_matches_oracle(monkeypatch, shape, storage):
+    """LBM_SC_SHAPE selects the step kernel's tile shape at create: every shape, on a lattice that
+    leaves partial tiles in both directions, equals the oracle."""
+    nx, ny, steps = 131, 45, 40
+    monkeypatch.setenv("LBM_SC_SHAPE", str(shape))
+    f0 = as_storage(noisy_populations(nx, ny, shape), storage)
+    with ShanChen(nx, ny, 0.9, 1.2, -1.2, storage) as sc:
+        sc.set_populations(f0)
+        sc.step(steps)
+        fg = sc.populations()
+    fo = orc.sc_run(dict(nx=nx, ny=ny, omega1=0.9, omega2=1.2, g=-1.2, storage=0 if storage == F64 else 1), f0, steps)
+    assert np.array_equal(fg, fo)

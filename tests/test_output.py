@@ -37,11 +37,12 @@ def test_cpp_and_python_writers_are_byte_identical(tmp_path, nx, ny):
     subprocess.run([SELFTEST, str(tmp_path), str(nx), str(ny)], input=blob, check=True)
     output.write_speed_ppm(tmp_path / "p.ppm", nx, ny, ux, uy, types)
     output.write_vectors_csv(tmp_path / "p_vectors.csv", nx, ny, ux, uy)
+    output.write_scalar_ppm(tmp_path / "p_rho.ppm", nx, ny, rho)
     output.write_vtk(tmp_path / "p.vtk", nx, ny, 1, rho, ux, uy)
     with output.MlupsCsv(str(tmp_path), "GPU", "p", nx) as csv:
         for it in range(100, 501, 100):
             csv.row(it, nx * ny, 1e-3 * it / 7.0)
-    for a, b in (("c.ppm", "p.ppm"), ("c_vectors.csv", "p_vectors.csv"), ("c.vtk", "p.vtk"),
+    for a, b in (("c.ppm", "p.ppm"), ("c_rho.ppm", "p_rho.ppm"), ("c_vectors.csv", "p_vectors.csv"), ("c.vtk", "p.vtk"),
                  (f"GPU_c_mlups_{nx}.csv", f"GPU_p_mlups_{nx}.csv")):
         assert (tmp_path / a).read_bytes() == (tmp_path / b).read_bytes(), a
 
@@ -94,3 +95,16 @@ def test_mlups_csv(tmp_path):
     assert int(it) == 1000
     assert float(m) == pytest.approx(128 * 128 * 1000 / 0.5 / 1e6)
     assert float(m10) == pytest.approx(10 * float(m))
+
+
+def test_scalar_image_meaning():
+    nx, ny = 6, 4
+    a = np.zeros(nx * ny)
+    a[2 + nx * 3] = 2.0   # the maximum, top row
+    a[4 + nx * 0] = -1.0  # the minimum, bottom row
+    img = output.scalar_image(nx, ny, a)
+    assert tuple(img[0, 2]) == (255, 0, 0)            # 1.0: red
+    assert tuple(img[ny - 1, 4]) == tuple(output.spectrum(0.0))
+    assert tuple(img[1, 1]) == tuple(output.spectrum(1.0 / 3.0))
+    flat = output.scalar_image(nx, ny, np.ones(nx * ny))
+    assert (flat == output.spectrum(0.5)).all()
