@@ -305,8 +305,10 @@ lbm_status lbm_sc_init(lbm_sc_handle h, const double *rho1, const double *rho2);
 lbm_status lbm_sc_step(lbm_sc_handle h, int64_t nsteps);
 /* As lbm_sc_step, then waits; *ms = device time of the launches (HIP events). */
 lbm_status lbm_sc_step_timed(lbm_sc_handle h, int64_t nsteps, double *ms);
-/* Fields after the last step, N doubles each (any may be NULL): the fluids' densities and the
- * mixture velocity (j1 + j2) / (rho1 + rho2) of the streamed populations.  Blocks. */
+/* Fields after the last step, N doubles each (any may be NULL), moments of the current
+ * (post-collision) populations: the fluids' densities (the collision conserves them, so they are
+ * the step's densities that the reference displays, :113-114,183, up to rounding) and the mixture
+ * velocity (j1 + j2) / (rho1 + rho2).  Blocks. */
 lbm_status lbm_sc_get_fields(lbm_sc_handle h, double *rho1, double *rho2, double *ux, double *uy);
 /* Populations as doubles: 9 N of fluid 1 then 9 N of fluid 2, population i of node n at
  * n + N i.  Blocks. */
