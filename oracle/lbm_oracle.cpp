@@ -26,6 +26,10 @@
 //  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
 //                       (non-equilibrium bounce-back) velocity inlet, zero-gradient outlet at a
 //                       reference density (non-equilibrium extrapolation of every population).
+//  * Shan-Chen          cpp/d2q9_shanchen.cpp:39-181 with its work-in-progress defects corrected
+//                       (oracle_sc_run); pinned in tests/test_shanchen.py by a numpy restatement,
+//                       the single-fluid reduction at G = 0, conservation, a point-perturbation
+//                       closed form and the sign of the coupling.
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
 //
