@@ -223,7 +223,10 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     bool upd[NW];
     static_for<NW>([&](auto KI) {
       constexpr int k = decltype(KI)::value;
-      const unsigned e = threadIdx.x + (unsigned)(k * NT);
+      // a partial last pass is dealt from the last thread down, so the first wavefronts (which
+      // carry the boundary list and, with flip, the held rows) get no position in it
+      constexpr bool REV = LIST && k == NW - 1 && W % NT != 0;
+      const unsigned e = (REV ? (unsigned)(NT - 1) - threadIdx.x : threadIdx.x) + (unsigned)(k * NT);
       const int i = s + (int)(e % (unsigned)WX);
       const int j = flip ? RY - 1 - s - (int)(e / (unsigned)WX) : s + (int)(e / (unsigned)WX);
       const int pos = i + j * RX;
