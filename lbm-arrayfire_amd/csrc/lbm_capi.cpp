@@ -135,6 +135,10 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
   tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
   p.inlet_profile = d->inlet_profile;
   p.meq_mode = d->meq_mode;
+  // Non-temporal population stores for lattices whose buffers cannot stay in the 256 MB
+  // Infinity Cache together (over 128 MiB each); LBM_NT_STORE=0 / 1 forces them off / on.
+  p.nt_store = h->buf_bytes() > ((size_t)128 << 20) ? 1 : 0;
+  if (const char *e = std::getenv("LBM_NT_STORE")) p.nt_store = std::atoi(e) != 0 ? 1 : 0;
   p.N = h->N;
   p.stride = h->stride;
   p.omega = d->omega;

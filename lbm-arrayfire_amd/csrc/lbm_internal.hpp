@@ -18,6 +18,7 @@ struct StepParams {
   int32_t inlet_profile;
   int32_t meq_mode;
   int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
+  int32_t nt_store;         // populations written with non-temporal stores (large lattices)
   int64_t N;                // nodes
   int64_t stride;           // elements between consecutive population arrays (>= N)
   const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
@@ -54,6 +55,7 @@ inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t 
 
 hipError_t launch_step(const Config &c, const StepParams &p, hipStream_t s);
 hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s);
+hipError_t launch_step_scalar_nt(const Config &c, const StepParams &p, hipStream_t s);  // Q = 19, 27
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s);
 // Batched launches: member m of a batch owns tiles [prefix[m], prefix[m+1]) of the batch's tile
 // range.  Everything a launch needs about the members is read from a device-resident table whose

@@ -1,50 +1,16 @@
-// lbm_kernels.hip — node-per-thread kernels: the fused step, field extraction, initial states
-// and the mass / kinetic-energy monitors, plus the host-side dispatch over
-// (lattice, storage, boundary mode, collision).
+// lbm_kernels.hip — node-per-thread kernels: the fused step (ordinary stores; the step body is in
+// lbm_step.hpp), its batched form, field extraction, initial states and the mass /
+// kinetic-energy monitors, and their host-side launchers.
 #include <algorithm>
 
-#include "lbm_device.hpp"
+#include "lbm_step.hpp"
 
 namespace lbm {
-
-// One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
-// solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
-template <int Q, int ST, int BC, int CO, class P>
-__device__ __forceinline__ void step_tile(const P &p, int tile) {
-  using C = typename Codec<ST, Q>::C;
-  const NodePos q = node_of_tile(p, tile);
-  if (!q.valid) return;
-  const uint8_t t = p.type[q.n];
-  C f[Q], out[Q], rho, ux, uy, uz;
-  if constexpr (BC == BC_REF) {
-    pull_periodic<Q, ST>(p, q, f);
-    moments<Q>(f, rho, ux, uy, uz);
-    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
-    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
-    if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK: out[opp(j)] = f[j], j >= 1
-      static_for<Q - 1>([&](auto J) {
-        constexpr int j = decltype(J)::value + 1;
-        out[Lat<Q>::opp[j]] = f[j];
-      });
-    }
-  } else {
-    pull_physical<Q, ST>(p, q, t, f);
-    if (t & NODE_SOLID) return;  // walls keep their populations; nothing writes them
-    moments<Q>(f, rho, ux, uy, uz);
-    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
-  }
-  store_all<Q, ST>(p, q.n, out);
-}
-
-template <int Q, int ST, int BC, int CO>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
-  step_tile<Q, ST, BC, CO>(p, xcd_tile((int)blockIdx.x, p.ntiles));
-}
 
 // Persistent batched step: every member of one configuration group, one launch.
 template <int Q, int ST, int BC, int CO, bool ONE>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
-  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO>(p, tile); });
+  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO, false>(p, tile); });
 }
 
 // rho, u the reference holds after a step: moments of the pulled populations with the overrides
@@ -70,9 +36,6 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fields(StepParams p, double *
   if (uy_o) uy_o[q.n] = (double)uy;
   if (uz_o) uz_o[q.n] = (double)uz;
 }
-
-template <class F>
-static hipError_t with_storage_fwd(const Config &c, F &&f);
 
 // f = f_eq(rho, u) (cpp/d2q9_bgk_lid.cpp:100-107) into both buffers.
 template <int Q, int ST>
@@ -135,7 +98,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_equilibrium(StepParams p
 
 hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double rho0, const double *u0,
                                    uint64_t seed, double amp, hipStream_t s) {
-  return with_storage_fwd(c, [&](auto Qc, auto STc) -> hipError_t {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
     hipLaunchKernelGGL((k_init_equilibrium<decltype(Qc)::value, decltype(STc)::value>), dim3(p.ntiles),
                        dim3(BLOCK_THREADS), 0, s, p, rho0, u0[0], u0[1], u0[2], seed, amp);
     return hipGetLastError();
@@ -339,52 +302,11 @@ hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *
   return hipGetLastError();
 }
 
-// ---- dispatch ----------------------------------------------------------------------------
-template <class F>
-static hipError_t with_storage(const Config &c, F &&f) {
-  auto byQ = [&](auto Qc) -> hipError_t {
-    switch (c.storage) {
-      case ST_F64: return f(Qc, IC<ST_F64>{});
-      case ST_F32: return f(Qc, IC<ST_F32>{});
-      case ST_F16S: return f(Qc, IC<ST_F16S>{});
-      case ST_BF16S: return f(Qc, IC<ST_BF16S>{});
-    }
-    return hipErrorInvalidValue;
-  };
-  if (c.Q == 9) return byQ(IC<9>{});
-  if (c.Q == 19) return byQ(IC<19>{});
-  if (c.Q == 27) return byQ(IC<27>{});
-  return hipErrorInvalidValue;
-}
-
-template <class F>
-static hipError_t with_storage_fwd(const Config &c, F &&f) {
-  return with_storage(c, f);
-}
-
-template <class F>
-static hipError_t with_full(const Config &c, F &&f) {
-  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
-    constexpr int Q = decltype(Qc)::value;
-    auto byBC = [&](auto BCc) -> hipError_t {
-      if (c.collision == CO_BGK) return f(Qc, STc, BCc, IC<CO_BGK>{});
-      if constexpr (Q == 27) {
-        if (c.collision == CO_MRT_RAW) return f(Qc, STc, BCc, IC<CO_MRT_RAW>{});
-        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
-      }
-      if constexpr (Q == 19) {  // the orthogonal basis only
-        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
-      }
-      return hipErrorInvalidValue;
-    };
-    return c.bc == BC_REF ? byBC(IC<BC_REF>{}) : byBC(IC<BC_PHYS>{});
-  });
-}
-
 hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s) {
+  if (p.nt_store && c.Q != 9) return launch_step_scalar_nt(c, p, s);  // lbm_kernels_nt.hip
   return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
     hipLaunchKernelGGL((k_step<decltype(Qc)::value, decltype(STc)::value, decltype(BCc)::value,
-                               decltype(COc)::value>),
+                               decltype(COc)::value, false>),
                        dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
     return hipGetLastError();
   });

@@ -1,0 +1,82 @@
+// lbm_step.hpp — the node-per-thread fused step (tile body and kernel) and the host dispatch
+// over (lattice, storage, boundary mode, collision), shared by the translation units that
+// instantiate the step kernel with ordinary and with non-temporal population stores.
+#pragma once
+
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+// One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
+// solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
+template <int Q, int ST, int BC, int CO, bool NT, class P>
+__device__ __forceinline__ void step_tile(const P &p, int tile) {
+  using C = typename Codec<ST, Q>::C;
+  const NodePos q = node_of_tile(p, tile);
+  if (!q.valid) return;
+  const uint8_t t = p.type[q.n];
+  C f[Q], out[Q], rho, ux, uy, uz;
+  if constexpr (BC == BC_REF) {
+    pull_periodic<Q, ST>(p, q, f);
+    moments<Q>(f, rho, ux, uy, uz);
+    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+    if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK: out[opp(j)] = f[j], j >= 1
+      static_for<Q - 1>([&](auto J) {
+        constexpr int j = decltype(J)::value + 1;
+        out[Lat<Q>::opp[j]] = f[j];
+      });
+    }
+  } else {
+    pull_physical<Q, ST>(p, q, t, f);
+    if (t & NODE_SOLID) return;  // walls keep their populations; nothing writes them
+    moments<Q>(f, rho, ux, uy, uz);
+    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+  }
+  store_all<Q, ST, NT>(p, q.n, out);
+}
+
+// NT: populations written with non-temporal stores (lattices too large for the caches)
+template <int Q, int ST, int BC, int CO, bool NT>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
+  step_tile<Q, ST, BC, CO, NT>(p, xcd_tile((int)blockIdx.x, p.ntiles));
+}
+
+// ---- dispatch ----------------------------------------------------------------------------
+template <class F>
+inline hipError_t with_storage(const Config &c, F &&f) {
+  auto byQ = [&](auto Qc) -> hipError_t {
+    switch (c.storage) {
+      case ST_F64: return f(Qc, IC<ST_F64>{});
+      case ST_F32: return f(Qc, IC<ST_F32>{});
+      case ST_F16S: return f(Qc, IC<ST_F16S>{});
+      case ST_BF16S: return f(Qc, IC<ST_BF16S>{});
+    }
+    return hipErrorInvalidValue;
+  };
+  if (c.Q == 9) return byQ(IC<9>{});
+  if (c.Q == 19) return byQ(IC<19>{});
+  if (c.Q == 27) return byQ(IC<27>{});
+  return hipErrorInvalidValue;
+}
+
+template <class F>
+inline hipError_t with_full(const Config &c, F &&f) {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    constexpr int Q = decltype(Qc)::value;
+    auto byBC = [&](auto BCc) -> hipError_t {
+      if (c.collision == CO_BGK) return f(Qc, STc, BCc, IC<CO_BGK>{});
+      if constexpr (Q == 27) {
+        if (c.collision == CO_MRT_RAW) return f(Qc, STc, BCc, IC<CO_MRT_RAW>{});
+        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
+      }
+      if constexpr (Q == 19) {  // the orthogonal basis only
+        if (c.collision == CO_MRT_GS) return f(Qc, STc, BCc, IC<CO_MRT_GS>{});
+      }
+      return hipErrorInvalidValue;
+    };
+    return c.bc == BC_REF ? byBC(IC<BC_REF>{}) : byBC(IC<BC_PHYS>{});
+  });
+}
+
+}  // namespace lbm

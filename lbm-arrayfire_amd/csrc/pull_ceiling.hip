@@ -1,10 +1,11 @@
 // pull_ceiling.hip — the attainable rate of the D3Q27 step's memory access pattern, measured with
 // no arithmetic.  Every node of a 256^3 lattice pulls its 27 populations from x - c_i (periodic)
 // and stores them, in the library's padded SoA layout (population_stride), with 4-byte and 2-byte
-// populations; and a 16-byte-per-lane streaming copy moves the same number of bytes.  Prints one
-// JSON line of GB/s, counted as the step kernels count them (2 Q s bytes per node).  bench.py
-// reports these beside the D3Q27 kernels' rates: they bound what any step kernel in this layout
-// can reach on the device it runs on.
+// populations; and a 16-byte-per-lane streaming copy moves the same number of bytes.  Each with
+// ordinary and with non-temporal stores (the step kernels use the latter on lattices this large).
+// Prints one JSON line of GB/s, counted as the step kernels count them (2 Q s bytes per node).
+// bench.py reports these beside the D3Q27 kernels' rates: they bound what any step kernel in this
+// layout can reach on the device it runs on.
 #include <cstdio>
 
 #include "lbm_device.hpp"
@@ -15,7 +16,7 @@ namespace {
 
 constexpr int N3 = 256, Q = 27;
 
-template <class T>
+template <class T, bool NT>
 __global__ void __launch_bounds__(256) k_pull(const T *__restrict__ in, T *__restrict__ out, int64_t stride) {
   const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
   const int z = (int)blockIdx.z;
@@ -32,11 +33,23 @@ __global__ void __launch_bounds__(256) k_pull(const T *__restrict__ in, T *__res
     v[i] = in[(size_t)i * (size_t)stride + (size_t)sx + (size_t)N3 * ((size_t)sy + (size_t)N3 * (size_t)sz)];
   });
   const size_t n = (size_t)x + (size_t)N3 * ((size_t)y + (size_t)N3 * (size_t)z);
-  static_for<Q>([&](auto I) { out[(size_t)decltype(I)::value * (size_t)stride + n] = v[decltype(I)::value]; });
+  static_for<Q>([&](auto I) {
+    T *dst = out + (size_t)decltype(I)::value * (size_t)stride + n;
+    if constexpr (NT) __builtin_nontemporal_store(v[decltype(I)::value], dst);
+    else *dst = v[decltype(I)::value];
+  });
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <bool NT>
 __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ in, float4 *__restrict__ out, size_t n4) {
-  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256) out[k] = in[k];
+  const v4f *src = (const v4f *)in;
+  v4f *dst = (v4f *)out;
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256) {
+    if constexpr (NT) __builtin_nontemporal_store(src[k], dst + k);
+    else dst[k] = src[k];
+  }
 }
 
 bool ok(hipError_t e, const char *what) {
@@ -62,7 +75,7 @@ double time_launches(L &&launch) {
   return ms * 1e-3 / 20;
 }
 
-template <class T>
+template <class T, bool NT>
 double pull_gbs() {
   const int64_t N = (int64_t)N3 * N3 * N3, stride = population_stride(N, (int)sizeof(T));
   const size_t bytes = (size_t)stride * Q * sizeof(T);
@@ -70,20 +83,21 @@ double pull_gbs() {
   if (!ok(hipMalloc(&a, bytes), "hipMalloc") || !ok(hipMalloc(&b, bytes), "hipMalloc")) return -1;
   if (!ok(hipMemset(a, 0, bytes), "hipMemset") || !ok(hipMemset(b, 0, bytes), "hipMemset")) return -1;
   const double t = time_launches([&](int odd) {
-    hipLaunchKernelGGL(k_pull<T>, dim3(N3 / 64, N3 / 4, N3), dim3(256), 0, 0, odd ? b : a, odd ? a : b, stride);
+    hipLaunchKernelGGL((k_pull<T, NT>), dim3(N3 / 64, N3 / 4, N3), dim3(256), 0, 0, odd ? b : a, odd ? a : b, stride);
   });
   (void)hipFree(a);
   (void)hipFree(b);
   return t > 0 ? 2.0 * (double)N * Q * sizeof(T) / t / 1e9 : -1;
 }
 
+template <bool NT>
 double copy_gbs(size_t bytes) {
   float4 *a = nullptr, *b = nullptr;
   if (!ok(hipMalloc(&a, bytes), "hipMalloc") || !ok(hipMalloc(&b, bytes), "hipMalloc")) return -1;
   if (!ok(hipMemset(a, 0, bytes), "hipMemset") || !ok(hipMemset(b, 0, bytes), "hipMemset")) return -1;
   const size_t n4 = bytes / sizeof(float4);
   const double t = time_launches([&](int odd) {
-    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, 0, odd ? b : a, odd ? a : b, n4);
+    hipLaunchKernelGGL(k_copy<NT>, dim3(1024), dim3(256), 0, 0, odd ? b : a, odd ? a : b, n4);
   });
   (void)hipFree(a);
   (void)hipFree(b);
@@ -93,12 +107,14 @@ double copy_gbs(size_t bytes) {
 }  // namespace
 
 int main() {
-  const double p4 = pull_gbs<float>(), p2 = pull_gbs<uint16_t>();
+  const double p4 = pull_gbs<float, false>(), p2 = pull_gbs<uint16_t, false>();
+  const double p4n = pull_gbs<float, true>(), p2n = pull_gbs<uint16_t, true>();
   const size_t cb = (size_t)N3 * N3 * N3 * Q * sizeof(float);
-  const double c = copy_gbs(cb);
-  if (p4 < 0 || p2 < 0 || c < 0) return 1;
+  const double c = copy_gbs<false>(cb), cn = copy_gbs<true>(cb);
+  if (p4 < 0 || p2 < 0 || c < 0 || p4n < 0 || p2n < 0 || cn < 0) return 1;
   std::printf("{\"lattice\": \"256^3 D3Q27, padded SoA\", \"pull_4byte_gbs\": %.1f, \"pull_2byte_gbs\": %.1f, "
-              "\"copy_gbs\": %.1f, \"copy_bytes\": %zu}\n",
-              p4, p2, c, cb);
+              "\"copy_gbs\": %.1f, \"pull_4byte_nt_gbs\": %.1f, \"pull_2byte_nt_gbs\": %.1f, \"copy_nt_gbs\": %.1f, "
+              "\"copy_bytes\": %zu}\n",
+              p4, p2, c, p4n, p2n, cn, cb);
   return 0;
 }
