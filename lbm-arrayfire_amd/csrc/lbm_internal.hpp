@@ -18,7 +18,6 @@ struct StepParams {
   int32_t inlet_profile;
   int32_t meq_mode;
   int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
-  int32_t nt_store;         // populations written with non-temporal stores (large lattices)
   int64_t N;                // nodes
   int64_t stride;           // elements between consecutive population arrays (>= N)
   const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
@@ -32,6 +31,8 @@ struct StepParams {
 
 struct Config {
   int Q, storage, bc, collision, engine;
+  int nt = 0;  // 3D lattices too large for the caches: non-temporal population stores (single
+               // lattices only; batches use ordinary stores)
 };
 
 // Tile shape of the node-per-thread kernels: one wavefront per 64-node row segment,

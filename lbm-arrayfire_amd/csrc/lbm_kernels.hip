@@ -7,12 +7,6 @@
 
 namespace lbm {
 
-// Persistent batched step: every member of one configuration group, one launch.
-template <int Q, int ST, int BC, int CO, bool ONE>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
-  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO, false>(p, tile); });
-}
-
 // rho, u the reference holds after a step: moments of the pulled populations with the overrides
 // (REFERENCE), or of the boundary-completed pulled populations (PHYSICAL; 0 at solid nodes).
 template <int Q, int ST, int BC>
@@ -303,7 +297,7 @@ hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *
 }
 
 hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t s) {
-  if (p.nt_store && c.Q != 9) return launch_step_scalar_nt(c, p, s);  // lbm_kernels_nt.hip
+  if (c.nt) return launch_step_scalar_nt(c, p, s);  // lbm_kernels_nt.hip
   return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
     hipLaunchKernelGGL((k_step<decltype(Qc)::value, decltype(STc)::value, decltype(BCc)::value,
                                decltype(COc)::value, false>),

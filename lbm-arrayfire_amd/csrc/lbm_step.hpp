@@ -42,6 +42,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
   step_tile<Q, ST, BC, CO, NT>(p, xcd_tile((int)blockIdx.x, p.ntiles));
 }
 
+// Persistent batched step: every member of one configuration group, one launch.
+template <int Q, int ST, int BC, int CO, bool ONE>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
+  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO, false>(p, tile); });
+}
+
 // ---- dispatch ----------------------------------------------------------------------------
 template <class F>
 inline hipError_t with_storage(const Config &c, F &&f) {
