@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -82,6 +83,7 @@ struct lbm_batch_ctx {
   bool dirty = true;
   int phase = 0;  // the next launch uses parameter block [phase]
   hipGraphExec_t graph = nullptr;
+  bool use_graph = true;  // LBM_BATCH_GRAPH=0: direct launches instead of graph replays
   int64_t steps = 0;
   lbm::BatchCounters counters{};
 
@@ -265,11 +267,16 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
     b->phase = 0;
     --left;
   }
-  if (left >= 2) {
+  if (left >= 2 && b->use_graph) {
     lbm_status st = ensure_graph(b);
     if (st != LBM_OK) return st;
+    for (; left >= 2; left -= 2) BCHK(hipGraphLaunch(b->graph, b->stream));
   }
-  for (; left >= 2; left -= 2) BCHK(hipGraphLaunch(b->graph, b->stream));
+  for (; left >= 2; left -= 2) {
+    lbm_status st = launch_phase(b, 0, b->stream);
+    if (st == LBM_OK) st = launch_phase(b, 1, b->stream);
+    if (st != LBM_OK) return st;
+  }
   if (left == 1) {
     lbm_status st = launch_phase(b, 0, b->stream);
     if (st != LBM_OK) return st;
@@ -305,6 +312,7 @@ lbm_status lbm_batch_create(lbm_batch *out) {
     lbm_batch_destroy(b);
     return set_error(LBM_EDEVICE, "batch stream/event creation failed");
   }
+  if (const char *e = std::getenv("LBM_BATCH_GRAPH")) b->use_graph = std::atoi(e) != 0;
   *out = b;
   return LBM_OK;
 }
@@ -350,7 +358,7 @@ lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms) {
     lbm_status st = rebuild(b);
     if (st != LBM_OK) return st;
   }
-  if (nsteps >= 2 && b->phase == 0 && has_single_groups(b)) {  // capture outside the timed region
+  if (nsteps >= 2 && b->phase == 0 && b->use_graph && has_single_groups(b)) {  // capture outside the timed region
     lbm_status st = ensure_graph(b);
     if (st != LBM_OK) return st;
   }
