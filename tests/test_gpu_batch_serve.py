@@ -44,7 +44,10 @@ def make(spec):
     return lat
 
 
-def test_batch_matches_individual_stepping():
+@pytest.mark.parametrize("graphs", ["1", "0"])
+def test_batch_matches_individual_stepping(monkeypatch, graphs):
+    """Replayed two-step graphs (default) and direct launches (LBM_BATCH_GRAPH=0) alike."""
+    monkeypatch.setenv("LBM_BATCH_GRAPH", graphs)
     specs = member_specs()
     alone = []
     for sp in specs:
