@@ -302,135 +302,151 @@ BatchCounters batch_counters(lbm_batch b) { return b ? b->counters : BatchCounte
 extern "C" {
 
 lbm_status lbm_batch_create(lbm_batch *out) {
-  if (!out) return set_error(LBM_EINVAL, "out is NULL");
-  *out = nullptr;
-  lbm_batch_ctx *b = new (std::nothrow) lbm_batch_ctx();
-  if (!b) return set_error(LBM_ENOMEM, "host allocation failed");
-  if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&b->order, hipEventDisableTiming) != hipSuccess) {
-    lbm_batch_destroy(b);
-    return set_error(LBM_EDEVICE, "batch stream/event creation failed");
-  }
-  if (const char *e = std::getenv("LBM_BATCH_GRAPH")) b->use_graph = std::atoi(e) != 0;
-  *out = b;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!out) return set_error(LBM_EINVAL, "out is NULL");
+    *out = nullptr;
+    lbm_batch_ctx *b = new (std::nothrow) lbm_batch_ctx();
+    if (!b) return set_error(LBM_ENOMEM, "host allocation failed");
+    if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&b->order, hipEventDisableTiming) != hipSuccess) {
+      lbm_batch_destroy(b);
+      return set_error(LBM_EDEVICE, "batch stream/event creation failed");
+    }
+    if (const char *e = std::getenv("LBM_BATCH_GRAPH")) b->use_graph = std::atoi(e) != 0;
+    *out = b;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_batch_add(lbm_batch b, lbm_handle h) {
-  if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
-  if (h->batch) return set_error(LBM_EINVAL, "the lattice already belongs to a batch");
-  if (h->device != b->device) return set_error(LBM_EINVAL, "the lattice lives on another device");
-  // the batch's stream runs after everything already queued on the lattice's own stream
-  BCHK(hipEventRecord(b->order, h->stream));
-  BCHK(hipStreamWaitEvent(b->stream, b->order, 0));
-  h->batch = b;
-  h->borrowed = b->stream;
-  b->members.push_back(h);
-  b->dirty = true;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
+    if (h->batch) return set_error(LBM_EINVAL, "the lattice already belongs to a batch");
+    if (h->device != b->device) return set_error(LBM_EINVAL, "the lattice lives on another device");
+    // the batch's stream runs after everything already queued on the lattice's own stream
+    BCHK(hipEventRecord(b->order, h->stream));
+    BCHK(hipStreamWaitEvent(b->stream, b->order, 0));
+    h->batch = b;
+    h->borrowed = b->stream;
+    b->members.push_back(h);
+    b->dirty = true;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_batch_remove(lbm_batch b, lbm_handle h) {
-  if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
-  auto it = std::find(b->members.begin(), b->members.end(), h);
-  if (it == b->members.end()) return set_error(LBM_EINVAL, "the lattice is not a member of this batch");
-  // the lattice's own stream runs after the batch work queued so far
-  BCHK(hipEventRecord(b->order, b->stream));
-  BCHK(hipStreamWaitEvent(h->stream, b->order, 0));
-  b->members.erase(it);
-  h->batch = nullptr;
-  h->borrowed = nullptr;
-  b->dirty = true;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b || !h) return set_error(LBM_EINVAL, "NULL argument");
+    auto it = std::find(b->members.begin(), b->members.end(), h);
+    if (it == b->members.end()) return set_error(LBM_EINVAL, "the lattice is not a member of this batch");
+    // the lattice's own stream runs after the batch work queued so far
+    BCHK(hipEventRecord(b->order, b->stream));
+    BCHK(hipStreamWaitEvent(h->stream, b->order, 0));
+    b->members.erase(it);
+    h->batch = nullptr;
+    h->borrowed = nullptr;
+    b->dirty = true;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_batch_step(lbm_batch b, int64_t nsteps) {
-  if (!b) return set_error(LBM_EINVAL, "batch is NULL");
-  if (nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
-  return enqueue(b, nsteps);
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b) return set_error(LBM_EINVAL, "batch is NULL");
+    if (nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
+    return enqueue(b, nsteps);
+  });
 }
 
 lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms) {
-  if (!b) return set_error(LBM_EINVAL, "batch is NULL");
-  if (nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
-  if (b->dirty) {
-    lbm_status st = rebuild(b);
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b) return set_error(LBM_EINVAL, "batch is NULL");
+    if (nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
+    if (b->dirty) {
+      lbm_status st = rebuild(b);
+      if (st != LBM_OK) return st;
+    }
+    if (nsteps >= 2 && b->phase == 0 && b->use_graph && has_single_groups(b)) {  // capture outside the timed region
+      lbm_status st = ensure_graph(b);
+      if (st != LBM_OK) return st;
+    }
+    BCHK(hipEventRecord(b->ev0, b->stream));
+    lbm_status st = enqueue(b, nsteps);
     if (st != LBM_OK) return st;
-  }
-  if (nsteps >= 2 && b->phase == 0 && b->use_graph && has_single_groups(b)) {  // capture outside the timed region
-    lbm_status st = ensure_graph(b);
-    if (st != LBM_OK) return st;
-  }
-  BCHK(hipEventRecord(b->ev0, b->stream));
-  lbm_status st = enqueue(b, nsteps);
-  if (st != LBM_OK) return st;
-  BCHK(hipEventRecord(b->ev1, b->stream));
-  BCHK(hipEventSynchronize(b->ev1));
-  float t = 0;
-  BCHK(hipEventElapsedTime(&t, b->ev0, b->ev1));
-  if (ms) *ms = (double)t;
-  return LBM_OK;
+    BCHK(hipEventRecord(b->ev1, b->stream));
+    BCHK(hipEventSynchronize(b->ev1));
+    float t = 0;
+    BCHK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = (double)t;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_batch_sync(lbm_batch b) {
-  if (!b) return set_error(LBM_EINVAL, "batch is NULL");
-  BCHK(hipStreamSynchronize(b->stream));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b) return set_error(LBM_EINVAL, "batch is NULL");
+    BCHK(hipStreamSynchronize(b->stream));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_batch_tune(lbm_batch b, int32_t trials) {
-  if (!b) return set_error(LBM_EINVAL, "batch is NULL");
-  if (b->dirty) {
-    lbm_status st = rebuild(b);
-    if (st != LBM_OK) return st;
-  }
-  const int cus = device_cu_count(b->device);
-  const int reps = trials > 0 ? trials : 5;
-  for (auto &g : b->groups) {
-    int best = g.grid;
-    float best_ms = 1e30f;
-    for (int f : {1, 2, 4, 8, 16, 0}) {  // 0 = one workgroup per tile
-      const int grid = f ? std::min(g.total_tiles, f * cus) : g.total_tiles;
-      if (grid < 1) continue;
-      // trial launches read the current buffers and write only the other ones
-      const BatchArgs a = g.args(g.multi_k ? g.phase : b->phase, grid);
-      BCHK(launch_step_batch(g.cfg, a, grid, b->stream));  // warm-up
-      BCHK(hipEventRecord(b->ev0, b->stream));
-      for (int r = 0; r < reps; ++r) BCHK(launch_step_batch(g.cfg, a, grid, b->stream));
-      BCHK(hipEventRecord(b->ev1, b->stream));
-      BCHK(hipEventSynchronize(b->ev1));
-      float ms = 0;
-      BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = grid;
-      }
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b) return set_error(LBM_EINVAL, "batch is NULL");
+    if (b->dirty) {
+      lbm_status st = rebuild(b);
+      if (st != LBM_OK) return st;
     }
-    g.tuned = best;
-    g.grid = best;
-  }
-  b->drop_graph();
-  return LBM_OK;
+    const int cus = device_cu_count(b->device);
+    const int reps = trials > 0 ? trials : 5;
+    for (auto &g : b->groups) {
+      int best = g.grid;
+      float best_ms = 1e30f;
+      for (int f : {1, 2, 4, 8, 16, 0}) {  // 0 = one workgroup per tile
+        const int grid = f ? std::min(g.total_tiles, f * cus) : g.total_tiles;
+        if (grid < 1) continue;
+        // trial launches read the current buffers and write only the other ones
+        const BatchArgs a = g.args(g.multi_k ? g.phase : b->phase, grid);
+        BCHK(launch_step_batch(g.cfg, a, grid, b->stream));  // warm-up
+        BCHK(hipEventRecord(b->ev0, b->stream));
+        for (int r = 0; r < reps; ++r) BCHK(launch_step_batch(g.cfg, a, grid, b->stream));
+        BCHK(hipEventRecord(b->ev1, b->stream));
+        BCHK(hipEventSynchronize(b->ev1));
+        float ms = 0;
+        BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = grid;
+        }
+      }
+      g.tuned = best;
+      g.grid = best;
+    }
+    b->drop_graph();
+    return LBM_OK;
+  });
 }
 
 int32_t lbm_batch_size(lbm_batch b) { return b ? (int32_t)b->members.size() : -1; }
 
 lbm_status lbm_batch_destroy(lbm_batch b) {
-  if (!b) return LBM_OK;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (lbm_ctx *m : b->members) {
-    m->batch = nullptr;
-    m->borrowed = nullptr;
-  }
-  b->drop_graph();
-  for (auto &g : b->groups) g.release();
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
-  if (b->order) (void)hipEventDestroy(b->order);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
-  delete b;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!b) return LBM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (lbm_ctx *m : b->members) {
+      m->batch = nullptr;
+      m->borrowed = nullptr;
+    }
+    b->drop_graph();
+    for (auto &g : b->groups) g.release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->order) (void)hipEventDestroy(b->order);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LBM_OK;
+  });
 }
 
 }  // extern "C"

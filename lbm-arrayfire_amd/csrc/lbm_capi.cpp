@@ -92,12 +92,14 @@ int lbm_abi_version(void) { return LBM_ABI_VERSION; }
 const char *lbm_last_error(void) { return g_err.c_str(); }
 
 lbm_status lbm_set_device(int device) {
-  int n = 0;
-  HIPCHK(hipGetDeviceCount(&n));
-  if (device < 0 || device >= n) return fail(LBM_EINVAL, "device index out of range");
-  HIPCHK(hipSetDevice(device));
-  g_device = device;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LBM_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    g_device = device;
+    return LBM_OK;
+  });
 }
 
 size_t lbm_footprint_bytes(const lbm_desc *d) {
@@ -111,135 +113,139 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
 }
 
 lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *out) {
-  if (!out) return fail(LBM_EINVAL, "out is NULL");
-  *out = nullptr;
-  const std::string why = check_desc(d);
-  if (!why.empty()) return fail(LBM_EINVAL, why);
-  if (!node_type) return fail(LBM_EINVAL, "node_type is NULL");
-  lbm_ctx *h = new (std::nothrow) lbm_ctx();
-  if (!h) return fail(LBM_ENOMEM, "host allocation failed");
-  h->d = *d;
-  h->device = g_device;
-  h->Q = d->lattice;
-  h->N = (int64_t)d->nx * d->ny * d->nz;
-  h->esize = storage_bytes(d->storage);
-  h->stride = population_stride(h->N, h->esize);
-  int engine = d->mrt_engine;
-  if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
-  h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!out) return fail(LBM_EINVAL, "out is NULL");
+    *out = nullptr;
+    const std::string why = check_desc(d);
+    if (!why.empty()) return fail(LBM_EINVAL, why);
+    if (!node_type) return fail(LBM_EINVAL, "node_type is NULL");
+    lbm_ctx *h = new (std::nothrow) lbm_ctx();
+    if (!h) return fail(LBM_ENOMEM, "host allocation failed");
+    h->d = *d;
+    h->device = g_device;
+    h->Q = d->lattice;
+    h->N = (int64_t)d->nx * d->ny * d->nz;
+    h->esize = storage_bytes(d->storage);
+    h->stride = population_stride(h->N, h->esize);
+    int engine = d->mrt_engine;
+    if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
+    h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
 
-  StepParams &p = h->base;
-  p.nx = d->nx;
-  p.ny = d->ny;
-  p.nz = d->nz;
-  tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
-  p.inlet_profile = d->inlet_profile;
-  p.meq_mode = d->meq_mode;
-  // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
-  // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
-  // at 160^3 F16S, -18 % at 128^3 F16S); LBM_NT_STORE=0 / 1 forces them off / on.
-  h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) ? 1 : 0;
-  if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;
-  if (d->lattice == 9) h->cfg.nt = 0;
-  p.N = h->N;
-  p.stride = h->stride;
-  p.omega = d->omega;
-  for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
-  p.u_in = d->u_in;
-  p.rho_bc = d->rho_bc;
-  for (int k = 0; k < 27; ++k) {
-    p.s[k] = d->s[k];
-    p.s32[k] = (float)d->s[k];
-  }
-
-  // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an
-  // inlet and every node of column nx - 1 solid or an outlet (the inlet replaces the populations it
-  // pulls across the wrap, the outlet takes all of its populations from its west neighbour); y:
-  // rows 0 and ny - 1 are solid (nodes pull nothing from beyond a wall).  Multi-step tiles hold the
-  // halo positions across such a wrap instead of updating them.
-  if (d->bc_mode == LBM_BC_PHYSICAL && d->nz == 1) {
-    bool xcut = d->nx >= 2, ycut = d->ny >= 2;
-    for (int y = 0; y < d->ny && xcut; ++y) {
-      const uint8_t a = node_type[(size_t)y * d->nx], b = node_type[(size_t)y * d->nx + d->nx - 1];
-      xcut = (a & (LBM_NODE_SOLID | LBM_NODE_INLET)) && (b & (LBM_NODE_SOLID | LBM_NODE_OUTLET));
+    StepParams &p = h->base;
+    p.nx = d->nx;
+    p.ny = d->ny;
+    p.nz = d->nz;
+    tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
+    p.inlet_profile = d->inlet_profile;
+    p.meq_mode = d->meq_mode;
+    // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
+    // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
+    // at 160^3 F16S, -18 % at 128^3 F16S); LBM_NT_STORE=0 / 1 forces them off / on.
+    h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) ? 1 : 0;
+    if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;
+    if (d->lattice == 9) h->cfg.nt = 0;
+    p.N = h->N;
+    p.stride = h->stride;
+    p.omega = d->omega;
+    for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
+    p.u_in = d->u_in;
+    p.rho_bc = d->rho_bc;
+    for (int k = 0; k < 27; ++k) {
+      p.s[k] = d->s[k];
+      p.s32[k] = (float)d->s[k];
     }
-    for (int x = 0; x < d->nx && ycut; ++x)
-      ycut = (node_type[x] & LBM_NODE_SOLID) && (node_type[(size_t)(d->ny - 1) * d->nx + x] & LBM_NODE_SOLID);
-    p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
-  }
 
-  // Multi-step launches for the lattices that qualify: the default shape unless LBM_MULTISTEP
-  // names another shape index, or "off" (single steps only).
-  if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
-    int v = MULTI_DEFAULT;
-    if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
-    const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
-    if (k > 1) {
-      h->multi_variant = v;
-      h->multi_k = k;
+    // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an
+    // inlet and every node of column nx - 1 solid or an outlet (the inlet replaces the populations it
+    // pulls across the wrap, the outlet takes all of its populations from its west neighbour); y:
+    // rows 0 and ny - 1 are solid (nodes pull nothing from beyond a wall).  Multi-step tiles hold the
+    // halo positions across such a wrap instead of updating them.
+    if (d->bc_mode == LBM_BC_PHYSICAL && d->nz == 1) {
+      bool xcut = d->nx >= 2, ycut = d->ny >= 2;
+      for (int y = 0; y < d->ny && xcut; ++y) {
+        const uint8_t a = node_type[(size_t)y * d->nx], b = node_type[(size_t)y * d->nx + d->nx - 1];
+        xcut = (a & (LBM_NODE_SOLID | LBM_NODE_INLET)) && (b & (LBM_NODE_SOLID | LBM_NODE_OUTLET));
+      }
+      for (int x = 0; x < d->nx && ycut; ++x)
+        ycut = (node_type[x] & LBM_NODE_SOLID) && (node_type[(size_t)(d->ny - 1) * d->nx + x] & LBM_NODE_SOLID);
+      p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
     }
-  }
 
-  auto cleanup = [&](lbm_status s, const std::string &m) {
-    lbm_destroy(h);
-    return fail(s, m);
-  };
-  if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-    return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-    return cleanup(LBM_EDEVICE, "hipEventCreate failed");
-  for (int b = 0; b < 2; ++b)
-    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
-      return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
-  if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
-  if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
-    return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
-  // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
-  // nodes whose pull needs the boundary path (NODE_SLOW).
-  {
-    uint8_t *raw = nullptr;
-    if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
-    hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(raw);
-    if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
-  }
-  p.type = h->types;
-  if (d->collision != LBM_BGK && d->lattice == 27) {
-    std::vector<float> tab(64 * MFMA_TAB);
-    build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
-    if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
-      return cleanup(LBM_ENOMEM, "MRT table allocation failed");
-    if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(LBM_EDEVICE, "MRT table upload failed");
-    p.mrt_tab = h->mrt_tab;
-  }
-  *out = h;
-  return LBM_OK;
+    // Multi-step launches for the lattices that qualify: the default shape unless LBM_MULTISTEP
+    // names another shape index, or "off" (single steps only).
+    if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
+      int v = MULTI_DEFAULT;
+      if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
+      const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
+      if (k > 1) {
+        h->multi_variant = v;
+        h->multi_k = k;
+      }
+    }
+
+    auto cleanup = [&](lbm_status s, const std::string &m) {
+      lbm_destroy(h);
+      return fail(s, m);
+    };
+    if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "hipEventCreate failed");
+    for (int b = 0; b < 2; ++b)
+      if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
+        return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
+    if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
+    if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
+    // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
+    // nodes whose pull needs the boundary path (NODE_SLOW).
+    {
+      uint8_t *raw = nullptr;
+      if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
+      hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
+      if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      (void)hipFree(raw);
+      if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
+    }
+    p.type = h->types;
+    if (d->collision != LBM_BGK && d->lattice == 27) {
+      std::vector<float> tab(64 * MFMA_TAB);
+      build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
+      if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
+        return cleanup(LBM_ENOMEM, "MRT table allocation failed");
+      if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(LBM_EDEVICE, "MRT table upload failed");
+      p.mrt_tab = h->mrt_tab;
+    }
+    *out = h;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_destroy(lbm_handle h) {
-  if (!h) return LBM_OK;
-  if (h->batch) return fail(LBM_EINVAL, "remove the lattice from its batch before destroying it");
-  if (h->graph) (void)hipGraphExecDestroy(h->graph);
-  (void)hipSetDevice(h->device);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (int b = 0; b < 2; ++b)
-    if (h->f[b]) (void)hipFree(h->f[b]);
-  if (h->types) (void)hipFree(h->types);
-  if (h->partials) (void)hipFree(h->partials);
-  if (h->red) (void)hipFree(h->red);
-  if (h->fields) (void)hipFree(h->fields);
-  if (h->mrt_tab) (void)hipFree(h->mrt_tab);
-  if (h->probe) (void)hipFree(h->probe);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
-  delete h;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return LBM_OK;
+    if (h->batch) return fail(LBM_EINVAL, "remove the lattice from its batch before destroying it");
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int b = 0; b < 2; ++b)
+      if (h->f[b]) (void)hipFree(h->f[b]);
+    if (h->types) (void)hipFree(h->types);
+    if (h->partials) (void)hipFree(h->partials);
+    if (h->red) (void)hipFree(h->red);
+    if (h->fields) (void)hipFree(h->fields);
+    if (h->mrt_tab) (void)hipFree(h->mrt_tab);
+    if (h->probe) (void)hipFree(h->probe);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LBM_OK;
+  });
 }
 
 static lbm_status ensure_fields(lbm_handle h) {
@@ -249,42 +255,48 @@ static lbm_status ensure_fields(lbm_handle h) {
 
 lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, const double *uy,
                            const double *uz) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  lbm_status st = ensure_fields(h);
-  if (st != LBM_OK) return st;
-  const double *src[4] = {rho, ux, uy, uz};
-  const double *dev[4] = {nullptr, nullptr, nullptr, nullptr};
-  for (int k = 0; k < 4; ++k)
-    if (src[k]) {
-      double *dst = h->fields + (size_t)k * h->N;
-      HIPCHK(hipMemcpyAsync(dst, src[k], (size_t)h->N * sizeof(double), hipMemcpyHostToDevice, h->strm()));
-      dev[k] = dst;
-    }
-  HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  h->steps = 0;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    lbm_status st = ensure_fields(h);
+    if (st != LBM_OK) return st;
+    const double *src[4] = {rho, ux, uy, uz};
+    const double *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int k = 0; k < 4; ++k)
+      if (src[k]) {
+        double *dst = h->fields + (size_t)k * h->N;
+        HIPCHK(hipMemcpyAsync(dst, src[k], (size_t)h->N * sizeof(double), hipMemcpyHostToDevice, h->strm()));
+        dev[k] = dst;
+      }
+    HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    h->steps = 0;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], uint64_t seed,
                                 double noise_amp) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  const double u[3] = {u0 ? u0[0] : 0.0, u0 ? u0[1] : 0.0, u0 ? u0[2] : 0.0};
-  HIPCHK(launch_init_equilibrium(h->cfg, h->params(h->cur), rho0, u, seed, noise_amp, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  h->steps = 0;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    const double u[3] = {u0 ? u0[0] : 0.0, u0 ? u0[1] : 0.0, u0 ? u0[2] : 0.0};
+    HIPCHK(launch_init_equilibrium(h->cfg, h->params(h->cur), rho0, u, seed, noise_amp, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    h->steps = 0;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_init_reference(lbm_handle h, double rho0) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  h->steps = 0;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    h->steps = 0;
+    return LBM_OK;
+  });
 }
 
 // Steps are enqueued as replays of a captured two-step graph (cur -> other -> cur); an odd step
@@ -340,193 +352,226 @@ static lbm_status enqueue_steps(lbm_handle h, int64_t nsteps) {
 }
 
 lbm_status lbm_step(lbm_handle h, int64_t nsteps) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
-  if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
-  HIPCHK(hipSetDevice(h->device));
-  return enqueue_steps(h, nsteps);
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+    if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
+    HIPCHK(hipSetDevice(h->device));
+    return enqueue_steps(h, nsteps);
+  });
 }
 
 lbm_status lbm_step_timed(lbm_handle h, int64_t nsteps, double *ms) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
-  if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipEventRecord(h->ev0, h->strm()));
-  lbm_status st = enqueue_steps(h, nsteps);
-  if (st != LBM_OK) return st;
-  HIPCHK(hipEventRecord(h->ev1, h->strm()));
-  HIPCHK(hipEventSynchronize(h->ev1));
-  float t = 0;
-  HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
-  if (ms) *ms = (double)t;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    if (nsteps < 0) return fail(LBM_EINVAL, "nsteps < 0");
+    if (h->batch) return fail(LBM_EINVAL, "the lattice is a batch member: step the batch");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipEventRecord(h->ev0, h->strm()));
+    lbm_status st = enqueue_steps(h, nsteps);
+    if (st != LBM_OK) return st;
+    HIPCHK(hipEventRecord(h->ev1, h->strm()));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    if (ms) *ms = (double)t;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sync(lbm_handle h) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    return LBM_OK;
+  });
 }
 
 int64_t lbm_steps_done(lbm_handle h) { return h ? h->steps : -1; }
 
 lbm_status lbm_check_reciprocal(uint32_t lo, uint32_t hi, uint64_t *mismatches) {
-  if (!mismatches) return fail(LBM_EINVAL, "mismatches is NULL");
-  HIPCHK(hipSetDevice(g_device));
-  unsigned long long *d = nullptr;
-  HIPCHK(hipMalloc((void **)&d, sizeof(*d)));
-  unsigned long long n = 0;
-  hipError_t e = hipMemcpy(d, &n, sizeof(n), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_check_reciprocal(lo, hi, d, nullptr);
-  if (e == hipSuccess) e = hipMemcpy(&n, d, sizeof(n), hipMemcpyDeviceToHost);
-  (void)hipFree(d);
-  if (e != hipSuccess) return fail(LBM_EDEVICE, std::string("reciprocal check failed: ") + hipGetErrorString(e));
-  *mismatches = (uint64_t)n;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!mismatches) return fail(LBM_EINVAL, "mismatches is NULL");
+    HIPCHK(hipSetDevice(g_device));
+    unsigned long long *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, sizeof(*d)));
+    unsigned long long n = 0;
+    hipError_t e = hipMemcpy(d, &n, sizeof(n), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_check_reciprocal(lo, hi, d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(&n, d, sizeof(n), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(LBM_EDEVICE, std::string("reciprocal check failed: ") + hipGetErrorString(e));
+    *mismatches = (uint64_t)n;
+    return LBM_OK;
+  });
 }
 
 int32_t lbm_steps_per_launch(lbm_handle h) { return h ? (h->multi_k > 1 ? h->multi_k : 1) : -1; }
 
 lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, double *uz) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  lbm_status st = ensure_fields(h);
-  if (st != LBM_OK) return st;
-  double *dst[4] = {rho, ux, uy, uz};
-  double *dev[4];
-  for (int k = 0; k < 4; ++k) dev[k] = dst[k] ? h->fields + (size_t)k * h->N : nullptr;
-  HIPCHK(launch_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
-  for (int k = 0; k < 4; ++k)
-    if (dst[k])
-      HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    lbm_status st = ensure_fields(h);
+    if (st != LBM_OK) return st;
+    double *dst[4] = {rho, ux, uy, uz};
+    double *dev[4];
+    for (int k = 0; k < 4; ++k) dev[k] = dst[k] ? h->fields + (size_t)k * h->N : nullptr;
+    HIPCHK(launch_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
+    for (int k = 0; k < 4; ++k)
+      if (dst[k])
+        HIPCHK(hipMemcpyAsync(dst[k], dev[k], (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_get_fields_at(lbm_handle h, const int64_t *nodes, int32_t count, double *rho, double *ux,
                              double *uy, double *uz) {
-  if (!h || (!nodes && count > 0) || count < 0) return fail(LBM_EINVAL, "bad argument");
-  if (count == 0) return LBM_OK;
-  for (int32_t k = 0; k < count; ++k)
-    if (nodes[k] < 0 || nodes[k] >= h->N) return fail(LBM_EINVAL, "node index out of range");
-  HIPCHK(hipSetDevice(h->device));
-  lbm_status st = ensure_fields(h);
-  if (st != LBM_OK) return st;
-  const size_t need = (size_t)count * (sizeof(int64_t) + 4 * sizeof(double));
-  if (need > h->probe_bytes) {
-    if (h->probe) HIPCHK(hipFree(h->probe));
-    h->probe = nullptr;
-    HIPCHK(hipMalloc(&h->probe, need));
-    h->probe_bytes = need;
-  }
-  int64_t *dnodes = (int64_t *)h->probe;
-  double *dout = (double *)((char *)h->probe + (size_t)count * sizeof(int64_t));
-  HIPCHK(hipMemcpyAsync(dnodes, nodes, (size_t)count * sizeof(int64_t), hipMemcpyHostToDevice, h->strm()));
-  double *f = h->fields;
-  HIPCHK(launch_fields(h->cfg, h->params(h->cur), f, f + h->N, f + 2 * h->N, f + 3 * h->N, h->strm()));
-  HIPCHK(launch_gather_fields(f, h->N, dnodes, count, dout, h->strm()));
-  std::vector<double> host((size_t)4 * count);
-  HIPCHK(hipMemcpyAsync(host.data(), dout, host.size() * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  double *dst[4] = {rho, ux, uy, uz};
-  for (int a = 0; a < 4; ++a)
-    if (dst[a]) std::copy(host.begin() + (long)a * count, host.begin() + (long)(a + 1) * count, dst[a]);
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || (!nodes && count > 0) || count < 0) return fail(LBM_EINVAL, "bad argument");
+    if (count == 0) return LBM_OK;
+    for (int32_t k = 0; k < count; ++k)
+      if (nodes[k] < 0 || nodes[k] >= h->N) return fail(LBM_EINVAL, "node index out of range");
+    HIPCHK(hipSetDevice(h->device));
+    lbm_status st = ensure_fields(h);
+    if (st != LBM_OK) return st;
+    const size_t need = (size_t)count * (sizeof(int64_t) + 4 * sizeof(double));
+    if (need > h->probe_bytes) {
+      if (h->probe) HIPCHK(hipFree(h->probe));
+      h->probe = nullptr;
+      HIPCHK(hipMalloc(&h->probe, need));
+      h->probe_bytes = need;
+    }
+    int64_t *dnodes = (int64_t *)h->probe;
+    double *dout = (double *)((char *)h->probe + (size_t)count * sizeof(int64_t));
+    HIPCHK(hipMemcpyAsync(dnodes, nodes, (size_t)count * sizeof(int64_t), hipMemcpyHostToDevice, h->strm()));
+    double *f = h->fields;
+    HIPCHK(launch_fields(h->cfg, h->params(h->cur), f, f + h->N, f + 2 * h->N, f + 3 * h->N, h->strm()));
+    HIPCHK(launch_gather_fields(f, h->N, dnodes, count, dout, h->strm()));
+    std::vector<double> host((size_t)4 * count);
+    HIPCHK(hipMemcpyAsync(host.data(), dout, host.size() * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    double *dst[4] = {rho, ux, uy, uz};
+    for (int a = 0; a < 4; ++a)
+      if (dst[a]) std::copy(host.begin() + (long)a * count, host.begin() + (long)(a + 1) * count, dst[a]);
+    return LBM_OK;
+  });
 }
 
 size_t lbm_state_bytes(lbm_handle h) { return h ? h->pop_bytes() : 0; }
 
 lbm_status lbm_get_state(lbm_handle h, void *dst) {
-  if (!h || !dst) return fail(LBM_EINVAL, "NULL argument");
-  HIPCHK(hipSetDevice(h->device));
-  const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
-  HIPCHK(hipMemcpy2DAsync(dst, row, h->f[h->cur], pitch, row, (size_t)h->Q, hipMemcpyDeviceToHost, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !dst) return fail(LBM_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(h->device));
+    const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
+    HIPCHK(hipMemcpy2DAsync(dst, row, h->f[h->cur], pitch, row, (size_t)h->Q, hipMemcpyDeviceToHost, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_set_state(lbm_handle h, const void *src) {
-  if (!h || !src) return fail(LBM_EINVAL, "NULL argument");
-  HIPCHK(hipSetDevice(h->device));
-  // Both buffers: in PHYSICAL mode solid nodes are never rewritten by a step.
-  const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
-  for (int b = 0; b < 2; ++b)
-    HIPCHK(hipMemcpy2DAsync(h->f[b], pitch, src, row, row, (size_t)h->Q, hipMemcpyHostToDevice, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !src) return fail(LBM_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(h->device));
+    // Both buffers: in PHYSICAL mode solid nodes are never rewritten by a step.
+    const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
+    for (int b = 0; b < 2; ++b)
+      HIPCHK(hipMemcpy2DAsync(h->f[b], pitch, src, row, row, (size_t)h->Q, hipMemcpyHostToDevice, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_get_populations(lbm_handle h, double *f) {
-  if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
-  std::vector<unsigned char> raw(h->pop_bytes());
-  lbm_status st = lbm_get_state(h, raw.data());
-  if (st != LBM_OK) return st;
-  const int64_t N = h->N;
-  for (int i = 0; i < h->Q; ++i) {
-    const double w = weight(h->Q, i);
-    for (int64_t n = 0; n < N; ++n) {
-      const size_t k = (size_t)(n + N * i);
-      switch (h->d.storage) {
-        case LBM_F64: f[k] = reinterpret_cast<const double *>(raw.data())[k]; break;
-        case LBM_F32: f[k] = reinterpret_cast<const float *>(raw.data())[k]; break;
-        case LBM_F16S: f[k] = dec_half(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
-        default: f[k] = dec_bf16(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
+    std::vector<unsigned char> raw(h->pop_bytes());
+    lbm_status st = lbm_get_state(h, raw.data());
+    if (st != LBM_OK) return st;
+    const int64_t N = h->N;
+    for (int i = 0; i < h->Q; ++i) {
+      const double w = weight(h->Q, i);
+      for (int64_t n = 0; n < N; ++n) {
+        const size_t k = (size_t)(n + N * i);
+        switch (h->d.storage) {
+          case LBM_F64: f[k] = reinterpret_cast<const double *>(raw.data())[k]; break;
+          case LBM_F32: f[k] = reinterpret_cast<const float *>(raw.data())[k]; break;
+          case LBM_F16S: f[k] = dec_half(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
+          default: f[k] = dec_bf16(reinterpret_cast<const uint16_t *>(raw.data())[k], w); break;
+        }
       }
     }
-  }
-  return LBM_OK;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_set_populations(lbm_handle h, const double *f) {
-  if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
-  std::vector<unsigned char> raw(h->pop_bytes());
-  const int64_t N = h->N;
-  for (int i = 0; i < h->Q; ++i) {
-    const double w = weight(h->Q, i);
-    for (int64_t n = 0; n < N; ++n) {
-      const size_t k = (size_t)(n + N * i);
-      switch (h->d.storage) {
-        case LBM_F64: reinterpret_cast<double *>(raw.data())[k] = f[k]; break;
-        case LBM_F32: reinterpret_cast<float *>(raw.data())[k] = (float)f[k]; break;
-        case LBM_F16S: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_half(f[k], w); break;
-        default: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_bf16(f[k], w); break;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
+    std::vector<unsigned char> raw(h->pop_bytes());
+    const int64_t N = h->N;
+    for (int i = 0; i < h->Q; ++i) {
+      const double w = weight(h->Q, i);
+      for (int64_t n = 0; n < N; ++n) {
+        const size_t k = (size_t)(n + N * i);
+        switch (h->d.storage) {
+          case LBM_F64: reinterpret_cast<double *>(raw.data())[k] = f[k]; break;
+          case LBM_F32: reinterpret_cast<float *>(raw.data())[k] = (float)f[k]; break;
+          case LBM_F16S: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_half(f[k], w); break;
+          default: reinterpret_cast<uint16_t *>(raw.data())[k] = enc_bf16(f[k], w); break;
+        }
       }
     }
-  }
-  return lbm_set_state(h, raw.data());
+    return lbm_set_state(h, raw.data());
+  });
 }
 
 lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int64_t *nonfinite) {
-  if (!h) return fail(LBM_EINVAL, "handle is NULL");
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(launch_monitors(h->cfg, h->params(h->cur), h->partials, h->red, h->strm()));
-  double out[3];
-  HIPCHK(hipMemcpyAsync(out, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  if (mass) *mass = out[0];
-  if (kinetic_energy) *kinetic_energy = out[1];
-  if (nonfinite) *nonfinite = (int64_t)out[2];
-  return out[2] > 0 ? fail(LBM_EUNSTABLE, "non-finite populations") : LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return fail(LBM_EINVAL, "handle is NULL");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(launch_monitors(h->cfg, h->params(h->cur), h->partials, h->red, h->strm()));
+    double out[3];
+    HIPCHK(hipMemcpyAsync(out, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    if (mass) *mass = out[0];
+    if (kinetic_energy) *kinetic_energy = out[1];
+    if (nonfinite) *nonfinite = (int64_t)out[2];
+    return out[2] > 0 ? fail(LBM_EUNSTABLE, "non-finite populations") : LBM_OK;
+  });
 }
 
 lbm_status lbm_force(lbm_handle h, const int32_t box[6], double force[3]) {
-  if (!h || !force) return fail(LBM_EINVAL, "NULL argument");
-  int b[6] = {0, h->d.nx, 0, h->d.ny, 0, h->d.nz};
-  if (box)
-    for (int k = 0; k < 6; ++k) b[k] = box[k];
-  HIPCHK(hipSetDevice(h->device));
-  HIPCHK(launch_force(h->cfg, h->params(h->cur), b, h->partials, h->red, h->strm()));
-  HIPCHK(hipMemcpyAsync(force, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
-  HIPCHK(hipStreamSynchronize(h->strm()));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !force) return fail(LBM_EINVAL, "NULL argument");
+    int b[6] = {0, h->d.nx, 0, h->d.ny, 0, h->d.nz};
+    if (box)
+      for (int k = 0; k < 6; ++k) b[k] = box[k];
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(launch_force(h->cfg, h->params(h->cur), b, h->partials, h->red, h->strm()));
+    HIPCHK(hipMemcpyAsync(force, h->red, 3 * sizeof(double), hipMemcpyDeviceToHost, h->strm()));
+    HIPCHK(hipStreamSynchronize(h->strm()));
+    return LBM_OK;
+  });
 }
 
 }  // extern "C"
 
 namespace lbm {
 lbm_status set_error(lbm_status s, const std::string &m) { return fail(s, m); }
+
+lbm_status set_error_noexcept(lbm_status s, const char *m) noexcept {
+  try {
+    g_err = m;
+  } catch (...) {
+    g_err.clear();
+  }
+  return s;
+}
 
 int64_t population_stride(int64_t N, int esize) {
   static const bool pad = [] {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,21 @@ struct lbm_ctx {
 namespace lbm {
 // Sets the thread-local last-error message and returns s.
 lbm_status set_error(lbm_status s, const std::string &m);
+lbm_status set_error_noexcept(lbm_status s, const char *m) noexcept;
+// The body of a C-ABI entry point: no C++ exception crosses the ABI (a host allocation failure
+// returns LBM_ENOMEM, any other exception LBM_EINTERNAL, with the message in lbm_last_error).
+template <class F>
+inline lbm_status abi_guard(F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return set_error_noexcept(LBM_ENOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return set_error_noexcept(LBM_EINTERNAL, e.what());
+  } catch (...) {
+    return set_error_noexcept(LBM_EINTERNAL, "unknown C++ exception");
+  }
+}
 // Number of compute units of the handle's device (cached).
 int device_cu_count(int device);
 // Work counters of a batch: table rewrites and step-graph captures.

@@ -150,173 +150,191 @@ lbm_status admit(lbm_server_ctx *s) {
 extern "C" {
 
 lbm_status lbm_server_create(size_t budget_bytes, int32_t steps_per_tick, lbm_server *out) {
-  if (!out) return set_error(LBM_EINVAL, "out is NULL");
-  *out = nullptr;
-  if (budget_bytes == 0 || steps_per_tick < 1) return set_error(LBM_EINVAL, "budget and steps_per_tick must be positive");
-  lbm_server_ctx *s = new (std::nothrow) lbm_server_ctx();
-  if (!s) return set_error(LBM_ENOMEM, "host allocation failed");
-  s->budget = budget_bytes;
-  s->steps_per_tick = steps_per_tick;
-  lbm_status st = lbm_batch_create(&s->batch);
-  if (st != LBM_OK) {
-    delete s;
-    return st;
-  }
-  *out = s;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!out) return set_error(LBM_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (budget_bytes == 0 || steps_per_tick < 1) return set_error(LBM_EINVAL, "budget and steps_per_tick must be positive");
+    lbm_server_ctx *s = new (std::nothrow) lbm_server_ctx();
+    if (!s) return set_error(LBM_ENOMEM, "host allocation failed");
+    s->budget = budget_bytes;
+    s->steps_per_tick = steps_per_tick;
+    lbm_status st = lbm_batch_create(&s->batch);
+    if (st != LBM_OK) {
+      delete s;
+      return st;
+    }
+    *out = s;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_server_submit(lbm_server s, const lbm_job_spec *spec, int64_t *job_id) {
-  if (!s || !spec || !spec->node_type) return set_error(LBM_EINVAL, "NULL argument");
-  if (spec->nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
-  const size_t fp = lbm_footprint_bytes(&spec->desc);
-  if (fp == 0) return set_error(LBM_EINVAL, "invalid job descriptor");
-  if (fp > s->budget) return set_error(LBM_ENOMEM, "job footprint exceeds the server's HBM budget");
-  Job *j = new (std::nothrow) Job();
-  if (!j) return set_error(LBM_ENOMEM, "host allocation failed");
-  j->id = s->next_id++;
-  j->spec = *spec;
-  const size_t N = (size_t)spec->desc.nx * spec->desc.ny * spec->desc.nz;
-  j->types.assign(spec->node_type, spec->node_type + N);
-  j->spec.node_type = nullptr;
-  j->footprint = fp;
-  j->submit_s = s->now();
-  s->queue.push_back(j);
-  if (job_id) *job_id = j->id;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s || !spec || !spec->node_type) return set_error(LBM_EINVAL, "NULL argument");
+    if (spec->nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
+    const size_t fp = lbm_footprint_bytes(&spec->desc);
+    if (fp == 0) return set_error(LBM_EINVAL, "invalid job descriptor");
+    if (fp > s->budget) return set_error(LBM_ENOMEM, "job footprint exceeds the server's HBM budget");
+    Job *j = new (std::nothrow) Job();
+    if (!j) return set_error(LBM_ENOMEM, "host allocation failed");
+    j->id = s->next_id++;
+    j->spec = *spec;
+    const size_t N = (size_t)spec->desc.nx * spec->desc.ny * spec->desc.nz;
+    j->types.assign(spec->node_type, spec->node_type + N);
+    j->spec.node_type = nullptr;
+    j->footprint = fp;
+    j->submit_s = s->now();
+    s->queue.push_back(j);
+    if (job_id) *job_id = j->id;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_server_tick(lbm_server s, int32_t *active, int32_t *queued) {
-  if (!s) return set_error(LBM_EINVAL, "server is NULL");
-  lbm_status st = admit(s);
-  if (st != LBM_OK) return st;
-  // zero-step jobs finish at admission
-  for (size_t i = 0; i < s->active.size();) {
-    Job *j = s->active[i];
-    if (j->done >= j->spec.nsteps) {
-      s->active.erase(s->active.begin() + (long)i);
-      retire(s, j, true);
-    } else {
-      ++i;
-    }
-  }
-  if (!s->active.empty()) {
-    int64_t k = s->steps_per_tick;
-    for (Job *j : s->active) k = std::min<int64_t>(k, j->spec.nsteps - j->done);
-    const double t_launch = s->now();
-    st = lbm_batch_step(s->batch, k);
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s) return set_error(LBM_EINVAL, "server is NULL");
+    lbm_status st = admit(s);
     if (st != LBM_OK) return st;
-    const double t_wait = s->now();
-    st = lbm_batch_sync(s->batch);
-    if (st != LBM_OK) return st;
-    const double t_done = s->now();
-    s->stats.launch_s += t_wait - t_launch;
-    s->stats.wait_s += t_done - t_wait;
-    ++s->stats.ticks;
-    s->stats.steps += k;
-    ++s->ticks;
-    const bool check = s->check_every > 0 && s->ticks % s->check_every == 0;
+    // zero-step jobs finish at admission
     for (size_t i = 0; i < s->active.size();) {
       Job *j = s->active[i];
-      j->done += k;
-      bool bad = false;
-      if (check && j->done < j->spec.nsteps) {
-        const double t_check = s->now();
-        int64_t nf = 0;
-        lbm_monitors(j->h, nullptr, nullptr, &nf);
-        bad = nf > 0;
-        s->stats.check_s += s->now() - t_check;
-      }
-      if (j->done >= j->spec.nsteps || bad) {
+      if (j->done >= j->spec.nsteps) {
         s->active.erase(s->active.begin() + (long)i);
-        retire(s, j, !bad);
+        retire(s, j, true);
       } else {
         ++i;
       }
     }
-    st = admit(s);  // memory freed by retired jobs goes to waiting jobs right away
-    if (st != LBM_OK) return st;
-  }
-  if (active) *active = (int32_t)s->active.size();
-  if (queued) *queued = (int32_t)s->queue.size();
-  return LBM_OK;
+    if (!s->active.empty()) {
+      int64_t k = s->steps_per_tick;
+      for (Job *j : s->active) k = std::min<int64_t>(k, j->spec.nsteps - j->done);
+      const double t_launch = s->now();
+      st = lbm_batch_step(s->batch, k);
+      if (st != LBM_OK) return st;
+      const double t_wait = s->now();
+      st = lbm_batch_sync(s->batch);
+      if (st != LBM_OK) return st;
+      const double t_done = s->now();
+      s->stats.launch_s += t_wait - t_launch;
+      s->stats.wait_s += t_done - t_wait;
+      ++s->stats.ticks;
+      s->stats.steps += k;
+      ++s->ticks;
+      const bool check = s->check_every > 0 && s->ticks % s->check_every == 0;
+      for (size_t i = 0; i < s->active.size();) {
+        Job *j = s->active[i];
+        j->done += k;
+        bool bad = false;
+        if (check && j->done < j->spec.nsteps) {
+          const double t_check = s->now();
+          int64_t nf = 0;
+          lbm_monitors(j->h, nullptr, nullptr, &nf);
+          bad = nf > 0;
+          s->stats.check_s += s->now() - t_check;
+        }
+        if (j->done >= j->spec.nsteps || bad) {
+          s->active.erase(s->active.begin() + (long)i);
+          retire(s, j, !bad);
+        } else {
+          ++i;
+        }
+      }
+      st = admit(s);  // memory freed by retired jobs goes to waiting jobs right away
+      if (st != LBM_OK) return st;
+    }
+    if (active) *active = (int32_t)s->active.size();
+    if (queued) *queued = (int32_t)s->queue.size();
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_server_poll(lbm_server s, lbm_job_result *out, int32_t cap, int32_t *n) {
-  if (!s || (!out && cap > 0)) return set_error(LBM_EINVAL, "NULL argument");
-  const int32_t k = (int32_t)std::min<size_t>((size_t)std::max(cap, 0), s->finished.size());
-  for (int32_t i = 0; i < k; ++i) out[i] = s->finished[(size_t)i];
-  s->finished.erase(s->finished.begin(), s->finished.begin() + k);
-  if (n) *n = k;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s || (!out && cap > 0)) return set_error(LBM_EINVAL, "NULL argument");
+    const int32_t k = (int32_t)std::min<size_t>((size_t)std::max(cap, 0), s->finished.size());
+    for (int32_t i = 0; i < k; ++i) out[i] = s->finished[(size_t)i];
+    s->finished.erase(s->finished.begin(), s->finished.begin() + k);
+    if (n) *n = k;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_server_take(lbm_server s, int64_t job_id, double *rho, double *ux, double *uy, double *uz) {
-  if (!s) return set_error(LBM_EINVAL, "server is NULL");
-  auto it = s->kept.find(job_id);
-  if (it == s->kept.end()) return set_error(LBM_EINVAL, "no kept fields for this job");
-  const Fields &f = it->second;
-  if (rho) std::copy(f.rho.begin(), f.rho.end(), rho);
-  if (ux) std::copy(f.ux.begin(), f.ux.end(), ux);
-  if (uy) std::copy(f.uy.begin(), f.uy.end(), uy);
-  if (uz) std::copy(f.uz.begin(), f.uz.end(), uz);
-  s->kept.erase(it);
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s) return set_error(LBM_EINVAL, "server is NULL");
+    auto it = s->kept.find(job_id);
+    if (it == s->kept.end()) return set_error(LBM_EINVAL, "no kept fields for this job");
+    const Fields &f = it->second;
+    if (rho) std::copy(f.rho.begin(), f.rho.end(), rho);
+    if (ux) std::copy(f.ux.begin(), f.ux.end(), ux);
+    if (uy) std::copy(f.uy.begin(), f.uy.end(), uy);
+    if (uz) std::copy(f.uz.begin(), f.uz.end(), uz);
+    s->kept.erase(it);
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_server_cancel(lbm_server s, int64_t job_id) {
-  if (!s) return set_error(LBM_EINVAL, "server is NULL");
-  for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
-    if ((*it)->id == job_id) {
-      delete *it;
-      s->queue.erase(it);
-      return LBM_OK;
-    }
-  for (size_t i = 0; i < s->active.size(); ++i)
-    if (s->active[i]->id == job_id) {
-      Job *j = s->active[i];
-      s->active.erase(s->active.begin() + (long)i);
-      lbm_batch_sync(s->batch);
-      lbm_batch_remove(s->batch, j->h);
-      lbm_destroy(j->h);
-      s->used -= j->footprint;
-      delete j;
-      return LBM_OK;
-    }
-  return set_error(LBM_EINVAL, "unknown or finished job");
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s) return set_error(LBM_EINVAL, "server is NULL");
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+      if ((*it)->id == job_id) {
+        delete *it;
+        s->queue.erase(it);
+        return LBM_OK;
+      }
+    for (size_t i = 0; i < s->active.size(); ++i)
+      if (s->active[i]->id == job_id) {
+        Job *j = s->active[i];
+        s->active.erase(s->active.begin() + (long)i);
+        lbm_batch_sync(s->batch);
+        lbm_batch_remove(s->batch, j->h);
+        lbm_destroy(j->h);
+        s->used -= j->footprint;
+        delete j;
+        return LBM_OK;
+      }
+    return set_error(LBM_EINVAL, "unknown or finished job");
+  });
 }
 
 lbm_status lbm_server_set_check_every(lbm_server s, int32_t ticks) {
-  if (!s || ticks < 0) return set_error(LBM_EINVAL, "bad argument");
-  s->check_every = ticks;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s || ticks < 0) return set_error(LBM_EINVAL, "bad argument");
+    s->check_every = ticks;
+    return LBM_OK;
+  });
 }
 
 size_t lbm_server_used_bytes(lbm_server s) { return s ? s->used : 0; }
 
 lbm_status lbm_server_get_stats(lbm_server s, lbm_server_stats *out) {
-  if (!s || !out) return set_error(LBM_EINVAL, "NULL argument");
-  *out = s->stats;
-  const BatchCounters c = batch_counters(s->batch);
-  out->batch_rebuilds = c.rebuilds;
-  out->graph_captures = c.captures;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s || !out) return set_error(LBM_EINVAL, "NULL argument");
+    *out = s->stats;
+    const BatchCounters c = batch_counters(s->batch);
+    out->batch_rebuilds = c.rebuilds;
+    out->graph_captures = c.captures;
+    return LBM_OK;
+  });
 }
 
 double lbm_server_clock(lbm_server s) { return s ? s->now() : -1.0; }
 
 lbm_status lbm_server_destroy(lbm_server s) {
-  if (!s) return LBM_OK;
-  if (s->batch) lbm_batch_sync(s->batch);
-  for (Job *j : s->active) {
-    lbm_batch_remove(s->batch, j->h);
-    lbm_destroy(j->h);
-    delete j;
-  }
-  for (Job *j : s->queue) delete j;
-  lbm_batch_destroy(s->batch);
-  delete s;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!s) return LBM_OK;
+    if (s->batch) lbm_batch_sync(s->batch);
+    for (Job *j : s->active) {
+      lbm_batch_remove(s->batch, j->h);
+      lbm_destroy(j->h);
+      delete j;
+    }
+    for (Job *j : s->queue) delete j;
+    lbm_batch_destroy(s->batch);
+    delete s;
+    return LBM_OK;
+  });
 }
 
 }  // extern "C"

@@ -301,145 +301,161 @@ lbm_status sc_steps(lbm_sc_handle h, int64_t nsteps) {
 extern "C" {
 
 lbm_status lbm_sc_create(const lbm_sc_desc *d, lbm_sc_handle *out) {
-  if (!out) return sc_fail(LBM_EINVAL, "out is NULL");
-  *out = nullptr;
-  if (!d) return sc_fail(LBM_EINVAL, "descriptor is NULL");
-  if (d->nx < 1 || d->ny < 1) return sc_fail(LBM_EINVAL, "lattice dimensions must be >= 1");
-  if ((uint64_t)d->nx * (uint64_t)d->ny >= (1ull << 31)) return sc_fail(LBM_EINVAL, "lattice too large");
-  if (d->storage != LBM_F64 && d->storage != LBM_F32) return sc_fail(LBM_EINVAL, "Shan-Chen storage must be F64 or F32");
-  if (!(d->omega1 > 0 && d->omega1 < 2 && d->omega2 > 0 && d->omega2 < 2))
-    return sc_fail(LBM_EINVAL, "relaxation rates must lie in (0, 2)");
-  if (!std::isfinite(d->g)) return sc_fail(LBM_EINVAL, "G is not finite");
-  lbm_sc_ctx *h = new (std::nothrow) lbm_sc_ctx();
-  if (!h) return sc_fail(LBM_ENOMEM, "host allocation failed");
-  h->d = *d;
-  h->N = (int64_t)d->nx * d->ny;
-  h->esize = d->storage == LBM_F64 ? 8 : 4;
-  h->stride = lbm::population_stride(h->N, h->esize);
-  if (const char *e = std::getenv("LBM_SC_SHAPE")) {
-    const int v = std::atoi(e);
-    if (v < 0 || v >= lbm::kScNumShapes) {
-      delete h;
-      return sc_fail(LBM_EINVAL, "LBM_SC_SHAPE out of range");
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!out) return sc_fail(LBM_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!d) return sc_fail(LBM_EINVAL, "descriptor is NULL");
+    if (d->nx < 1 || d->ny < 1) return sc_fail(LBM_EINVAL, "lattice dimensions must be >= 1");
+    if ((uint64_t)d->nx * (uint64_t)d->ny >= (1ull << 31)) return sc_fail(LBM_EINVAL, "lattice too large");
+    if (d->storage != LBM_F64 && d->storage != LBM_F32) return sc_fail(LBM_EINVAL, "Shan-Chen storage must be F64 or F32");
+    if (!(d->omega1 > 0 && d->omega1 < 2 && d->omega2 > 0 && d->omega2 < 2))
+      return sc_fail(LBM_EINVAL, "relaxation rates must lie in (0, 2)");
+    if (!std::isfinite(d->g)) return sc_fail(LBM_EINVAL, "G is not finite");
+    lbm_sc_ctx *h = new (std::nothrow) lbm_sc_ctx();
+    if (!h) return sc_fail(LBM_ENOMEM, "host allocation failed");
+    h->d = *d;
+    h->N = (int64_t)d->nx * d->ny;
+    h->esize = d->storage == LBM_F64 ? 8 : 4;
+    h->stride = lbm::population_stride(h->N, h->esize);
+    if (const char *e = std::getenv("LBM_SC_SHAPE")) {
+      const int v = std::atoi(e);
+      if (v < 0 || v >= lbm::kScNumShapes) {
+        delete h;
+        return sc_fail(LBM_EINVAL, "LBM_SC_SHAPE out of range");
+      }
+      h->shape = v;
     }
-    h->shape = v;
-  }
-  auto cleanup = [&](lbm_status s, const std::string &m) {
-    lbm_sc_destroy(h);
-    return sc_fail(s, m);
-  };
-  if (hipGetDevice(&h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipGetDevice failed");
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-    return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
-  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-    return cleanup(LBM_EDEVICE, "hipEventCreate failed");
-  for (int b = 0; b < 2; ++b)
-    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess) return cleanup(LBM_ENOMEM, "population buffer allocation failed");
-  if (hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)) != hipSuccess)
-    return cleanup(LBM_ENOMEM, "field buffer allocation failed");
-  *out = h;
-  return LBM_OK;
+    auto cleanup = [&](lbm_status s, const std::string &m) {
+      lbm_sc_destroy(h);
+      return sc_fail(s, m);
+    };
+    if (hipGetDevice(&h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipGetDevice failed");
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "hipEventCreate failed");
+    for (int b = 0; b < 2; ++b)
+      if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess) return cleanup(LBM_ENOMEM, "population buffer allocation failed");
+    if (hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "field buffer allocation failed");
+    *out = h;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_init(lbm_sc_handle h, const double *rho1, const double *rho2) {
-  if (!h || !rho1 || !rho2) return sc_fail(LBM_EINVAL, "handle or density is NULL");
-  SCCHK(hipSetDevice(h->device));
-  const size_t nb = (size_t)h->N * sizeof(double);
-  SCCHK(hipMemcpyAsync(h->fields, rho1, nb, hipMemcpyHostToDevice, h->stream));
-  SCCHK(hipMemcpyAsync(h->fields + h->N, rho2, nb, hipMemcpyHostToDevice, h->stream));
-  h->cur = 0;
-  const lbm::ScParams p = h->params();
-  SCCHK(sc_typed(h, [&](auto z) {
-    using T = decltype(z);
-    hipLaunchKernelGGL(lbm::k_sc_init<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, h->fields, h->fields + h->N);
-    return hipGetLastError();
-  }));
-  SCCHK(hipStreamSynchronize(h->stream));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !rho1 || !rho2) return sc_fail(LBM_EINVAL, "handle or density is NULL");
+    SCCHK(hipSetDevice(h->device));
+    const size_t nb = (size_t)h->N * sizeof(double);
+    SCCHK(hipMemcpyAsync(h->fields, rho1, nb, hipMemcpyHostToDevice, h->stream));
+    SCCHK(hipMemcpyAsync(h->fields + h->N, rho2, nb, hipMemcpyHostToDevice, h->stream));
+    h->cur = 0;
+    const lbm::ScParams p = h->params();
+    SCCHK(sc_typed(h, [&](auto z) {
+      using T = decltype(z);
+      hipLaunchKernelGGL(lbm::k_sc_init<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, h->fields, h->fields + h->N);
+      return hipGetLastError();
+    }));
+    SCCHK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_step(lbm_sc_handle h, int64_t nsteps) {
-  if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
-  if (nsteps < 0) return sc_fail(LBM_EINVAL, "nsteps < 0");
-  SCCHK(hipSetDevice(h->device));
-  return sc_steps(h, nsteps);
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
+    if (nsteps < 0) return sc_fail(LBM_EINVAL, "nsteps < 0");
+    SCCHK(hipSetDevice(h->device));
+    return sc_steps(h, nsteps);
+  });
 }
 
 lbm_status lbm_sc_step_timed(lbm_sc_handle h, int64_t nsteps, double *ms) {
-  if (!h || !ms) return sc_fail(LBM_EINVAL, "handle or ms is NULL");
-  SCCHK(hipSetDevice(h->device));
-  SCCHK(hipEventRecord(h->ev0, h->stream));
-  const lbm_status st = sc_steps(h, nsteps);
-  if (st != LBM_OK) return st;
-  SCCHK(hipEventRecord(h->ev1, h->stream));
-  SCCHK(hipEventSynchronize(h->ev1));
-  float t = 0;
-  SCCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
-  *ms = t;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !ms) return sc_fail(LBM_EINVAL, "handle or ms is NULL");
+    SCCHK(hipSetDevice(h->device));
+    SCCHK(hipEventRecord(h->ev0, h->stream));
+    const lbm_status st = sc_steps(h, nsteps);
+    if (st != LBM_OK) return st;
+    SCCHK(hipEventRecord(h->ev1, h->stream));
+    SCCHK(hipEventSynchronize(h->ev1));
+    float t = 0;
+    SCCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = t;
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_get_fields(lbm_sc_handle h, double *rho1, double *rho2, double *ux, double *uy) {
-  if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
-  SCCHK(hipSetDevice(h->device));
-  lbm::ScParams p = h->params();
-  double *F = h->fields;
-  SCCHK(sc_typed(h, [&](auto z) {
-    using T = decltype(z);
-    hipLaunchKernelGGL(lbm::k_sc_fields<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, F, F + h->N, F + 2 * h->N,
-                       F + 3 * h->N);
-    return hipGetLastError();
-  }));
-  double *dst[4] = {rho1, rho2, ux, uy};
-  for (int k = 0; k < 4; ++k)
-    if (dst[k]) SCCHK(hipMemcpyAsync(dst[k], F + (size_t)k * h->N, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  SCCHK(hipStreamSynchronize(h->stream));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return sc_fail(LBM_EINVAL, "handle is NULL");
+    SCCHK(hipSetDevice(h->device));
+    lbm::ScParams p = h->params();
+    double *F = h->fields;
+    SCCHK(sc_typed(h, [&](auto z) {
+      using T = decltype(z);
+      hipLaunchKernelGGL(lbm::k_sc_fields<T>, dim3(h->grid()), dim3(256), 0, h->stream, p, F, F + h->N, F + 2 * h->N,
+                         F + 3 * h->N);
+      return hipGetLastError();
+    }));
+    double *dst[4] = {rho1, rho2, ux, uy};
+    for (int k = 0; k < 4; ++k)
+      if (dst[k]) SCCHK(hipMemcpyAsync(dst[k], F + (size_t)k * h->N, (size_t)h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SCCHK(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_get_populations(lbm_sc_handle h, double *f) {
-  if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
-  SCCHK(hipSetDevice(h->device));
-  SCCHK(hipStreamSynchronize(h->stream));
-  std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize);
-  SCCHK(hipMemcpy(tmp.data(), h->f[h->cur], tmp.size(), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 18; ++i)
-    for (int64_t n = 0; n < h->N; ++n) {
-      const size_t src = (size_t)i * (size_t)h->stride + (size_t)n;
-      f[(size_t)i * (size_t)h->N + (size_t)n] =
-          h->esize == 8 ? ((const double *)tmp.data())[src] : (double)((const float *)tmp.data())[src];
-    }
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
+    SCCHK(hipSetDevice(h->device));
+    SCCHK(hipStreamSynchronize(h->stream));
+    std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize);
+    SCCHK(hipMemcpy(tmp.data(), h->f[h->cur], tmp.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 18; ++i)
+      for (int64_t n = 0; n < h->N; ++n) {
+        const size_t src = (size_t)i * (size_t)h->stride + (size_t)n;
+        f[(size_t)i * (size_t)h->N + (size_t)n] =
+            h->esize == 8 ? ((const double *)tmp.data())[src] : (double)((const float *)tmp.data())[src];
+      }
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_set_populations(lbm_sc_handle h, const double *f) {
-  if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
-  SCCHK(hipSetDevice(h->device));
-  SCCHK(hipStreamSynchronize(h->stream));
-  std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize, 0);
-  for (int i = 0; i < 18; ++i)
-    for (int64_t n = 0; n < h->N; ++n) {
-      const size_t dst = (size_t)i * (size_t)h->stride + (size_t)n;
-      const double v = f[(size_t)i * (size_t)h->N + (size_t)n];
-      if (h->esize == 8) ((double *)tmp.data())[dst] = v;
-      else ((float *)tmp.data())[dst] = (float)v;
-    }
-  SCCHK(hipMemcpy(h->f[h->cur], tmp.data(), tmp.size(), hipMemcpyHostToDevice));
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h || !f) return sc_fail(LBM_EINVAL, "handle or buffer is NULL");
+    SCCHK(hipSetDevice(h->device));
+    SCCHK(hipStreamSynchronize(h->stream));
+    std::vector<char> tmp((size_t)h->stride * 18 * (size_t)h->esize, 0);
+    for (int i = 0; i < 18; ++i)
+      for (int64_t n = 0; n < h->N; ++n) {
+        const size_t dst = (size_t)i * (size_t)h->stride + (size_t)n;
+        const double v = f[(size_t)i * (size_t)h->N + (size_t)n];
+        if (h->esize == 8) ((double *)tmp.data())[dst] = v;
+        else ((float *)tmp.data())[dst] = (float)v;
+      }
+    SCCHK(hipMemcpy(h->f[h->cur], tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    return LBM_OK;
+  });
 }
 
 lbm_status lbm_sc_destroy(lbm_sc_handle h) {
-  if (!h) return LBM_OK;
-  (void)hipSetDevice(h->device);
-  if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (int b = 0; b < 2; ++b)
-    if (h->f[b]) (void)hipFree(h->f[b]);
-  if (h->fields) (void)hipFree(h->fields);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
-  delete h;
-  return LBM_OK;
+  return lbm::abi_guard([&]() -> lbm_status {
+    if (!h) return LBM_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int b = 0; b < 2; ++b)
+      if (h->f[b]) (void)hipFree(h->f[b]);
+    if (h->fields) (void)hipFree(h->fields);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LBM_OK;
+  });
 }
 
 }  // extern "C"
