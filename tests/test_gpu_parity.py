@@ -315,3 +315,30 @@ def test_force_matches_restatement(storage):
     print(f"force {got} restatement {want}")
     assert abs(got[0]) > 1e-4
     assert np.allclose(got, want, rtol=1e-12, atol=1e-15)
+
+
+# The step kernels with non-temporal population stores run on 3D lattices over 256 MiB per buffer
+# (the benchmark's); LBM_NT_STORE=1 forces them on the small parity lattices.  Only the store kind
+# differs, so the populations must be bit-identical to the ordinary kernels' and, in FP64, to the
+# oracle's.
+NT_CASES = [c for c in PARITY if c[0].startswith("d3q")]
+
+
+@pytest.mark.parametrize("storage", [F64, F32, F16S, BF16S])
+@pytest.mark.parametrize("name,make,steps", NT_CASES, ids=[p[0] for p in NT_CASES])
+def test_nontemporal_store_kernels_are_bitwise(monkeypatch, name, make, steps, storage):
+    cfg, types = make(storage)
+    cfg = dict(cfg, mrt_engine=MRT_SCALAR) if cfg.get("collision", 0) else cfg
+    types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1)
+    rho, u = noisy_fields(types, cfg["lattice"], 21)
+    out = {}
+    for nt in ("0", "1"):
+        monkeypatch.setenv("LBM_NT_STORE", nt)
+        with Lattice(cfg, types) as lat:
+            lat.init_fields(rho, *u)
+            lat.step(min(steps, 40))
+            out[nt] = lat.populations()
+    assert np.array_equal(out["0"], out["1"]), name
+    if storage == F64:
+        fo = orc.run(cfg, types, orc.init_fields(cfg, types, rho, *u), min(steps, 40))
+        assert rel_l2(out["1"], fo) <= 1e-12
