@@ -714,13 +714,48 @@ __device__ __forceinline__ void collide_mrt_scalar(const P &p, const C *f, C rho
   mrt_inverse<C, CO>(m, out);
 }
 
+// Parity of the D3Q19 orthogonal rows under c -> -c.  Every row is a polynomial in c of definite
+// parity, so m_k sums over the 9 opposite-pair sums (even rows, plus the rest population) or the
+// 9 pair differences (odd rows), and the inverse builds both populations of a pair from one even
+// and one odd sum.
+namespace d19 {
+constexpr int pair_rep(int p) {  // the p-th direction i with i < opp(i)
+  int n = 0;
+  for (int i = 1; i < 19; ++i)
+    if (i < Lat<19>::opp[i]) {
+      if (n == p) return i;
+      ++n;
+    }
+  return -1;
+}
+constexpr bool row_parity(int k, int sign) {
+  for (int i = 0; i < 19; ++i)
+    if (M19[k][i] != sign * M19[k][Lat<19>::opp[i]]) return false;
+  return true;
+}
+constexpr bool every_row_has_parity() {
+  for (int k = 0; k < 19; ++k)
+    if (!row_parity(k, 1) && !row_parity(k, -1)) return false;
+  return true;
+}
+static_assert(every_row_has_parity(), "each D3Q19 orthogonal row is even or odd in c");
+static_assert(pair_rep(8) > 0 && pair_rep(9) == -1, "D3Q19 has 9 opposite pairs");
+}  // namespace d19
+
 // D3Q19 MRT in the orthogonal basis M19 (d'Humieres et al. 2002): m = M19 f, projected
 // equilibrium moments meq = M19 f_eq (closed form MEQ19), m* = m - s (m - meq),
-// out = M19^T diag(1 / |row|^2) m*.  FP32: out = f - M19^T ((s / |row|^2) (m - meq)), FMAs.
+// out = M19^T diag(1 / |row|^2) m*.  FP32: out = f - M19^T ((s / |row|^2) (m - meq)), FMAs, both
+// transforms in the pair form above (half the terms of the dense rows).
 template <class C, class P>
 __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
   const C phi[10] = {(C)1, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
-  C d[19];
+  C d[19], ps[9], pd[9];  // FP32: sums and differences of the opposite pairs
+  if constexpr (sizeof(C) == 4)
+    static_for<9>([&](auto PI) {
+      constexpr int i = d19::pair_rep(decltype(PI)::value), o = Lat<19>::opp[i];
+      ps[decltype(PI)::value] = f[i] + f[o];
+      pd[decltype(PI)::value] = f[i] - f[o];
+    });
   static_for<19>([&](auto K) {
     constexpr int k = decltype(K)::value;
     C meq = (C)-0.0;
@@ -733,18 +768,43 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
       }
     });
     C m = (C)-0.0;
-    static_for<19>([&](auto I) { m = add_int_term<C, M19[k][decltype(I)::value]>(m, f[decltype(I)::value]); });
+    if constexpr (sizeof(C) == 4) {
+      if constexpr (d19::row_parity(k, 1)) {
+        m = add_int_term<C, M19[k][0]>(m, f[0]);
+        static_for<9>([&](auto PI) { m = add_int_term<C, M19[k][d19::pair_rep(decltype(PI)::value)]>(m, ps[decltype(PI)::value]); });
+      } else {
+        static_for<9>([&](auto PI) { m = add_int_term<C, M19[k][d19::pair_rep(decltype(PI)::value)]>(m, pd[decltype(PI)::value]); });
+      }
+    } else {
+      static_for<19>([&](auto I) { m = add_int_term<C, M19[k][decltype(I)::value]>(m, f[decltype(I)::value]); });
+    }
     meq = rho * meq;
     if constexpr (sizeof(C) == 4) d[k] = (p.s32[k] * (1.0f / (float)M19_NORM2[k])) * (m - meq);
     else d[k] = m - (m - meq) * (C)p.s[k];  // m*
   });
-  static_for<19>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    if constexpr (sizeof(C) == 4) {
-      C acc = (C)-0.0;
-      static_for<19>([&](auto K) { acc = add_int_term<C, M19[decltype(K)::value][i]>(acc, d[decltype(K)::value]); });
-      out[i] = f[i] - acc;
-    } else {
+  if constexpr (sizeof(C) == 4) {
+    // out_i = f_i - sum_k M19[k][i] d_k: for a pair (i, opp i) the even rows give E, the odd rows
+    // +-O, so out_i = f_i - (E + O) and out_opp = f_opp - (E - O)
+    C e0 = (C)-0.0;
+    static_for<19>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      if constexpr (d19::row_parity(k, 1)) e0 = add_int_term<C, M19[k][0]>(e0, d[k]);
+    });
+    out[0] = f[0] - e0;
+    static_for<9>([&](auto PI) {
+      constexpr int i = d19::pair_rep(decltype(PI)::value), o = Lat<19>::opp[i];
+      C e = (C)-0.0, q = (C)-0.0;
+      static_for<19>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if constexpr (d19::row_parity(k, 1)) e = add_int_term<C, M19[k][i]>(e, d[k]);
+        else q = add_int_term<C, M19[k][i]>(q, d[k]);
+      });
+      out[i] = f[i] - (e + q);
+      out[o] = f[o] - (e - q);
+    });
+  } else {
+    static_for<19>([&](auto I) {
+      constexpr int i = decltype(I)::value;
       C acc = (C)-0.0;
       static_for<19>([&](auto K) {
         constexpr int k = decltype(K)::value;
@@ -752,8 +812,8 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
         if constexpr (c != (C)0) acc = acc + d[k] * c;
       });
       out[i] = acc;
-    }
-  });
+    });
+  }
 }
 
 template <int Q, int CO, class C, class P>

@@ -148,3 +148,32 @@ def test_d3q19_tables():
         rho, u = 1 + 0.1 * rng.standard_normal(), 0.1 * rng.standard_normal(3)
         fe = feq(rho, u, c.astype(float), w)
         assert np.allclose(rho * E @ phi(u), D @ fe, rtol=0, atol=1e-14)
+
+
+def test_d3q19_pair_form_equals_dense_transforms():
+    """The FP32 kernels evaluate m = M19 f and M19^T d on the opposite pairs: even rows over
+    f_0 and the 9 sums f_i + f_opp(i), odd rows over the 9 differences, and per pair
+    out_i = E + O, out_opp = E - O.  Restated here from the rows' parities (computed from the
+    generated matrix, with the opposite map found by search) and checked against the dense
+    products on random vectors; a row put in the wrong class fails it."""
+    D = np.array([[float(v) for v in row] for row in gen.d3q19_rows()])
+    c, _, opp = lattice(19)
+    even = [bool(np.all(D[k] == D[k][opp])) for k in range(19)]
+    odd = [bool(np.all(D[k] == -D[k][opp])) for k in range(19)]
+    assert all(e != o for e, o in zip(even, odd))  # every row has exactly one parity
+    reps = [i for i in range(1, 19) if i < opp[i]]
+    assert len(reps) == 9
+    rng = np.random.default_rng(19)
+    for _ in range(10):
+        f, d = rng.standard_normal(19), rng.standard_normal(19)
+        ps = np.array([f[i] + f[opp[i]] for i in reps])
+        pd = np.array([f[i] - f[opp[i]] for i in reps])
+        m = np.array([D[k][0] * f[0] + D[k][reps] @ ps if even[k] else D[k][reps] @ pd for k in range(19)])
+        assert np.allclose(m, D @ f, rtol=0, atol=1e-12)
+        out = np.zeros(19)
+        out[0] = sum(D[k][0] * d[k] for k in range(19) if even[k])
+        for i in reps:
+            E = sum(D[k][i] * d[k] for k in range(19) if even[k])
+            O = sum(D[k][i] * d[k] for k in range(19) if odd[k])
+            out[i], out[opp[i]] = E + O, E - O
+        assert np.allclose(out, D.T @ d, rtol=0, atol=1e-12)
