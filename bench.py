@@ -179,18 +179,19 @@ def _rounded(d):
     return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
 
 
-def serving_leg(njobs, scale):
+def serving_leg(njobs, scale, sides="north-star"):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
-    32^3-192^3, FP16-shifted storage) through the continuous-batching server, in real time and
-    saturated, against the same jobs run one at a time with the plain solver."""
+    32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
+    continuous-batching server, in real time and saturated, against the same jobs run one at a
+    time with the plain solver."""
     from lbm_amd import serving
 
-    jobs = serving.make_mix(njobs, step_scale=scale)
+    jobs = serving.make_mix(njobs, step_scale=scale, sides=sides)
     budget = 64 << 30
     rt = serving.serve(jobs, budget, realtime=True)
     sat = serving.serve(jobs, budget, realtime=False)
     seq = serving.sequential(jobs)
-    return {"jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
+    return {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
             "storage": "F16S", "realtime": _rounded(rt), "saturated": _rounded(sat),
             "one_at_a_time": _rounded(seq),
             "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
@@ -303,6 +304,7 @@ def main():
         except Exception as e:  # noqa: BLE001 — report, keep the headline line
             also[k] = {"error": str(e)}
     serving = serving_leg(args.serve_jobs, args.serve_scale) if args.serve_jobs > 0 else None
+    serving_small = serving_leg(args.serve_jobs, args.serve_scale * 2, "small") if args.serve_jobs > 0 else None
     try:
         shanchen = None if args.also == "none" else shanchen_leg()
     except Exception as e:  # noqa: BLE001 — report, keep the headline line
@@ -343,6 +345,7 @@ def main():
         "also": also,
         "ceilings_d3q27": ceilings,
         "serving": serving,
+        "serving_small_jobs": serving_small,
         "shanchen": shanchen,
     }
     print(json.dumps(out))

@@ -2,7 +2,7 @@
 
 The mix: jobs arrive as a Poisson process; half are D2Q9 BGK square lattices with side
 log-uniform in [64, 2048], half D3Q27 MRT (Gram-Schmidt basis) cubes with side log-uniform in
-[32, 192]; Re uniform in [50, 1000]; geometry is a lid cavity or a channel with a cylinder/sphere
+[32, 192] (the "small" mix: [32, 256] and [16, 48]); Re uniform in [50, 1000]; geometry is a lid cavity or a channel with a cylinder/sphere
 of radius 5-15 % of the height; steps uniform in [1000, 20000] times ``step_scale``; FP32
 arithmetic with FP16-shifted storage.
 
@@ -22,15 +22,23 @@ NU_MIN = 1.0 / 150.0
 U_LAT = 0.05
 
 
-def make_job(rng, step_scale=1.0):
+# side ranges (log-uniform) of the 3D and 2D jobs and the viscosity floor: the north-star mix, and
+# a mix of small jobs (each launch-latency-bound when run alone), where batching their steps pays
+# most; on lattices of 16-48 nodes the floor is raised to nu >= 1/30 (tau >= 0.6) to keep every
+# job stable
+SIDES = {"north-star": ((32, 192), (64, 2048), NU_MIN), "small": ((16, 48), (32, 256), 1.0 / 30.0)}
+
+
+def make_job(rng, step_scale=1.0, sides="north-star"):
+    (lo3, hi3), (lo2, hi2), nu_min = SIDES[sides]
     d3 = rng.random() < 0.5
-    side = int(round(math.exp(rng.uniform(math.log(32), math.log(192))) if d3 else
-                     math.exp(rng.uniform(math.log(64), math.log(2048)))))
+    side = int(round(math.exp(rng.uniform(math.log(lo3), math.log(hi3))) if d3 else
+                     math.exp(rng.uniform(math.log(lo2), math.log(hi2)))))
     re = float(rng.uniform(50, 1000))
     lid = rng.random() < 0.5
     steps = max(1, int(round(int(rng.integers(1000, 20001)) * step_scale)))
     L = side - 2
-    nu = max(U_LAT * L / re, NU_MIN)
+    nu = max(U_LAT * L / re, nu_min)
     re_run = U_LAT * L / nu
     om = cases.omega_from_nu(nu)
     if d3:
@@ -58,9 +66,9 @@ def make_job(rng, step_scale=1.0):
                 seed=int(rng.integers(1 << 62)), nodes=side ** (3 if d3 else 2))
 
 
-def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0):
+def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="north-star"):
     rng = np.random.default_rng(seed)
-    jobs = [make_job(rng, step_scale) for _ in range(njobs)]
+    jobs = [make_job(rng, step_scale, sides) for _ in range(njobs)]
     t = np.cumsum(rng.exponential(1.0 / rate, njobs))
     for j, a in zip(jobs, t):
         j["arrival_s"] = float(a)
