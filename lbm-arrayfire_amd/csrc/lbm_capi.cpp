@@ -66,6 +66,9 @@ std::string check_desc(const lbm_desc *d) {
   if (d->collision != LBM_BGK && d->lattice == 9) return "MRT is implemented for D3Q19 and D3Q27";
   if (d->collision == LBM_MRT_RAW && d->lattice == 19) return "D3Q19 MRT uses the orthogonal basis (LBM_MRT_GS)";
   if (d->storage < 0 || d->storage > 3) return "unknown storage format";
+  // 2-byte populations are addressed by 32-bit byte offsets from each population's base
+  if ((d->storage == LBM_F16S || d->storage == LBM_BF16S) && N >= (1ull << 31))
+    return "F16S / BF16S storage holds at most 2^31 - 1 nodes";
   if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
   if (d->mrt_engine < 0 || d->mrt_engine > 4) return "unknown MRT engine";
   if (d->mrt_engine == LBM_MRT_MFMA_BF16X3) return "the BF16x3 MFMA engine is not available in this build";

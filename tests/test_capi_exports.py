@@ -42,6 +42,8 @@ def test_desc_layout_matches_header():
     dict(bc_mode=3),
     dict(nx=0),
     dict(omega=float("nan")),
+    dict(storage=2, nx=65536, ny=32768),  # F16S: 2^31 nodes
+    dict(storage=3, nx=65536, ny=32768),  # BF16S: 2^31 nodes
 ])
 def test_create_rejects_bad_descriptors(lbm_lib, bad):
     cfg = dict(lattice=9, nx=8, ny=8, nz=1, omega=1.0)
