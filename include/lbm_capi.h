@@ -58,7 +58,8 @@ typedef enum { LBM_BGK = 0, LBM_MRT_RAW = 1, LBM_MRT_GS = 2 } lbm_collision;
 
 /* Population storage in HBM.  F16S stores the shifted value (f - w_i) * 2^15 as IEEE half,
  * BF16S stores f - w_i as bfloat16 (both round to nearest even) with FP32 collision arithmetic;
- * F64 computes in FP64, all others in FP32. */
+ * F64 computes in FP64, all others in FP32.  Lattices hold fewer than 2^32 nodes, and F16S /
+ * BF16S lattices fewer than 2^31 (lbm_create returns LBM_EINVAL otherwise). */
 typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_storage;
 
 /* Boundary semantics.  REFERENCE reproduces the reference: every axis periodic, full-way
