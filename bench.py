@@ -179,7 +179,7 @@ def _rounded(d):
     return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
 
 
-def serving_leg(njobs, scale, sides="north-star"):
+def serving_leg(njobs, scale, sides="mixed"):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
     continuous-batching server, in real time and saturated, against the same jobs run one at a

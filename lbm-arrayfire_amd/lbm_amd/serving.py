@@ -22,14 +22,14 @@ NU_MIN = 1.0 / 150.0
 U_LAT = 0.05
 
 
-# side ranges (log-uniform) of the 3D and 2D jobs and the viscosity floor: the north-star mix, and
+# side ranges (log-uniform) of the 3D and 2D jobs and the viscosity floor: the mixed workload, and
 # a mix of small jobs (each launch-latency-bound when run alone), where batching their steps pays
 # most; on lattices of 16-48 nodes the floor is raised to nu >= 1/30 (tau >= 0.6) to keep every
 # job stable
-SIDES = {"north-star": ((32, 192), (64, 2048), NU_MIN), "small": ((16, 48), (32, 256), 1.0 / 30.0)}
+SIDES = {"mixed": ((32, 192), (64, 2048), NU_MIN), "small": ((16, 48), (32, 256), 1.0 / 30.0)}
 
 
-def make_job(rng, step_scale=1.0, sides="north-star"):
+def make_job(rng, step_scale=1.0, sides="mixed"):
     (lo3, hi3), (lo2, hi2), nu_min = SIDES[sides]
     d3 = rng.random() < 0.5
     side = int(round(math.exp(rng.uniform(math.log(lo3), math.log(hi3))) if d3 else
@@ -66,7 +66,7 @@ def make_job(rng, step_scale=1.0, sides="north-star"):
                 seed=int(rng.integers(1 << 62)), nodes=side ** (3 if d3 else 2))
 
 
-def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="north-star"):
+def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="mixed"):
     rng = np.random.default_rng(seed)
     jobs = [make_job(rng, step_scale, sides) for _ in range(njobs)]
     t = np.cumsum(rng.exponential(1.0 / rate, njobs))

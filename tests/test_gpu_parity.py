@@ -1,6 +1,6 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same inputs.
 
-Tolerances (north-star bar): FP64 within 1e-12 relative L2 on rho/u and populations; FP32 storage
+Tolerances: FP64 within 1e-12 relative L2 on rho/u and populations; FP32 storage
 against the FP32 oracle within 1e-5 (same operation order, usually bit-identical); shifted
 FP16/BF16 storage within 1e-3 relative L2 on rho/u against the FP64 oracle.
 """
