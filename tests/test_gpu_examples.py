@@ -87,3 +87,30 @@ def test_d2q9_shanchen_example_matches_library(tmp_path):
     assert dims == (n, n, 1) and np.array_equal(rho, want[0]) and np.array_equal(u[:, 0], want[2])
     img = (tmp_path / f"shanchen_{steps}.ppm").read_bytes()
     assert img == f"P6\n{n} {n}\n255\n".encode() + output.scalar_image(n, n, want[0]).tobytes()
+
+
+CH_EXE = os.path.join(ROOT, "lbm-arrayfire_amd", "lib", "d3q19_mrt_channel")
+
+
+@pytest.mark.skipif(not os.path.exists(CH_EXE), reason="examples not built (run build())")
+def test_d3q19_channel_example_matches_library(tmp_path):
+    from lbm_amd import cases
+    from lbm_amd.capi import MRT_GS
+
+    nx, ny, nz, r, steps, re = 64, 24, 24, 4, 200, 10.0
+    env = dict(os.environ, LBM_GPU_NAME="T")
+    res = subprocess.run([CH_EXE, "0", str(nx), str(ny), str(nz), str(r), str(steps), str(re), "f32", "100",
+                          "out.bin", "viz"], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "C_d" in res.stdout and (tmp_path / "channel_200.vtk").exists()
+    rows = (tmp_path / f"T_d3q19_mrt_channel_mlups_{nx}.csv").read_text().splitlines()
+    assert [int(x.split(",")[0]) for x in rows[1:]] == [100, 200]
+    got = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(4, nx * ny * nz)
+    cfg, types = cases.d3q19_sphere_channel(nx, ny, nz, (nx // 4, ny // 2, nz // 2), r, re=re, storage=F32,
+                                            collision=MRT_GS)
+    with Lattice(cfg, types.reshape(-1)) as lat:
+        lat.init_equilibrium(1.0, (0.05, 0.0, 0.0), 1, 0.0)
+        lat.step(steps)
+        want = lat.fields()
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
