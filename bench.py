@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 
 def workloads():
     from lbm_amd import cases
-    from lbm_amd.capi import BF16S, F16S, F32, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
     W = {
         "d2q9_bgk_karman_1024x256_fp32": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
@@ -37,6 +37,8 @@ def workloads():
         # the same lattices with one launch per step (the multi-step launches switched off)
         "d2q9_bgk_karman_1024x256_fp32_1step": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
         "d2q9_bgk_karman_1024x256_f16s_1step": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F16S),
+        "d2q9_bgk_karman_1024x256_fp64": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F64),
+        "d2q9_bgk_karman_1024x256_fp64_1step": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F64),
         "d3q27_mrt_gs_lid_256_fp32": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32),
         "d3q27_mrt_gs_lid_256_f16s": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S),
         "d3q27_mrt_gs_lid_256_fp32_mfma2": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_MFMA_F32),

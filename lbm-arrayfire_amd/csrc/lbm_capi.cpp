@@ -174,10 +174,10 @@ lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *o
       p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
     }
 
-    // Multi-step launches for the lattices that qualify: the default shape unless LBM_MULTISTEP
-    // names another shape index, or "off" (single steps only).
+    // Multi-step launches for the lattices that qualify: the default shape (FP64: the default
+    // one-copy shape) unless LBM_MULTISTEP names another shape index, or "off" (single steps only).
     if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
-      int v = MULTI_DEFAULT;
+      int v = d->storage == LBM_F64 ? MULTI_DEFAULT_F64 : MULTI_DEFAULT;
       if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
       const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
       if (k > 1) {

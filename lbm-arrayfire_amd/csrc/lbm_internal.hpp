@@ -84,6 +84,7 @@ hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int g
 // does not qualify; the launch reads p.fin and leaves the state K steps later in p.fout.
 int multi_variant_count();
 constexpr int MULTI_DEFAULT = 9;  // the default launch shape (64 x 16 tiles, 5 steps), also batched
+constexpr int MULTI_DEFAULT_F64 = 3;  // FP64: 64 x 16 tiles, 4 steps, one LDS copy (133 KB)
 int multi_steps_per_launch(const Config &c, int variant);
 hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, hipStream_t s);
 // Multi-step tiles of a nx x ny lattice in the given shape; batched multi-step launches (shape

@@ -301,10 +301,14 @@ constexpr MultiVariant kMultiVariants[] = {LBM_MULTI_SHAPES(LBM_MULTI_ENTRY)};
 
 template <int ST, int BC, int TX, int K, int TY, int NT, bool DB>
 hipError_t launch_multi_shape(const StepParams &p, hipStream_t s) {
-  const int gxm = (p.nx + TX - 1) / TX, gym = (p.ny + TY - 1) / TY;
-  hipLaunchKernelGGL((k_step_multi<ST, BC, TX, K, TY, NT, DB>), dim3(gxm * gym), dim3(NT), 0, s, p, gxm,
-                     gxm * gym);
-  return hipGetLastError();
+  if constexpr (ST == ST_F64 && DB) {
+    return hipErrorInvalidValue;  // two FP64 copies of a region exceed the 160 KB of LDS
+  } else {
+    const int gxm = (p.nx + TX - 1) / TX, gym = (p.ny + TY - 1) / TY;
+    hipLaunchKernelGGL((k_step_multi<ST, BC, TX, K, TY, NT, DB>), dim3(gxm * gym), dim3(NT), 0, s, p, gxm,
+                       gxm * gym);
+    return hipGetLastError();
+  }
 }
 
 template <int ST, int BC>
@@ -350,8 +354,9 @@ hipError_t launch_step_multi_batch(const Config &c, const BatchArgs &b, int grid
 }
 
 int multi_steps_per_launch(const Config &c, int variant) {
-  if (c.Q != 9 || c.collision != CO_BGK || c.storage == ST_F64) return 0;  // and nx, ny < 65536
+  if (c.Q != 9 || c.collision != CO_BGK) return 0;  // and nx, ny < 65536
   if (variant < 0 || variant >= multi_variant_count()) return 0;
+  if (c.storage == ST_F64 && kMultiVariants[variant].DB) return 0;  // FP64: one-copy shapes only
   return kMultiVariants[variant].K;
 }
 
@@ -363,6 +368,7 @@ hipError_t launch_step_multi(const Config &c, int variant, const StepParams &p, 
                           : launch_multi_variant<ST, BC_PHYS>(variant, p, s);
   };
   switch (c.storage) {
+    case ST_F64: return byBC(IC<ST_F64>{});
     case ST_F32: return byBC(IC<ST_F32>{});
     case ST_F16S: return byBC(IC<ST_F16S>{});
     case ST_BF16S: return byBC(IC<ST_BF16S>{});
