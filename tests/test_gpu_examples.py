@@ -1,6 +1,7 @@
-"""The C++ host examples on the device: the D2Q9 lid program (the reference's cpp/d2q9_bgk_lid.cpp
-inputs and outputs) writes its headless window snapshots and MLUPS CSV, and the fields in its VTK
-snapshot equal the same lattice stepped through the library from Python, bit for bit."""
+"""The C++ host examples on the device: each program (the reference examples' inputs and outputs:
+D2Q9 lid, two-fluid Shan-Chen, D3Q19 MRT channel, D3Q27 MRT lid) writes its headless snapshots
+and MLUPS CSV, and the fields it writes equal the same lattice stepped through the library from
+Python, bit for bit."""
 import os
 import subprocess
 
