@@ -258,8 +258,8 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=50000)
+    ap.add_argument("--warmup", type=int, default=20000)
     ap.add_argument("--workload", default="d2q9_bgk_karman_1024x256_fp32")
     ap.add_argument("--also", default="default",
                     help="comma list of extra workloads timed briefly, 'none', or 'default'")
