@@ -101,3 +101,22 @@ def test_fp64_default_shape_and_double_copy_shapes_fall_back(monkeypatch):
     monkeypatch.setenv("LBM_MULTISTEP", "9")  # two FP64 copies would not fit: single steps
     with Lattice(cfg, t.reshape(-1)) as lat:
         assert lat.steps_per_launch == 1
+
+
+def test_wide_lattice_falls_back_to_single_steps_and_matches_oracle(monkeypatch):
+    """Multi-step tiles pack x | y << 16: a side of 65536 or more takes single steps, on the same
+    trajectory as the oracle."""
+    monkeypatch.delenv("LBM_MULTISTEP", raising=False)
+    cfg, _ = cases.d2q9_lid_reference(8, storage=F32)
+    cfg = dict(cfg, nx=70000, ny=4)
+    types = geo.lid_cavity_2d_reference(70000, 4).reshape(-1)
+    rng = np.random.default_rng(13)
+    ux = 1e-3 * rng.standard_normal(types.size)
+    uy = 1e-3 * rng.standard_normal(types.size)
+    with Lattice(cfg, types) as lat:
+        assert lat.steps_per_launch == 1
+        lat.init_fields(None, ux, uy, None)
+        lat.step(7)
+        got = lat.populations()
+    want = orc.run(cfg, types, orc.init_fields(cfg, types, None, ux, uy, None), 7)
+    assert np.array_equal(got, want)
