@@ -285,16 +285,17 @@ def main():
     copy_gbs = copy_bandwidth()
     names = [] if args.also == "none" else (
         [k for k in workloads() if k != args.workload] if args.also == "default" else args.also.split(","))
-    ceilings = pull_ceilings() if any(k.startswith("d3q27") for k in names + [args.workload]) else None
+    ceilings = pull_ceilings() if any(k.startswith(("d3q27", "d3q19")) for k in names + [args.workload]) else None
     r = measure(args.workload, args.steps, args.warmup, repeats=4)
     also = {}
     for k in names:
         try:
             m = measure(k, args.also_steps, 20)
             ceil = None
-            if ceilings and m["cfg"]["lattice"] == 27 and m["cfg"]["storage"] in (1, 2, 3):
-                # the benched D3Q27 lattices are over 128 MiB per buffer: non-temporal stores
-                ceil = ceilings["pull_4byte_nt_gbs" if m["cfg"]["storage"] == 1 else "pull_2byte_nt_gbs"]
+            if ceilings and m["cfg"]["lattice"] in (19, 27) and m["cfg"]["storage"] in (1, 2, 3):
+                # the benched 3D lattices are over 256 MiB per buffer: non-temporal stores
+                pre = "d3q19_" if m["cfg"]["lattice"] == 19 else ""
+                ceil = ceilings.get(pre + ("pull_4byte_nt_gbs" if m["cfg"]["storage"] == 1 else "pull_2byte_nt_gbs"))
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
                        "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
                        **({"frac_of_pull_ceiling": round(m["achieved_gbs"] / ceil, 4)} if ceil else {}),
@@ -345,7 +346,7 @@ def main():
         "wall_s_timed": round(r["wall_s"], 4),
         "nonfinite": r["nonfinite"],
         "also": also,
-        "ceilings_d3q27": ceilings,
+        "ceilings_3d": ceilings,
         "serving": serving,
         "serving_small_jobs": serving_small,
         "shanchen": shanchen,

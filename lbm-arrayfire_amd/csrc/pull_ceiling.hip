@@ -1,7 +1,8 @@
-// pull_ceiling.hip — the attainable rate of the D3Q27 step's memory access pattern, measured with
-// no arithmetic.  Every node of a 256^3 lattice pulls its 27 populations from x - c_i (periodic)
+// pull_ceiling.hip — the attainable rate of the D3Q27 (and D3Q19) step's memory access pattern,
+// measured with no arithmetic.  Every node of a 256^3 lattice pulls its 27 populations from x - c_i (periodic)
 // and stores them, in the library's padded SoA layout (population_stride), with 4-byte and 2-byte
-// populations; and a 16-byte-per-lane streaming copy moves the same number of bytes.  Each with
+// populations (D3Q19: its 19 directions, non-temporal stores); and a 16-byte-per-lane streaming copy
+// moves the same number of bytes.  Each with
 // ordinary and with non-temporal stores (the step kernels use the latter on lattices this large).
 // Prints one JSON line of GB/s, counted as the step kernels count them (2 Q s bytes per node).
 // bench.py reports these beside the D3Q27 kernels' rates: they bound what any step kernel in this
@@ -14,9 +15,9 @@ using namespace lbm;
 
 namespace {
 
-constexpr int N3 = 256, Q = 27;
+constexpr int N3 = 256;
 
-template <class T, bool NT>
+template <class T, bool NT, int Q = 27>
 __global__ void __launch_bounds__(256) k_pull(const T *__restrict__ in, T *__restrict__ out, int64_t stride) {
   const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63), y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
   const int z = (int)blockIdx.z;
@@ -75,7 +76,7 @@ double time_launches(L &&launch) {
   return ms * 1e-3 / 20;
 }
 
-template <class T, bool NT>
+template <class T, bool NT, int Q = 27>
 double pull_gbs() {
   const int64_t N = (int64_t)N3 * N3 * N3, stride = population_stride(N, (int)sizeof(T));
   const size_t bytes = (size_t)stride * Q * sizeof(T);
@@ -83,7 +84,7 @@ double pull_gbs() {
   if (!ok(hipMalloc(&a, bytes), "hipMalloc") || !ok(hipMalloc(&b, bytes), "hipMalloc")) return -1;
   if (!ok(hipMemset(a, 0, bytes), "hipMemset") || !ok(hipMemset(b, 0, bytes), "hipMemset")) return -1;
   const double t = time_launches([&](int odd) {
-    hipLaunchKernelGGL((k_pull<T, NT>), dim3(N3 / 64, N3 / 4, N3), dim3(256), 0, 0, odd ? b : a, odd ? a : b, stride);
+    hipLaunchKernelGGL((k_pull<T, NT, Q>), dim3(N3 / 64, N3 / 4, N3), dim3(256), 0, 0, odd ? b : a, odd ? a : b, stride);
   });
   (void)hipFree(a);
   (void)hipFree(b);
@@ -109,12 +110,13 @@ double copy_gbs(size_t bytes) {
 int main() {
   const double p4 = pull_gbs<float, false>(), p2 = pull_gbs<uint16_t, false>();
   const double p4n = pull_gbs<float, true>(), p2n = pull_gbs<uint16_t, true>();
-  const size_t cb = (size_t)N3 * N3 * N3 * Q * sizeof(float);
+  const double q4n = pull_gbs<float, true, 19>(), q2n = pull_gbs<uint16_t, true, 19>();
+  const size_t cb = (size_t)N3 * N3 * N3 * 27 * sizeof(float);
   const double c = copy_gbs<false>(cb), cn = copy_gbs<true>(cb);
-  if (p4 < 0 || p2 < 0 || c < 0 || p4n < 0 || p2n < 0 || cn < 0) return 1;
-  std::printf("{\"lattice\": \"256^3 D3Q27, padded SoA\", \"pull_4byte_gbs\": %.1f, \"pull_2byte_gbs\": %.1f, "
+  if (p4 < 0 || p2 < 0 || c < 0 || p4n < 0 || p2n < 0 || cn < 0 || q4n < 0 || q2n < 0) return 1;
+  std::printf("{\"lattice\": \"256^3, padded SoA (D3Q27; D3Q19 where named)\", \"pull_4byte_gbs\": %.1f, \"pull_2byte_gbs\": %.1f, "
               "\"copy_gbs\": %.1f, \"pull_4byte_nt_gbs\": %.1f, \"pull_2byte_nt_gbs\": %.1f, \"copy_nt_gbs\": %.1f, "
-              "\"copy_bytes\": %zu}\n",
-              p4, p2, c, p4n, p2n, cn, cb);
+              "\"d3q19_pull_4byte_nt_gbs\": %.1f, \"d3q19_pull_2byte_nt_gbs\": %.1f, \"copy_bytes\": %zu}\n",
+              p4, p2, c, p4n, p2n, cn, q4n, q2n, cb);
   return 0;
 }
