@@ -297,6 +297,7 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
 
 namespace lbm {
 BatchCounters batch_counters(lbm_batch b) { return b ? b->counters : BatchCounters{}; }
+hipStream_t batch_stream(lbm_batch b) { return b ? b->stream : nullptr; }
 }  // namespace lbm
 
 extern "C" {

@@ -115,117 +115,133 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
          (size_t)nt * 3 * sizeof(double) + 4 * sizeof(double);
 }
 
+}  // extern "C"
+
+namespace lbm {
+// lbm_create with the handle's work on the given stream (borrowed, not destroyed with the handle)
+// instead of a stream of its own; nullptr creates one.  The job server creates its lattices on
+// its batch's stream: a HIP stream costs about 0.6 ms to create and 0.2 ms to destroy, which
+// dominated the host time of small jobs.
+lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, lbm_handle *out) {
+  if (!out) return fail(LBM_EINVAL, "out is NULL");
+  *out = nullptr;
+  const std::string why = check_desc(d);
+  if (!why.empty()) return fail(LBM_EINVAL, why);
+  if (!node_type) return fail(LBM_EINVAL, "node_type is NULL");
+  lbm_ctx *h = new (std::nothrow) lbm_ctx();
+  if (!h) return fail(LBM_ENOMEM, "host allocation failed");
+  h->d = *d;
+  h->device = g_device;
+  h->Q = d->lattice;
+  h->N = (int64_t)d->nx * d->ny * d->nz;
+  h->esize = storage_bytes(d->storage);
+  h->stride = population_stride(h->N, h->esize);
+  int engine = d->mrt_engine;
+  if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
+  h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
+
+  StepParams &p = h->base;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.nz = d->nz;
+  tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
+  p.inlet_profile = d->inlet_profile;
+  p.meq_mode = d->meq_mode;
+  // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
+  // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
+  // at 160^3 F16S, -18 % at 128^3 F16S); LBM_NT_STORE=0 / 1 forces them off / on.
+  h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) ? 1 : 0;
+  if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;
+  if (d->lattice == 9) h->cfg.nt = 0;
+  p.N = h->N;
+  p.stride = h->stride;
+  p.omega = d->omega;
+  for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
+  p.u_in = d->u_in;
+  p.rho_bc = d->rho_bc;
+  for (int k = 0; k < 27; ++k) {
+    p.s[k] = d->s[k];
+    p.s32[k] = (float)d->s[k];
+  }
+
+  // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an
+  // inlet and every node of column nx - 1 solid or an outlet (the inlet replaces the populations it
+  // pulls across the wrap, the outlet takes all of its populations from its west neighbour); y:
+  // rows 0 and ny - 1 are solid (nodes pull nothing from beyond a wall).  Multi-step tiles hold the
+  // halo positions across such a wrap instead of updating them.
+  if (d->bc_mode == LBM_BC_PHYSICAL && d->nz == 1) {
+    bool xcut = d->nx >= 2, ycut = d->ny >= 2;
+    for (int y = 0; y < d->ny && xcut; ++y) {
+      const uint8_t a = node_type[(size_t)y * d->nx], b = node_type[(size_t)y * d->nx + d->nx - 1];
+      xcut = (a & (LBM_NODE_SOLID | LBM_NODE_INLET)) && (b & (LBM_NODE_SOLID | LBM_NODE_OUTLET));
+    }
+    for (int x = 0; x < d->nx && ycut; ++x)
+      ycut = (node_type[x] & LBM_NODE_SOLID) && (node_type[(size_t)(d->ny - 1) * d->nx + x] & LBM_NODE_SOLID);
+    p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
+  }
+
+  // Multi-step launches for the lattices that qualify: the default shape (FP64: the default
+  // one-copy shape) unless LBM_MULTISTEP names another shape index, or "off" (single steps only).
+  if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
+    int v = d->storage == LBM_F64 ? MULTI_DEFAULT_F64 : MULTI_DEFAULT;
+    if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
+    const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
+    if (k > 1) {
+      h->multi_variant = v;
+      h->multi_k = k;
+    }
+  }
+
+  auto cleanup = [&](lbm_status s, const std::string &m) {
+    lbm_destroy(h);
+    return fail(s, m);
+  };
+  if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
+  if (shared) {
+    h->stream = shared;
+    h->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
+  }
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "hipEventCreate failed");
+  for (int b = 0; b < 2; ++b)
+    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
+  if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
+  if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
+    return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
+  // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
+  // nodes whose pull needs the boundary path (NODE_SLOW).
+  {
+    uint8_t *raw = nullptr;
+    if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
+    hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(raw);
+    if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
+  }
+  p.type = h->types;
+  if (d->collision != LBM_BGK && d->lattice == 27) {
+    std::vector<float> tab(64 * MFMA_TAB);
+    build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
+    if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
+      return cleanup(LBM_ENOMEM, "MRT table allocation failed");
+    if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(LBM_EDEVICE, "MRT table upload failed");
+    p.mrt_tab = h->mrt_tab;
+  }
+  *out = h;
+  return LBM_OK;
+}
+}  // namespace lbm
+
+extern "C" {
+
 lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *out) {
-  return lbm::abi_guard([&]() -> lbm_status {
-    if (!out) return fail(LBM_EINVAL, "out is NULL");
-    *out = nullptr;
-    const std::string why = check_desc(d);
-    if (!why.empty()) return fail(LBM_EINVAL, why);
-    if (!node_type) return fail(LBM_EINVAL, "node_type is NULL");
-    lbm_ctx *h = new (std::nothrow) lbm_ctx();
-    if (!h) return fail(LBM_ENOMEM, "host allocation failed");
-    h->d = *d;
-    h->device = g_device;
-    h->Q = d->lattice;
-    h->N = (int64_t)d->nx * d->ny * d->nz;
-    h->esize = storage_bytes(d->storage);
-    h->stride = population_stride(h->N, h->esize);
-    int engine = d->mrt_engine;
-    if (engine == LBM_MRT_AUTO) engine = d->storage == LBM_F64 ? MRT_ENGINE_SCALAR : MRT_ENGINE_AUTO_FP32;
-    h->cfg = Config{d->lattice, d->storage, d->bc_mode, d->collision, engine};
-
-    StepParams &p = h->base;
-    p.nx = d->nx;
-    p.ny = d->ny;
-    p.nz = d->nz;
-    tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
-    p.inlet_profile = d->inlet_profile;
-    p.meq_mode = d->meq_mode;
-    // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
-    // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
-    // at 160^3 F16S, -18 % at 128^3 F16S); LBM_NT_STORE=0 / 1 forces them off / on.
-    h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) ? 1 : 0;
-    if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;
-    if (d->lattice == 9) h->cfg.nt = 0;
-    p.N = h->N;
-    p.stride = h->stride;
-    p.omega = d->omega;
-    for (int k = 0; k < 3; ++k) p.u_wall[k] = d->u_wall[k];
-    p.u_in = d->u_in;
-    p.rho_bc = d->rho_bc;
-    for (int k = 0; k < 27; ++k) {
-      p.s[k] = d->s[k];
-      p.s32[k] = (float)d->s[k];
-    }
-
-    // Periodic wraps that carry nothing in PHYSICAL mode.  x: every node of column 0 is solid or an
-    // inlet and every node of column nx - 1 solid or an outlet (the inlet replaces the populations it
-    // pulls across the wrap, the outlet takes all of its populations from its west neighbour); y:
-    // rows 0 and ny - 1 are solid (nodes pull nothing from beyond a wall).  Multi-step tiles hold the
-    // halo positions across such a wrap instead of updating them.
-    if (d->bc_mode == LBM_BC_PHYSICAL && d->nz == 1) {
-      bool xcut = d->nx >= 2, ycut = d->ny >= 2;
-      for (int y = 0; y < d->ny && xcut; ++y) {
-        const uint8_t a = node_type[(size_t)y * d->nx], b = node_type[(size_t)y * d->nx + d->nx - 1];
-        xcut = (a & (LBM_NODE_SOLID | LBM_NODE_INLET)) && (b & (LBM_NODE_SOLID | LBM_NODE_OUTLET));
-      }
-      for (int x = 0; x < d->nx && ycut; ++x)
-        ycut = (node_type[x] & LBM_NODE_SOLID) && (node_type[(size_t)(d->ny - 1) * d->nx + x] & LBM_NODE_SOLID);
-      p.wrap_cut = (xcut ? 1 : 0) | (ycut ? 2 : 0);
-    }
-
-    // Multi-step launches for the lattices that qualify: the default shape (FP64: the default
-    // one-copy shape) unless LBM_MULTISTEP names another shape index, or "off" (single steps only).
-    if (d->nz == 1 && d->nx < 65536 && d->ny < 65536) {  // the multi-step tiles pack x, y in 16 bits
-      int v = d->storage == LBM_F64 ? MULTI_DEFAULT_F64 : MULTI_DEFAULT;
-      if (const char *e = std::getenv("LBM_MULTISTEP")) v = std::strcmp(e, "off") == 0 ? -1 : std::atoi(e);
-      const int k = v >= 0 ? multi_steps_per_launch(h->cfg, v) : 0;
-      if (k > 1) {
-        h->multi_variant = v;
-        h->multi_k = k;
-      }
-    }
-
-    auto cleanup = [&](lbm_status s, const std::string &m) {
-      lbm_destroy(h);
-      return fail(s, m);
-    };
-    if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-      return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-      return cleanup(LBM_EDEVICE, "hipEventCreate failed");
-    for (int b = 0; b < 2; ++b)
-      if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
-        return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
-    if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
-    if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
-      return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
-    // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
-    // nodes whose pull needs the boundary path (NODE_SLOW).
-    {
-      uint8_t *raw = nullptr;
-      if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
-      hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
-      if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-      (void)hipFree(raw);
-      if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
-    }
-    p.type = h->types;
-    if (d->collision != LBM_BGK && d->lattice == 27) {
-      std::vector<float> tab(64 * MFMA_TAB);
-      build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
-      if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
-        return cleanup(LBM_ENOMEM, "MRT table allocation failed");
-      if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(LBM_EDEVICE, "MRT table upload failed");
-      p.mrt_tab = h->mrt_tab;
-    }
-    *out = h;
-    return LBM_OK;
-  });
+  return lbm::abi_guard([&]() -> lbm_status { return lbm::create_lattice(d, node_type, nullptr, out); });
 }
 
 lbm_status lbm_destroy(lbm_handle h) {
@@ -245,7 +261,7 @@ lbm_status lbm_destroy(lbm_handle h) {
     if (h->probe) (void)hipFree(h->probe);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LBM_OK;
   });

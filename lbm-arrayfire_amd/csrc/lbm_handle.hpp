@@ -22,6 +22,7 @@ struct lbm_ctx {
   int64_t stride = 0;  // elements between population arrays
   int esize = 4;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: a stream lent at creation (not destroyed with the handle)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
@@ -60,6 +61,9 @@ namespace lbm {
 // Sets the thread-local last-error message and returns s.
 lbm_status set_error(lbm_status s, const std::string &m);
 lbm_status set_error_noexcept(lbm_status s, const char *m) noexcept;
+// lbm_create on a lent stream (nullptr: a stream of its own), and a batch's stream
+lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, lbm_handle *out);
+hipStream_t batch_stream(lbm_batch b);
 // The body of a C-ABI entry point: no C++ exception crosses the ABI (a host allocation failure
 // returns LBM_ENOMEM, any other exception LBM_EINTERNAL, with the message in lbm_last_error).
 template <class F>

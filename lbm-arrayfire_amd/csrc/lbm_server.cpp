@@ -111,7 +111,11 @@ lbm_status admit(lbm_server_ctx *s) {
       ++it;
       continue;
     }
-    st = lbm_create(&j->spec.desc, j->types.data(), &j->h);
+    // on the batch's stream: the job's work runs there anyway once it joins the batch, and a
+    // stream of its own would cost more host time to create and destroy than a small job steps
+    st = lbm::abi_guard([&]() -> lbm_status {
+      return lbm::create_lattice(&j->spec.desc, j->types.data(), lbm::batch_stream(s->batch), &j->h);
+    });
     if (st == LBM_ENOMEM) {  // the device is fuller than the budget says: wait
       st = LBM_OK;
       break;
