@@ -126,6 +126,36 @@ def dhumieres_rates(omega, s_e=1.19, s_eps=1.4, s_q=1.2, s_pi=1.4, s_m=1.98, con
     return s + [1.0] * (27 - len(s))
 
 
+def suga_rates(omega=None, s0=1.0, s1=1.0, sv=1.2, sb=0.6, s3=1.2, s3b=0.6, s4=1.2, s4b=1.2, s5=1.2,
+               s6=1.2):
+    """The rates of the reference's work-in-progress 27-velocity MRT channel, after Suga et al.
+    (2015) (cpp/d3q19_mrt_channel.cpp:71-110), in the D3Q27 row order shared by the raw and the
+    Gram-Schmidt bases: rho; j (3); the off-diagonal stresses xy, xz, yz; the trace; the normal
+    stress differences x^2-y^2, x^2-z^2; the six third-order rows; xyz; x^2y^2, x^2z^2, y^2z^2;
+    x^2yz, xy^2z, xyz^2; the three fifth-order rows; x^2y^2z^2.  omega, when given, replaces s_nu
+    on the five stress rows (the program's own s_nu = 1.2 gives nu = 1/9)."""
+    if omega is not None:
+        sv = omega
+    return [s0, s1, s1, s1, sv, sv, sv, sb, sv, sv] + [s3] * 6 + [s3b] + [s4] * 3 + [s4b] * 3 + \
+        [s5] * 3 + [s6]
+
+
+def d3q27_mrt_channel_suga(nx=320, ny=80, nz=80, u_in=0.1, storage=F32, collision=MRT_GS):
+    """The setup of the reference's work-in-progress MRT channel (cpp/d3q19_mrt_channel.cpp:22-34,
+    71-121, 160-167; named D3Q19 but run on the 27-velocity set): a duct nx x ny x nz with walls on
+    y = 0, ny - 1, z = 0, nz - 1, a sphere of radius ny / 9 at (nx / 4, ny / 2 + ny / 30,
+    nz / 2 + nz / 30), inlet speed 0.1 and Suga's rates (s_nu = 1.2, nu = 1/9, Re = u * 2r / nu).
+    The program's inlet and outlet are unfinished; here they are the Zou-He uniform inlet and the
+    zero-gradient outlet of BC_PHYSICAL, and the equilibrium moments are those of f_eq (the
+    program's singular moment matrix, which never sets row 4, is the raw matrix with row 4 = xy).
+    collision MRT_GS relaxes Suga's groups in the orthogonal basis, MRT_RAW in the raw one."""
+    cx, cy, cz, r = nx // 4, ny // 2 + ny // 30, nz // 2 + nz // 30, ny // 9
+    s = suga_rates()
+    cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
+               bc_mode=BC_PHYSICAL, omega=s[4], s=s, u_in=u_in, inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_3d_physical(nx, ny, nz, cx, cy, cz, r)
+
+
 def d3q19_lid(n=64, u_lid=0.05, re=100.0, storage=F32, collision=BGK, bc_mode=BC_REFERENCE):
     """D3Q19 lid cavity (BGK or MRT in the orthogonal basis), nu = u*n/Re."""
     nu = u_lid * n / re

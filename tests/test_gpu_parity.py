@@ -54,6 +54,9 @@ PARITY = [
     ("d3q27_sphere_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st), 100),
     ("d3q27_sphere_mrt_phys_40x20x20", lambda st: cases.d3q27_sphere_channel(
         40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st, collision=MRT_GS), 60),
+    ("d3q27_suga_gs_phys_40x18x18", lambda st: cases.d3q27_mrt_channel_suga(40, 18, 18, storage=st), 60),
+    ("d3q27_suga_raw_phys_40x18x18", lambda st: cases.d3q27_mrt_channel_suga(
+        40, 18, 18, storage=st, collision=MRT_RAW), 60),
     ("d3q19_bgk_lid_ref_20", lambda st: cases.d3q19_lid(20, re=100.0, storage=st), 100),
     ("d3q19_mrt_lid_ref_20", lambda st: cases.d3q19_lid(20, re=100.0, storage=st, collision=MRT_GS), 60),
     ("d3q19_sphere_phys_40x20x20", lambda st: cases.d3q19_sphere_channel(
@@ -256,6 +259,7 @@ MFMA_CASES = [
         geo.lid_cavity_3d_reference(70, 20, 12)), 40),
     ("d3q27_sphere_mrt_phys_40x20x20", lambda st, en: cases.d3q27_sphere_channel(
         40, 20, 20, (12, 10, 10), 4, re=10.0, storage=st, collision=MRT_GS), 60),
+    ("d3q27_suga_gs_phys_40x18x18", lambda st, en: cases.d3q27_mrt_channel_suga(40, 18, 18, storage=st), 60),
 ]
 
 

@@ -463,3 +463,72 @@ def test_d3q19_mrt_duct_rates_are_stable():
     bad = dict(cfg, s=cases.dhumieres_rates(cfg["omega"]))
     rho_bad = orc.fields(bad, t, orc.run(bad, t, f0, 100))[0][fluid]
     assert not (np.isfinite(rho_bad).all() and np.abs(rho_bad - 1).max() < 0.3)
+
+
+def test_suga_channel_case_has_the_programs_setup():
+    """cases.d3q27_mrt_channel_suga carries the constants of cpp/d3q19_mrt_channel.cpp: a 320 x 80
+    x 80 duct (:24-26), a sphere of radius ny / 9 = 8 at (nx / 4, ny / 2 + ny / 30, nz / 2 + nz / 30)
+    = (80, 42, 42) (:31-34), walls on the four sides (:164-167), and its rate groups (:71-110)
+    laid on the 27 rows in the order of the raw matrix (rust/examples/d3q27_mrt_lid.rs:167-193)."""
+    from lbm_amd.capi import D3Q27, MRT_GS
+    cfg, t = cases.d3q27_mrt_channel_suga()
+    assert (cfg["lattice"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["collision"]) == (D3Q27, 320, 80, 80, MRT_GS)
+    t = t.reshape(80, 80, 320)
+    solid = (t & 1) != 0
+    assert solid[42, 42, 80] and solid[42 + 8, 42, 80] and not solid[42 + 9, 42, 80]
+    assert solid[42, 42 - 8, 80] and not solid[42, 42 - 9, 80] and not solid[42, 42, 80 + 9]
+    assert solid[0].all() and solid[-1].all() and solid[:, 0].all() and solid[:, -1].all()
+    assert (t[1:-1, 1:-1, 0] & 4).all() and (t[1:-1, 1:-1, -1] & 8).all()
+    s = np.array(cfg["s"])
+    expect = {1.0: [0, 1, 2, 3], 1.2: [4, 5, 6, 8, 9] + list(range(10, 16)) + list(range(17, 27)),
+              0.6: [7, 16]}
+    for v, rows in expect.items():
+        assert np.all(s[rows] == v), v
+    assert sorted(sum(expect.values(), [])) == list(range(27))
+    assert abs((1 / cfg["omega"] - 0.5) / 3 - 1 / 9) < 1e-15  # nu from s_nu = 1.2 (:114-115)
+    s2 = cases.suga_rates(omega=1.7)
+    assert [s2[k] for k in (4, 5, 6, 8, 9)] == [1.7] * 5 and s2[7] == 0.6
+
+
+def test_suga_channel_reduced_gs_is_steady_raw_diverges():
+    """The Suga-rate duct at 40 x 18 x 18 from a uniform start at u = 0.1: relaxed in the
+    Gram-Schmidt basis it settles to the steady flow of BGK at the same s_nu (velocity within a few
+    percent; the rates differ only on modes that steady low-Re flow barely excites); relaxed in the
+    raw basis, as the reference program does, the same groups couple non-orthogonal moments and
+    the run diverges within 300 steps."""
+    from lbm_amd.capi import BGK, MRT_RAW
+    out = {}
+    for name, kw in (("gs", {}), ("bgk", dict(collision=BGK)), ("raw", dict(collision=MRT_RAW))):
+        cfg, t = cases.d3q27_mrt_channel_suga(40, 18, 18, storage=F64, **kw)
+        t = t.reshape(-1)
+        N = t.size
+        f0 = orc.init_fields(cfg, t, np.ones(N), np.full(N, 0.1), np.zeros(N), np.zeros(N))
+        out[name] = orc.fields(cfg, t, orc.run(cfg, t, f0, 400 if name != "raw" else 300))
+    fluid = (t & 1) == 0
+    rho, ux = out["gs"][0][fluid], out["gs"][1][fluid]
+    assert np.isfinite(rho).all() and np.abs(rho - 1).max() < 1.2
+    ux_bgk = out["bgk"][1][fluid]
+    assert np.linalg.norm(ux - ux_bgk) / np.linalg.norm(ux_bgk) < 0.05
+    assert not np.isfinite(out["raw"][0][fluid]).all()
+
+
+@pytest.mark.parametrize("basis", ["gs", "raw"])
+def test_suga_rates_conserve_mass_in_a_closed_box(basis):
+    """Steps from a perturbed state inside a closed box (the duct with inlet and outlet walled)
+    conserve mass to round-off: the conserved rows carry rate 1, and mass moves only by streaming
+    and bounce-back."""
+    from lbm_amd.capi import MRT_RAW
+    kw = {} if basis == "gs" else dict(collision=MRT_RAW)
+    cfg, t = cases.d3q27_mrt_channel_suga(40, 18, 18, storage=F64, **kw)
+    t = t.reshape(-1)
+    N = t.size
+    box = np.where((t & 1) == 0, 0, 1).astype(np.uint8)
+    box.reshape(18, 18, 40)[:, :, [0, -1]] = 1
+    rng = np.random.default_rng(5)
+    fb0 = orc.init_fields(cfg, box, 1 + 0.01 * rng.standard_normal(N), 0.01 * rng.standard_normal(N),
+                          np.zeros(N), np.zeros(N))
+    fb = orc.run(cfg, box, fb0, 20)
+    inside = box == 0
+    m0 = orc.fields(cfg, box, fb0)[0][inside].sum()
+    m1 = orc.fields(cfg, box, fb)[0][inside].sum()
+    assert abs(m1 - m0) < 1e-9 * m0
