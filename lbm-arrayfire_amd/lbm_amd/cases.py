@@ -105,6 +105,53 @@ def d3q27_bgk_lid_reference(n=64, u_lid=0.1, re=None, nu=None, storage=F32):
     return cfg, g.lid_cavity_3d_reference(n, n, n)
 
 
+def channel_mrt_rates(s0=1.3, s1=1.3, sv=1.0, sb=0.55, s3=1.2, s3b=0.55, s4=1.2, s4b=1.2, s5=1.2, s6=1.2):
+    """The 27 rates of the reference's D3Q27 MRT channels (rust/examples/d3q27_mrt_channel.rs:106-144,
+    cpp/d3q27_mrt_channel.cpp:116-150) in the raw row order; nu = (1/sv - 1/2)/3 = 1/6."""
+    return [s0, s1, s1, s1, sv, sv, sv, sb, sv, sv] + [s3] * 6 + [s3b] + [s4] * 3 + [s4b] * 3 + \
+        [s5] * 3 + [s6]
+
+
+def d3q27_bgk_channel_reference(nx=200, ny=60, nz=60, u_in=0.1, re=220.0, storage=F32):
+    """cpp/d3q27_bgk_channel.cpp:57-79,118-124,181-187: a sphere of radius ny/10+1 at
+    (nx/4+1, ny/2+ny/30, nz/2+nz/30) in a box periodic in y and z, inlet u = 0.1 on x = 0, rho = 1 on
+    x = 0 and x = nx-1, nu = u * 2r / Re (Re 220), in the reference's boundary semantics.  The
+    program's D3Q27 weights are wrong (:88-91); the lattice here has the correct ones.  Start with
+    lbm_init_fields of reference_channel_start (the program's f_eq of its initial fields, :134-160)."""
+    r = ny // 10 + 1
+    cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=BGK, storage=storage, bc_mode=BC_REFERENCE,
+               omega=omega_from_nu(u_in * 2 * r / re), u_in=u_in, inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_3d_reference(nx, ny, nz, nx // 4 + 1, ny // 2 + ny // 30, nz // 2 + nz // 30, r)
+
+
+def d3q27_mrt_channel_reference(nx=320, ny=80, nz=80, u_in=0.1, storage=F32, collision=MRT_RAW,
+                                engine=MRT_AUTO):
+    """cpp/d3q27_mrt_channel.cpp:60-72,116-150,204-210,327-333: pipe (y-ny/2)^2+(z-nz/2)^2 >=
+    (ny/2)^2-1 and a sphere of radius ny/10+1 at (nx/4, ny/2+ny/15, nz/2), inlet u = 0.1 on x = 0,
+    rho = 1 on x = 0 and x = nx-1, the channel rates (s_nu = 1.0, nu = 1/6) on the raw basis with
+    the reference equilibrium table, reference boundary semantics.  The program's moment matrix
+    never sets row 4 (:176-177, singular); the raw matrix here has row 4 = xy as the Rust
+    programs do.  Start with lbm_init_fields of reference_channel_start."""
+    r = ny // 10 + 1
+    s = channel_mrt_rates()
+    cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
+               bc_mode=BC_REFERENCE, mrt_engine=engine, meq_mode=MEQ_REFERENCE,
+               omega=s[4], s=s, u_in=u_in, inlet_profile=INLET_UNIFORM)
+    return cfg, g.channel_3d_reference(nx, ny, nz, nx // 4, ny // 2 + ny // 15, nz // 2, r, pipe=True)
+
+
+def reference_channel_start(types, u_in=0.1):
+    """The initial fields of the reference's channel programs (cpp/d3q27_bgk_channel.cpp:134-142,
+    cpp/d2q9_bgk_channel.cpp:107-118): rho = 1 and u = (u_in, 0, 0) everywhere, then rho = u = 0
+    at solid nodes.  Returns (rho, ux, uy, uz) for lbm_init_fields."""
+    t = np.asarray(types).reshape(-1)
+    solid = (t & 1) != 0
+    rho = np.where(solid, 0.0, 1.0)
+    ux = np.where(solid, 0.0, u_in)
+    z = np.zeros(t.size)
+    return rho, ux, z, z.copy()
+
+
 def d3q27_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
                          storage=F32, collision=BGK):
     """Duct with a sphere: uniform Zou-He inlet, zero-gradient outlet; Re = u * D / nu, D = 2r."""
@@ -146,7 +193,7 @@ def d3q27_mrt_channel_suga(nx=320, ny=80, nz=80, u_in=0.1, storage=F32, collisio
     y = 0, ny - 1, z = 0, nz - 1, a sphere of radius ny / 9 at (nx / 4, ny / 2 + ny / 30,
     nz / 2 + nz / 30), inlet speed 0.1 and Suga's rates (s_nu = 1.2, nu = 1/9, Re = u * 2r / nu).
     The program's inlet and outlet are unfinished; here they are the Zou-He uniform inlet and the
-    zero-gradient outlet of BC_PHYSICAL, and the equilibrium moments are those of f_eq (the
+    zero-gradient outlet of BC_PHYSICAL, and with MRT_GS the equilibrium moments are M f_eq (the
     program's singular moment matrix, which never sets row 4, is the raw matrix with row 4 = xy).
     collision MRT_GS relaxes Suga's groups in the orthogonal basis, MRT_RAW in the raw one."""
     cx, cy, cz, r = nx // 4, ny // 2 + ny // 30, nz // 2 + nz // 30, ny // 9

@@ -82,6 +82,24 @@ def channel_2d_physical(nx, ny, cx, cy, r):
     return t
 
 
+def channel_3d_reference(nx, ny, nz, ox, oy, oz, r, pipe=False):
+    """The 3D channels of cpp/d3q27_bgk_channel.cpp:118-124 and cpp/d3q27_mrt_channel.cpp:204-210
+    in the reference's boundary semantics: solid = sphere (x-ox)^2+(y-oy)^2+(z-oz)^2 <= r^2, with
+    pipe=True also every node with (y-ny/2)^2+(z-nz/2)^2 >= (ny/2)^2-1 (integer ny/2, nz/2); the
+    x = 0 plane gets the inlet velocity and the x = 0 and x = nx-1 planes rho = 1 after the solid
+    override (:181-187 / :327-333).  y and z are periodic (the BGK program's pipe mask is
+    overwritten by its sphere, :121-124)."""
+    x, y, z = _grid(nx, ny, nz)
+    t = np.zeros((nz, ny, nx), np.uint8)
+    solid = (x - ox) ** 2 + (y - oy) ** 2 + (z - oz) ** 2 <= r * r
+    if pipe:
+        solid |= (y - ny // 2) ** 2 + (z - nz // 2) ** 2 >= (ny // 2) ** 2 - 1
+    t[solid] = NODE_SOLID
+    t[x == 0] |= NODE_INLET | NODE_RHO_FIX
+    t[x == nx - 1] |= NODE_RHO_FIX
+    return t
+
+
 def channel_3d_physical(nx, ny, nz, cx, cy, cz, r):
     """Duct with a spherical obstacle: walls y = 0, ny-1 and z = 0, nz-1, Zou-He inlet on x = 0,
     zero-gradient outlet on x = nx-1."""

@@ -118,6 +118,9 @@ REF_CASES = {
                               geo.lid_cavity_3d_reference(5, 5, 5)),
     "d3q19_mrt_lid": lambda: (dict(cases.d3q19_lid(5, re=10.0, storage=F64, collision=MRT_GS)[0]),
                               geo.lid_cavity_3d_reference(5, 5, 5)),
+    "d3q27_bgk_channel_ref": lambda: cases.d3q27_bgk_channel_reference(10, 7, 7, re=10.0, storage=F64),
+    "d3q27_mrt_channel_ref": lambda: cases.d3q27_mrt_channel_reference(10, 8, 8, storage=F64),
+    "d3q27_mrt_gs_channel_ref": lambda: cases.d3q27_mrt_channel_reference(10, 8, 8, storage=F64, collision=MRT_GS),
 }
 
 
@@ -532,3 +535,55 @@ def test_suga_rates_conserve_mass_in_a_closed_box(basis):
     m0 = orc.fields(cfg, box, fb0)[0][inside].sum()
     m1 = orc.fields(cfg, box, fb)[0][inside].sum()
     assert abs(m1 - m0) < 1e-9 * m0
+
+
+def test_reference_3d_channels_have_the_programs_setup():
+    """The 3D channel cases carry the constants of cpp/d3q27_bgk_channel.cpp (200 x 60 x 60, sphere
+    r = ny/10+1 = 7 at (nx/4+1, ny/2+ny/30, nz/2+nz/30) = (51, 32, 32), Re 220, :57-79) and of
+    cpp/d3q27_mrt_channel.cpp (320 x 80 x 80, sphere r = 9 at (80, 45, 40), pipe
+    (y-40)^2+(z-40)^2 >= 1599, :60-72,206-208), the channel rates of
+    rust/examples/d3q27_mrt_channel.rs:106-144, and the boundary planes of both (:181-187)."""
+    from lbm_amd.capi import BC_REFERENCE, BGK, MRT_RAW
+    cfg, t = cases.d3q27_bgk_channel_reference()
+    assert (cfg["nx"], cfg["ny"], cfg["nz"], cfg["collision"], cfg["bc_mode"]) == (200, 60, 60, BGK, BC_REFERENCE)
+    assert abs(cfg["omega"] - 1 / (3 * 0.1 * 14 / 220 + 0.5)) < 1e-12
+    t = t.reshape(60, 60, 200)
+    solid = (t & 1) != 0
+    assert solid.sum() == sum((x - 51) ** 2 + (y - 32) ** 2 + (z - 32) ** 2 <= 49
+                              for x in range(44, 59) for y in range(25, 40) for z in range(25, 40))
+    assert solid[32, 32, 51] and solid[32, 32, 58] and not solid[32, 32, 59] and not solid[0].any()
+    assert ((t[:, :, 0] & 0x14) == 0x14).all() and ((t[:, :, -1] & 0x10) == 0x10).all()
+    assert not (t[:, :, 1:-1] & 0x14).any()
+    cfg, t = cases.d3q27_mrt_channel_reference()
+    assert (cfg["nx"], cfg["ny"], cfg["nz"], cfg["collision"]) == (320, 80, 80, MRT_RAW)
+    t = t.reshape(80, 80, 320)
+    solid = (t & 1) != 0
+    assert solid[40, 45, 80] and solid[40, 45 + 9, 80] and not solid[40, 45 + 10, 80]
+    y, z = np.meshgrid(np.arange(80), np.arange(80))
+    assert np.array_equal(solid[:, :, 200], (y - 40) ** 2 + (z - 40) ** 2 >= 1599)
+    assert not solid[40, 1, 200] and solid[40, 0, 200] and not solid[1, 40, 200] and solid[0, 40, 200]
+    s = np.array(cfg["s"])
+    expect = {1.3: [0, 1, 2, 3], 1.0: [4, 5, 6, 8, 9], 0.55: [7, 16],
+              1.2: list(range(10, 16)) + list(range(17, 27))}
+    for v, rows in expect.items():
+        assert np.all(s[rows] == v), v
+    assert sorted(sum(expect.values(), [])) == list(range(27))
+
+
+def test_reference_3d_channels_reach_steady_flow():
+    """Reduced 3D channels in the reference's boundary semantics, started from the programs' initial
+    fields, settle: the density excursion and the peak speed at 400 steps equal those at 300 steps
+    to 5e-3 (the BGK box, periodic in y and z, still accelerates by about 1e-3 per 100 steps)."""
+    from lbm_amd.capi import MRT_GS
+    for cfg, t in (cases.d3q27_bgk_channel_reference(40, 20, 20, re=20.0, storage=F64),
+                   cases.d3q27_mrt_channel_reference(48, 20, 20, storage=F64),
+                   cases.d3q27_mrt_channel_reference(48, 20, 20, storage=F64, collision=MRT_GS)):
+        t = t.reshape(-1)
+        fluid = (t & 1) == 0
+        f = orc.run(cfg, t, orc.init_fields(cfg, t, *cases.reference_channel_start(t, cfg["u_in"])), 300)
+        a = orc.fields(cfg, t, f)
+        b = orc.fields(cfg, t, orc.run(cfg, t, f, 100))
+        for k in (0, 1):
+            assert np.isfinite(b[k]).all()
+            assert abs(np.abs(b[k][fluid] - (k == 0)).max() - np.abs(a[k][fluid] - (k == 0)).max()) < 5e-3
+        assert np.abs(b[0][fluid] - 1).max() < 0.3 and np.abs(b[1][fluid]).max() < 0.2
