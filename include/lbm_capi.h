@@ -244,7 +244,10 @@ typedef struct {
 } lbm_job_result;
 
 /* budget_bytes: device memory the server may hold for resident jobs; steps_per_tick: most steps
- * a tick advances the resident jobs by. */
+ * a tick advances the resident jobs by.  The server runs on the device current at its creation
+ * (lbm_set_device), and its jobs' lattices share its one stream: make that device current again
+ * before each lbm_server_* call (a job admitted with another device current fails with
+ * LBM_EINVAL in its result). */
 lbm_status lbm_server_create(size_t budget_bytes, int32_t steps_per_tick, lbm_server *out);
 /* Queues a job (LBM_ENOMEM if it could never fit the budget). */
 lbm_status lbm_server_submit(lbm_server s, const lbm_job_spec *spec, int64_t *job_id);

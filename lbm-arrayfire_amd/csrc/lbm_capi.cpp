@@ -198,8 +198,11 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   };
   if (hipSetDevice(h->device) != hipSuccess) return cleanup(LBM_EDEVICE, "hipSetDevice failed");
   if (shared) {
-    h->stream = shared;
     h->own_stream = false;
+    hipDevice_t sd = -1;
+    if (hipStreamGetDevice(shared, &sd) != hipSuccess || sd != h->device)
+      return cleanup(LBM_EINVAL, "the lent stream belongs to another device than the current one");
+    h->stream = shared;
   } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     return cleanup(LBM_EDEVICE, "hipStreamCreate failed");
   }
