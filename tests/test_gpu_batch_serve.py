@@ -230,3 +230,24 @@ def test_server_rejects_and_cancels():
         ids = [r["job_id"] for r in srv.poll()]
         assert ids == [b]
         assert srv.used_bytes == 0
+
+
+def test_server_destroyed_with_resident_jobs():
+    """Destroying a server while jobs are resident releases their lattices before the batch whose
+    stream they share; lattices created afterwards step as before."""
+    cfg, t = cases.d2q9_lid_reference(64, storage=F32)
+    with Lattice(cfg, t) as lat:
+        lat.init_equilibrium(1.0, (0, 0, 0), 3, 1e-4)
+        lat.step(50)
+        want = lat.fields()
+    for _ in range(2):
+        with Server(1 << 30, steps_per_tick=10) as srv:
+            for seed in range(3):
+                srv.submit(cfg, t, 1000, seed=seed, noise_amp=1e-4)
+            active, queued = srv.tick()
+            assert active == 3 and queued == 0
+    with Lattice(cfg, t) as lat:
+        lat.init_equilibrium(1.0, (0, 0, 0), 3, 1e-4)
+        lat.step(50)
+        for a, b in zip(lat.fields(), want):
+            assert np.array_equal(a, b)
