@@ -218,10 +218,13 @@ int32_t lbm_batch_size(lbm_batch b);
 lbm_status lbm_batch_destroy(lbm_batch b);
 
 /* ---- serving: continuous batching of simulation jobs ------------------------------------ */
-/* No reference counterpart.  Jobs are admitted while their footprint (lbm_footprint_bytes) fits
- * the HBM budget; admitted jobs stay resident and are stepped together, and jobs that arrive
- * while others run join at the next tick.  A job's populations are exactly those it would reach
- * run alone. */
+/* No reference counterpart.  Jobs are admitted while their footprint fits the HBM budget;
+ * admitted jobs stay resident and are stepped together, and jobs that arrive while others run
+ * join at the next tick.  A job's populations are exactly those it would reach run alone.  The
+ * server keeps the device buffers of finished jobs for later ones, in size classes (2^k times 1,
+ * 1.25, 1.5 or 1.75): a job's footprint is lbm_footprint_bytes with each buffer rounded up to its
+ * class (at most 25 % more), and idle kept buffers count against the budget until a job needs
+ * the room. */
 typedef struct {
   lbm_desc desc;
   const uint8_t *node_type; /* N bytes, copied at submit */
@@ -263,6 +266,7 @@ lbm_status lbm_server_take(lbm_server s, int64_t job_id, double *rho, double *ux
 lbm_status lbm_server_cancel(lbm_server s, int64_t job_id);
 /* Monitors of every resident job are read every `ticks` ticks (0 = only at completion). */
 lbm_status lbm_server_set_check_every(lbm_server s, int32_t ticks);
+/* Footprint of the resident jobs (idle kept buffers not included). */
 size_t lbm_server_used_bytes(lbm_server s);
 double lbm_server_clock(lbm_server s);
 

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -118,11 +119,80 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
 }  // extern "C"
 
 namespace lbm {
+size_t DevPool::size_class(size_t n) {
+  if (n <= 256) return 256;
+  size_t base = 256;
+  while (base * 2 <= n) base *= 2;
+  for (size_t m = 4; m <= 7; ++m)
+    if (base / 4 * m >= n) return base / 4 * m;
+  return base * 2;
+}
+
+hipError_t DevPool::get(void **p, size_t n) {
+  const size_t c = size_class(n);
+  auto it = idle.find(c);
+  if (it != idle.end()) {
+    *p = it->second;
+    idle.erase(it);
+    held -= c;
+    return hipSuccess;
+  }
+  return hipMalloc(p, c);
+}
+
+void DevPool::put(void *p, size_t n) {
+  const size_t c = size_class(n);
+  idle.emplace(c, p);
+  held += c;
+}
+
+void DevPool::trim() {
+  for (auto &kv : idle) (void)hipFree(kv.second);
+  idle.clear();
+  held = 0;
+}
+
+void DevPool::trim_except(const std::vector<size_t> &keep) {
+  std::map<size_t, int> want;
+  for (size_t c : keep) ++want[c];
+  for (auto it = idle.begin(); it != idle.end();) {
+    auto w = want.find(it->first);
+    if (w != want.end() && w->second > 0) {
+      --w->second;
+      ++it;
+      continue;
+    }
+    (void)hipFree(it->second);
+    held -= it->first;
+    it = idle.erase(it);
+  }
+}
+
+std::vector<size_t> pooled_blocks(const lbm_desc *d) {
+  if (!check_desc(d).empty()) return {};
+  const size_t N = (size_t)d->nx * d->ny * d->nz;
+  int32_t gx, gy, gz, nt;
+  tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
+  const size_t buf = DevPool::size_class((size_t)population_stride((int64_t)N, storage_bytes(d->storage)) *
+                                         (size_t)d->lattice * (size_t)storage_bytes(d->storage));
+  std::vector<size_t> b = {buf, buf, DevPool::size_class(N), DevPool::size_class((size_t)nt * 3 * sizeof(double)),
+                           DevPool::size_class(4 * sizeof(double))};
+  if (d->collision != LBM_BGK && d->lattice == 27) b.push_back(DevPool::size_class(64 * MFMA_TAB * sizeof(float)));
+  return b;
+}
+
+size_t pooled_footprint(const lbm_desc *d) {
+  size_t fp = 0;
+  for (size_t c : pooled_blocks(d)) fp += c;
+  return fp;
+}
+
 // lbm_create with the handle's work on the given stream (borrowed, not destroyed with the handle)
 // instead of a stream of its own; nullptr creates one.  The job server creates its lattices on
 // its batch's stream: a HIP stream costs about 0.6 ms to create and 0.2 ms to destroy, which
 // dominated the host time of small jobs.
-lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, lbm_handle *out) {
+lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, DevPool *pool,
+                          lbm_handle *out) {
   if (!out) return fail(LBM_EINVAL, "out is NULL");
   *out = nullptr;
   const std::string why = check_desc(d);
@@ -192,6 +262,8 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
     }
   }
 
+  h->pool = pool;
+  auto dmalloc = [&](void **ptr, size_t n) { return pool ? pool->get(ptr, n) : hipMalloc(ptr, n); };
   auto cleanup = [&](lbm_status s, const std::string &m) {
     lbm_destroy(h);
     return fail(s, m);
@@ -209,28 +281,31 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return cleanup(LBM_EDEVICE, "hipEventCreate failed");
   for (int b = 0; b < 2; ++b)
-    if (hipMalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
+    if (dmalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
       return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
-  if (hipMalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
-  if (hipMalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
+  if (dmalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
+  if (dmalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
+      dmalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
     return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
   // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
   // nodes whose pull needs the boundary path (NODE_SLOW).
   {
     uint8_t *raw = nullptr;
-    if (hipMalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
+    if (dmalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
     hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(raw);
+    if (pool)
+      pool->put(raw, (size_t)h->N);
+    else
+      (void)hipFree(raw);
     if (e != hipSuccess) return cleanup(LBM_EDEVICE, std::string("node-type upload failed: ") + hipGetErrorString(e));
   }
   p.type = h->types;
   if (d->collision != LBM_BGK && d->lattice == 27) {
     std::vector<float> tab(64 * MFMA_TAB);
     build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
-    if (hipMalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
+    if (dmalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
       return cleanup(LBM_ENOMEM, "MRT table allocation failed");
     if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(LBM_EDEVICE, "MRT table upload failed");
@@ -244,7 +319,7 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
 extern "C" {
 
 lbm_status lbm_create(const lbm_desc *d, const uint8_t *node_type, lbm_handle *out) {
-  return lbm::abi_guard([&]() -> lbm_status { return lbm::create_lattice(d, node_type, nullptr, out); });
+  return lbm::abi_guard([&]() -> lbm_status { return lbm::create_lattice(d, node_type, nullptr, nullptr, out); });
 }
 
 lbm_status lbm_destroy(lbm_handle h) {
@@ -254,13 +329,19 @@ lbm_status lbm_destroy(lbm_handle h) {
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (int b = 0; b < 2; ++b)
-      if (h->f[b]) (void)hipFree(h->f[b]);
-    if (h->types) (void)hipFree(h->types);
-    if (h->partials) (void)hipFree(h->partials);
-    if (h->red) (void)hipFree(h->red);
+    auto dfree = [&](void *ptr, size_t n) {
+      if (!ptr) return;
+      if (h->pool)
+        h->pool->put(ptr, n);
+      else
+        (void)hipFree(ptr);
+    };
+    for (int b = 0; b < 2; ++b) dfree(h->f[b], h->buf_bytes());
+    dfree(h->types, (size_t)h->N);
+    dfree(h->partials, (size_t)h->base.ntiles * 3 * sizeof(double));
+    dfree(h->red, 4 * sizeof(double));
+    dfree(h->mrt_tab, 64 * MFMA_TAB * sizeof(float));
     if (h->fields) (void)hipFree(h->fields);
-    if (h->mrt_tab) (void)hipFree(h->mrt_tab);
     if (h->probe) (void)hipFree(h->probe);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);

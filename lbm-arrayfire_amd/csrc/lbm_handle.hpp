@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <exception>
 #include <new>
 #include <string>
@@ -14,6 +15,10 @@
 
 struct lbm_batch_ctx;
 
+namespace lbm {
+struct DevPool;
+}
+
 struct lbm_ctx {
   lbm_desc d{};
   int device = 0;
@@ -23,6 +28,7 @@ struct lbm_ctx {
   int esize = 4;
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false: a stream lent at creation (not destroyed with the handle)
+  lbm::DevPool *pool = nullptr;  // device buffers come from and return to this pool (job server)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
@@ -61,8 +67,29 @@ namespace lbm {
 // Sets the thread-local last-error message and returns s.
 lbm_status set_error(lbm_status s, const std::string &m);
 lbm_status set_error_noexcept(lbm_status s, const char *m) noexcept;
-// lbm_create on a lent stream (nullptr: a stream of its own), and a batch's stream
-lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, lbm_handle *out);
+// Device buffers kept for reuse by the job server, so that a finished job's buffers serve the
+// next job without a hipFree / hipMalloc pair (each about 30-40 us of host time and a device
+// synchronisation).  Blocks are handed out in size classes (2^k times 1, 1.25, 1.5 or 1.75, at
+// least 256 B), so lattices of nearby sizes share them; idle blocks stay allocated until trim().
+// Used from one thread at a time (the server's).
+struct DevPool {
+  std::multimap<size_t, void *> idle;
+  size_t held = 0;  // bytes in idle blocks
+  static size_t size_class(size_t n);
+  hipError_t get(void **p, size_t n);
+  void put(void *p, size_t n);
+  void trim();
+  // frees every idle block except one per entry of `keep` (block sizes, each a size class)
+  void trim_except(const std::vector<size_t> &keep);
+};
+// The blocks a lattice takes from a DevPool (sizes rounded to their classes; the transient
+// node-type staging block not included), and their sum.
+std::vector<size_t> pooled_blocks(const lbm_desc *d);
+size_t pooled_footprint(const lbm_desc *d);
+// lbm_create on a lent stream (nullptr: a stream of its own) with buffers from a pool (nullptr:
+// hipMalloc / hipFree), and a batch's stream
+lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, DevPool *pool,
+                          lbm_handle *out);
 hipStream_t batch_stream(lbm_batch b);
 // The body of a C-ABI entry point: no C++ exception crosses the ABI (a host allocation failure
 // returns LBM_ENOMEM, any other exception LBM_EINTERNAL, with the message in lbm_last_error).

@@ -53,6 +53,7 @@ struct lbm_server_ctx {
   std::vector<lbm_job_result> finished;
   std::map<int64_t, Fields> kept;
   lbm_batch batch = nullptr;
+  lbm::DevPool pool;  // device buffers of retired jobs, reused by later ones
   lbm_server_stats stats{};
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -113,10 +114,15 @@ lbm_status admit(lbm_server_ctx *s) {
     }
     // on the batch's stream: the job's work runs there anyway once it joins the batch, and a
     // stream of its own would cost more host time to create and destroy than a small job steps
+    // idle pooled blocks count against the budget: when the job needs the room, release those it
+    // will not reuse, so that once it has taken the rest the device holds used + footprint, within
+    // the budget as checked above
+    if (s->used + j->footprint + s->pool.held > s->budget) s->pool.trim_except(lbm::pooled_blocks(&j->spec.desc));
     st = lbm::abi_guard([&]() -> lbm_status {
-      return lbm::create_lattice(&j->spec.desc, j->types.data(), lbm::batch_stream(s->batch), &j->h);
+      return lbm::create_lattice(&j->spec.desc, j->types.data(), lbm::batch_stream(s->batch), &s->pool, &j->h);
     });
     if (st == LBM_ENOMEM) {  // the device is fuller than the budget says: wait
+      s->pool.trim();
       st = LBM_OK;
       break;
     }
@@ -176,7 +182,7 @@ lbm_status lbm_server_submit(lbm_server s, const lbm_job_spec *spec, int64_t *jo
   return lbm::abi_guard([&]() -> lbm_status {
     if (!s || !spec || !spec->node_type) return set_error(LBM_EINVAL, "NULL argument");
     if (spec->nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
-    const size_t fp = lbm_footprint_bytes(&spec->desc);
+    const size_t fp = lbm::pooled_footprint(&spec->desc);  // buffers come from the server's pool
     if (fp == 0) return set_error(LBM_EINVAL, "invalid job descriptor");
     if (fp > s->budget) return set_error(LBM_ENOMEM, "job footprint exceeds the server's HBM budget");
     Job *j = new (std::nothrow) Job();
@@ -335,6 +341,7 @@ lbm_status lbm_server_destroy(lbm_server s) {
       delete j;
     }
     for (Job *j : s->queue) delete j;
+    s->pool.trim();
     lbm_batch_destroy(s->batch);
     delete s;
     return LBM_OK;

@@ -251,3 +251,30 @@ def test_server_destroyed_with_resident_jobs():
         lat.step(50)
         for a, b in zip(lat.fields(), want):
             assert np.array_equal(a, b)
+
+
+def test_server_reuses_buffers_of_finished_jobs():
+    """A budget that holds one job at a time makes each job run on the device buffers the previous
+    one left (64^2 and 62^2 D2Q9 lattices share every size class): each job still equals its run
+    alone bit for bit, whatever the reused buffers held."""
+    from lbm_amd.capi import footprint_bytes
+    specs = [(cases.d2q9_lid_reference(64, storage=F32), 1), (cases.d2q9_lid_reference(62, storage=F32), 2),
+             (cases.d2q9_lid_reference(64, storage=F32), 3), (cases.d2q9_lid_reference(62, storage=F32), 4)]
+    alone = []
+    for (cfg, t), seed in specs:
+        with Lattice(cfg, t) as lat:
+            lat.init_equilibrium(1.0, (0, 0, 0), seed, 1e-3)
+            lat.step(300)
+            alone.append(lat.fields())
+    budget = int(1.5 * footprint_bytes(specs[0][0][0]))
+    with Server(budget, steps_per_tick=100) as srv:
+        ids = [srv.submit(cfg, t, 300, seed=seed, noise_amp=1e-3, want_fields=True) for (cfg, t), seed in specs]
+        done = []
+        while len(done) < len(ids):
+            active, queued = srv.tick()
+            assert active <= 1 and srv.used_bytes <= budget
+            done += srv.poll()
+        assert all(r["status"] == 0 for r in done)
+        for jid, want in zip(ids, alone):
+            for a, b in zip(srv.take(jid), want):
+                assert np.array_equal(a, b)
