@@ -1,18 +1,20 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the fused LBM time step on one MI355X.
 
-Default workload: D2Q9 BGK channel with a cylinder (Karman street), 1024x256 nodes, FP32,
-cylinder r = 20 at (256,128), parabolic Zou-He inlet u_max = 0.1, zero-gradient outlet,
-no-slip walls, Re = 150, 1e-4 seeded noise (seed 0x5EED0002).  A "step" is one lattice time
-step over the whole lattice; MLUPS counts every node (solid included), 1e6 lattice updates/s.
+Default workload (the largest single-GPU configuration): D3Q27 BGK duct with a sphere,
+512x256x256 nodes, FP32, sphere r = 24 at (128,128,128), uniform Zou-He inlet u = 0.05,
+zero-gradient outlet, no-slip walls, Re = 200, 1e-4 seeded noise (seed 0x5EED0004).  A "step" is
+one lattice time step over the whole lattice; MLUPS counts every node (solid included), 1e6
+lattice updates/s.  ``value`` is the median of five timed regions of ``--steps`` steps each.
 
-Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes of the dominant kernel
-(2 * Q * bytes-per-population per node update, times the node updates one launch performs: one
-step per launch, or several for the D2Q9 multi-step kernel) over its average launch duration,
-measured with HIP events on the lattice's stream; ``traffic`` is the HBM bytes per launch that
-rocprofv3 counted for the same kernel (profiles/traffic.json).  ``cpu_baseline`` times the CPU
-oracle (a restatement of the reference step, OpenMP) on a bounded sample of the same workload on
-this host.
+Prints ONE JSON line.  ``roofline.achieved`` is the algorithmic bytes of the dominant kernel
+(2 * Q * bytes-per-population per node update, times the node updates one launch performs) over
+its average launch duration, measured with HIP events on the lattice's stream; ``traffic`` is the
+HBM bytes per launch that rocprofv3 counted for the same kernel (profiles/traffic.json) and
+``traffic_frac`` those bytes over the same launch duration against the 8 TB/s peak.
+``cpu_baseline`` times the CPU oracle (a restatement of the reference step, OpenMP) on a bounded
+sample of the same workload on this host, plus the reference's multi-pass step order on the
+128^2 D2Q9 lid and 128^3 D3Q27 MRT lid shapes.
 """
 import argparse
 import json
@@ -61,7 +63,15 @@ def workloads():
     return W
 
 
-def measure(name, steps, warmup, repeats=0):
+def seed_of(name):
+    """The recorded noise seed of each configuration: 0x5EED0004 for the sphere ducts, 0x5EED0002
+    otherwise."""
+    return 0x5EED0004 if "sphere" in name else 0x5EED0002
+
+
+def measure(name, steps, warmup, regions=1):
+    """Times `regions` regions of `steps` steps each (HIP events on the lattice's stream around each
+    region, the stream synchronised before and after); the figures are those of the median region."""
     from lbm_amd import Lattice
     from lbm_amd.capi import STORAGE_BYTES
 
@@ -76,35 +86,39 @@ def measure(name, steps, warmup, repeats=0):
             os.environ.pop("LBM_MULTISTEP", None)
         else:
             os.environ["LBM_MULTISTEP"] = saved
-    N_lat = cfg["nx"] * cfg["ny"] * cfg.get("nz", 1)
     with lat:
         k = lat.steps_per_launch
         # whole launches only, so the timed region holds steps // k launches of one kernel
         steps, warmup = max(k, steps - steps % k), warmup - warmup % k
         u0 = (cfg.get("u_in", 0.0), 0.0, 0.0)
-        lat.init_equilibrium(1.0, u0, 0x5EED0002, 1e-4)
+        lat.init_equilibrium(1.0, u0, seed_of(name), 1e-4)
         lat.step(warmup)
         lat.sync()
-        t0 = time.perf_counter()
-        ms = lat.step_timed(steps)
-        wall = time.perf_counter() - t0
-        # the same timed region repeated, for the spread (value is the first region above)
-        rep = [N_lat * steps / (lat.step_timed(steps) * 1e3) for _ in range(repeats)]
+        region_ms, walls = [], []
+        for _ in range(regions):
+            lat.sync()
+            t0 = time.perf_counter()
+            region_ms.append(lat.step_timed(steps))  # returns after the stream has drained
+            walls.append(time.perf_counter() - t0)
         _, _, bad = lat.monitors()
     N = lat.N
     Q = cfg["lattice"]
+    order = sorted(range(regions), key=lambda i: region_ms[i])
+    mid = order[regions // 2]
+    ms = region_ms[mid]
     # algorithmic bytes: every population read and written once per node update, k updates of
     # every node per launch
     bytes_per_launch = 2 * Q * STORAGE_BYTES[cfg["storage"]] * N * k
     launch_s = ms * 1e-3 / (steps // k)
     achieved = bytes_per_launch / launch_s / 1e9
-    return dict(cfg=cfg, N=N, Q=Q, steps=steps, warmup=warmup, ms=ms, wall_s=wall,
+    return dict(cfg=cfg, N=N, Q=Q, steps=steps, warmup=warmup, ms=ms, wall_s=walls[mid],
                 mlups=N * steps / (ms * 1e3), ms_per_step=ms / steps, steps_per_launch=k,
                 launch_us=launch_s * 1e6, bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
-                repeat_mlups=rep, nonfinite=int(bad))
+                region_mlups=[N * steps / (t * 1e3) for t in region_ms], nonfinite=int(bad),
+                nt=bool(Q != 9 and N * Q * STORAGE_BYTES[cfg["storage"]] > (256 << 20)))
 
 
-def pcie_inclusive_leg(name, step_counts=(1000, 50000)):
+def pcie_inclusive_leg(name, step_counts=None):
     """The workload as a host caller runs it end to end through the C ABI: create the lattice
     (node types uploaded), initialise it from host fields (rho, ux, uy uploaded), step, read rho
     and u back to host, destroy.  Host wall time around all of it; never the headline value, whose
@@ -114,6 +128,8 @@ def pcie_inclusive_leg(name, step_counts=(1000, 50000)):
 
     cfg, types = workloads()[name]()
     N = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    if step_counts is None:  # about 0.02 s and 2 s of steps at 25k MLUPS
+        step_counts = tuple(max(1, int(v * 25e9 / N)) for v in (0.02, 2.0))
     rho, ux, uy = np.ones(N), np.full(N, 0.05), np.zeros(N)
     up = N * (1 + 3 * 8)  # node types + three double fields
     down = N * 4 * 8  # rho, ux, uy, uz as doubles
@@ -155,53 +171,88 @@ def shanchen_leg(steps=200, warmup=20, n=1024):
     return out
 
 
-def kernel_name(cfg, k):
+def kernel_name(cfg, k, nt=False):
     """The kernel the timed launches run (as rocprofv3 lists it, template arguments abridged)."""
     if k > 1:
         return f"lbm::k_step_multi<storage {cfg['storage']}, bc {cfg.get('bc_mode', 0)}, {k} steps per launch>"
-    return f"lbm::k_step<{cfg['lattice']}, {cfg['storage']}, {cfg.get('bc_mode', 0)}, {cfg.get('collision', 0)}>"
+    tail = ", true" if nt else ""
+    return f"lbm::k_step<{cfg['lattice']}, {cfg['storage']}, {cfg.get('bc_mode', 0)}, {cfg.get('collision', 0)}{tail}>"
 
 
 def traffic_for(name):
+    """The counted-traffic record of a workload (profiles/traffic.json), or None."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
-        return json.load(open(p)).get(name, {}).get("hbm_bytes_per_launch")
+        return json.load(open(p)).get(name)
     except (OSError, ValueError):
         return None
 
 
-def cpu_baseline(name, target_s=10.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-
-    cfg, types = workloads()[name]()
-    types = types.reshape(-1)
-    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-    t = orc.time_steps(cfg, types, 5, threads=threads)
-    n = max(5, int(target_s / max(t / 5, 1e-6)))
-    t = orc.time_steps(cfg, types, n, threads=threads)
-    N = types.size
-    model = ""
+def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    out = {"value": N * n / (t * 1e6), "unit": "MLUPS", "cores": threads, "kind": "port",
-           "sample": f"{name}: {n} steps of the CPU oracle (restated reference step, same boundary "
-                     f"rules, FP32, OpenMP {threads} threads) in {t:.1f} s on {model}"}
-    # the same step on one core, and in FP64 on all cores (about 3 s each)
-    n1 = max(2, int(3.0 / max(t / n * threads, 1e-6)))
-    t1 = orc.time_steps(cfg, types, n1, threads=1)
-    out["single_core_mlups"] = N * n1 / (t1 * 1e6)
-    cfg64 = dict(cfg, storage=0)
-    n64 = max(2, int(3.0 / max(2 * t / n, 1e-6)))
-    t64 = orc.time_steps(cfg64, types, n64, threads=threads)
-    out["fp64_mlups"] = N * n64 / (t64 * 1e6)
+    return ""
+
+
+def _oracle_rate(orc, cfg, types, threads, target_s, warmup=1):
+    """MLUPS of the oracle over about target_s seconds (at least one timed step)."""
+    N = types.size
+    t1 = orc.time_steps(cfg, types, 1, threads=threads, warmup=warmup)
+    n = max(1, int(target_s / max(t1, 1e-6)))
+    t = t1 if n == 1 else orc.time_steps(cfg, types, n, threads=threads, warmup=0)
+    return N * n / (t * 1e6), n, t
+
+
+def cpu_baseline(name, target_s=10.0):
+    """The CPU oracle on this host.  value: the headline workload itself (FP32, OpenMP on every
+    core the process may use), a bounded number of steps.  Beside it, FP64 on all cores, one core
+    on a 16-plane slab of the same duct (the per-node work of the full lattice, a step short
+    enough for one core), and the reference's multi-pass step order (materialised pull index,
+    gather, reductions, overrides, collision at every node, bounce-back scatter) on the 128^2
+    D2Q9 lid cavity and the 128^3 D3Q27 MRT (Gram-Schmidt) lid cavity, FP32 and FP64, all cores
+    and one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from lbm_amd import cases
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    cfg, types = workloads()[name]()
+    types = types.reshape(-1)
+    v, n, t = _oracle_rate(orc, cfg, types, threads, target_s)
+    model = _cpu_model()
+    out = {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"{name}: {n} steps of the whole lattice by the CPU oracle (restated reference step, same "
+                     f"boundary rules, FP32, OpenMP {threads} threads) in {t:.1f} s after one untimed step, "
+                     f"on {model}"}
+    out["fp64_mlups"] = _oracle_rate(orc, dict(cfg, storage=0), types, threads, 3.0)[0]
+    if cfg["lattice"] == 27 and cfg.get("nz", 1) > 16:
+        nz = 16
+        slab = (cfg["nx"] * cfg["ny"] * nz)
+        cfg1, types1 = dict(cfg, nz=nz), types[:slab].copy()
+        types1[-cfg["nx"] * cfg["ny"]:] = types.reshape(cfg["nz"], -1)[-1]  # keep the closing wall plane
+        out["single_core_mlups"] = _oracle_rate(orc, cfg1, types1, 1, 3.0)[0]
+        out["single_core_sample"] = f"planes z < {nz} of the duct (the wall plane z = nz-1 closing them)"
+    else:
+        out["single_core_mlups"] = _oracle_rate(orc, cfg, types, 1, 3.0)[0]
+    ref = {}
+    for key, make in (("d2q9_bgk_lid_128x128_reference_order", lambda st: cases.d2q9_lid_reference(128, storage=st)),
+                      ("d3q27_mrt_gs_lid_128cubed_reference_order",
+                       lambda st: cases.d3q27_mrt_lid_gs(128, re=400.0, storage=st))):
+        row = {}
+        for st, tag in ((1, "fp32"), (0, "fp64")):
+            c, ty = make(st)
+            ty = ty.reshape(-1)
+            row[f"{tag}_mlups_{threads}_threads"] = _oracle_rate(orc, c, ty, threads, 2.0)[0]
+            row[f"{tag}_mlups_1_thread"] = _oracle_rate(orc, c, ty, 1, 2.0)[0]
+        ref[key] = row
+    out["reference_order"] = ref
+    out["cpu_model"] = model
     return out
 
 
@@ -209,11 +260,15 @@ def _rounded(d):
     return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
 
 
-def serving_leg(njobs, scale, sides="mixed"):
+def serving_leg(njobs, scale, sides="mixed", accuracy_every=10):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
     continuous-batching server, in real time and saturated, against the same jobs run one at a
-    time with the plain solver."""
+    time with the plain solver.  Accuracy: every accuracy_every-th job is served again (alone in a
+    server of its own kind of traffic: the sample, saturated) with its fields kept, and compared
+    with the same job run in FP64 storage."""
+    import numpy as np
+
     from lbm_amd import serving
 
     jobs = serving.make_mix(njobs, step_scale=scale, sides=sides)
@@ -221,11 +276,21 @@ def serving_leg(njobs, scale, sides="mixed"):
     rt = serving.serve(jobs, budget, realtime=True)
     sat = serving.serve(jobs, budget, realtime=False)
     seq = serving.sequential(jobs)
-    return {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
-            "storage": "F16S", "realtime": _rounded(rt), "saturated": _rounded(sat),
-            "one_at_a_time": _rounded(seq),
-            "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
-            "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+    out = {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
+           "storage": "F16S", "realtime": _rounded(rt), "saturated": _rounded(sat),
+           "one_at_a_time": _rounded(seq),
+           "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
+           "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+    if accuracy_every:
+        sample = jobs[::accuracy_every]
+        acc = serving.serve(sample, budget, realtime=False, keep=range(len(sample)))
+        errs = [serving.fp64_errors(j, acc["fields"][k]) for k, j in enumerate(sample) if k in acc["fields"]]
+        out["accuracy_vs_fp64"] = {
+            "sample": f"every {accuracy_every}th job ({len(sample)} jobs, {len(errs)} finished)",
+            "max_rel_l2_rho": float(max(e[0] for e in errs)) if errs else None,
+            "max_rel_l2_u": float(max(e[1] for e in errs)) if errs else None,
+            "median_rel_l2_u": float(np.median([e[1] for e in errs])) if errs else None}
+    return out
 
 
 def copy_bandwidth(mib=1024):
@@ -286,9 +351,9 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50000)
-    ap.add_argument("--warmup", type=int, default=20000)
-    ap.add_argument("--workload", default="d2q9_bgk_karman_1024x256_fp32")
+    ap.add_argument("--steps", type=int, default=100, help="steps per timed region (five regions)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="d3q27_bgk_sphere_512x256x256_fp32")
     ap.add_argument("--also", default="default",
                     help="comma list of extra workloads timed briefly, 'none', or 'default'")
     ap.add_argument("--also-steps", type=int, default=100)
@@ -314,7 +379,7 @@ def main():
     names = [] if args.also == "none" else (
         [k for k in workloads() if k != args.workload] if args.also == "default" else args.also.split(","))
     ceilings = pull_ceilings() if any(k.startswith(("d3q27", "d3q19")) for k in names + [args.workload]) else None
-    r = measure(args.workload, args.steps, args.warmup, repeats=4)
+    r = measure(args.workload, args.steps, args.warmup, regions=5)
     also = {}
     for k in names:
         try:
@@ -328,6 +393,9 @@ def main():
                 ceil = ceilings.get(pre + ("pull_4byte" if c["storage"] == 1 else "pull_2byte") + ("_nt_gbs" if nt else "_gbs"))
             also[k] = {"mlups": round(m["mlups"], 1), "ms_per_step": round(m["ms_per_step"], 5),
                        "steps_per_launch": m["steps_per_launch"], "achieved_gbs": round(m["achieved_gbs"], 1),
+                       **({"rate_kind": "algorithmic-equivalent: several steps per launch from an on-chip "
+                                        "tile, HBM traffic about 0.24x the algorithmic bytes"}
+                          if m["steps_per_launch"] > 1 else {}),
                        **({"frac_of_pull_ceiling": round(m["achieved_gbs"] / ceil, 4)} if ceil else {}),
                        **({"algorithmic_mrt_tflops": round(m["mlups"] * 1e6 * mrt_flop_per_lup(m["Q"]) / 1e12, 2),
                            "frac_fp32_matrix_peak": round(m["mlups"] * 1e6 * mrt_flop_per_lup(m["Q"]) / 157.3e12, 4)}
@@ -348,6 +416,13 @@ def main():
         pcie = {"error": str(e)}
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     cfg = r["cfg"]
+    tr = traffic_for(args.workload)
+    traffic = tr.get("hbm_bytes_per_launch") if tr else None
+    launch_s = r["launch_us"] * 1e-6
+    # HBM-bound when the counted traffic is (nearly) the algorithmic traffic; a launch that moves
+    # much less (several steps from on-chip tiles) is bound by issue and latency instead
+    ratio = traffic / r["bytes_per_launch"] if traffic else None
+    bound = "hbm" if ratio is None or ratio >= 0.9 else "issue/latency (temporal blocking)"
     out = {
         "metric": METRIC,
         "value": round(r["mlups"], 1),
@@ -367,15 +442,19 @@ def main():
                    "storage": ["F64", "F32", "F16S", "BF16S"][cfg["storage"]],
                    "bc_mode": ["REFERENCE", "PHYSICAL"][cfg.get("bc_mode", 0)],
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": bound, "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_for(args.workload),
-                     "kernel": kernel_name(cfg, r["steps_per_launch"]),
+                     "traffic": traffic,
+                     "traffic_gbs": None if not traffic else round(traffic / launch_s / 1e9, 1),
+                     "traffic_frac": None if not traffic else round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic_over_algorithmic": None if ratio is None else round(ratio, 4),
+                     "traffic_source": None if not tr else "profiles/traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+                     "kernel": kernel_name(cfg, r["steps_per_launch"], r["nt"]),
                      "steps_per_launch": r["steps_per_launch"], "avg_launch_us": round(r["launch_us"], 3),
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"],
-                     "attainable_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
-        "repeats": {"mlups": [round(r["mlups"], 1)] + [round(v, 1) for v in r["repeat_mlups"]],
-                    "median": round(sorted([r["mlups"]] + r["repeat_mlups"])[2], 1)},
+                     "attainable_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
+                     "frac_of_attainable_copy": None if not copy_gbs else round(r["achieved_gbs"] / copy_gbs, 4)},
+        "regions": {"mlups": [round(v, 1) for v in r["region_mlups"]], "value_is": "median of the regions"},
         "cpu_baseline": cpu,
         "wall_s_timed": round(r["wall_s"], 4),
         "nonfinite": r["nonfinite"],

@@ -223,8 +223,9 @@ lbm_status lbm_batch_destroy(lbm_batch b);
  * join at the next tick.  A job's populations are exactly those it would reach run alone.  The
  * server keeps the device buffers of finished jobs for later ones, in size classes (2^k times 1,
  * 1.25, 1.5 or 1.75): a job's footprint is lbm_footprint_bytes with each buffer rounded up to its
- * class (at most 25 % more), and idle kept buffers count against the budget until a job needs
- * the room. */
+ * class (at most 25 % more), plus, for a job submitted with want_fields, the 4 N doubles its field
+ * readout takes on the device; idle kept buffers count against the budget until a job needs the
+ * room.  A job whose field readout fails is reported with that status and keeps no fields. */
 typedef struct {
   lbm_desc desc;
   const uint8_t *node_type; /* N bytes, copied at submit */

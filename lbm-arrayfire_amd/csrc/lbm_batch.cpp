@@ -160,10 +160,13 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     int32_t *pmulti = (int32_t *)(g.h_tab + g.pmulti_off);
     StepParams *par0 = (StepParams *)(g.h_tab + g.par_off[0]);
     StepParams *par1 = (StepParams *)(g.h_tab + g.par_off[1]);
-    // multi-step launches when every member uses the default multi-step shape
+    // multi-step launches when every member uses the default multi-step shape; a group that
+    // switches between multi-step and single-step launches enters or leaves the captured graph
+    const int was_multi = g.multi_k;
     g.multi_k = multi_steps_per_launch(g.cfg, MULTI_DEFAULT);
     for (lbm_ctx *m : g.members)
       if (m->multi_variant != MULTI_DEFAULT || m->d.nz != 1) g.multi_k = 0;
+    if ((was_multi != 0) != (g.multi_k != 0)) recapture = true;
     prefix[0] = pmulti[0] = 0;
     for (int i = 0; i < n; ++i) {
       const lbm_ctx *m = g.members[(size_t)i];

@@ -168,7 +168,7 @@ void DevPool::trim_except(const std::vector<size_t> &keep) {
   }
 }
 
-std::vector<size_t> pooled_blocks(const lbm_desc *d) {
+std::vector<size_t> pooled_blocks(const lbm_desc *d, bool fields) {
   if (!check_desc(d).empty()) return {};
   const size_t N = (size_t)d->nx * d->ny * d->nz;
   int32_t gx, gy, gz, nt;
@@ -178,12 +178,13 @@ std::vector<size_t> pooled_blocks(const lbm_desc *d) {
   std::vector<size_t> b = {buf, buf, DevPool::size_class(N), DevPool::size_class((size_t)nt * 3 * sizeof(double)),
                            DevPool::size_class(4 * sizeof(double))};
   if (d->collision != LBM_BGK && d->lattice == 27) b.push_back(DevPool::size_class(64 * MFMA_TAB * sizeof(float)));
+  if (fields) b.push_back(DevPool::size_class(field_scratch_bytes((int64_t)N)));
   return b;
 }
 
-size_t pooled_footprint(const lbm_desc *d) {
+size_t pooled_footprint(const lbm_desc *d, bool fields) {
   size_t fp = 0;
-  for (size_t c : pooled_blocks(d)) fp += c;
+  for (size_t c : pooled_blocks(d, fields)) fp += c;
   return fp;
 }
 
@@ -341,7 +342,7 @@ lbm_status lbm_destroy(lbm_handle h) {
     dfree(h->partials, (size_t)h->base.ntiles * 3 * sizeof(double));
     dfree(h->red, 4 * sizeof(double));
     dfree(h->mrt_tab, 64 * MFMA_TAB * sizeof(float));
-    if (h->fields) (void)hipFree(h->fields);
+    dfree(h->fields, field_scratch_bytes(h->N));
     if (h->probe) (void)hipFree(h->probe);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -351,8 +352,13 @@ lbm_status lbm_destroy(lbm_handle h) {
   });
 }
 
+// Device scratch of the field readout (rho, ux, uy, uz as doubles), allocated on first use; a
+// lattice of the job server takes it from the server's pool (and returns it there).
 static lbm_status ensure_fields(lbm_handle h) {
-  if (!h->fields) HIPCHK(hipMalloc((void **)&h->fields, 4 * (size_t)h->N * sizeof(double)));
+  if (!h->fields) {
+    const size_t n = field_scratch_bytes(h->N);
+    HIPCHK(h->pool ? h->pool->get((void **)&h->fields, n) : hipMalloc((void **)&h->fields, n));
+  }
   return LBM_OK;
 }
 

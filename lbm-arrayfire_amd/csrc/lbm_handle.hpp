@@ -84,8 +84,11 @@ struct DevPool {
 };
 // The blocks a lattice takes from a DevPool (sizes rounded to their classes; the transient
 // node-type staging block not included), and their sum.
-std::vector<size_t> pooled_blocks(const lbm_desc *d);
-size_t pooled_footprint(const lbm_desc *d);
+// fields: the lattice's field readout scratch (field_scratch_bytes) too, for jobs that keep their
+// fields.
+std::vector<size_t> pooled_blocks(const lbm_desc *d, bool fields);
+size_t pooled_footprint(const lbm_desc *d, bool fields);
+inline size_t field_scratch_bytes(int64_t N) { return 4 * (size_t)N * sizeof(double); }
 // lbm_create on a lent stream (nullptr: a stream of its own) with buffers from a pool (nullptr:
 // hipMalloc / hipFree), and a batch's stream
 lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream_t shared, DevPool *pool,

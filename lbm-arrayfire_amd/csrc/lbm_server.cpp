@@ -87,8 +87,10 @@ lbm_status retire(lbm_server_ctx *s, Job *j, bool check_fields) {
     f.ux.resize(N);
     f.uy.resize(N);
     f.uz.resize(N);
-    lbm_get_fields(j->h, f.rho.data(), f.ux.data(), f.uy.data(), f.uz.data());
-    s->kept[j->id] = std::move(f);
+    // a failed readout is the job's result: no fields are kept for it
+    const lbm_status fs = lbm_get_fields(j->h, f.rho.data(), f.ux.data(), f.uy.data(), f.uz.data());
+    if (fs == LBM_OK) s->kept[j->id] = std::move(f);
+    else r.status = fs;
   }
   lbm_batch_remove(s->batch, j->h);
   lbm_destroy(j->h);
@@ -117,7 +119,8 @@ lbm_status admit(lbm_server_ctx *s) {
     // idle pooled blocks count against the budget: when the job needs the room, release those it
     // will not reuse, so that once it has taken the rest the device holds used + footprint, within
     // the budget as checked above
-    if (s->used + j->footprint + s->pool.held > s->budget) s->pool.trim_except(lbm::pooled_blocks(&j->spec.desc));
+    if (s->used + j->footprint + s->pool.held > s->budget)
+      s->pool.trim_except(lbm::pooled_blocks(&j->spec.desc, j->spec.want_fields != 0));
     st = lbm::abi_guard([&]() -> lbm_status {
       return lbm::create_lattice(&j->spec.desc, j->types.data(), lbm::batch_stream(s->batch), &s->pool, &j->h);
     });
@@ -182,7 +185,9 @@ lbm_status lbm_server_submit(lbm_server s, const lbm_job_spec *spec, int64_t *jo
   return lbm::abi_guard([&]() -> lbm_status {
     if (!s || !spec || !spec->node_type) return set_error(LBM_EINVAL, "NULL argument");
     if (spec->nsteps < 0) return set_error(LBM_EINVAL, "nsteps < 0");
-    const size_t fp = lbm::pooled_footprint(&spec->desc);  // buffers come from the server's pool
+    // buffers come from the server's pool; a job that keeps its fields also holds the field
+    // readout scratch (4 doubles per node) when it retires
+    const size_t fp = lbm::pooled_footprint(&spec->desc, spec->want_fields != 0);
     if (fp == 0) return set_error(LBM_EINVAL, "invalid job descriptor");
     if (fp > s->budget) return set_error(LBM_ENOMEM, "job footprint exceeds the server's HBM budget");
     Job *j = new (std::nothrow) Job();

@@ -195,6 +195,17 @@ def _dp(a):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _sized(a, n, name):
+    """a as a contiguous float64 vector of exactly n entries (None stays None); the C ABI reads n
+    values from it, so a wrong size raises here instead of reading past the buffer."""
+    if a is None:
+        return None
+    v = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if v.size != n:
+        raise ValueError(f"{name} has {v.size} entries, expected {n}")
+    return v
+
+
 def check(status):
     if status != 0:
         raise LbmError(status, load().lbm_last_error().decode())
@@ -248,8 +259,7 @@ class Lattice:
         self.close()
 
     def init_fields(self, rho=None, ux=None, uy=None, uz=None):
-        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
-                for a in (rho, ux, uy, uz)]
+        arrs = [_sized(a, self.N, name) for a, name in zip((rho, ux, uy, uz), ("rho", "ux", "uy", "uz"))]
         check(self.lib.lbm_init_fields(self.h, *[_dp(a) for a in arrs]))
 
     def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), seed=0, noise_amp=0.0):
@@ -297,7 +307,7 @@ class Lattice:
         return f
 
     def set_populations(self, f):
-        f = np.ascontiguousarray(f, dtype=np.float64)
+        f = _sized(f, self.N * self.Q, "populations")
         check(self.lib.lbm_set_populations(self.h, _dp(f)))
 
     def state(self):
@@ -306,7 +316,10 @@ class Lattice:
         return b
 
     def set_state(self, b):
-        b = np.ascontiguousarray(b, dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8).reshape(-1)
+        want = self.N * self.Q * STORAGE_BYTES[self.cfg.get("storage", F32)]  # lbm_state_bytes
+        if b.size != want:
+            raise ValueError(f"state has {b.size} bytes, the lattice holds {want}")
         check(self.lib.lbm_set_state(self.h, b.ctypes.data_as(ctypes.c_void_p)))
 
     def monitors(self):
@@ -394,6 +407,9 @@ class Server:
     def submit(self, cfg, node_type, nsteps, rho0=1.0, u0=(0.0, 0.0, 0.0), seed=0, noise_amp=0.0,
                want_fields=False):
         types = np.ascontiguousarray(node_type, dtype=np.uint8).reshape(-1)
+        n = int(np.prod([cfg["nx"], cfg["ny"], cfg.get("nz", 1)]))
+        if types.size != n:
+            raise ValueError(f"node_type has {types.size} entries, the job's lattice has {n} nodes")
         spec = JobSpec()
         spec.desc = make_desc(cfg)
         spec.node_type = types.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
@@ -475,9 +491,7 @@ class ShanChen:
 
     def init(self, rho1, rho2):
         """Both fluids at rest: f_i = w_i rho_s."""
-        r1 = np.ascontiguousarray(rho1, dtype=np.float64).reshape(-1)
-        r2 = np.ascontiguousarray(rho2, dtype=np.float64).reshape(-1)
-        assert r1.size == self.N and r2.size == self.N
+        r1, r2 = _sized(rho1, self.N, "rho1"), _sized(rho2, self.N, "rho2")
         check(self.lib.lbm_sc_init(self.h, _dp(r1), _dp(r2)))
 
     def step(self, n=1):
@@ -502,8 +516,7 @@ class ShanChen:
         return f
 
     def set_populations(self, f):
-        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
-        assert f.size == 18 * self.N
+        f = _sized(f, 18 * self.N, "populations")
         check(self.lib.lbm_sc_set_populations(self.h, _dp(f)))
 
     def close(self):

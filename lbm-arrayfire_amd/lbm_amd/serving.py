@@ -2,9 +2,10 @@
 
 The mix: jobs arrive as a Poisson process; half are D2Q9 BGK square lattices with side
 log-uniform in [64, 2048], half D3Q27 MRT (Gram-Schmidt basis) cubes with side log-uniform in
-[32, 192] (the "small" mix: [32, 256] and [16, 48]); Re uniform in [50, 1000]; geometry is a lid cavity or a channel with a cylinder/sphere
-of radius 5-15 % of the height; steps uniform in [1000, 20000] times ``step_scale``; FP32
-arithmetic with FP16-shifted storage.
+[32, 192] (the "small" mix: [32, 256] and [16, 48]; the "tiny" mix, small enough for the CPU
+oracle: [24, 64] and [10, 20]); Re uniform in [50, 1000]; geometry is a lid cavity or a channel
+with a cylinder/sphere of radius 5-15 % of the height; steps uniform in [1000, 20000] times
+``step_scale``; FP32 arithmetic with FP16-shifted storage (``storage`` selects another).
 
 Lattice-velocity policy (the reference fixes u and derives nu from Re): u = 0.05 for lids and
 inlets, nu = u L / Re, clamped to nu >= 1/150 (tau >= 0.52) so every job is stable; the Reynolds
@@ -26,10 +27,11 @@ U_LAT = 0.05
 # a mix of small jobs (each launch-latency-bound when run alone), where batching their steps pays
 # most; on lattices of 16-48 nodes the floor is raised to nu >= 1/30 (tau >= 0.6) to keep every
 # job stable
-SIDES = {"mixed": ((32, 192), (64, 2048), NU_MIN), "small": ((16, 48), (32, 256), 1.0 / 30.0)}
+SIDES = {"mixed": ((32, 192), (64, 2048), NU_MIN), "small": ((16, 48), (32, 256), 1.0 / 30.0),
+         "tiny": ((10, 20), (24, 64), 1.0 / 30.0)}
 
 
-def make_job(rng, step_scale=1.0, sides="mixed"):
+def make_job(rng, step_scale=1.0, sides="mixed", storage=F16S):
     (lo3, hi3), (lo2, hi2), nu_min = SIDES[sides]
     d3 = rng.random() < 0.5
     side = int(round(math.exp(rng.uniform(math.log(lo3), math.log(hi3))) if d3 else
@@ -42,7 +44,7 @@ def make_job(rng, step_scale=1.0, sides="mixed"):
     re_run = U_LAT * L / nu
     om = cases.omega_from_nu(nu)
     if d3:
-        cfg = dict(lattice=D3Q27, nx=side, ny=side, nz=side, collision=MRT_GS, storage=F16S,
+        cfg = dict(lattice=D3Q27, nx=side, ny=side, nz=side, collision=MRT_GS, storage=storage,
                    bc_mode=BC_PHYSICAL, omega=om, s=cases.shear_mrt_rates(om))
         if lid:
             cfg["u_wall"] = (U_LAT, 0.0, 0.0)
@@ -52,7 +54,7 @@ def make_job(rng, step_scale=1.0, sides="mixed"):
             cfg.update(u_in=U_LAT, inlet_profile=INLET_UNIFORM)
             types = geo.channel_3d_physical(side, side, side, side // 4, side // 2, side // 2, r)
     else:
-        cfg = dict(lattice=D2Q9, nx=side, ny=side, nz=1, collision=BGK, storage=F16S,
+        cfg = dict(lattice=D2Q9, nx=side, ny=side, nz=1, collision=BGK, storage=storage,
                    bc_mode=BC_PHYSICAL, omega=om)
         if lid:
             cfg["u_wall"] = (U_LAT, 0.0, 0.0)
@@ -66,19 +68,23 @@ def make_job(rng, step_scale=1.0, sides="mixed"):
                 seed=int(rng.integers(1 << 62)), nodes=side ** (3 if d3 else 2))
 
 
-def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="mixed"):
+def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="mixed", storage=F16S):
     rng = np.random.default_rng(seed)
-    jobs = [make_job(rng, step_scale, sides) for _ in range(njobs)]
+    jobs = [make_job(rng, step_scale, sides, storage) for _ in range(njobs)]
     t = np.cumsum(rng.exponential(1.0 / rate, njobs))
     for j, a in zip(jobs, t):
         j["arrival_s"] = float(a)
     return jobs
 
 
-def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True):
+def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True, keep=()):
     """Runs the mix through the server.  realtime: submit each job at its arrival time (wall clock,
-    between ticks); otherwise submit everything at t = 0.  Returns per-job results and totals."""
+    between ticks); otherwise submit everything at t = 0.  keep: indices of jobs whose final
+    rho/u fields are kept (returned under "fields", by job index).  Returns per-job results and
+    totals."""
     res = []
+    keep = set(keep)
+    fields = {}
     with Server(budget_bytes, steps_per_tick=steps_per_tick, check_every=32) as srv:
         t0 = time.perf_counter()
         k = 0
@@ -87,7 +93,8 @@ def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True):
             now = time.perf_counter() - t0
             while k < len(jobs) and (not realtime or jobs[k]["arrival_s"] <= now):
                 j = jobs[k]
-                ids[srv.submit(j["cfg"], j["types"], j["steps"], 1.0, j["u0"], j["seed"], 1e-4)] = k
+                ids[srv.submit(j["cfg"], j["types"], j["steps"], 1.0, j["u0"], j["seed"], 1e-4,
+                               want_fields=k in keep)] = k
                 k += 1
             active, queued = srv.tick()
             res += srv.poll()
@@ -95,14 +102,33 @@ def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True):
                 time.sleep(max(0.0, min(0.002, jobs[k]["arrival_s"] - (time.perf_counter() - t0))))
         wall = time.perf_counter() - t0
         stats = srv.stats()
+        for r in res:
+            if ids[r["job_id"]] in keep and r["status"] == 0:
+                fields[ids[r["job_id"]]] = srv.take(r["job_id"])
     # latency from the job's arrival (realtime) or its submission (everything at t = 0)
     lat = np.array([r["finish_s"] - (jobs[ids[r["job_id"]]]["arrival_s"] if realtime else r["submit_s"])
                     for r in res])
     lups = sum(r["nodes"] * r["steps"] for r in res)
     bad = sum(1 for r in res if r["status"] != 0)
-    return dict(wall_s=wall, jobs=len(res), failed=bad, p50_s=float(np.percentile(lat, 50)),
-                p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6,
-                server=stats)
+    out = dict(wall_s=wall, jobs=len(res), failed=bad, p50_s=float(np.percentile(lat, 50)),
+               p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6,
+               server=stats)
+    if keep:
+        out["fields"] = fields
+    return out
+
+
+def fp64_errors(job, fields, steps=None):
+    """rel-L2 of rho and of the velocity vector field of a served job's fields against the same job
+    run alone with FP64 storage (same initial state, same step count)."""
+    cfg = dict(job["cfg"], storage=0)
+    with Lattice(cfg, job["types"]) as lat:
+        lat.init_equilibrium(1.0, job["u0"], job["seed"], 1e-4)
+        lat.step(job["steps"] if steps is None else steps)
+        ref = lat.fields()
+    e_rho = float(np.linalg.norm(fields[0] - ref[0]) / np.linalg.norm(ref[0]))
+    ua, ub = np.stack(fields[1:4]), np.stack(ref[1:4])
+    return e_rho, float(np.linalg.norm(ua - ub) / max(np.linalg.norm(ub), 1e-300))
 
 
 def sequential(jobs):

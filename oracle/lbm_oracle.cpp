@@ -690,14 +690,14 @@ int fields_impl(const oracle_desc *d, const uint8_t *types, const double *f, dou
 }
 
 template <class T>
-double time_impl(const oracle_desc *d, const uint8_t *types, int64_t nsteps) {
+double time_impl(const oracle_desc *d, const uint8_t *types, int64_t nsteps, int64_t warmup) {
   Sim<T> sim(*d, types);
   const int Q = sim.L.Q;
   sim.F.assign((size_t)(sim.N * Q), (T)0);
   for (int i = 0; i < Q; ++i)
     for (int64_t n = 0; n < sim.N; ++n) sim.F[(size_t)(n + sim.N * i)] = sim.w[i];
   if (d->bc_mode == 0) sim.build_pull_map(false);
-  sim.step();  // warm-up (page in temporaries)
+  for (int64_t t = 0; t < warmup; ++t) sim.step();  // warm-up (page in temporaries)
   const auto t0 = std::chrono::steady_clock::now();
   for (int64_t t = 0; t < nsteps; ++t) sim.step();
   const auto t1 = std::chrono::steady_clock::now();
@@ -906,9 +906,10 @@ int oracle_fields(const oracle_desc *d, const uint8_t *types, const double *f, d
 }
 
 // Seconds for nsteps steps from a rest state (CPU baseline timing).
-double oracle_time_steps(const oracle_desc *d, const uint8_t *types, int64_t nsteps) {
-  if (!desc_ok(d)) return -1.0;
-  return is_f64(d) ? time_impl<double>(d, types, nsteps) : time_impl<float>(d, types, nsteps);
+// Seconds for nsteps steps after `warmup` untimed ones (populations start at w_i).
+double oracle_time_steps(const oracle_desc *d, const uint8_t *types, int64_t nsteps, int64_t warmup) {
+  if (!desc_ok(d) || warmup < 0) return -1.0;
+  return is_f64(d) ? time_impl<double>(d, types, nsteps, warmup) : time_impl<float>(d, types, nsteps, warmup);
 }
 
 // Codecs exposed for tests (checked against numpy's float16 conversion).

@@ -51,7 +51,7 @@ def lib():
         L.oracle_init_reference.argtypes = [dsc, u8p, ctypes.c_double, dp]
         L.oracle_run.argtypes = [dsc, u8p, dp, ctypes.c_int64, ctypes.c_int]
         L.oracle_fields.argtypes = [dsc, u8p, dp, dp, dp, dp, dp]
-        L.oracle_time_steps.argtypes = [dsc, u8p, ctypes.c_int64]
+        L.oracle_time_steps.argtypes = [dsc, u8p, ctypes.c_int64, ctypes.c_int64]
         L.oracle_time_steps.restype = ctypes.c_double
         L.oracle_tables.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), dp,
                                     ctypes.POINTER(ctypes.c_int32)]
@@ -147,11 +147,12 @@ def fields(cfg, types, f):
     return rho, ux, uy, uz
 
 
-def time_steps(cfg, types, nsteps, threads=0):
+def time_steps(cfg, types, nsteps, threads=0, warmup=1):
+    """Seconds the oracle takes for nsteps steps after `warmup` untimed ones."""
     if threads:
         lib().oracle_set_threads(int(threads))
     d = make_desc(cfg)
-    return lib().oracle_time_steps(ctypes.byref(d), _u8(types), int(nsteps))
+    return lib().oracle_time_steps(ctypes.byref(d), _u8(types), int(nsteps), int(warmup))
 
 
 def max_threads():

@@ -64,3 +64,41 @@ def test_footprint_bytes(lbm_lib):
     assert fp >= 2 * N * 27 * 2 + N
     assert fp < 2 * (N * 2 + 8448) * 27 + N + 64 * 1024  # population arrays may be padded
     assert capi.footprint_bytes(dict(cfg, lattice=5)) == 0
+
+
+class _NoCall:
+    """Stands in for the library: any C call made by a wrapper under test is a failure."""
+
+    def __getattr__(self, name):
+        raise AssertionError(f"{name} called although the arguments were invalid")
+
+
+def _unbound(cls, **attrs):
+    obj = cls.__new__(cls)
+    obj.lib = _NoCall()
+    obj.h = ctypes.c_void_p()
+    for k, v in attrs.items():
+        setattr(obj, k, v)
+    return obj
+
+
+def test_wrappers_reject_wrongly_sized_buffers():
+    """A buffer of the wrong length raises ValueError before the C ABI (which reads N, N*Q or
+    lbm_state_bytes elements from it) is called."""
+    lat = _unbound(capi.Lattice, N=48, Q=9, cfg=dict(lattice=9, nx=8, ny=6, storage=capi.F32))
+    with pytest.raises(ValueError):
+        lat.init_fields(np.ones(47))
+    with pytest.raises(ValueError):
+        lat.init_fields(None, np.zeros(49))
+    with pytest.raises(ValueError):
+        lat.set_populations(np.zeros(48 * 9 - 1))
+    with pytest.raises(ValueError):
+        lat.set_state(np.zeros(48 * 9 * 4 + 1, np.uint8))
+    srv = _unbound(capi.Server, _keep={})
+    with pytest.raises(ValueError):
+        srv.submit(dict(lattice=9, nx=8, ny=6), np.zeros(47, np.uint8), 10)
+    sc = _unbound(capi.ShanChen, N=20, nx=5, ny=4)
+    with pytest.raises(ValueError):
+        sc.init(np.ones(20), np.ones(19))
+    with pytest.raises(ValueError):
+        sc.set_populations(np.ones(18 * 20 + 1))

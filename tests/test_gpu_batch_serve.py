@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from lbm_amd import Batch, Lattice, LbmError, Server, cases, geometry as geo
-from lbm_amd.capi import F16S, F32, F64, MRT_GS, MRT_SCALAR
+from lbm_amd.capi import F16S, F32, F64, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
 pytestmark = pytest.mark.gpu
 
@@ -19,10 +19,14 @@ def member_specs():
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d2q9_karman(128, 48, 32, 24, 5, re=30.0, storage=F16S)
     specs.append((cfg, t, ("eq", (0.1, 0, 0), 5, 1e-4)))
-    cfg, t = cases.d3q27_mrt_lid_gs(24, re=100.0, storage=F16S)  # MFMA engine
+    cfg, t = cases.d3q27_mrt_lid_gs(24, re=100.0, storage=F16S, engine=MRT_MFMA_F32)  # two-stage MFMA
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d3q27_mrt_lid_gs(20, re=100.0, storage=F16S)  # the scalar engine (AUTO)
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F32)
     specs.append((cfg, t, ("eq", (0.05, 0, 0), 9, 1e-4)))
+    cfg, t = cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F32, collision=MRT_GS)
+    specs.append((dict(cfg, mrt_engine=MRT_MFMA_FUSED), t, ("eq", (0.05, 0, 0), 11, 1e-4)))  # fused MFMA
     cfg, t = cases.d3q27_mrt_lid_reference(12, storage=F64, engine=MRT_SCALAR)
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d3q27_bgk_lid_reference(20, u_lid=0.05, re=50.0, storage=F32)
@@ -266,7 +270,8 @@ def test_server_reuses_buffers_of_finished_jobs():
             lat.init_equilibrium(1.0, (0, 0, 0), seed, 1e-3)
             lat.step(300)
             alone.append(lat.fields())
-    budget = int(1.5 * footprint_bytes(specs[0][0][0]))
+    # a job that keeps its fields also holds their readout scratch, 4 doubles per node
+    budget = int(1.5 * (footprint_bytes(specs[0][0][0]) + 4 * 8 * 64 * 64))
     with Server(budget, steps_per_tick=100) as srv:
         ids = [srv.submit(cfg, t, 300, seed=seed, noise_amp=1e-3, want_fields=True) for (cfg, t), seed in specs]
         done = []
@@ -278,3 +283,132 @@ def test_server_reuses_buffers_of_finished_jobs():
         for jid, want in zip(ids, alone):
             for a, b in zip(srv.take(jid), want):
                 assert np.array_equal(a, b)
+
+
+def test_batch_group_switching_to_single_steps_recaptures(monkeypatch):
+    """One membership change swaps a D2Q9 lattice that advances several steps per launch for one of
+    the same shape created with LBM_MULTISTEP=off: its group leaves the multi-step launches and
+    joins the replayed two-step graph (and back again), and every member still equals its run
+    alone."""
+    cfg2, t2 = cases.d2q9_lid_reference(64, storage=F32)
+    cfg3, t3 = cases.d3q27_bgk_lid_reference(12, u_lid=0.05, re=50.0, storage=F32)
+
+    def lattice(cfg, t, single=False):
+        if single:
+            monkeypatch.setenv("LBM_MULTISTEP", "off")
+        try:
+            lat = Lattice(cfg, t)
+        finally:
+            monkeypatch.delenv("LBM_MULTISTEP", raising=False)
+        lat.init_reference(1.0)
+        return lat
+
+    want = {}
+    for key, cfg, t, single, n in (("a", cfg2, t2, False, 20), ("b", cfg2, t2, True, 10), ("c", cfg3, t3, False, 30)):
+        with lattice(cfg, t, single) as lat:
+            lat.step(n)
+            want[key] = lat.state()
+    a, b, c = lattice(cfg2, t2), lattice(cfg2, t2, True), lattice(cfg3, t3)
+    assert a.steps_per_launch > 1 and b.steps_per_launch == 1
+    with Batch([c, a]) as bt:
+        bt.step(10)  # c replays the captured graph, a runs multi-step launches
+        bt.remove(a)
+        bt.add(b)  # same group, same tiles: now single steps, inside the graph
+        bt.step(10)
+        bt.remove(b)
+        bt.add(a)  # and out of it again
+        bt.step(10)
+        got = {"a": a.state(), "b": b.state(), "c": c.state()}
+    for key in want:
+        assert np.array_equal(got[key], want[key]), key
+    for lat in (a, b, c):
+        lat.close()
+
+
+def equilibrium_start(cfg, types, seed, u0, amp=1e-4):
+    """rho, ux, uy, uz that lbm_init_equilibrium(1, u0, seed, amp) builds (its documented SplitMix64
+    draws, vectorised): u0 plus amp * (2 r - 1) on every component of every non-solid node."""
+    types = np.asarray(types).reshape(-1)
+    N, D = types.size, (2 if cfg["lattice"] == 9 else 3)
+    k = np.arange(N * D, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (k + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    r = (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    u = np.zeros((3, N))
+    for a in range(3):
+        u[a] = u0[a]
+    fluid = (types & 1) == 0
+    noise = (amp * (2.0 * r - 1.0)).reshape(N, D)
+    for a in range(D):
+        u[a, fluid] += noise[fluid, a]
+    return np.ones(N), u[0], u[1], u[2]
+
+
+def test_equilibrium_start_restates_the_device_generator():
+    cfg, t = cases.d2q9_karman(40, 24, 10, 12, 3, storage=F64)
+    t = t.reshape(-1)
+    start = equilibrium_start(cfg, t, 0x5EED0005, (0.05, 0, 0))
+    with Lattice(cfg, t) as a, Lattice(cfg, t) as b:
+        a.init_equilibrium(1.0, (0.05, 0.0, 0.0), 0x5EED0005, 1e-4)
+        b.init_fields(*start)
+        assert np.array_equal(a.populations(), b.populations())
+
+
+def field_errors(a, b):
+    e_rho = float(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]))
+    ua, ub = np.stack(a[1:4]), np.stack(b[1:4])
+    return e_rho, float(np.linalg.norm(ua - ub) / max(np.linalg.norm(ub), 1e-300))
+
+
+def tiny_mix():
+    """Jobs of the serving mix's generator at sizes the CPU oracle runs in seconds: every
+    combination of lattice (D2Q9 BGK, D3Q27 MRT-GS), geometry (lid cavity, channel with an
+    obstacle) and storage (F32, F16S), two of each, 20-400 steps."""
+    from lbm_amd import serving
+    picked = []
+    for storage in (F32, F16S):
+        seen = {}
+        for job in serving.make_mix(80, seed=0x5EED0005 + storage, step_scale=0.02, sides="tiny", storage=storage):
+            key = (job["cfg"]["lattice"], "u_wall" in job["cfg"])
+            if seen.get(key, 0) < 2:
+                seen[key] = seen.get(key, 0) + 1
+                picked.append(job)
+        assert len(seen) == 4
+    return picked
+
+
+def test_server_mix_matches_oracle():
+    """A tiny serving mix through the continuous-batching server (jobs joining and leaving between
+    ticks of 37 steps): each job's rho/u fields against the CPU oracle running the same job from
+    the same initial state.  F32 jobs: within 1e-5 rel-L2 of the oracle's FP32 run.  F16S jobs:
+    within 2e-5 of the oracle's run through the same codec, and within 1e-3 of the FP64 oracle
+    (the accuracy the serving figure is quoted at)."""
+    import oracle as orc
+    from lbm_amd import serving
+    jobs = tiny_mix()
+    res = serving.serve(jobs, 1 << 30, steps_per_tick=37, realtime=False, keep=range(len(jobs)))
+    assert res["failed"] == 0 and len(res["fields"]) == len(jobs)
+    worst = {}
+    for k, job in enumerate(jobs):
+        cfg, t = job["cfg"], job["types"]
+        start = equilibrium_start(cfg, t, job["seed"], job["u0"])
+        got = res["fields"][k]
+        fo = orc.run(cfg, t, orc.init_fields(cfg, t, *start), job["steps"])
+        e_o = field_errors(got, orc.fields(cfg, t, fo))
+        cfg64 = dict(cfg, storage=F64)
+        f64 = orc.run(cfg64, t, orc.init_fields(cfg64, t, *start), job["steps"])
+        e_64 = field_errors(got, orc.fields(cfg64, t, f64))
+        kind = f"{'D2Q9' if cfg['lattice'] == 9 else 'D3Q27'} {'lid' if 'u_wall' in cfg else 'channel'} " \
+               f"{'F32' if cfg['storage'] == F32 else 'F16S'}"
+        print(f"job {k} {kind} side {cfg['nx']} steps {job['steps']}: vs oracle rho {e_o[0]:.2e} u {e_o[1]:.2e}; "
+              f"vs FP64 rho {e_64[0]:.2e} u {e_64[1]:.2e}")
+        worst[kind] = max(worst.get(kind, 0.0), *e_o)
+        if cfg["storage"] == F32:
+            assert max(e_o) <= 1e-5, k
+        else:
+            assert max(e_o) <= 2e-5, k
+            assert max(e_64) <= 1e-3, k
+    print("largest difference from the oracle per kind:", worst)
