@@ -284,12 +284,23 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10):
     if accuracy_every:
         sample = jobs[::accuracy_every]
         acc = serving.serve(sample, budget, realtime=False, keep=range(len(sample)))
-        errs = [serving.fp64_errors(j, acc["fields"][k]) for k, j in enumerate(sample) if k in acc["fields"]]
+        errs, worst = [], None
+        for k, j in enumerate(sample):
+            if k not in acc["fields"]:
+                continue
+            e = serving.fp64_errors(j, acc["fields"][k])
+            errs.append(e)
+            if worst is None or e[1] > worst["rel_l2_u"]:
+                c = j["cfg"]
+                worst = {"job": f"{'D2Q9 BGK' if c['lattice'] == 9 else 'D3Q27 MRT-GS'} "
+                                f"{'lid' if 'u_wall' in c else 'channel'} side {c['nx']}, {j['steps']} steps, "
+                                f"Re {j['re_run']:.0f}", "rel_l2_rho": e[0], "rel_l2_u": e[1]}
         out["accuracy_vs_fp64"] = {
             "sample": f"every {accuracy_every}th job ({len(sample)} jobs, {len(errs)} finished)",
             "max_rel_l2_rho": float(max(e[0] for e in errs)) if errs else None,
             "max_rel_l2_u": float(max(e[1] for e in errs)) if errs else None,
-            "median_rel_l2_u": float(np.median([e[1] for e in errs])) if errs else None}
+            "median_rel_l2_u": float(np.median([e[1] for e in errs])) if errs else None,
+            "worst_job": worst}
     return out
 
 
@@ -358,6 +369,7 @@ def main():
                     help="comma list of extra workloads timed briefly, 'none', or 'default'")
     ap.add_argument("--also-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--serve-jobs", type=int, default=120,
                     help="jobs of the serving mix (0 skips the serving leg)")
     ap.add_argument("--serve-scale", type=float, default=0.05,
@@ -411,7 +423,7 @@ def main():
     except Exception as e:  # noqa: BLE001 — report, keep the headline line
         shanchen = {"error": str(e)}
     try:
-        pcie = pcie_inclusive_leg(args.workload)
+        pcie = None if args.no_pcie else pcie_inclusive_leg(args.workload)
     except Exception as e:  # noqa: BLE001 — report, keep the headline line
         pcie = {"error": str(e)}
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
