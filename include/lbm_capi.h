@@ -180,6 +180,16 @@ lbm_status lbm_get_fields(lbm_handle h, double *rho, double *ux, double *uy, dou
  * Blocks.  No reference counterpart (the reference reads whole fields for display). */
 lbm_status lbm_get_fields_at(lbm_handle h, const int64_t *nodes, int32_t count, double *rho, double *ux,
                              double *uy, double *uz);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc): lbm_get_fields_at with the same
+ * arguments and the same values, bit for bit, computed at the listed nodes only and calling no
+ * device allocation.  At most LBM_NOALLOC_MAX_NODES nodes per call (LBM_EINVAL beyond).  The node
+ * list and the results are staged in one fixed device array per GPU (640 KiB, held until the
+ * process ends) that every call on that GPU shares: calls that overlap on one GPU, from several
+ * threads, race.  Blocks. */
+#define LBM_NOALLOC_MAX_NODES 16384
+lbm_status lbm_get_fields_at_noalloc(lbm_handle h, const int64_t *nodes, int32_t count, double *rho,
+                                     double *ux, double *uy, double *uz);
 /* Current post-collision populations as doubles (decoded for shifted storage), N*Q.  Blocks. */
 lbm_status lbm_get_populations(lbm_handle h, double *f);
 lbm_status lbm_set_populations(lbm_handle h, const double *f);

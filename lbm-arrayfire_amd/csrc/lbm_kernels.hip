@@ -12,23 +12,14 @@ namespace lbm {
 template <int Q, int ST, int BC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fields(StepParams p, double *rho_o, double *ux_o,
                                                           double *uy_o, double *uz_o) {
-  using C = typename Codec<ST, Q>::C;
   const NodePos q = node_of_thread(p);
   if (!q.valid) return;
-  const uint8_t t = p.type[q.n];
-  C f[Q], rho = 0, ux = 0, uy = 0, uz = 0;
-  if constexpr (BC == BC_REF) {
-    pull_periodic<Q, ST>(p, q, f);
-    moments<Q>(f, rho, ux, uy, uz);
-    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
-  } else if (!(t & NODE_SOLID)) {
-    pull_physical<Q, ST>(p, q, t, f);
-    moments<Q>(f, rho, ux, uy, uz);
-  }
-  if (rho_o) rho_o[q.n] = (double)rho;
-  if (ux_o) ux_o[q.n] = (double)ux;
-  if (uy_o) uy_o[q.n] = (double)uy;
-  if (uz_o) uz_o[q.n] = (double)uz;
+  double r[4];
+  fields_of_node<Q, ST, BC>(p, q, r);
+  if (rho_o) rho_o[q.n] = r[0];
+  if (ux_o) ux_o[q.n] = r[1];
+  if (uy_o) uy_o[q.n] = r[2];
+  if (uz_o) uz_o[q.n] = r[3];
 }
 
 // f = f_eq(rho, u) (cpp/d2q9_bgk_lid.cpp:100-107) into both buffers.

@@ -36,6 +36,28 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
   store_all<Q, ST, NT>(p, q.n, out);
 }
 
+// rho, u the reference holds after a step at node q (cpp/d2q9_bgk_lid.cpp:131-146): moments of the
+// pulled populations with the overrides (REFERENCE), or of the boundary-completed pulled
+// populations (PHYSICAL; 0 at solid nodes).  out = {rho, ux, uy, uz}.
+template <int Q, int ST, int BC, class P>
+__device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, double *out) {
+  using C = typename Codec<ST, Q>::C;
+  const uint8_t t = p.type[q.n];
+  C f[Q], rho = 0, ux = 0, uy = 0, uz = 0;
+  if constexpr (BC == BC_REF) {
+    pull_periodic<Q, ST>(p, q, f);
+    moments<Q>(f, rho, ux, uy, uz);
+    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+  } else if (!(t & NODE_SOLID)) {
+    pull_physical<Q, ST>(p, q, t, f);
+    moments<Q>(f, rho, ux, uy, uz);
+  }
+  out[0] = (double)rho;
+  out[1] = (double)ux;
+  out[2] = (double)uy;
+  out[3] = (double)uz;
+}
+
 // NT: populations written with non-temporal stores (lattices too large for the caches)
 template <int Q, int ST, int BC, int CO, bool NT>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {

@@ -25,6 +25,7 @@ NODE_FLUID, NODE_SOLID, NODE_MOVING, NODE_INLET, NODE_OUTLET, NODE_RHO_FIX = 0, 
 STATUS = {0: "LBM_OK", 1: "LBM_EINVAL", 2: "LBM_ENOMEM", 3: "LBM_EDEVICE", 4: "LBM_EUNSTABLE",
           5: "LBM_EINTERNAL", 6: "LBM_EAGAIN"}
 STORAGE_BYTES = {F64: 8, F32: 4, F16S: 2, BF16S: 2}
+NOALLOC_MAX_NODES = 16384  # LBM_NOALLOC_MAX_NODES
 
 # Every symbol include/lbm_capi.h declares (checked by tests/test_capi_exports.py).
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "lbm_destroy", "lbm_init_fields", "lbm_init_equilibrium", "lbm_init_reference", "lbm_step",
     "lbm_step_timed", "lbm_sync", "lbm_check_reciprocal", "lbm_steps_done", "lbm_steps_per_launch",
     "lbm_get_fields",
-    "lbm_get_fields_at",
+    "lbm_get_fields_at", "lbm_get_fields_at_noalloc",
     "lbm_get_populations",
     "lbm_set_populations", "lbm_state_bytes", "lbm_get_state", "lbm_set_state", "lbm_monitors",
     "lbm_force",
@@ -148,6 +149,7 @@ def load(path=None):
     L.lbm_steps_per_launch.restype = ctypes.c_int32
     L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
     L.lbm_get_fields_at.argtypes = [H, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, dp, dp, dp, dp]
+    L.lbm_get_fields_at_noalloc.argtypes = [H, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, dp, dp, dp, dp]
     L.lbm_get_populations.argtypes = [H, dp]
     L.lbm_set_populations.argtypes = [H, dp]
     L.lbm_state_bytes.argtypes = [H]
@@ -299,6 +301,15 @@ class Lattice:
         out = [np.zeros(idx.size) for _ in range(4)]
         check(self.lib.lbm_get_fields_at(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                          int(idx.size), *[_dp(a) for a in out]))
+        return out
+
+    def fields_at_noalloc(self, nodes):
+        """fields_at through lbm_get_fields_at_noalloc (no device allocation; at most
+        NOALLOC_MAX_NODES nodes)."""
+        idx = np.ascontiguousarray(nodes, dtype=np.int64).reshape(-1)
+        out = [np.zeros(idx.size) for _ in range(4)]
+        check(self.lib.lbm_get_fields_at_noalloc(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                 int(idx.size), *[_dp(a) for a in out]))
         return out
 
     def populations(self):
