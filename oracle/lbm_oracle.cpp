@@ -271,7 +271,10 @@ struct Sim {
   static int wrap(int a, int n) { return ((a % n) + n) % n; }
 
   // nb_index[p(x,i)] = p((x - c_i) mod dims, i); f32 builds round the index through float.
-  void build_pull_map(bool f32_compat) {
+  // Returns false when a rounded index falls outside the N*Q populations (above 2^24 consecutive
+  // floats are 2, 4, ... apart, and the last index can round up to N*Q): the reference's FP32
+  // build would then read past its array, which this restatement refuses to do.
+  bool build_pull_map(bool f32_compat) {
     nb.resize((size_t)(N * L.Q));
     for (int i = 0; i < L.Q; ++i)
       for (int z = 0; z < nz; ++z)
@@ -280,9 +283,12 @@ struct Sim {
             const int64_t src = idx(wrap(x - L.cx[i], nx), wrap(y - L.cy[i], ny), wrap(z - L.cz[i], nz)) + N * i;
             nb[(size_t)(idx(x, y, z) + N * i)] = f32_compat ? (int64_t)(float)src : src;
           }
+    for (const int64_t v : nb)
+      if (v < 0 || v >= N * L.Q) return false;
     on.clear();
     for (int64_t n = 0; n < N; ++n)
       if (type[n] & SOLID) on.push_back(n);
+    return true;
   }
 
   T inlet_u(int y, int z) const {
@@ -609,7 +615,7 @@ template <class T>
 int run_impl(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32c) {
   Sim<T> sim(*d, types);
   sim.load(f);
-  if (d->bc_mode == 0) sim.build_pull_map(f32c != 0);
+  if (d->bc_mode == 0 && !sim.build_pull_map(f32c != 0)) return 3;  // rounded index out of range
   for (int64_t t = 0; t < nsteps; ++t) sim.step();
   sim.store(f);
   return 0;

@@ -133,6 +133,8 @@ def run(cfg, types, f, nsteps, f32_index_compat=False):
     d = make_desc(cfg)
     out = np.array(f, dtype=np.float64, copy=True)
     rc = lib().oracle_run(ctypes.byref(d), _u8(types), _dp(out), int(nsteps), int(f32_index_compat))
+    if rc == 3:
+        raise ValueError("f32_index_compat: a pull index rounded through float falls outside the lattice")
     assert rc == 0, rc
     return out
 

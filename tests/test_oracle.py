@@ -587,3 +587,97 @@ def test_reference_3d_channels_reach_steady_flow():
             assert np.isfinite(b[k]).all()
             assert abs(np.abs(b[k][fluid] - (k == 0)).max() - np.abs(a[k][fluid] - (k == 0)).max()) < 5e-3
         assert np.abs(b[0][fluid] - 1).max() < 0.3 and np.abs(b[1][fluid]).max() < 0.2
+
+
+def _fp32_cases():
+    """Every lattice x collision x boundary mode the oracle runs, on small grids."""
+    out = []
+    for bc in (BC_REFERENCE, BC_PHYSICAL):
+        ref = bc == BC_REFERENCE
+        out.append((f"d2q9_bgk_{bc}", cases.d2q9_lid_reference(16, storage=F32) if ref else
+                    cases.d2q9_karman(40, 20, 10, 10, 3, re=20.0, storage=F32)))
+        for q, cols in ((19, (BGK, MRT_GS)), (27, (BGK, MRT_RAW, MRT_GS))):
+            for co in cols:
+                if q == 19:
+                    c = cases.d3q19_lid(8, re=20.0, storage=F32, collision=co) if ref else \
+                        cases.d3q19_sphere_channel(20, 10, 10, (6, 5, 5), 2, re=5.0, storage=F32, collision=co)
+                elif ref:
+                    c = (cases.d3q27_mrt_lid_reference(8, storage=F32) if co == MRT_RAW else
+                         cases.d3q27_mrt_lid_gs(8, re=20.0, storage=F32) if co == MRT_GS else
+                         cases.d3q27_bgk_lid_reference(8, u_lid=0.05, re=20.0, storage=F32))
+                else:
+                    c = cases.d3q27_sphere_channel(20, 10, 10, (6, 5, 5), 2, re=5.0, storage=F32, collision=co)
+                out.append((f"d3q{q}_co{co}_bc{bc}", c))
+    return out
+
+
+@pytest.mark.parametrize("name,case", _fp32_cases(), ids=[c[0] for c in _fp32_cases()])
+def test_fp32_oracle_tracks_fp64_oracle(name, case):
+    """The oracle's FP32 path (FMA-contracted equilibrium and relaxation, u = j * (1/rho), FP32
+    moment transforms) against its FP64 path, from the same fields: 10 steps stay within 1e-6
+    rel-L2, the FP32 rounding level.  A term dropped or changed only in the FP32 path (a sign, an
+    FMA operand, a weight) moves it by far more."""
+    cfg, types = case
+    types = types.reshape(-1)
+    rng = np.random.default_rng(3)
+    N = types.size
+    u = [0.02 * rng.standard_normal(N) * 0.05 for _ in range(3)]
+    rho = 1 + 1e-3 * rng.standard_normal(N)
+    f32 = orc.run(cfg, types, orc.init_fields(cfg, types, rho, *u), 10)
+    cfg64 = dict(cfg, storage=F64)
+    f64 = orc.run(cfg64, types, orc.init_fields(cfg64, types, rho, *u), 10)
+    e = rel_l2(f32, f64)
+    assert 1e-9 < e <= 1e-6, (name, e)  # FP32 rounding present, and no larger
+
+
+def test_fp32_oracle_pin_sees_small_slips():
+    """The 1e-6 band of the pin above is tight enough to see slips far smaller than a wrong term:
+    the same FP32 run with one shear rate 0.1 % off, or the lid speed 0.01 % off, lands outside
+    it (4e-6 each)."""
+    cfg, types = cases.d3q27_mrt_lid_gs(8, re=20.0, storage=F32)
+    types = types.reshape(-1)
+    rng = np.random.default_rng(3)
+    N = types.size
+    u = [0.02 * rng.standard_normal(N) * 0.05 for _ in range(3)]
+    rho = 1 + 1e-3 * rng.standard_normal(N)
+    cfg64 = dict(cfg, storage=F64)
+    ref = orc.run(cfg64, types, orc.init_fields(cfg64, types, rho, *u), 10)
+    for bad in (dict(cfg, s=[v * 1.001 if k == 4 else v for k, v in enumerate(cfg["s"])]),
+                dict(cfg, u_wall=(0.05 * 1.0001, 0.0, 0.0))):
+        got = orc.run(bad, types, orc.init_fields(bad, types, rho, *u), 10)
+        assert rel_l2(got, ref) > 1e-6
+
+
+def test_f32_index_compat_rounds_pull_indices_as_float():
+    """Above 2^24 populations the reference's FP32 builds index through float arrays
+    (cpp/d2q9_bgk_lid.cpp:86): brute force over a 1365 x 1367 periodic D2Q9 box (N*Q = 16,793,595)
+    that the gathered index of every population equals fl32(its true index).  With omega = 0 and
+    F[p] = p (FP64 values, exact), one step leaves F'[p] = the index it was gathered from."""
+    nx, ny = 1365, 1367
+    N = nx * ny
+    cfg = dict(lattice=9, nx=nx, ny=ny, nz=1, collision=BGK, storage=F64, bc_mode=BC_REFERENCE, omega=0.0)
+    types = np.zeros(N, np.uint8)
+    f = np.arange(9 * N, dtype=np.float64)
+    got = orc.run(cfg, types, f, 1, f32_index_compat=True).astype(np.int64)
+    c, _, _ = orc.tables(9)
+    x = np.tile(np.arange(nx), ny)
+    y = np.repeat(np.arange(ny), nx)
+    want = np.empty(9 * N, np.int64)
+    for i in range(9):
+        src = (x - c[0, i]) % nx + nx * ((y - c[1, i]) % ny) + N * i
+        want[N * i:N * (i + 1)] = src.astype(np.float32).astype(np.int64)
+    assert np.array_equal(got, want)
+    exact = orc.run(cfg, types, f, 1, f32_index_compat=False).astype(np.int64)
+    # floats are 2 apart in [2^24, 2^25): exactly the odd indices there move (8,189 of them)
+    moved = (exact >= 2 ** 24) & (exact % 2 == 1)
+    assert np.array_equal(exact != want, moved) and moved.sum() > 8000
+
+
+def test_f32_index_compat_refuses_an_index_past_the_array():
+    """1366 x 1366: N*Q - 1 = 16,793,603 rounds up to N*Q as a float, an index past the lattice; the
+    oracle refuses instead of reading past its array."""
+    nx = ny = 1366
+    cfg = dict(lattice=9, nx=nx, ny=ny, nz=1, collision=BGK, storage=F64, bc_mode=BC_REFERENCE, omega=0.0)
+    assert int(np.float32(9 * nx * ny - 1)) == 9 * nx * ny
+    with pytest.raises(ValueError, match="outside"):
+        orc.run(cfg, np.zeros(nx * ny, np.uint8), np.zeros(9 * nx * ny), 1, f32_index_compat=True)
