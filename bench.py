@@ -334,6 +334,64 @@ def pull_ceilings():
     return None
 
 
+def accuracy_run(kind, steps=None):
+    """Accuracy of each storage (and MRT engine) against the FP64 kernel on one configuration at its
+    stated size: rel-L2 of rho and of the velocity vector field after the stated step count.
+    kind "sphere": the 512x256x256 D3Q27 BGK sphere duct (Re 200, 10,000 steps; F32, F16S, BF16S;
+    drag coefficient on the sphere's box).  kind "lid128": the 128^3 D3Q27 MRT-GS lid cavity (Re
+    400, 20,000 steps; FP32 scalar and both MFMA engines, F16S, BF16S).  Prints progress lines and
+    returns a dict."""
+    import numpy as np
+
+    from lbm_amd import Lattice, cases
+    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+
+    def errs(a, b):
+        ua, ub = np.stack(a[1:4]), np.stack(b[1:4])
+        return (float(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0])),
+                float(np.linalg.norm(ua - ub) / np.linalg.norm(ub)))
+
+    out = {"kind": kind}
+    if kind == "sphere":
+        steps = steps or 10000
+        c, r = (128, 128, 128), 24
+        box = (c[0] - r - 2, c[0] + r + 3, c[1] - r - 2, c[1] + r + 3, c[2] - r - 2, c[2] + r + 3)
+        runs = [("fp64", F64, None), ("fp32", F32, None), ("f16s", F16S, None), ("bf16s", BF16S, None)]
+    else:
+        steps = steps or 20000
+        runs = [("fp64", F64, MRT_SCALAR), ("fp32_scalar", F32, MRT_SCALAR), ("fp32_mfma2", F32, MRT_MFMA_F32),
+                ("fp32_fused", F32, MRT_MFMA_FUSED), ("f16s_scalar", F16S, MRT_SCALAR),
+                ("bf16s_scalar", BF16S, MRT_SCALAR)]
+    ref = None
+    for tag, st, en in runs:
+        if kind == "sphere":
+            cfg, types = cases.d3q27_sphere_channel(storage=st)
+        else:
+            cfg, types = cases.d3q27_mrt_lid_gs(128, re=400.0, storage=st, engine=en)
+        t0 = time.perf_counter()
+        with Lattice(cfg, types) as lat:
+            if kind == "sphere":
+                lat.init_equilibrium(1.0, (cfg["u_in"], 0.0, 0.0), 0x5EED0004, 1e-4)
+            else:
+                lat.init_fields()
+            lat.step(steps)
+            flds = lat.fields()
+            row = {"nonfinite": int(lat.monitors()[2])}
+            if kind == "sphere":
+                row["c_d"] = float(2 * lat.force(box)[0] / (cfg["u_in"] ** 2 * np.pi * r * r))
+        row["wall_s"] = round(time.perf_counter() - t0, 2)
+        if ref is None:
+            ref = (flds, row)
+        else:
+            row["rel_l2_rho"], row["rel_l2_u"] = errs(flds, ref[0])
+            if "c_d" in row:
+                row["c_d_rel_diff"] = abs(row["c_d"] / ref[1]["c_d"] - 1)
+        out[tag] = row
+        print(f"accuracy {kind} {tag}: {row}", flush=True)
+    out["steps"] = steps
+    return out
+
+
 def mrt_flop_per_lup(Q):
     """The two dense Q x Q transforms (M f and M^-1 m*) per node update."""
     return 2 * (2 * Q * Q)
@@ -370,6 +428,8 @@ def main():
     ap.add_argument("--also-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
+    ap.add_argument("--accuracy", choices=["sphere", "lid128"], default=None,
+                    help="only run the accuracy comparison of one configuration against FP64 and print it")
     ap.add_argument("--serve-jobs", type=int, default=120,
                     help="jobs of the serving mix (0 skips the serving leg)")
     ap.add_argument("--serve-scale", type=float, default=0.05,
@@ -381,6 +441,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.accuracy:
+        print(json.dumps(accuracy_run(args.accuracy)))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 typedef struct lbm_ctx *lbm_handle;
 typedef struct lbm_batch_ctx *lbm_batch;
@@ -87,6 +87,16 @@ typedef enum { LBM_MEQ_REFERENCE = 0, LBM_MEQ_PROJECTED = 1 } lbm_meq_mode;
 
 typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profile;
 
+/* How PHYSICAL mode closes a velocity-inlet node (no reference counterpart: the reference's
+ * inlets are the REFERENCE-mode velocity overrides).  ZOU_HE: the c_x = +1 unknowns by
+ * non-equilibrium bounce-back with the transverse correction (Zou & He 1997; Hecht & Harting 2010
+ * in 3D).  EXTRAPOLATED: every population f_eq(rho_e, u_in) + (g - f_eq(rho_e, u_e)), g the
+ * downstream (x + 1) neighbour's populations with density rho_e and velocity u_e (Guo, Zheng &
+ * Shi 2002), the outlet's closure with the roles of rho and u swapped.  On D3Q27 ducts at
+ * tau near 0.5 the Zou-He inlet feeds a growing mode at the inlet plane (the 128x64x64 sphere duct
+ * at tau = 0.536 diverges by step 1,500) where the extrapolated inlet stays stable. */
+typedef enum { LBM_INLET_ZOU_HE = 0, LBM_INLET_EXTRAPOLATED = 1 } lbm_inlet_closure;
+
 /* Node-type byte, one per node (the reference's BOUND mask, cpp/d2q9_bgk_lid.cpp:90-94,
  * generalised).  Bits combine. */
 #define LBM_NODE_FLUID 0x00u
@@ -95,7 +105,8 @@ typedef enum { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1 } lbm_inlet_profil
 #define LBM_NODE_MOVING 0x02u  /* REFERENCE: u := u_wall override (lid, :141-142).
                                   PHYSICAL (with SOLID): the wall moves with u_wall. */
 #define LBM_NODE_INLET 0x04u   /* REFERENCE: ux := inlet velocity override (channel :190).
-                                  PHYSICAL: Zou-He velocity inlet on the x = 0 face. */
+                                  PHYSICAL: velocity inlet on the x = 0 face, closed as
+                                  lbm_desc.inlet_closure selects. */
 #define LBM_NODE_OUTLET 0x08u  /* PHYSICAL: zero-gradient outlet on the x = nx-1 face: every
                                   population = f_eq(rho_bc, u) + f_neq of the upstream neighbour
                                   (u = the neighbour's velocity). */
@@ -111,6 +122,8 @@ typedef struct {
   int32_t mrt_engine;     /* lbm_mrt_engine */
   int32_t meq_mode;       /* lbm_meq_mode */
   int32_t inlet_profile;  /* lbm_inlet_profile */
+  int32_t inlet_closure;  /* lbm_inlet_closure (PHYSICAL mode) */
+  int32_t reserved;       /* 0 */
   double omega;           /* BGK relaxation rate 1/tau */
   double s[27];           /* MRT rates, one per moment row */
   double u_wall[3];       /* lid / moving-wall velocity */

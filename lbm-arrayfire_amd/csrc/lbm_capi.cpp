@@ -79,6 +79,7 @@ std::string check_desc(const lbm_desc *d) {
     return "the MFMA engines are implemented for D3Q27";
   if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
   if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
+  if (d->inlet_closure < 0 || d->inlet_closure > 1) return "unknown inlet closure";
   if (!std::isfinite(d->omega)) return "omega is not finite";
   for (int k = 0; k < 27; ++k)
     if (!std::isfinite(d->s[k])) return "MRT rate is not finite";
@@ -217,6 +218,7 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   p.nz = d->nz;
   tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
   p.inlet_profile = d->inlet_profile;
+  p.inlet_closure = d->inlet_closure;
   p.meq_mode = d->meq_mode;
   // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
   // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %

@@ -243,14 +243,17 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 
 // PHYSICAL-mode boundary rules applied to the periodically pulled populations f of a SLOW node
 // at (x, y, z) of type t: half-way bounce-back on links whose source is solid (plus the
-// moving-wall momentum term 6 w_i rho_w c_i.u_w); inlet nodes close their c_x = +1 unknowns for
-// the prescribed velocity; outlet nodes take f_eq(rho_bc, u_nb) plus the non-equilibrium part of
-// the upstream neighbour in every direction (zero normal gradient of u and f_neq).  The state is
-// read through accessors, so the single-step kernels (global memory) and the multi-step kernel
-// (an LDS tile) share one statement of the rules:
+// moving-wall momentum term 6 w_i rho_w c_i.u_w); inlet nodes either close their c_x = +1
+// unknowns for the prescribed velocity (Zou-He) or take f_eq(rho_e, u_in) plus the
+// non-equilibrium part of the downstream neighbour in every direction (extrapolated, p.inlet_closure
+// = 1); outlet nodes take f_eq(rho_bc, u_nb) plus the non-equilibrium part of the upstream
+// neighbour in every direction (zero normal gradient of u and f_neq).  The state is read through
+// accessors, so the single-step kernels (global memory) and the multi-step kernel (an LDS tile)
+// share one statement of the rules:
 //   src_type(IC<i>)  type of the node this node pulls population i from (node - c_i)
 //   own(IC<j>)       population j of this node in the state being advanced
 //   west(IC<j>)      population j of the node at x - 1;  west_type()  its type
+//   east(IC<j>)      population j of the node at x + 1;  east_type()  its type
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
 // 6 w_i rho_w c_i.u_w when the source is a moving wall.
 template <int Q, class C, class SrcType, class Own, class P>
@@ -275,14 +278,56 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, SrcType &&src_type,
   });
 }
 
-template <int Q, class C, class SrcType, class Own, class West, class WestType, class P>
+// Non-equilibrium extrapolation (Guo, Zheng & Shi, Chinese Physics 11 (2002) 366) from a
+// neighbour whose populations nbr(IC<j>) the node would have pulled had the closure not replaced
+// them: f_i = f_eq,i(rho_b, u_b) + (g_i - f_eq,i(rho_g, u_g)) with g the neighbour's populations
+// and rho_g, u_g their moments.  rho_b < 0 takes rho_b = rho_g (velocity inlet), otherwise u_b =
+// u_g (pressure outlet: f_eq is linear in rho, so g_i + (rho_b - rho_g) f_eq,i(1, u_g), one FMA in
+// FP32).  Without a fluid neighbour the node takes f_eq(rho_b, u_b) (rho_b = rho_bc).
+template <int Q, class C, class Nbr, class P>
+__device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, bool set_u, C ubx, Nbr &&nbr) {
+  C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
+  if (nb) {
+    static_for<Q>([&](auto I) { g[decltype(I)::value] = nbr(I); });
+    moments<Q>(g, rn, vx, vy, vz);
+  }
+  if (set_u) {  // velocity inlet: u_b = (ubx, 0, 0), rho_b = rho_g
+    const C rb = nb ? rn : (C)p.rho_bc;
+    const C us_b = usq_of<Q>(ubx, (C)0, (C)0), us_g = usq_of<Q>(vx, vy, vz);
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      const C eb = feq_i<Q, i>(rb, ubx, (C)0, (C)0, us_b);
+      f[i] = nb ? g[i] + (eb - feq_i<Q, i>(rn, vx, vy, vz, us_g)) : eb;
+    });
+    return;
+  }
+  const C usq = usq_of<Q>(vx, vy, vz);
+  const C dr = rho_b - rn;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if (nb) {
+      const C e = feq_i<Q, i>((C)1, vx, vy, vz, usq);
+      if constexpr (sizeof(C) == 4) f[i] = __builtin_fmaf(dr, e, g[i]);
+      else f[i] = g[i] + dr * e;
+    } else {
+      f[i] = feq_i<Q, i>(rho_b, vx, vy, vz, usq);
+    }
+  });
+}
+
+template <int Q, class C, class SrcType, class Own, class West, class WestType, class East, class EastType,
+          class P>
 __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f,
                                                 SrcType &&src_type, Own &&own, West &&west,
-                                                WestType &&west_type) {
+                                                WestType &&west_type, East &&east, EastType &&east_type) {
   const C rw = (C)p.rho_bc;
   if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
     wall_links<Q>(p, f, src_type, own);
-  if (t & NODE_INLET) {
+  if ((t & NODE_INLET) && p.inlet_closure == 1) {
+    // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
+    const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
+    extrapolate<Q>(p, f, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
+  } else if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = (C)-0.0, sm = (C)-0.0;  // sums from -0, as in moments()
     static_for<Q>([&](auto I) {
@@ -295,26 +340,9 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   } else if (t & NODE_OUTLET) {
     // zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi 2002), all i:
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
-    // post-collision populations; the node then holds rho_bc and u_nb exactly.  f_eq is linear in
-    // rho, so this is evaluated as g_i + (rho_bc - rho_nb) f_eq,i(1, u_nb).
-    C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
+    // post-collision populations; the node then holds rho_bc and u_nb exactly.
     const bool nb = x > 0 && !(west_type() & NODE_SOLID);
-    if (nb) {
-      static_for<Q>([&](auto I) { g[decltype(I)::value] = west(I); });
-      moments<Q>(g, rn, vx, vy, vz);
-    }
-    const C usq = usq_of<Q>(vx, vy, vz);
-    const C dr = rw - rn;
-    static_for<Q>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      if (nb) {
-        const C e = feq_i<Q, i>((C)1, vx, vy, vz, usq);
-        if constexpr (sizeof(C) == 4) f[i] = __builtin_fmaf(dr, e, g[i]);
-        else f[i] = g[i] + dr * e;
-      } else {
-        f[i] = feq_i<Q, i>(rw, vx, vy, vz, usq);
-      }
-    });
+    extrapolate<Q>(p, f, nb, rw, false, (C)0, west);
   }
 }
 
@@ -349,7 +377,12 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
         constexpr int j = decltype(J)::value;
         return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n - 1);
       },
-      [&]() -> uint8_t { return p.type[q.n - 1]; });
+      [&]() -> uint8_t { return p.type[q.n - 1]; },
+      [&](auto J) {
+        constexpr int j = decltype(J)::value;
+        return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n + 1);
+      },
+      [&]() -> uint8_t { return p.type[q.n + 1]; });
 }
 
 // ---- moments, overrides, collisions ------------------------------------------------------

@@ -16,6 +16,7 @@ struct StepParams {
   int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile
   int32_t ntiles;
   int32_t inlet_profile;
+  int32_t inlet_closure;    // 0 Zou-He (non-equilibrium bounce-back), 1 extrapolated (Guo et al.)
   int32_t meq_mode;
   int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
   int64_t N;                // nodes

@@ -92,7 +92,8 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   if constexpr (LIST) {
     // Each thread annotates the list entries it updates in every sub-step (so no barrier): bits
     // 0-10 the position, 11-18 solid pull sources i = 1..8 (bit 10 + i), 19-26 moving-wall ones
-    // (bit 18 + i), 27 the west neighbour is solid, 28 inlet, 29 outlet, 30 wall neighbour.
+    // (bit 18 + i), 27 the west neighbour is solid, 28 inlet, 29 outlet, 30 wall neighbour, 31 the
+    // east neighbour is solid.
     static_assert(RN <= 2048, "list entries hold the position in 11 bits");
     for (int e = (int)threadIdx.x; e < ns; e += NT) {
       const int pos = slow_list[e];
@@ -107,6 +108,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
           if (ts & NODE_SOLID) w |= (1u << (10 + q)) | ((ts & NODE_MOVING) ? 1u << (18 + q) : 0u);
         });
         if (lt[pos - 1] & NODE_SOLID) w |= 1u << 27;
+        if (lt[pos + 1] & NODE_SOLID) w |= 1u << 31;
       }
       slow_list[e] = (int)w;
     }
@@ -144,7 +146,9 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             },
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos]; },
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos - 1]; },
-            [&]() -> uint8_t { return lt[pos - 1]; });
+            [&]() -> uint8_t { return lt[pos - 1]; },
+            [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
+            [&]() -> uint8_t { return lt[pos + 1]; });
       }
       moments<Q>(f, rho, ux, uy, uz);
       collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
@@ -211,7 +215,9 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             },
             [&](auto J) -> C { return own[decltype(J)::value]; },
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos - 1]; },
-            [&]() -> uint8_t { return (w >> 27) & 1u ? NODE_SOLID : 0; });
+            [&]() -> uint8_t { return (w >> 27) & 1u ? NODE_SOLID : 0; },
+            [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
+            [&]() -> uint8_t { return (w >> 31) & 1u ? NODE_SOLID : 0; });
         moments<Q>(f, rho, ux, uy, uz);
         collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
         if constexpr (s == K) write_out(i, j, out);

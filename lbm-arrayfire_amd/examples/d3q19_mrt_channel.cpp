@@ -3,7 +3,7 @@
 // What the reference's work-in-progress program cpp/d3q19_mrt_channel.cpp sets out to run (an MRT
 // channel with a zero-gradient outlet, :9-10,299-301), on the library's D3Q19 lattice:
 // - geometry: walls on y = 0, ny - 1 and z = 0, nz - 1, a sphere of radius r at
-//   (nx/4, ny/2, nz/2), a uniform velocity inlet (non-equilibrium bounce-back) on x = 0 and a
+//   (nx/4, ny/2, nz/2), a uniform velocity inlet (non-equilibrium extrapolation) on x = 0 and a
 //   zero-gradient outlet at rho = 1 on x = nx - 1;
 // - collision: MRT in d'Humieres et al.'s (2002) orthogonal basis, s_nu = omega on the stress
 //   rows, their values on the others (1.19, 1.4, 1.2, 1.4) except s_m = 1.2 on the ghost rows
@@ -77,6 +77,7 @@ int main(int argc, char **argv) {
   for (int k = 0; k < 27; ++k) d.s[k] = k < 19 ? s19[k] : 1.0;
   d.u_in = u_in;
   d.inlet_profile = LBM_INLET_UNIFORM;
+  d.inlet_closure = LBM_INLET_EXTRAPOLATED;
   d.rho_bc = 1.0;
   std::printf("D3Q19 MRT channel %dx%dx%d, sphere r %d at (%d, %d, %d), Re %.1f, omega %.6f, %s\n", nx, ny, nz, r, cx,
               cy, cz, re, omega, sto.c_str());

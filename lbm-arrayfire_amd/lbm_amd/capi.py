@@ -20,6 +20,7 @@ BC_REFERENCE, BC_PHYSICAL = 0, 1
 MRT_AUTO, MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_BF16X3, MRT_MFMA_FUSED = 0, 1, 2, 3, 4
 MEQ_REFERENCE, MEQ_PROJECTED = 0, 1
 INLET_UNIFORM, INLET_PARABOLIC = 0, 1
+INLET_ZOU_HE, INLET_EXTRAPOLATED = 0, 1
 NODE_FLUID, NODE_SOLID, NODE_MOVING, NODE_INLET, NODE_OUTLET, NODE_RHO_FIX = 0, 1, 2, 4, 8, 16
 
 STATUS = {0: "LBM_OK", 1: "LBM_EINVAL", 2: "LBM_ENOMEM", 3: "LBM_EDEVICE", 4: "LBM_EUNSTABLE",
@@ -58,7 +59,8 @@ class LbmDesc(ctypes.Structure):
         ("lattice", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("nz", ctypes.c_int32), ("collision", ctypes.c_int32), ("storage", ctypes.c_int32),
         ("bc_mode", ctypes.c_int32), ("mrt_engine", ctypes.c_int32), ("meq_mode", ctypes.c_int32),
-        ("inlet_profile", ctypes.c_int32), ("omega", ctypes.c_double),
+        ("inlet_profile", ctypes.c_int32), ("inlet_closure", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("omega", ctypes.c_double),
         ("s", ctypes.c_double * 27), ("u_wall", ctypes.c_double * 3), ("u_in", ctypes.c_double),
         ("rho_bc", ctypes.c_double),
     ]
@@ -103,6 +105,7 @@ def make_desc(cfg):
     d.mrt_engine = cfg.get("mrt_engine", MRT_AUTO)
     d.meq_mode = cfg.get("meq_mode", MEQ_REFERENCE)
     d.inlet_profile = cfg.get("inlet_profile", INLET_UNIFORM)
+    d.inlet_closure = cfg.get("inlet_closure", INLET_ZOU_HE)
     d.omega = cfg.get("omega", 1.0)
     s = cfg.get("s", [1.0] * 27)
     for k in range(27):

@@ -5,7 +5,7 @@ omega follows the reference's unit convention nu = (tau - 1/2)/3, omega = 1/tau
 import numpy as np
 
 from . import geometry as g
-from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q19, D3Q27, F32, INLET_PARABOLIC,
+from .capi import (BC_PHYSICAL, BC_REFERENCE, BGK, D2Q9, D3Q19, D3Q27, F32, INLET_EXTRAPOLATED, INLET_PARABOLIC,
                    INLET_UNIFORM, MEQ_REFERENCE, MRT_AUTO, MRT_GS, MRT_RAW)
 
 
@@ -154,12 +154,14 @@ def reference_channel_start(types, u_in=0.1):
 
 def d3q27_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
                          storage=F32, collision=BGK):
-    """Duct with a sphere: uniform Zou-He inlet, zero-gradient outlet; Re = u * D / nu, D = 2r."""
+    """Duct with a sphere: uniform velocity inlet closed by non-equilibrium extrapolation (the
+    Zou-He closure destabilises this duct at tau = 0.536), zero-gradient outlet; Re = u * D / nu,
+    D = 2r."""
     nu = u_in * 2 * r / re
     om = omega_from_nu(nu)
     cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
                bc_mode=BC_PHYSICAL, omega=om, s=shear_mrt_rates(om), u_in=u_in,
-               inlet_profile=INLET_UNIFORM)
+               inlet_profile=INLET_UNIFORM, inlet_closure=INLET_EXTRAPOLATED)
     return cfg, g.channel_3d_physical(nx, ny, nz, center[0], center[1], center[2], r)
 
 
@@ -192,14 +194,15 @@ def d3q27_mrt_channel_suga(nx=320, ny=80, nz=80, u_in=0.1, storage=F32, collisio
     71-121, 160-167; named D3Q19 but run on the 27-velocity set): a duct nx x ny x nz with walls on
     y = 0, ny - 1, z = 0, nz - 1, a sphere of radius ny / 9 at (nx / 4, ny / 2 + ny / 30,
     nz / 2 + nz / 30), inlet speed 0.1 and Suga's rates (s_nu = 1.2, nu = 1/9, Re = u * 2r / nu).
-    The program's inlet and outlet are unfinished; here they are the Zou-He uniform inlet and the
-    zero-gradient outlet of BC_PHYSICAL, and with MRT_GS the equilibrium moments are M f_eq (the
+    The program's inlet and outlet are unfinished; here they are the uniform velocity inlet
+    (closed by non-equilibrium extrapolation) and the zero-gradient outlet of BC_PHYSICAL, and with MRT_GS the equilibrium moments are M f_eq (the
     program's singular moment matrix, which never sets row 4, is the raw matrix with row 4 = xy).
     collision MRT_GS relaxes Suga's groups in the orthogonal basis, MRT_RAW in the raw one."""
     cx, cy, cz, r = nx // 4, ny // 2 + ny // 30, nz // 2 + nz // 30, ny // 9
     s = suga_rates()
     cfg = dict(lattice=D3Q27, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
-               bc_mode=BC_PHYSICAL, omega=s[4], s=s, u_in=u_in, inlet_profile=INLET_UNIFORM)
+               bc_mode=BC_PHYSICAL, omega=s[4], s=s, u_in=u_in, inlet_profile=INLET_UNIFORM,
+               inlet_closure=INLET_EXTRAPOLATED)
     return cfg, g.channel_3d_physical(nx, ny, nz, cx, cy, cz, r)
 
 
@@ -215,14 +218,14 @@ def d3q19_lid(n=64, u_lid=0.05, re=100.0, storage=F32, collision=BGK, bc_mode=BC
 
 def d3q19_sphere_channel(nx=512, ny=256, nz=256, center=(128, 128, 128), r=24, u_in=0.05, re=200.0,
                          storage=F32, collision=BGK):
-    """D3Q19 duct with a sphere: uniform Zou-He inlet, zero-gradient outlet; Re = u * D / nu.
-    MRT rates: d'Humieres et al.'s except s_m = 1.2 on the ghost rows mx, my, mz.  Their 1.98 lets
-    the inlet/outlet closures drive those modes unstable in a duct (the oracle, FP64, 64 x 24 x 24,
-    r 4, Re 10: divergence between steps 50 and 100 with s_m = 1.98; stable 400 steps with
-    s_m <= 1.5, and with any of the other rates changed instead it still diverges)."""
+    """D3Q19 duct with a sphere: uniform velocity inlet (non-equilibrium extrapolation),
+    zero-gradient outlet; Re = u * D / nu.
+    MRT rates: d'Humieres et al.'s except s_m = 1.2 on the ghost rows mx, my, mz.  With the Zou-He
+    inlet their 1.98 drives those modes unstable in a duct (the oracle, FP64, 40 x 20 x 20, r 4,
+    Re 10: divergence within 100 steps); with the extrapolated inlet used here both stay bounded."""
     nu = u_in * 2 * r / re
     om = omega_from_nu(nu)
     cfg = dict(lattice=D3Q19, nx=nx, ny=ny, nz=nz, collision=collision, storage=storage,
                bc_mode=BC_PHYSICAL, omega=om, s=dhumieres_rates(om, s_m=1.2), u_in=u_in,
-               inlet_profile=INLET_UNIFORM)
+               inlet_profile=INLET_UNIFORM, inlet_closure=INLET_EXTRAPOLATED)
     return cfg, g.channel_3d_physical(nx, ny, nz, center[0], center[1], center[2], r)

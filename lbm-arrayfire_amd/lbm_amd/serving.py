@@ -17,7 +17,7 @@ import time
 import numpy as np
 
 from . import cases, geometry as geo
-from .capi import BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, INLET_UNIFORM, MRT_GS, Lattice, Server
+from .capi import BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, INLET_EXTRAPOLATED, INLET_UNIFORM, MRT_GS, Lattice, Server
 
 NU_MIN = 1.0 / 150.0
 U_LAT = 0.05
@@ -51,7 +51,7 @@ def make_job(rng, step_scale=1.0, sides="mixed", storage=F16S):
             types = geo.lid_cavity_3d_physical(side, side, side)
         else:
             r = max(1, int(round(rng.uniform(0.05, 0.15) * side)))
-            cfg.update(u_in=U_LAT, inlet_profile=INLET_UNIFORM)
+            cfg.update(u_in=U_LAT, inlet_profile=INLET_UNIFORM, inlet_closure=INLET_EXTRAPOLATED)
             types = geo.channel_3d_physical(side, side, side, side // 4, side // 2, side // 2, r)
     else:
         cfg = dict(lattice=D2Q9, nx=side, ny=side, nz=1, collision=BGK, storage=storage,

@@ -23,9 +23,10 @@
 //                       on 27 velocities): D3Q27's rest, face and edge directions, weights
 //                       1/3, 1/18, 1/36; MRT in the orthogonal basis of d'Humieres et al. (2002)
 //                       with projected equilibrium moments; initial state f_eq(rho0, 0)
-//  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, Zou-He
-//                       (non-equilibrium bounce-back) velocity inlet, zero-gradient outlet at a
-//                       reference density (non-equilibrium extrapolation of every population).
+//  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, velocity
+//                       inlet by Zou-He (non-equilibrium bounce-back) or by non-equilibrium
+//                       extrapolation, zero-gradient outlet at a reference density
+//                       (non-equilibrium extrapolation of every population).
 //  * Shan-Chen          cpp/d2q9_shanchen.cpp:39-181 with its work-in-progress defects corrected
 //                       (oracle_sc_run); pinned in tests/test_shanchen.py by a numpy restatement,
 //                       the single-fluid reduction at G = 0, conservation, a point-perturbation
@@ -55,7 +56,8 @@
 
 extern "C" {
 typedef struct {
-  int32_t lattice, nx, ny, nz, collision, storage, bc_mode, mrt_engine, meq_mode, inlet_profile;
+  int32_t lattice, nx, ny, nz, collision, storage, bc_mode, mrt_engine, meq_mode, inlet_profile, inlet_closure,
+      reserved;
   double omega;
   double s[27];
   double u_wall[3];
@@ -493,7 +495,25 @@ struct Sim {
         f[i] = Fin[(size_t)(src + N * i)];
       }
     }
-    if (t & INLET) {  // velocity inlet, unknowns c_x = +1
+    if ((t & INLET) && d.inlet_closure == 1) {
+      // Extrapolated velocity inlet (Guo, Zheng & Shi, Chinese Physics 11 (2002) 366): every
+      // population f_eq(rho_e, u_in) + (g - f_eq(rho_e, u_e)), g = the downstream (x + 1)
+      // neighbour's populations as the pull reads them, rho_e and u_e their moments; f_eq(rho_bc,
+      // u_in) when that neighbour is solid or absent.  The node then holds rho_e and u_in.
+      const T uin = inlet_u(y, z);
+      T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
+      const bool nb = x + 1 < nx && !(type[n + 1] & SOLID);
+      if (nb) {
+        for (int i = 0; i < L.Q; ++i) g[i] = Fin[(size_t)(n + 1 + N * i)];
+        moments(g, rn, vx, vy, vz);
+      }
+      const T rb = nb ? rn : rw;
+      const T us_b = usq_term(uin, (T)0, (T)0), us_g = usq_term(vx, vy, vz);
+      for (int i = 0; i < L.Q; ++i) {
+        const T eb = feq(i, rb, uin, (T)0, (T)0, us_b);
+        f[i] = nb ? g[i] + (eb - feq(i, rn, vx, vy, vz, us_g)) : eb;
+      }
+    } else if (t & INLET) {  // velocity inlet (Zou-He), unknowns c_x = +1
       const T uin = inlet_u(y, z);
       T s0 = 0, sm = 0;
       for (int i = 0; i < L.Q; ++i) {

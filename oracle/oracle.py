@@ -19,7 +19,8 @@ class OracleDesc(ctypes.Structure):
         ("lattice", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
         ("nz", ctypes.c_int32), ("collision", ctypes.c_int32), ("storage", ctypes.c_int32),
         ("bc_mode", ctypes.c_int32), ("mrt_engine", ctypes.c_int32), ("meq_mode", ctypes.c_int32),
-        ("inlet_profile", ctypes.c_int32), ("omega", ctypes.c_double),
+        ("inlet_profile", ctypes.c_int32), ("inlet_closure", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("omega", ctypes.c_double),
         ("s", ctypes.c_double * 27), ("u_wall", ctypes.c_double * 3), ("u_in", ctypes.c_double),
         ("rho_bc", ctypes.c_double),
     ]
@@ -92,6 +93,7 @@ def make_desc(cfg):
     d.mrt_engine = cfg.get("mrt_engine", 0)
     d.meq_mode = cfg.get("meq_mode", 0)
     d.inlet_profile = cfg.get("inlet_profile", 0)
+    d.inlet_closure = cfg.get("inlet_closure", 0)
     d.omega = cfg.get("omega", 1.0)
     s = cfg.get("s", [1.0] * 27)
     for k in range(27):

@@ -180,7 +180,23 @@ def step_physical(F, types, cfg):
                 new = f[opp[i]] + 6 * w[i] * rho * cu - 18 * w[i] * (c[i][1] * Ny + c[i][2] * Nz)
                 f[i] = np.where(sel, new, f[i])
 
-    if inl.any():
+    if inl.any() and cfg.get("inlet_closure", 0) == 1:
+        # f_eq(rho_e, u_in) + (g - f_eq(rho_e, u_e)): g the downstream neighbour's previous
+        # post-collision populations, rho_e / u_e their moments (f_eq(rho_bc, u_in) without one)
+        uin = inlet_profile(cfg, types.shape)
+        Fp = np.roll(F, shift=-1, axis=3)
+        rp = Fp.sum(0)
+        up = np.einsum("ia,i...->a...", c.astype(float), Fp) / np.where(rp == 0, 1.0, rp)
+        nb_ok = ~np.roll(solid, shift=-1, axis=2)
+        nb_ok[..., -1] = False
+        rb = np.where(nb_ok, rp, rw)
+        ub = np.stack([uin, 0 * uin, 0 * uin])
+        fb = feq(rb, ub, c, w)
+        fg = feq(rp, up, c, w)
+        for i in range(Q):
+            new = np.where(nb_ok, Fp[i] + (fb[i] - fg[i]), fb[i])
+            f[i] = np.where(inl, new, f[i])
+    elif inl.any():
         uin = inlet_profile(cfg, types.shape)
         neg = [i for i in range(Q) if c[i][0] == -1]
         rho_in = (sum(f[i] for i in zero) + 2 * sum(f[i] for i in neg)) / (1 - uin)

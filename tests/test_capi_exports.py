@@ -25,13 +25,14 @@ def test_header_matches_binding_list():
 def test_library_exports_every_declared_symbol(lbm_lib):
     missing = [s for s in declared_functions() if not hasattr(lbm_lib, s)]
     assert not missing, missing
-    assert lbm_lib.lbm_abi_version() == 1
+    assert lbm_lib.lbm_abi_version() == 2
 
 
 def test_desc_layout_matches_header():
-    # 10 int32 fields, then 1 + 27 + 3 + 1 + 1 doubles
-    assert ctypes.sizeof(capi.LbmDesc) == 40 + 8 * 33
-    assert capi.LbmDesc.omega.offset == 40
+    # 12 int32 fields, then 1 + 27 + 3 + 1 + 1 doubles
+    assert ctypes.sizeof(capi.LbmDesc) == 48 + 8 * 33
+    assert capi.LbmDesc.omega.offset == 48
+    assert capi.LbmDesc.inlet_closure.offset == 40
 
 
 @pytest.mark.parametrize("bad", [
@@ -42,6 +43,7 @@ def test_desc_layout_matches_header():
     dict(bc_mode=3),
     dict(nx=0),
     dict(omega=float("nan")),
+    dict(inlet_closure=2),
     dict(storage=2, nx=65536, ny=32768),  # F16S: 2^31 nodes
     dict(storage=3, nx=65536, ny=32768),  # BF16S: 2^31 nodes
 ])

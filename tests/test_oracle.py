@@ -164,6 +164,13 @@ PHYS_CASES = {
     "d3q19_sphere_mrt": lambda: (dict(cases.d3q19_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64,
                                                                   collision=MRT_GS)[0]),
                                  geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+    # both inlet closures on every lattice
+    "d2q9_karman_extrapolated_inlet": lambda: (dict(cases.d2q9_karman(24, 12, 7, 6, 2, storage=F64)[0],
+                                                    inlet_closure=1), geo.channel_2d_physical(24, 12, 7, 6, 2)),
+    "d3q27_sphere_zou_he_inlet": lambda: (dict(cases.d3q27_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64)[0],
+                                               inlet_closure=0), geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
+    "d3q19_sphere_zou_he_inlet": lambda: (dict(cases.d3q19_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64)[0],
+                                               inlet_closure=0), geo.channel_3d_physical(12, 7, 7, 4, 3, 3, 1)),
 }
 
 
@@ -349,6 +356,54 @@ def test_zou_he_inlet_recovers_prescribed_velocity():
     assert np.abs(uy[inlet]).max() < 1e-14
 
 
+@pytest.mark.parametrize("Q", [9, 27])
+def test_extrapolated_inlet_holds_prescribed_velocity_and_downstream_density(Q):
+    """The extrapolated inlet node's moments are exactly u_in and the density of its downstream
+    neighbour's previous populations (the non-equilibrium part carries neither mass nor
+    momentum); a wall-adjacent inlet node included."""
+    if Q == 9:
+        cfg, types = cases.d2q9_karman(32, 14, 10, 7, 2, storage=F64)
+    else:
+        cfg, types = cases.d3q27_sphere_channel(16, 9, 9, (6, 4, 4), 1, storage=F64)
+    cfg = dict(cfg, inlet_closure=1)
+    types = types.reshape(-1)
+    N = types.size
+    rng = np.random.default_rng(13)
+    u = np.zeros((3, N))
+    u[0] = 0.05 + 0.01 * rng.standard_normal(N)
+    u[1:3] = 0.01 * rng.standard_normal((2, N))
+    f = orc.run(cfg, types, orc.init_fields(cfg, types, 1 + 0.01 * rng.standard_normal(N), *u), 3)
+    rho, ux, uy, uz = orc.fields(cfg, types, f)
+    inlet = np.flatnonzero((types & 4) != 0)
+    assert inlet.size > 0
+    down = f.reshape(Q, N)[:, inlet + 1].sum(0)
+    nx, ny = cfg["nx"], cfg["ny"]
+    y = (inlet // nx) % ny
+    if Q == 9:
+        H = ny - 2
+        expect = 4 * 0.1 * (y - 0.5) * (H - (y - 0.5)) / H ** 2
+    else:
+        expect = np.full(inlet.size, cfg["u_in"])
+    assert np.abs(rho[inlet] - down).max() < 1e-14
+    assert np.abs(ux[inlet] - expect).max() < 1e-14
+    assert np.abs(uy[inlet]).max() < 1e-14 and np.abs(uz[inlet]).max() < 1e-14
+
+
+def test_extrapolated_inlet_keeps_sphere_duct_stable():
+    """The 3D duct that the Zou-He inlet destabilises: 80 x 40 x 40, sphere r = 4, BGK at
+    tau = 0.536 (u = 0.05, Re 33).  With the extrapolated inlet it stays bounded for 2,000 steps;
+    the velocity never exceeds twice the inlet speed."""
+    cfg, types = cases.d3q27_sphere_channel(80, 40, 40, (20, 20, 20), 4, re=33.333333333333336, storage=F32)
+    assert abs(1 / cfg["omega"] - 0.536) < 1e-9 and cfg["inlet_closure"] == 1
+    types = types.reshape(-1)
+    N = types.size
+    f = orc.init_fields(cfg, types, None, np.full(N, 0.05) * ((types & 1) == 0))
+    f = orc.run(cfg, types, f, 2000)
+    rho, ux, uy, uz = orc.fields(cfg, types, f)
+    sp = np.sqrt(ux ** 2 + uy ** 2 + uz ** 2)
+    assert np.isfinite(sp).all() and sp.max() < 0.1
+
+
 def test_poiseuille_profile_develops():
     """Empty channel, parabolic velocity inlet, zero-gradient outlet at rho_bc, half-way walls:
     the flow becomes steady with a parabolic cross-section (walls at y = 0.5 and ny - 1.5), the
@@ -453,19 +508,25 @@ def test_d3q19_mrt_conserves_mass_and_momentum():
 
 def test_d3q19_mrt_duct_rates_are_stable():
     """The D3Q19 MRT duct case (uniform velocity inlet, zero-gradient outlet, sphere) stays bounded
-    with its rates (s_m = 1.2 on the ghost rows); with d'Humieres et al.'s s_m = 1.98 the same
-    duct diverges within 100 steps."""
+    with its rates (s_m = 1.2 on the ghost rows) under both inlet closures; d'Humieres et al.'s
+    s_m = 1.98 diverges within 100 steps with the Zou-He inlet and stays bounded for 1,000 steps
+    with the extrapolated one."""
     from lbm_amd.capi import MRT_GS
     cfg, t = cases.d3q19_sphere_channel(40, 20, 20, (10, 10, 10), 4, re=10.0, storage=F64, collision=MRT_GS)
     t = t.reshape(-1)
     N = t.size
     f0 = orc.init_fields(cfg, t, np.ones(N), np.full(N, 0.05), np.zeros(N), np.zeros(N))
     fluid = (t & 1) == 0  # the fields are 0 at solid nodes
-    rho = orc.fields(cfg, t, orc.run(cfg, t, f0, 300))[0][fluid]
-    assert np.isfinite(rho).all() and np.abs(rho - 1).max() < 0.3
-    bad = dict(cfg, s=cases.dhumieres_rates(cfg["omega"]))
-    rho_bad = orc.fields(bad, t, orc.run(bad, t, f0, 100))[0][fluid]
-    assert not (np.isfinite(rho_bad).all() and np.abs(rho_bad - 1).max() < 0.3)
+
+    def bounded(c, steps):
+        rho = orc.fields(c, t, orc.run(c, t, f0, steps))[0][fluid]
+        return np.isfinite(rho).all() and np.abs(rho - 1).max() < 0.3
+
+    for closure in (0, 1):
+        assert bounded(dict(cfg, inlet_closure=closure), 300)
+    dh = cases.dhumieres_rates(cfg["omega"])
+    assert not bounded(dict(cfg, s=dh, inlet_closure=0), 100)
+    assert bounded(dict(cfg, s=dh, inlet_closure=1), 1000)
 
 
 def test_suga_channel_case_has_the_programs_setup():
