@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 
 def workloads():
     from lbm_amd import cases
-    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
     W = {
         "d2q9_bgk_karman_1024x256_fp32": lambda: cases.d2q9_karman(1024, 256, 256, 128, 20, storage=F32),
@@ -49,6 +49,8 @@ def workloads():
         "d3q27_mrt_gs_lid_256_f16s_mfma2": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_MFMA_F32),
         "d3q27_mrt_gs_lid_256_f16s_fused": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_MFMA_FUSED),
         "d3q27_mrt_gs_lid_256_f16s_scalar": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_SCALAR),
+        "d3q27_mrt_gs_lid_256_fp32_bf16x3": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F32, engine=MRT_MFMA_BF16X3),
+        "d3q27_mrt_gs_lid_256_f16s_bf16x3": lambda: cases.d3q27_mrt_lid_gs(256, re=400.0, storage=F16S, engine=MRT_MFMA_BF16X3),
         "d3q27_bgk_sphere_512x256x256_fp32": lambda: cases.d3q27_sphere_channel(storage=F32),
         "d3q27_bgk_sphere_512x256x256_f16s": lambda: cases.d3q27_sphere_channel(storage=F16S),
         "d3q27_bgk_sphere_512x256x256_bf16s": lambda: cases.d3q27_sphere_channel(storage=BF16S),
@@ -344,7 +346,7 @@ def accuracy_run(kind, steps=None):
     import numpy as np
 
     from lbm_amd import Lattice, cases
-    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+    from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
     def errs(a, b):
         ua, ub = np.stack(a[1:4]), np.stack(b[1:4])
@@ -360,7 +362,8 @@ def accuracy_run(kind, steps=None):
     else:
         steps = steps or 20000
         runs = [("fp64", F64, MRT_SCALAR), ("fp32_scalar", F32, MRT_SCALAR), ("fp32_mfma2", F32, MRT_MFMA_F32),
-                ("fp32_fused", F32, MRT_MFMA_FUSED), ("f16s_scalar", F16S, MRT_SCALAR),
+                ("fp32_fused", F32, MRT_MFMA_FUSED), ("fp32_bf16x3", F32, MRT_MFMA_BF16X3),
+                ("f16s_scalar", F16S, MRT_SCALAR), ("f16s_bf16x3", F16S, MRT_MFMA_BF16X3),
                 ("bf16s_scalar", BF16S, MRT_SCALAR)]
     ref = None
     for tag, st, en in runs:

@@ -70,14 +70,20 @@ typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_stora
  * counterpart). */
 typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
-/* How the MRT moment transforms are evaluated. */
+/* How the MRT moment transforms are evaluated.  The MFMA engines evaluate the collision in
+ * correction form, f' = f - M^-1 S M g with g = f - f_eq(rho, u) (equal to the reference's
+ * f' = M^-1 (m - S (m - meq)), rust/examples/d3q27_mrt_lid.rs:336,385-387, because meq = M f_eq;
+ * the reference table's row-26 difference is carried along), so the matrix units see only the
+ * small non-equilibrium part. */
 typedef enum {
   LBM_MRT_AUTO = 0,         /* SCALAR: the fastest measured engine on gfx950 for every storage */
   LBM_MRT_SCALAR = 1,       /* per-node unrolled FMA code with compile-time matrices */
-  LBM_MRT_MFMA_F32 = 2,     /* m = M f and f = M^-1 m* on v_mfma_f32_16x16x4_f32, 64-node tiles */
-  LBM_MRT_MFMA_BF16X3 = 3,  /* reserved: transforms on bf16 MFMA (not available in this build) */
-  LBM_MRT_MFMA_FUSED = 4    /* whole collision as one FP32 MFMA contraction with the operator
-                               [M^-1 (I - S) M | M^-1 S E] precomputed on the host */
+  LBM_MRT_MFMA_F32 = 2,     /* d = M g and c = M^-1 S d on v_mfma_f32_16x16x4_f32, 64-node tiles */
+  LBM_MRT_MFMA_BF16X3 = 3,  /* the same transforms on v_mfma_f32_16x16x32_bf16: g and S d split into
+                               three bfloat16 terms each, M and 24 M^-1 (or M^T) exact in bfloat16,
+                               FP32 accumulation */
+  LBM_MRT_MFMA_FUSED = 4    /* c = M^-1 S M g as one FP32 MFMA contraction with the operator
+                               precomputed on the host */
 } lbm_mrt_engine;
 
 /* Equilibrium moments for MRT_RAW: REFERENCE = the reference's hand-written table

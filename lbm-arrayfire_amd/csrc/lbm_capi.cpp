@@ -72,11 +72,11 @@ std::string check_desc(const lbm_desc *d) {
     return "F16S / BF16S storage holds at most 2^31 - 1 nodes";
   if (d->bc_mode < 0 || d->bc_mode > 1) return "unknown boundary mode";
   if (d->mrt_engine < 0 || d->mrt_engine > 4) return "unknown MRT engine";
-  if (d->mrt_engine == LBM_MRT_MFMA_BF16X3) return "the BF16x3 MFMA engine is not available in this build";
-  if ((d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED) && d->storage == LBM_F64)
-    return "the FP32 MFMA engines need FP32 arithmetic (storage F32, F16S or BF16S)";
-  if ((d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED) && d->lattice != 27)
-    return "the MFMA engines are implemented for D3Q27";
+  const bool mfma = d->mrt_engine == LBM_MRT_MFMA_F32 || d->mrt_engine == LBM_MRT_MFMA_FUSED ||
+                    d->mrt_engine == LBM_MRT_MFMA_BF16X3;
+  if (mfma && d->storage == LBM_F64)
+    return "the MFMA engines need FP32 arithmetic (storage F32, F16S or BF16S)";
+  if (mfma && d->lattice != 27) return "the MFMA engines are implemented for D3Q27";
   if (d->meq_mode < 0 || d->meq_mode > 1) return "unknown equilibrium-moment mode";
   if (d->inlet_profile < 0 || d->inlet_profile > 1) return "unknown inlet profile";
   if (d->inlet_closure < 0 || d->inlet_closure > 1) return "unknown inlet closure";
@@ -219,6 +219,7 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
   p.inlet_profile = d->inlet_profile;
   p.inlet_closure = d->inlet_closure;
+  p.meq_fix26 = d->collision == LBM_MRT_RAW && d->meq_mode == LBM_MEQ_REFERENCE ? 1 : 0;
   p.meq_mode = d->meq_mode;
   // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
   // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
@@ -307,7 +308,7 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   p.type = h->types;
   if (d->collision != LBM_BGK && d->lattice == 27) {
     std::vector<float> tab(64 * MFMA_TAB);
-    build_mfma_table(d->collision, d->meq_mode, d->s, h->cfg.engine == MRT_ENGINE_FUSED, tab.data());
+    if (uses_mfma(h->cfg)) build_mfma_table(d->collision, d->s, h->cfg.engine, tab.data());
     if (dmalloc((void **)&h->mrt_tab, tab.size() * sizeof(float)) != hipSuccess)
       return cleanup(LBM_ENOMEM, "MRT table allocation failed");
     if (hipMemcpy(h->mrt_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)

@@ -17,6 +17,7 @@ struct StepParams {
   int32_t ntiles;
   int32_t inlet_profile;
   int32_t inlet_closure;    // 0 Zou-He (non-equilibrium bounce-back), 1 extrapolated (Guo et al.)
+  int32_t meq_fix26;        // MFMA MRT: raw basis with the reference's equilibrium table (row 26)
   int32_t meq_mode;
   int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
   int64_t N;                // nodes
@@ -39,7 +40,7 @@ struct Config {
 // Tile shape of the node-per-thread kernels: one wavefront per 64-node row segment,
 // four rows per 256-thread workgroup.
 constexpr int TILE_X = 64, TILE_Y = 4, BLOCK_THREADS = 256;
-// Per-lane constants of the MFMA MRT kernel: M (2x8), s.E (2x4), 1 - s (2x4), M^-1 (2x8).
+// Per-lane constants of the MFMA MRT kernels (at most 40 floats used; see build_mfma_table).
 constexpr int MFMA_TAB = 48;
 
 // Population arrays are laid out with a stride of round_up(N * s, 4 KiB) + 4 KiB + 256 B bytes, so
@@ -99,11 +100,11 @@ enum { MRT_ENGINE_AUTO = 0, MRT_ENGINE_SCALAR = 1, MRT_ENGINE_MFMA = 2, MRT_ENGI
        MRT_ENGINE_AUTO_FP32 = MRT_ENGINE_SCALAR };
 inline bool uses_mfma(const Config &c) {
   return c.Q == 27 && c.collision != CO_BGK && c.storage != ST_F64 &&
-         (c.engine == MRT_ENGINE_MFMA || c.engine == MRT_ENGINE_FUSED);
+         (c.engine == MRT_ENGINE_MFMA || c.engine == MRT_ENGINE_FUSED || c.engine == MRT_ENGINE_BF16X3);
 }
-// Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice (two-stage
-// transform layout, or the fused collision operator when fused is set).
-void build_mfma_table(int collision, int meq_mode, const double *s, bool fused, float *tab);
+// Fills 64 * MFMA_TAB floats of per-lane MFMA operands for a D3Q27 MRT lattice and the given
+// engine (MRT_ENGINE_MFMA, MRT_ENGINE_FUSED or MRT_ENGINE_BF16X3).
+void build_mfma_table(int collision, const double *s, int engine, float *tab);
 // Reference-style fields (rho, u after the boundary overrides) of the state p.fin.
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);

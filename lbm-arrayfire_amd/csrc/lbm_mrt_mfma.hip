@@ -1,47 +1,79 @@
 // lbm_mrt_mfma.hip — D3Q27 MRT step with the moment transforms on MFMA (gfx950).
 //
 // The reference evaluates the MRT collision as three N x 27 x 27 GEMMs through ArrayFire's BLAS
-// (rust/examples/d3q27_mrt_lid.rs:336 m = f M^T, :385 (m - meq) S^T, :387 f = m* M^-T).  Here one
-// wavefront owns 64 nodes:
-//   phase 1  node per lane: pull + boundary rules + moments + overrides (same code as the scalar
-//            kernel), then the 27 pulled populations and the 13 equilibrium monomials
-//            phi = (rho, rho u_a, rho u_a^2, rho u_a u_b, u_a^2) are staged in LDS, one row of
-//            MT_STRIDE floats per node;
-//   phase 2  per 16-node tile, on v_mfma_f32_16x16x4_f32 (rows padded 27 -> 32):
-//              m  = M f                    2 row tiles x 8 K-steps   (16 MFMA)
-//              m *= (1 - s)                per-row lane constants
-//              m += (s . E) phi            2 x 4 K-steps             (8 MFMA)  meq = E phi
-//              f' = M^-1 m                 2 x 8 K-steps             (16 MFMA)
-//            The K order of the inverse is permuted so that the accumulator of the forward
-//            transform (lane l holds rows 4(l/16)+r of node l%16) is directly the B operand of
-//            the inverse: no data movement between the two transforms;
-//   phase 3  f' goes back through LDS to node-per-lane order and is stored like the scalar path
-//            (full-way bounce-back at solid nodes in REFERENCE mode).
-// Per-lane A operands (M, s.E, M^-1 entries and the 1 - s scales, 48 floats) come from a table
-// the host builds for the lattice's basis and rates (build_mfma_table).
+// (rust/examples/d3q27_mrt_lid.rs:336 m = f M^T, :385 (m - meq) S^T, :387 f = m* M^-T).  The
+// engines here evaluate the same map in correction form,
+//   f' = f - M^-1 S (M f - meq) = f - M^-1 S M g,   g = f - f_eq(rho, u),
+// which holds because meq = M f_eq (projected equilibrium moments); for the reference's own table
+// (raw basis, meq_mode REFERENCE) row 26 differs and its difference delta26 = (M f_eq)_26 -
+// meq_ref_26 is added to that row.  The matrix units see only g, the small non-equilibrium part,
+// so their accumulation errors scale with |g| rather than with |f|.  The conserved rows (rho, j)
+// do not come from M g: the FP32 f_eq values do not sum to rho exactly, and relaxing that offset
+// every step drifts the mass (1.4e-4 over 20,000 steps of the 128^3 lid).  Their deviations
+// (rho_f - rho, j_f - rho u, with rho_f, j_f the moments of f and rho, u after the overrides:
+// exactly 0 where no override applies) are computed on the vector unit and enter through the four
+// spare K slots 28..31 (unit columns of the forward operator, or the fused operator's columns).  This matters: FP32 MFMA
+// accumulation does not round to nearest (measured on MI355X: 57 % of 16x16x4 results equal the
+// correctly rounded value, mean error -0.2 ulp), and an engine that forms f' = M^-1 m* from the
+// full moments drifts by that bias every step (1.4e-4 in rho over 20,000 steps of the 128^3 lid).
 //
-// The fused engine folds the three stages into one operator on the host,
-//   f' = M^-1 (I - S) M f + M^-1 S E phi = [C | D] [f; phi],
-// and runs it as a single K = 48 contraction per tile (24 MFMA instead of 40).
+// One wavefront owns 64 nodes:
+//   phase 1  node per lane: pull + boundary rules + moments + overrides (the scalar kernel's
+//            code), then g, delta26 and the conserved deviations are staged in LDS, one row of ROW
+//            floats per node (slots 0..26 g, 27 delta26, 28..31 the deviations of rho, jx, jy,
+//            jz); f stays in registers;
+//   phase 2  per 16-node tile, the lane group gq = lane / 16 supplies the populations (K slots)
+//            8 gq .. 8 gq + 7 of node lane % 16; the accumulator of the first transform (lane l
+//            holds rows 4 (l / 16) + r and 16 + 4 (l / 16) + r of node l % 16) is directly the
+//            B operand of the second (its K order is permuted to match), so nothing moves
+//            between the transforms:
+//              MFMA_F32   d = M g on v_mfma_f32_16x16x4_f32 (2 row tiles x 8 K-steps),
+//                         r = (s / n) d (+ delta26), c = (n M^-1) r (16 MFMA);
+//              FUSED      c = [M^-1 S' M | M^-1 s_26 e_26 | M^-1 S_c] [g; delta26; dev] (16 MFMA),
+//                         S' the rates with the conserved rows' set to 0, S_c those rows';
+//              BF16X3     as MFMA_F32 on v_mfma_f32_16x16x32_bf16: g and r are split into three
+//                         bfloat16 terms each (hi + mid + lo carry 24 significant bits), M and
+//                         n M^-1 (integers up to 24: the raw basis' 24 M^-1, the Gram-Schmidt
+//                         M^T) are exact in bfloat16, products are exact in FP32;
+//            n is 24 (raw basis) or the squared row norms (Gram-Schmidt, M^-1 = M^T / n);
+//   phase 3  c goes back through LDS to node-per-lane order, f' = f - c is stored like the scalar
+//            path (full-way bounce-back at solid nodes in REFERENCE mode).
+// Per-lane A operands and rates come from a table the host builds for the lattice's basis and
+// rates (build_mfma_table).
 #include "lbm_device.hpp"
 
 namespace lbm {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// LDS row of one node.  Two-stage engine: f[0..26], 0, phi[0..12] at 28..40, pad to 44.
-// Fused engine: f[0..26], phi[0..12] at 27..39, zeros to 52 (K = 48 slots, 12 per lane group).
-template <bool FUSED> constexpr int mt_stride() { return FUSED ? 52 : 44; }
-template <bool FUSED> constexpr int phi_at() { return FUSED ? 27 : 28; }
-constexpr int MT_STRIDE_MAX = 52;
+// LDS row of one node: g[0..26], delta26, the conserved rows' deviations; 36 floats so that the 16
+// nodes a b128 read of a lane group touches spread over the banks.
+constexpr int ROW = 36;
 
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
-template <int ST, int BC, bool FUSED, class Params>
-__device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (*lds)[64 * MT_STRIDE_MAX]) {
-  constexpr int MT_STRIDE = mt_stride<FUSED>();
+// x = hi + mid + lo, each term a bfloat16 rounded to nearest from what the previous ones leave
+__device__ __forceinline__ void split3(const float *x, bf16x8 &hi, bf16x8 &mid, bf16x8 &lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const __bf16 h = (__bf16)x[j];
+    const float r1 = x[j] - (float)h;
+    const __bf16 m = (__bf16)r1;
+    const float r2 = r1 - (float)m;
+    hi[j] = h;
+    mid[j] = m;
+    lo[j] = (__bf16)r2;
+  }
+}
+
+template <int ST, int BC, int ENG, class Params>
+__device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (*lds)[64 * ROW]) {
   using CD = Codec<ST, 27>;
   using C = typename CD::C;
   static_assert(sizeof(C) == 4, "MFMA path computes in FP32");
@@ -53,106 +85,146 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
   const NodePos q = node_of_tile(p, tile);
   const uint8_t t = q.valid ? p.type[q.n] : (uint8_t)NODE_SOLID;
   const bool active = q.valid && !(BC == BC_PHYS && (t & NODE_SOLID));
+  C f[27];
   {
-    C f[27], rho = 0, ux = 0, uy = 0, uz = 0;
+    C rho = 0, ux = 0, uy = 0, uz = 0;
     if (q.valid) {
       if constexpr (BC == BC_REF) pull_periodic<27, ST>(p, q, f);
       else pull_physical<27, ST>(p, q, t, f);
     }
+    float row[ROW];
+    float dev[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
       moments<27>(f, rho, ux, uy, uz);
+      const C r0 = rho, x0 = ux, y0 = uy, z0 = uz;
       if constexpr (BC == BC_REF) overrides<27>(p, t, q.y, q.z, rho, ux, uy, uz);
+      // conserved rows: rho_f - rho and rho_f u_f - rho u, exactly 0 without an override
+      dev[0] = r0 - rho;
+      dev[1] = r0 * x0 - rho * ux;
+      dev[2] = r0 * y0 - rho * uy;
+      dev[3] = r0 * z0 - rho * uz;
+      const C usq = usq_of<27>(ux, uy, uz);
+      static_for<27>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        row[i] = f[i] - feq_i<27, i>(rho, ux, uy, uz, usq);
+      });
+      float d26 = 0.f;
+      if (p.meq_fix26) {  // the reference table's row 26 against M f_eq
+        const float phi[10] = {1.0f, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+        float acc = 0.0f;
+        static_for<10>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          constexpr float cf = (float)MEQ_RAW[26][j];
+          if constexpr (cf != 0.0f) acc = __builtin_fmaf(cf, phi[j], acc);
+        });
+        float ref[27];
+        meq_reference<float>(rho, ux, uy, uz, ref);
+        d26 = rho * acc - ref[26];
+      }
+      row[27] = d26;
     } else {
-      static_for<27>([&](auto I) { f[decltype(I)::value] = 0; });
+      static_for<27>([&](auto I) {
+        f[decltype(I)::value] = 0;
+        row[decltype(I)::value] = 0.f;
+      });
+      row[27] = 0.f;
     }
-    float row[MT_STRIDE];
-    static_for<27>([&](auto I) { row[decltype(I)::value] = f[decltype(I)::value]; });
-    constexpr int P = phi_at<FUSED>();
-    const float phi[13] = {rho, rho * ux, rho * uy, rho * uz, rho * ux * ux, rho * uy * uy, rho * uz * uz,
-                           rho * ux * uy, rho * ux * uz, rho * uy * uz, ux * ux, uy * uy, uz * uz};
 #pragma unroll
-    for (int k = 27; k < MT_STRIDE; ++k) row[k] = 0.f;
+    for (int k = 0; k < 4; ++k) row[28 + k] = dev[k];
 #pragma unroll
-    for (int k = 0; k < 13; ++k) row[P + k] = phi[k];
-    f32x4 *dst = (f32x4 *)(L + lane * MT_STRIDE);
+    for (int k = 32; k < ROW; ++k) row[k] = 0.f;
+    f32x4 *dst = (f32x4 *)(L + lane * ROW);
 #pragma unroll
-    for (int k = 0; k < MT_STRIDE / 4; ++k)
-      dst[k] = f32x4{row[4 * k], row[4 * k + 1], row[4 * k + 2], row[4 * k + 3]};
+    for (int k = 0; k < ROW / 4; ++k) dst[k] = f32x4{row[4 * k], row[4 * k + 1], row[4 * k + 2], row[4 * k + 3]};
   }
-  // lane constants: two-stage AF[2][8], AP[2][4], SC[2][4], AI[2][8]; fused AC[2][12]
-  f32x4 tab[12];
+  // per-lane constants (see build_mfma_table)
+  f32x4 tab[10];
   {
     const f32x4 *src = (const f32x4 *)(p.mrt_tab + lane * MFMA_TAB);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) tab[k] = src[k];
+    for (int k = 0; k < 10; ++k) tab[k] = src[k];
   }
   __syncthreads();
 
   // ---- phase 2: 4 tiles of 16 nodes on MFMA ----------------------------------------------
-  if constexpr (FUSED) {
-    // f' = [M^-1 (I - S) M | M^-1 S E] [f; phi]: one K = 48 contraction, 2 row tiles x 12 steps
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float *row = L + (16 * j + c) * MT_STRIDE;
-      const f32x4 b0 = *(const f32x4 *)(row + 12 * g);
-      const f32x4 b1 = *(const f32x4 *)(row + 12 * g + 4);
-      const f32x4 b2 = *(const f32x4 *)(row + 12 * g + 8);
-      f32x4 o0 = {0, 0, 0, 0}, o1 = {0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s < 12; ++s) {
-        const float b = s < 4 ? b0[s & 3] : (s < 8 ? b1[s & 3] : b2[s & 3]);
-        o0 = mfma4(tab[s >> 2][s & 3], b, o0);        // AC[0][s]
-        o1 = mfma4(tab[3 + (s >> 2)][s & 3], b, o1);  // AC[1][s]
-      }
-      *(f32x4 *)(row + 4 * g) = o0;
-      *(f32x4 *)(row + 16 + 4 * g) = o1;
-    }
-  } else {
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    float *row = L + (16 * j + c) * MT_STRIDE;
-    const f32x4 g0 = *(const f32x4 *)(row + 8 * g);
-    const f32x4 g1 = *(const f32x4 *)(row + 8 * g + 4);
-    const f32x4 ph = *(const f32x4 *)(row + 28 + 4 * g);
-    f32x4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const float b = s < 4 ? g0[s & 3] : g1[s & 3];
-      a0 = mfma4(tab[s >> 2][s & 3], b, a0);          // AF[0][s]
-      a1 = mfma4(tab[2 + (s >> 2)][s & 3], b, a1);    // AF[1][s]
-    }
-    a0 *= tab[6];  // SC[0][r] = 1 - s_row
-    a1 *= tab[7];  // SC[1][r]
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      a0 = mfma4(tab[4][s], ph[s], a0);  // AP[0][s] = s_row E[row][4g+s]
-      a1 = mfma4(tab[5][s], ph[s], a1);  // AP[1][s]
-    }
+    float *row = L + (16 * j + c) * ROW;
+    const f32x4 b0 = *(const f32x4 *)(row + 8 * g);      // g of populations 8 gq .. 8 gq + 3
+    const f32x4 b1 = *(const f32x4 *)(row + 8 * g + 4);  // and 8 gq + 4 .. 8 gq + 7
     f32x4 o0 = {0, 0, 0, 0}, o1 = {0, 0, 0, 0};
+    if constexpr (ENG == MRT_ENGINE_FUSED) {
+      // c = C [g; delta26; dev]: A = the fused operator, K slot (t, gq) = population 8 gq + t
+      // (slot 27 delta26, slots 28..31 the conserved deviations)
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const float b = s < 4 ? a0[s & 3] : a1[s & 3];  // m*[16(s/4) + 4g + s%4][node]
-      o0 = mfma4(tab[8 + (s >> 2)][s & 3], b, o0);    // AI[0][s]
-      o1 = mfma4(tab[10 + (s >> 2)][s & 3], b, o1);   // AI[1][s]
+      for (int s = 0; s < 8; ++s) {
+        const float b = s < 4 ? b0[s & 3] : b1[s & 3];
+        o0 = mfma4(tab[s >> 2][s & 3], b, o0);        // AC[0][s]
+        o1 = mfma4(tab[2 + (s >> 2)][s & 3], b, o1);  // AC[1][s]
+      }
+    } else {
+      const float d26 = row[27];
+      f32x4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+      if constexpr (ENG == MRT_ENGINE_BF16X3) {
+        // tab[0..1] forward A (R = 0, 1), tab[2..3] inverse A, as packed bfloat16 x 8
+        const float x[8] = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+        bf16x8 hi, mid, lo;
+        split3(x, hi, mid, lo);
+        const bf16x8 af0 = __builtin_bit_cast(bf16x8, tab[0]), af1 = __builtin_bit_cast(bf16x8, tab[1]);
+        a0 = mfma_bf16(af0, lo, a0);  // smallest terms first
+        a1 = mfma_bf16(af1, lo, a1);
+        a0 = mfma_bf16(af0, mid, a0);
+        a1 = mfma_bf16(af1, mid, a1);
+        a0 = mfma_bf16(af0, hi, a0);
+        a1 = mfma_bf16(af1, hi, a1);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const float b = s < 4 ? b0[s & 3] : b1[s & 3];
+          a0 = mfma4(tab[s >> 2][s & 3], b, a0);        // AF[0][s]
+          a1 = mfma4(tab[2 + (s >> 2)][s & 3], b, a1);  // AF[1][s]
+        }
+      }
+      // rows 4 gq + r and 16 + 4 gq + r of node c: row 26 is (R = 1, gq = 2, r = 2)
+      if (g == 2) a1[2] += d26;
+      const f32x4 rate0 = tab[ENG == MRT_ENGINE_BF16X3 ? 4 : 8], rate1 = tab[ENG == MRT_ENGINE_BF16X3 ? 5 : 9];
+      a0 *= rate0;
+      a1 *= rate1;
+      if constexpr (ENG == MRT_ENGINE_BF16X3) {
+        const float x[8] = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+        bf16x8 hi, mid, lo;
+        split3(x, hi, mid, lo);
+        const bf16x8 ai0 = __builtin_bit_cast(bf16x8, tab[2]), ai1 = __builtin_bit_cast(bf16x8, tab[3]);
+        o0 = mfma_bf16(ai0, lo, o0);
+        o1 = mfma_bf16(ai1, lo, o1);
+        o0 = mfma_bf16(ai0, mid, o0);
+        o1 = mfma_bf16(ai1, mid, o1);
+        o0 = mfma_bf16(ai0, hi, o0);
+        o1 = mfma_bf16(ai1, hi, o1);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const float b = s < 4 ? a0[s & 3] : a1[s & 3];  // r[16 (s / 4) + 4 gq + s % 4][node]
+          o0 = mfma4(tab[4 + (s >> 2)][s & 3], b, o0);   // AI[0][s]
+          o1 = mfma4(tab[6 + (s >> 2)][s & 3], b, o1);   // AI[1][s]
+        }
+      }
     }
-    *(f32x4 *)(row + 4 * g) = o0;       // f'[4g + r]
-    *(f32x4 *)(row + 16 + 4 * g) = o1;  // f'[16 + 4g + r]
-  }
+    *(f32x4 *)(row + 4 * g) = o0;       // c[4 gq + r]
+    *(f32x4 *)(row + 16 + 4 * g) = o1;  // c[16 + 4 gq + r]
   }
   __syncthreads();
 
-  // ---- phase 3: node per lane again, store ----------------------------------------------
+  // ---- phase 3: node per lane again, f' = f - c, store ------------------------------------
   C out[27];
   {
-    const float *src = L + lane * MT_STRIDE;
-    static_for<27>([&](auto I) { out[decltype(I)::value] = src[decltype(I)::value]; });
+    const float *src = L + lane * ROW;
+    static_for<27>([&](auto I) { out[decltype(I)::value] = f[decltype(I)::value] - src[decltype(I)::value]; });
   }
   __syncthreads();  // the LDS rows are free for the next tile of a persistent workgroup
   if (!active) return;
   if constexpr (BC == BC_REF) {
     if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK, as the scalar kernel
-      C f[27];
-      pull_periodic<27, ST>(p, q, f);
       static_for<26>([&](auto J) {
         constexpr int jj = decltype(J)::value + 1;
         out[Lat<27>::opp[jj]] = f[jj];
@@ -162,23 +234,28 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
   store_all<27, ST>(p, q.n, out);
 }
 
-template <int ST, int BC, bool FUSED>
+template <int ST, int BC, int ENG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
-  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE_MAX];
-  mrt_mfma_tile<ST, BC, FUSED>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * ROW];
+  mrt_mfma_tile<ST, BC, ENG>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
 }
 
-template <int ST, int BC, bool FUSED>
+template <int ST, int BC, int ENG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch_mrt_mfma(BatchArgs b) {
-  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * MT_STRIDE_MAX];
-  for_batch_tiles(b, [&](const auto &p, int tile) { mrt_mfma_tile<ST, BC, FUSED>(p, tile, lds); });
+  __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * ROW];
+  for_batch_tiles(b, [&](const auto &p, int tile) { mrt_mfma_tile<ST, BC, ENG>(p, tile, lds); });
 }
 
 template <class F>
 static hipError_t with_mfma_variant(const Config &c, F &&f) {
   auto byST = [&](auto STc) -> hipError_t {
     auto byBC = [&](auto BCc) -> hipError_t {
-      return c.engine == MRT_ENGINE_FUSED ? f(STc, BCc, std::true_type{}) : f(STc, BCc, std::false_type{});
+      switch (c.engine) {
+        case MRT_ENGINE_MFMA: return f(STc, BCc, IC<MRT_ENGINE_MFMA>{});
+        case MRT_ENGINE_FUSED: return f(STc, BCc, IC<MRT_ENGINE_FUSED>{});
+        case MRT_ENGINE_BF16X3: return f(STc, BCc, IC<MRT_ENGINE_BF16X3>{});
+      }
+      return hipErrorInvalidValue;
     };
     return c.bc == BC_REF ? byBC(IC<BC_REF>{}) : byBC(IC<BC_PHYS>{});
   };
@@ -191,16 +268,16 @@ static hipError_t with_mfma_variant(const Config &c, F &&f) {
 }
 
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
-  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
-    hipLaunchKernelGGL((k_step_batch_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),
+  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Ec) -> hipError_t {
+    hipLaunchKernelGGL((k_step_batch_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Ec)::value>),
                        dim3(grid), dim3(BLOCK_THREADS), 0, s, b);
     return hipGetLastError();
   });
 }
 
 hipError_t launch_step_mrt_mfma(const Config &c, const StepParams &p, hipStream_t s) {
-  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Fc) -> hipError_t {
-    hipLaunchKernelGGL((k_step_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Fc)::value>),
+  return with_mfma_variant(c, [&](auto STc, auto BCc, auto Ec) -> hipError_t {
+    hipLaunchKernelGGL((k_step_mrt_mfma<decltype(STc)::value, decltype(BCc)::value, decltype(Ec)::value>),
                        dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
     return hipGetLastError();
   });

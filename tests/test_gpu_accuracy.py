@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from lbm_amd import Lattice, cases
-from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
 pytestmark = pytest.mark.gpu
 
@@ -39,15 +39,16 @@ def test_d3q27_mrt_lid_128_re400_20000_steps():
     n, steps = 128, 20000
     runs = {}
     for tag, st, en in (("fp64", F64, MRT_SCALAR), ("fp32_scalar", F32, MRT_SCALAR), ("fp32_mfma2", F32, MRT_MFMA_F32),
-                        ("fp32_fused", F32, MRT_MFMA_FUSED), ("f16s_scalar", F16S, MRT_SCALAR)):
+                        ("fp32_fused", F32, MRT_MFMA_FUSED), ("fp32_bf16x3", F32, MRT_MFMA_BF16X3),
+                        ("f16s_scalar", F16S, MRT_SCALAR)):
         cfg, types = cases.d3q27_mrt_lid_gs(n, re=400.0, storage=st, engine=en)
         runs[tag] = run_fields(cfg, types, steps, lambda lat: lat.init_fields())
     errs = {k: errors(v, runs["fp64"]) for k, v in runs.items() if k != "fp64"}
     for k, (er, eu) in errs.items():
         print(f"128^3 MRT-GS lid Re 400, {steps} steps, {k} vs FP64: rho {er:.2e}, u {eu:.2e}")
     assert max(errs["fp32_scalar"]) <= 5e-5
-    for k in ("fp32_mfma2", "fp32_fused"):
-        assert max(errs[k]) <= 1e-3, k
+    for k in ("fp32_mfma2", "fp32_fused", "fp32_bf16x3"):
+        assert max(errs[k]) <= 1e-4, k
     # F16S: each step rounds every population to 2^-11 of its deviation from w_i (about 4e-5 of f
     # where u is 0.05), and at tau = 0.548 these errors add up over 20,000 steps to a few percent of
     # u; the configuration asks for FP32, the bound only records the level

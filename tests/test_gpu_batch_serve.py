@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from lbm_amd import Batch, Lattice, LbmError, Server, cases, geometry as geo
-from lbm_amd.capi import F16S, F32, F64, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
 
 pytestmark = pytest.mark.gpu
 
@@ -22,6 +22,8 @@ def member_specs():
     cfg, t = cases.d3q27_mrt_lid_gs(24, re=100.0, storage=F16S, engine=MRT_MFMA_F32)  # two-stage MFMA
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d3q27_mrt_lid_gs(20, re=100.0, storage=F16S)  # the scalar engine (AUTO)
+    specs.append((cfg, t, ("ref",)))
+    cfg, t = cases.d3q27_mrt_lid_reference(18, storage=BF16S, engine=MRT_MFMA_BF16X3)  # BF16x3, raw basis
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d3q27_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F32)
     specs.append((cfg, t, ("eq", (0.05, 0, 0), 9, 1e-4)))

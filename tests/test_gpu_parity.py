@@ -9,7 +9,8 @@ import pytest
 
 import oracle as orc
 from lbm_amd import Lattice, cases, geometry as geo
-from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_RAW, MRT_SCALAR
+from lbm_amd.capi import (BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_RAW,
+                          MRT_SCALAR)
 
 pytestmark = pytest.mark.gpu
 
@@ -103,6 +104,8 @@ SHIFTED = [
                                                                          engine=MRT_MFMA_F32), 600, 1e-3, 5e-3),
     ("d3q27_mrt_gs_lid_ref_24_fused", lambda st: cases.d3q27_mrt_lid_gs(24, re=100.0, storage=st,
                                                                          engine=MRT_MFMA_FUSED), 600, 1e-3, 5e-3),
+    ("d3q27_mrt_gs_lid_ref_24_bf16x3", lambda st: cases.d3q27_mrt_lid_gs(24, re=100.0, storage=st,
+                                                                          engine=MRT_MFMA_BF16X3), 600, 1e-3, 5e-3),
     ("d2q9_channel_phys_256x64", lambda st: cases.d2q9_karman(256, 64, 64, 32, 6, re=20.0, storage=st), 3000,
      2e-3, 5e-2),
     ("d3q19_mrt_lid_ref_24", lambda st: cases.d3q19_lid(24, re=100.0, storage=st, collision=MRT_GS), 600,
@@ -164,7 +167,8 @@ MRT_SHIFTED = [
 ]
 
 
-@pytest.mark.parametrize("engine", [MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_FUSED], ids=["scalar", "mfma2", "fused"])
+@pytest.mark.parametrize("engine", [MRT_SCALAR, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_MFMA_BF16X3],
+                         ids=["scalar", "mfma2", "fused", "bf16x3"])
 @pytest.mark.parametrize("storage", [F16S, BF16S], ids=["f16s", "bf16s"])
 @pytest.mark.parametrize("name,make,steps", MRT_SHIFTED, ids=[p[0] for p in MRT_SHIFTED])
 def test_mrt_shifted_storage_matches_oracle_emulation(name, make, steps, storage, engine):
@@ -308,11 +312,10 @@ MFMA_CASES = [
 
 @pytest.mark.parametrize("name,make,steps", MFMA_CASES, ids=[p[0] for p in MFMA_CASES])
 def test_mrt_mfma_against_fp64(name, make, steps):
-    """FP32 MFMA moment transforms (two-stage and fused operator) and the FP32 scalar-FMA variant,
-    each against the FP64 oracle."""
-    from lbm_amd.capi import MRT_MFMA_F32, MRT_MFMA_FUSED
+    """The MFMA engines (two-stage FP32, fused-operator FP32, BF16x3 split) and the FP32
+    scalar-FMA variant, each against the FP64 oracle."""
     errs = {}
-    for en in (MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR):
+    for en in (MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_MFMA_BF16X3, MRT_SCALAR):
         cfg, types = make(F32, en)
         cfg = dict(cfg, mrt_engine=en)
         types = np.ascontiguousarray(types, dtype=np.uint8).reshape(-1)
@@ -325,7 +328,7 @@ def test_mrt_mfma_against_fp64(name, make, steps):
         fo = orc.run(cfg64, types, orc.init_fields(cfg64, types, rho, *u), steps)
         errs[en] = rel_l2(fg, fo)
     print(f"{name}: rel-L2 vs FP64 oracle: MFMA {errs[MRT_MFMA_F32]:.2e}, fused MFMA "
-          f"{errs[MRT_MFMA_FUSED]:.2e}, scalar {errs[MRT_SCALAR]:.2e}")
+          f"{errs[MRT_MFMA_FUSED]:.2e}, BF16x3 {errs[MRT_MFMA_BF16X3]:.2e}, scalar {errs[MRT_SCALAR]:.2e}")
     assert max(errs.values()) <= 1e-5
 
 
