@@ -227,6 +227,7 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) ? 1 : 0;
   if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;
   if (d->lattice == 9) h->cfg.nt = 0;
+  p.stream_order = h->cfg.nt;
   p.N = h->N;
   p.stride = h->stride;
   p.omega = d->omega;

@@ -18,6 +18,7 @@ struct StepParams {
   int32_t inlet_profile;
   int32_t inlet_closure;    // 0 Zou-He (non-equilibrium bounce-back), 1 extrapolated (Guo et al.)
   int32_t meq_fix26;        // MFMA MRT: raw basis with the reference's equilibrium table (row 26)
+  int32_t stream_order;     // MFMA MRT: tiles in stream_tile order (lattices too large for the caches)
   int32_t meq_mode;
   int32_t wrap_cut;         // bit 0 / 1: nothing crosses the periodic x / y wrap (PHYSICAL, 2D)
   int64_t N;                // nodes

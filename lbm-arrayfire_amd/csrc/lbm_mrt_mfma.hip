@@ -237,7 +237,8 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
 template <int ST, int BC, int ENG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_mrt_mfma(StepParams p) {
   __shared__ __attribute__((aligned(16))) float lds[BLOCK_THREADS / 64][64 * ROW];
-  mrt_mfma_tile<ST, BC, ENG>(p, xcd_tile((int)blockIdx.x, p.ntiles), lds);
+  mrt_mfma_tile<ST, BC, ENG>(p, p.stream_order ? stream_tile((int)blockIdx.x, p.gx) : xcd_tile((int)blockIdx.x, p.ntiles),
+                             lds);
 }
 
 template <int ST, int BC, int ENG>

@@ -58,10 +58,11 @@ __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, dou
   out[3] = (double)uz;
 }
 
-// NT: populations written with non-temporal stores (lattices too large for the caches)
+// NT: populations written with non-temporal stores (lattices too large for the caches), whose
+// tiles run in the skewed natural order (stream_tile).
 template <int Q, int ST, int BC, int CO, bool NT>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
-  step_tile<Q, ST, BC, CO, NT>(p, xcd_tile((int)blockIdx.x, p.ntiles));
+  step_tile<Q, ST, BC, CO, NT>(p, NT ? stream_tile((int)blockIdx.x, p.gx) : xcd_tile((int)blockIdx.x, p.ntiles));
 }
 
 // Persistent batched step: every member of one configuration group, one launch.
