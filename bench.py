@@ -531,7 +531,12 @@ def main():
                      "steps_per_launch": r["steps_per_launch"], "avg_launch_us": round(r["launch_us"], 3),
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"],
                      "attainable_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
-                     "frac_of_attainable_copy": None if not copy_gbs else round(r["achieved_gbs"] / copy_gbs, 4)},
+                     "frac_of_attainable_copy": None if not copy_gbs else round(r["achieved_gbs"] / copy_gbs, 4),
+                     **({"pull_ceiling_gbs": ceilings.get("pull_4byte_nt_gbs" if cfg["storage"] == 1 else "pull_2byte_nt_gbs"),
+                         "frac_of_pull_ceiling": round(r["achieved_gbs"] / ceilings.get(
+                             "pull_4byte_nt_gbs" if cfg["storage"] == 1 else "pull_2byte_nt_gbs"), 4),
+                         "device_copy_gbs": ceilings.get("copy_nt_gbs")}
+                        if ceilings and cfg["lattice"] == 27 and cfg["storage"] in (1, 2, 3) and r["nt"] else {})},
         "regions": {"mlups": [round(v, 1) for v in r["region_mlups"]], "value_is": "median of the regions"},
         "cpu_baseline": cpu,
         "wall_s_timed": round(r["wall_s"], 4),
