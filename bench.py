@@ -262,7 +262,7 @@ def _rounded(d):
     return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
 
 
-def serving_leg(njobs, scale, sides="mixed", accuracy_every=10):
+def serving_leg(njobs, scale, sides="mixed", accuracy_every=10, modes=("realtime", "saturated", "sequential")):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
     continuous-batching server, in real time and saturated, against the same jobs run one at a
@@ -275,14 +275,17 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10):
 
     jobs = serving.make_mix(njobs, step_scale=scale, sides=sides)
     budget = 64 << 30
-    rt = serving.serve(jobs, budget, realtime=True)
-    sat = serving.serve(jobs, budget, realtime=False)
-    seq = serving.sequential(jobs)
+    prog = 100 if njobs >= 500 else 0
+    rt = serving.serve(jobs, budget, realtime=True, progress=prog) if "realtime" in modes else None
+    sat = serving.serve(jobs, budget, realtime=False, progress=prog) if "saturated" in modes else None
+    seq = serving.sequential(jobs, progress=prog) if "sequential" in modes else None
+    served = sat or rt
     out = {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
-           "storage": "F16S", "realtime": _rounded(rt), "saturated": _rounded(sat),
-           "one_at_a_time": _rounded(seq),
-           "throughput_vs_one_at_a_time": round(sat["mlups"] / seq["mlups"], 3),
-           "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)"}
+           "storage": "F16S", "realtime": rt and _rounded(rt), "saturated": sat and _rounded(sat),
+           "one_at_a_time": seq and _rounded(seq),
+           "throughput_vs_one_at_a_time": round(served["mlups"] / seq["mlups"], 3) if served and seq else None,
+           "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)",
+           "total_lattice_updates": sum(j["nodes"] * j["steps"] for j in jobs)}
     if accuracy_every:
         sample = jobs[::accuracy_every]
         acc = serving.serve(sample, budget, realtime=False, keep=range(len(sample)))
@@ -292,6 +295,8 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10):
                 continue
             e = serving.fp64_errors(j, acc["fields"][k])
             errs.append(e)
+            if prog:
+                print(f"accuracy: job {k + 1}/{len(sample)} rel-L2 rho {e[0]:.2e} u {e[1]:.2e}", file=sys.stderr, flush=True)
             if worst is None or e[1] > worst["rel_l2_u"]:
                 c = j["cfg"]
                 worst = {"job": f"{'D2Q9 BGK' if c['lattice'] == 9 else 'D3Q27 MRT-GS'} "
@@ -431,6 +436,11 @@ def main():
     ap.add_argument("--also-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) leg")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the serving leg of the mixed mix and print it (with --serve-jobs, "
+                         "--serve-scale, --serve-modes)")
+    ap.add_argument("--serve-modes", default="realtime,saturated,sequential",
+                    help="serving runs of --serve-only: realtime, saturated, sequential")
     ap.add_argument("--accuracy", choices=["sphere", "lid128"], default=None,
                     help="only run the accuracy comparison of one configuration against FP64 and print it")
     ap.add_argument("--serve-jobs", type=int, default=120,
@@ -447,6 +457,10 @@ def main():
         return
     if args.accuracy:
         print(json.dumps(accuracy_run(args.accuracy)))
+        return
+    if args.serve_only:
+        print(json.dumps(serving_leg(args.serve_jobs, args.serve_scale, "mixed", accuracy_every=50,
+                                     modes=tuple(args.serve_modes.split(",")))))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "

@@ -178,10 +178,12 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     g.total_tiles = prefix[n];
     g.total_multi = pmulti[n];
     g.phase = 0;
+    size_t group_bytes = 0;  // both population buffers of every member
+    for (const lbm_ctx *m : g.members) group_bytes += 2 * m->buf_bytes();
     hdr->nmembers = n;
     hdr->total_tiles = g.total_tiles;
     hdr->total_multi = g.total_multi;
-    hdr->pad_ = 0;
+    hdr->stream = group_bytes > ((size_t)256 << 20) ? 1 : 0;
     hdr->prefix = (const int32_t *)(g.d_tab + g.prefix_off);
     hdr->prefix_multi = (const int32_t *)(g.d_tab + g.pmulti_off);
     hdr->members[0] = (const StepParams *)(g.d_tab + g.par_off[0]);
@@ -191,8 +193,13 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     BCHK(hipEventRecord(g.uploaded, b->stream));
     // default: one workgroup per tile, dispatched by the hardware as they free up (measured
     // faster than statically chunked persistent grids, whose chunks differ in cost: partial
-    // tiles, wall nodes, different members); the tuner may pick a persistent grid instead
-    const int grid = g.tuned > 0 ? std::min(g.tuned, std::max(g.total_tiles, 1)) : std::max(g.total_tiles, 1);
+    // tiles, wall nodes, different members); the tuner may pick a persistent grid instead.  The
+    // grid keeps room for 1/8 more tiles and shrinks only below half, so members joining and
+    // leaving mostly keep the launch shape and the captured graph (spare workgroups exit at once).
+    const int need = std::max(g.total_tiles, 1);
+    int grid = g.grid;
+    if (g.tuned > 0) grid = std::min(g.tuned, need);
+    else if (grid < need || grid > 2 * need) grid = need + need / 8 + 1;
     if (grid != g.grid) recapture = true;
     g.grid = grid;
   }

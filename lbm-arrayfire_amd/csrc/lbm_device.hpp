@@ -151,7 +151,14 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
   const_as<int32_t> prefix = (const_as<int32_t>)(MULTI ? tab->prefix_multi : tab->prefix);
   const_as<StepParams> members = (const_as<StepParams>)tab->members[b.phase];
   const int nb = (int)gridDim.x;
-  const int blk = xcd_tile((int)blockIdx.x, nb);
+  // one workgroup per tile: the grid may hold more workgroups than tiles (it is sized with some
+  // room, so membership changes need no new launch shape); the spare ones leave at once, and the
+  // tiles are dealt over the tile count so every XCD gets its share.  Groups too large for the
+  // caches take their tiles in stream order (stream_tile within each member), the others in XCD
+  // slabs.
+  const bool stream = !MULTI && tab->stream;
+  if (ONE && (int)blockIdx.x >= total) return;
+  const int blk = ONE ? (stream ? (int)blockIdx.x : xcd_tile((int)blockIdx.x, total)) : xcd_tile((int)blockIdx.x, nb);
   const int chunk = ONE ? 1 : (total + nb - 1) / nb;
   const int t0 = blk * chunk;
   const int t1 = t0 + chunk < total ? t0 + chunk : total;
@@ -163,7 +170,8 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
     else hi = mid - 1;
   }
   if constexpr (ONE) {
-    f(members[lo], t0 - prefix[lo]);
+    const int local = t0 - prefix[lo];
+    f(members[lo], stream ? stream_tile(local, members[lo].gx) : local);
     return;
   }
   int m = lo;

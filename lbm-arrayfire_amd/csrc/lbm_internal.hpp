@@ -69,7 +69,7 @@ struct BatchTable {
   int32_t nmembers;
   int32_t total_tiles;
   int32_t total_multi;           // multi-step tiles (D2Q9 BGK groups), 0 otherwise
-  int32_t pad_;
+  int32_t stream;                // members too large for the caches together: tiles in stream order
   const int32_t *prefix;         // nmembers + 1 tile offsets
   const int32_t *prefix_multi;   // nmembers + 1 multi-step tile offsets (or null)
   const StepParams *members[2];  // parameter blocks of both buffer orientations

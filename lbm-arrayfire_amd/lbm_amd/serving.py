@@ -12,6 +12,7 @@ inlets, nu = u L / Re, clamped to nu >= 1/150 (tau >= 0.52) so every job is stab
 number actually run is recorded with the job.
 """
 import math
+import sys
 import time
 
 import numpy as np
@@ -77,11 +78,11 @@ def make_mix(njobs, seed=0x5EED0005, rate=50.0, step_scale=1.0, sides="mixed", s
     return jobs
 
 
-def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True, keep=()):
+def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True, keep=(), progress=0):
     """Runs the mix through the server.  realtime: submit each job at its arrival time (wall clock,
     between ticks); otherwise submit everything at t = 0.  keep: indices of jobs whose final
-    rho/u fields are kept (returned under "fields", by job index).  Returns per-job results and
-    totals."""
+    rho/u fields are kept (returned under "fields", by job index).  progress: print a line to
+    stderr every that many finished jobs.  Returns per-job results and totals."""
     res = []
     keep = set(keep)
     fields = {}
@@ -97,7 +98,11 @@ def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True, keep=()):
                                want_fields=k in keep)] = k
                 k += 1
             active, queued = srv.tick()
+            before = len(res)
             res += srv.poll()
+            if progress and len(res) // progress > before // progress:
+                print(f"serve: {len(res)}/{len(jobs)} jobs done at {time.perf_counter() - t0:.1f} s, "
+                      f"{active} resident, {queued} queued", file=sys.stderr, flush=True)
             if active == 0 and queued == 0 and k < len(jobs):
                 time.sleep(max(0.0, min(0.002, jobs[k]["arrival_s"] - (time.perf_counter() - t0))))
         wall = time.perf_counter() - t0
@@ -131,12 +136,15 @@ def fp64_errors(job, fields, steps=None):
     return e_rho, float(np.linalg.norm(ua - ub) / max(np.linalg.norm(ub), 1e-300))
 
 
-def sequential(jobs):
+def sequential(jobs, progress=0):
     """The plain solver, one job at a time in arrival order (no idle time between jobs)."""
     t0 = time.perf_counter()
     lups = 0
     t_setup = t_run = t_end = 0.0
-    for j in jobs:
+    for k, j in enumerate(jobs):
+        if progress and k and k % progress == 0:
+            print(f"sequential: {k}/{len(jobs)} jobs done at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
         a = time.perf_counter()
         with Lattice(j["cfg"], j["types"]) as lat:
             lat.init_equilibrium(1.0, j["u0"], j["seed"], 1e-4)

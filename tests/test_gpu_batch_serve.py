@@ -416,3 +416,35 @@ def test_server_mix_matches_oracle():
                           max(e_64) > (2e-5 if cfg["storage"] == F32 else 1e-3)) else None
     print("largest rel-L2 per kind (vs oracle in the job's storage, vs FP64 oracle):", worst)
     assert not bad, bad
+
+
+def test_batch_group_larger_than_the_caches_matches_alone():
+    """Two D3Q27 lattices whose buffers together exceed 256 MiB: the group takes its tiles in
+    stream order (and a grid with spare workgroups); each member still equals its run alone, also
+    after a third member joins and one leaves."""
+    specs = [cases.d3q27_mrt_lid_gs(96, re=200.0, storage=F32), cases.d3q27_mrt_lid_gs(100, re=200.0, storage=F32),
+             cases.d3q27_mrt_lid_gs(44, re=100.0, storage=F32)]  # one group: 407 MB, later 209 MB
+    want = []
+    for (cfg, t), n in zip(specs, (12, 7, 10)):
+        with Lattice(cfg, t) as lat:
+            lat.init_reference(1.0)
+            lat.step(n)
+            want.append(lat.state())
+    lats = []
+    for cfg, t in specs:
+        lat = Lattice(cfg, t)
+        lat.init_reference(1.0)
+        lats.append(lat)
+    with Batch(lats[:2]) as b:
+        b.step(4)
+        b.add(lats[2])
+        b.step(3)
+        b.remove(lats[1])
+        b.step(5)
+        b.remove(lats[2])
+        lats[2].step(2)
+        got = [lat.state() for lat in lats]
+    for k in range(3):
+        assert np.array_equal(got[k], want[k]), k
+    for lat in lats:
+        lat.close()
