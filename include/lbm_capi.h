@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 typedef struct lbm_ctx *lbm_handle;
 typedef struct lbm_batch_ctx *lbm_batch;
@@ -211,8 +211,12 @@ lbm_status lbm_get_fields_at_noalloc(lbm_handle h, const int64_t *nodes, int32_t
                                      double *ux, double *uy, double *uz);
 /* Current post-collision populations as doubles (decoded for shifted storage), N*Q.  Blocks. */
 lbm_status lbm_get_populations(lbm_handle h, double *f);
+/* Restarts the step count of the stochastic rounding (shifted storages), as the lbm_init_* calls do. */
 lbm_status lbm_set_populations(lbm_handle h, const double *f);
-/* Raw storage-format bytes of the current populations (bitwise checkpoint / restore). */
+/* Bitwise checkpoint / restore: the raw storage-format bytes of the current populations (Q arrays
+ * of N elements), then 8 bytes holding the lattice's step count (uint32, then 4 zero bytes), which
+ * the shifted storages' stochastic rounding draws on, so a restored lattice continues exactly as
+ * the original would have. */
 size_t lbm_state_bytes(lbm_handle h);
 lbm_status lbm_get_state(lbm_handle h, void *dst);
 lbm_status lbm_set_state(lbm_handle h, const void *src);

@@ -114,7 +114,7 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
   tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
   return 2 * (size_t)population_stride((int64_t)N, storage_bytes(d->storage)) * (size_t)d->lattice *
              (size_t)storage_bytes(d->storage) + N +
-         (size_t)nt * 3 * sizeof(double) + 4 * sizeof(double);
+         (size_t)nt * 3 * sizeof(double) + RED_BYTES;
 }
 
 }  // extern "C"
@@ -177,7 +177,7 @@ std::vector<size_t> pooled_blocks(const lbm_desc *d, bool fields) {
   const size_t buf = DevPool::size_class((size_t)population_stride((int64_t)N, storage_bytes(d->storage)) *
                                          (size_t)d->lattice * (size_t)storage_bytes(d->storage));
   std::vector<size_t> b = {buf, buf, DevPool::size_class(N), DevPool::size_class((size_t)nt * 3 * sizeof(double)),
-                           DevPool::size_class(4 * sizeof(double))};
+                           DevPool::size_class(RED_BYTES)};
   if (d->collision != LBM_BGK && d->lattice == 27) b.push_back(DevPool::size_class(64 * MFMA_TAB * sizeof(float)));
   if (fields) b.push_back(DevPool::size_class(field_scratch_bytes((int64_t)N)));
   return b;
@@ -290,8 +290,10 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
       return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
   if (dmalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
   if (dmalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
-      dmalloc((void **)&h->red, 4 * sizeof(double)) != hipSuccess)
+      dmalloc((void **)&h->red, RED_BYTES) != hipSuccess)
     return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
+  if (hipMemsetAsync(h->step_ctr(), 0, 2 * sizeof(uint32_t), h->stream) != hipSuccess)
+    return cleanup(LBM_EDEVICE, "step counter initialisation failed");
   // Node types: the user's bits as given; PHYSICAL mode then marks, on the device, the non-solid
   // nodes whose pull needs the boundary path (NODE_SLOW).
   {
@@ -344,7 +346,7 @@ lbm_status lbm_destroy(lbm_handle h) {
     for (int b = 0; b < 2; ++b) dfree(h->f[b], h->buf_bytes());
     dfree(h->types, (size_t)h->N);
     dfree(h->partials, (size_t)h->base.ntiles * 3 * sizeof(double));
-    dfree(h->red, 4 * sizeof(double));
+    dfree(h->red, RED_BYTES);
     dfree(h->mrt_tab, 64 * MFMA_TAB * sizeof(float));
     dfree(h->fields, field_scratch_bytes(h->N));
     if (h->probe) (void)hipFree(h->probe);
@@ -573,7 +575,8 @@ lbm_status lbm_get_fields_at(lbm_handle h, const int64_t *nodes, int32_t count, 
   });
 }
 
-size_t lbm_state_bytes(lbm_handle h) { return h ? h->pop_bytes() : 0; }
+// the state: the populations, then the step count (uint32) and 4 zero bytes
+size_t lbm_state_bytes(lbm_handle h) { return h ? h->pop_bytes() + 8 : 0; }
 
 lbm_status lbm_get_state(lbm_handle h, void *dst) {
   return lbm::abi_guard([&]() -> lbm_status {
@@ -581,7 +584,10 @@ lbm_status lbm_get_state(lbm_handle h, void *dst) {
     HIPCHK(hipSetDevice(h->device));
     const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
     HIPCHK(hipMemcpy2DAsync(dst, row, h->f[h->cur], pitch, row, (size_t)h->Q, hipMemcpyDeviceToHost, h->strm()));
+    uint32_t tail[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(tail, h->step_ctr() + h->cur, sizeof(uint32_t), hipMemcpyDeviceToHost, h->strm()));
     HIPCHK(hipStreamSynchronize(h->strm()));
+    std::memcpy((char *)dst + h->pop_bytes(), tail, sizeof(tail));
     return LBM_OK;
   });
 }
@@ -594,6 +600,10 @@ lbm_status lbm_set_state(lbm_handle h, const void *src) {
     const size_t row = (size_t)h->N * h->esize, pitch = (size_t)h->stride * h->esize;
     for (int b = 0; b < 2; ++b)
       HIPCHK(hipMemcpy2DAsync(h->f[b], pitch, src, row, row, (size_t)h->Q, hipMemcpyHostToDevice, h->strm()));
+    uint32_t ctr[2];
+    std::memcpy(&ctr[0], (const char *)src + h->pop_bytes(), sizeof(uint32_t));
+    ctr[1] = ctr[0];
+    HIPCHK(hipMemcpyAsync(h->step_ctr(), ctr, sizeof(ctr), hipMemcpyHostToDevice, h->strm()));
     HIPCHK(hipStreamSynchronize(h->strm()));
     return LBM_OK;
   });
@@ -602,7 +612,7 @@ lbm_status lbm_set_state(lbm_handle h, const void *src) {
 lbm_status lbm_get_populations(lbm_handle h, double *f) {
   return lbm::abi_guard([&]() -> lbm_status {
     if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
-    std::vector<unsigned char> raw(h->pop_bytes());
+    std::vector<unsigned char> raw(h->pop_bytes() + 8);
     lbm_status st = lbm_get_state(h, raw.data());
     if (st != LBM_OK) return st;
     const int64_t N = h->N;
@@ -625,7 +635,7 @@ lbm_status lbm_get_populations(lbm_handle h, double *f) {
 lbm_status lbm_set_populations(lbm_handle h, const double *f) {
   return lbm::abi_guard([&]() -> lbm_status {
     if (!h || !f) return fail(LBM_EINVAL, "NULL argument");
-    std::vector<unsigned char> raw(h->pop_bytes());
+    std::vector<unsigned char> raw(h->pop_bytes() + 8, 0);
     const int64_t N = h->N;
     for (int i = 0; i < h->Q; ++i) {
       const double w = weight(h->Q, i);
@@ -639,7 +649,7 @@ lbm_status lbm_set_populations(lbm_handle h, const double *f) {
         }
       }
     }
-    return lbm_set_state(h, raw.data());
+    return lbm_set_state(h, raw.data());  // the zero tail restarts the step count
   });
 }
 

@@ -33,58 +33,124 @@ __device__ __forceinline__ void static_for(F &&f) {
 
 // ---- storage codecs ----------------------------------------------------------------------
 template <int ST, int Q> struct Codec;
-// Each codec: ld / st convert one population between storage (T) and compute (C) types; enc is
-// st's conversion alone, for the streaming stores below.
+// Each codec converts one population between storage (T) and compute (C) types:
+//   ld<I>(p, k)      the compute value of population I stored at p[k];
+//   enc<I>(v, r)     the stored form of compute value v (r: random bits, shifted codecs only);
+//   enc_f<I>(f)      the stored form of a population f given unshifted, rounded to nearest (the
+//                    initial states);
+//   st<I>(p, k, v, r)  p[k] = enc<I>(v, r).
+// shifted: the compute value is the deviation g = f - w_i, not f (the 2-byte storages).
 template <int Q> struct Codec<ST_F64, Q> {
   using T = double;
   using C = double;
+  static constexpr bool shifted = false;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
-  template <int I> __device__ static __forceinline__ T enc(C v) { return v; }
-  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = v; }
+  template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
+  template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t) { p[k] = v; }
 };
 template <int Q> struct Codec<ST_F32, Q> {
   using T = float;
   using C = float;
+  static constexpr bool shifted = false;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
-  template <int I> __device__ static __forceinline__ T enc(C v) { return v; }
-  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = v; }
+  template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
+  template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t) { p[k] = v; }
 };
-// Shifted half storage: the stored value is (f - w_i) * 2^15 rounded to nearest even.  The exact
+// Shifted half storage: the stored value is (f - w_i) * 2^15 in IEEE half.  The exact
 // power-of-two scale keeps the small deviations of the low-weight D3Q27 directions out of the
 // half-precision subnormal range, so every stored value carries 11 significant bits.
+// The step kernels compute on g = f - w_i itself (decoded exactly: h * 2^-15) and store it with
+// stochastic rounding (v_cvt_sr_f16_f32: round up with probability equal to the fraction of the
+// half ulp that g exceeds the value below, from the random bits r).  A population whose state
+// does not change from one step to the next would otherwise round the same way every step, and in
+// a steady region those frozen errors act as a source: on a 256^2 lid cavity at tau = 1 the
+// velocity drifted to 1.2e-2 of the FP64 run after 8,000 steps with rounding to nearest, against
+// 3.8e-4 (and not growing) with stochastic rounding.  Initial states are rounded to nearest.
 // The 2-byte codecs load through a 32-bit byte offset from the population's (uniform) base, so
 // the loads take the scalar-base address form and no 64-bit address arithmetic per population
 // (-53 VALU instructions per node in the D3Q27 MRT F16S step, +2 to +4 % on the F16S workloads);
 // lbm_create limits 2-byte lattices to 2^31 - 1 nodes so the offsets are exact.
 constexpr float F16S_SCALE = 32768.0f, F16S_INV_SCALE = 1.0f / 32768.0f;
-// The scale is a power of two, so h * 2^-15 + w_i as one FMA rounds exactly like the separate
-// product and sum, and so does v * 2^15 - w_i * 2^15 against (v - w_i) * 2^15.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
 template <int Q> struct Codec<ST_F16S, Q> {
   using T = uint16_t;
   using C = float;
+  static constexpr bool shifted = true;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
     const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
-    return __builtin_fmaf((float)__builtin_bit_cast(_Float16, v), F16S_INV_SCALE, (float)Lat<Q>::w[I]);
+    return (float)__builtin_bit_cast(_Float16, v) * F16S_INV_SCALE;
   }
-  template <int I> __device__ static __forceinline__ T enc(C v) {
+  template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
+    const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32((half2_t){0, 0}, v * F16S_SCALE, r, false);
+    return __builtin_bit_cast(uint16_t, h[0]);
+  }
+  // (f - w_i) * 2^15 as one FMA: the scale is a power of two, so this rounds like the separate
+  // difference and product
+  template <int I> __device__ static __forceinline__ T enc_f(C v) {
     constexpr float ws = -(float)Lat<Q>::w[I] * F16S_SCALE;
     return __builtin_bit_cast(uint16_t, (_Float16)__builtin_fmaf(v, F16S_SCALE, ws));
   }
-  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = enc<I>(v); }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t r) { p[k] = enc<I>(v, r); }
 };
-// Shifted bfloat16 storage: the stored value is f - w_i rounded to nearest even.
+// Shifted bfloat16 storage: the stored value is f - w_i (stochastically rounded by the steps,
+// v_cvt_sr_bf16_f32; to nearest in the initial states).
 template <int Q> struct Codec<ST_BF16S, Q> {
   using T = uint16_t;
   using C = float;
+  static constexpr bool shifted = true;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
     const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
-    return __builtin_bit_cast(float, (uint32_t)v << 16) + (float)Lat<Q>::w[I];
+    return __builtin_bit_cast(float, (uint32_t)v << 16);
   }
-  template <int I> __device__ static __forceinline__ T enc(C v) {
+  template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
+    const bf162_t h = __builtin_amdgcn_cvt_sr_bf16_f32((bf162_t){0, 0}, v, r, false);
+    return __builtin_bit_cast(uint16_t, h[0]);
+  }
+  template <int I> __device__ static __forceinline__ T enc_f(C v) {
     return __builtin_bit_cast(uint16_t, (__bf16)(v - (float)Lat<Q>::w[I]));
   }
-  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v) { p[k] = enc<I>(v); }
+  template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t r) { p[k] = enc<I>(v, r); }
 };
+
+// ---- random bits of the stochastic rounding ------------------------------------------------
+// Node n's rounding bits in step t (t counts the lattice's steps from its initial state):
+// mix(n ^ key(t)), key(t) = mix(t + 0x9E3779B9), mix the "lowbias32" integer hash (C. Wellons);
+// population i takes them rotated left by 7 i (mod 32).  The conversions use the top bits (13 for
+// half, 16 for bfloat16).  A function of (n, t, i) alone, so a node's result does not depend on
+// how the lattice is cut into tiles, launches or batches.
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint32_t step_key(uint32_t t) { return mix32(t + 0x9E3779B9u); }
+__device__ __forceinline__ uint32_t node_bits(uint32_t n, uint32_t key) { return mix32(n ^ key); }
+template <int I>
+__device__ __forceinline__ uint32_t pop_bits(uint32_t h) {
+  constexpr int k = (7 * I) & 31;
+  if constexpr (k == 0) return h;
+  else return (h << k) | (h >> (32 - k));
+}
+
+// The step counter of a shifted-storage lattice: read from p.step_in by every workgroup; the
+// thread that owns node 0 of the lattice's first tile writes the next value to p.step_out (the
+// other slot of the lattice's pair, the one the next launch reads).  Returns key(t).
+template <bool SH, class P>
+__device__ __forceinline__ uint32_t step_begin(const P &p, int tile, uint32_t steps) {
+  if constexpr (!SH) {
+    return 0;
+  } else {
+    const uint32_t t = *p.step_in;
+    if (tile == 0 && threadIdx.x == 0) *p.step_out = t + steps;
+    return step_key(t);
+  }
+}
 
 // ---- XCD-aware tile order ---------------------------------------------------------------
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8).  Remapping
@@ -227,12 +293,18 @@ __device__ __forceinline__ C c_times(C u) {
 
 template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz);
+template <int Q>
+__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz);
 template <int Q, int i, class C>
 __device__ __forceinline__ C cdotu(C ux, C uy, C uz);
 template <int Q, int i, class C>
 __device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq);
 template <int Q, class C>
 __device__ __forceinline__ C usq_of(C ux, C uy, C uz);
+template <int Q>
+__device__ __forceinline__ float m15_of(float ux, float uy, float uz);
+template <int Q, int i>
+__device__ __forceinline__ float geq_i(float drho, float rho, float ux, float uy, float uz, float m15);
 
 // Non-equilibrium bounce-back closure of the unknowns (c_x == SGN) of an x-face node with
 // prescribed rho, u; transverse momentum corrected through N_a = sum_{c_x=0} f c_a - (2/3) rho u_a
@@ -303,12 +375,36 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, SrcType &&src_type,
 // and rho_g, u_g their moments.  rho_b < 0 takes rho_b = rho_g (velocity inlet), otherwise u_b =
 // u_g (pressure outlet: f_eq is linear in rho, so g_i + (rho_b - rho_g) f_eq,i(1, u_g), one FMA in
 // FP32).  Without a fluid neighbour the node takes f_eq(rho_b, u_b) (rho_b = rho_bc).
-template <int Q, class C, class Nbr, class P>
+// SH: populations in the shifted representation (f - w_i); the same rules, with the equilibria
+// as deviations from w_i and the density differences taken between density deviations.
+template <int Q, bool SH = false, class C, class Nbr, class P>
 __device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, bool set_u, C ubx, Nbr &&nbr) {
-  C g[Q], rn = 1, vx = 0, vy = 0, vz = 0;
+  C g[Q], rn = 1, drn = 0, vx = 0, vy = 0, vz = 0;
   if (nb) {
     static_for<Q>([&](auto I) { g[decltype(I)::value] = nbr(I); });
-    moments<Q>(g, rn, vx, vy, vz);
+    if constexpr (SH) moments_s<Q>(g, drn, rn, vx, vy, vz);
+    else moments<Q>(g, rn, vx, vy, vz);
+  }
+  if constexpr (SH) {
+    const float drbc = (float)(p.rho_bc - 1.0);
+    if (set_u) {  // velocity inlet
+      const float rb = nb ? rn : (float)p.rho_bc, drb = nb ? drn : drbc;
+      const float mb = m15_of<Q>(ubx, 0.0f, 0.0f), mg = m15_of<Q>(vx, vy, vz);
+      static_for<Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const float eb = geq_i<Q, i>(drb, rb, ubx, 0.0f, 0.0f, mb);
+        f[i] = nb ? g[i] + (eb - geq_i<Q, i>(drn, rn, vx, vy, vz, mg)) : eb;
+      });
+      return;
+    }
+    const float usq = usq_of<Q>(vx, vy, vz), mg = m15_of<Q>(vx, vy, vz);
+    const float dr = drbc - drn;  // rho_b - rho_g (rho_b = rho_bc for the outlet)
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      if (nb) f[i] = __builtin_fmaf(dr, feq_i<Q, i>(1.0f, vx, vy, vz, usq), g[i]);
+      else f[i] = geq_i<Q, i>(drbc, (float)p.rho_bc, vx, vy, vz, mg);
+    });
+    return;
   }
   if (set_u) {  // velocity inlet: u_b = (ubx, 0, 0), rho_b = rho_g
     const C rb = nb ? rn : (C)p.rho_bc;
@@ -334,8 +430,8 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, 
   });
 }
 
-template <int Q, class C, class SrcType, class Own, class West, class WestType, class East, class EastType,
-          class P>
+template <int Q, bool SH = false, class C, class SrcType, class Own, class West, class WestType, class East,
+          class EastType, class P>
 __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type, East &&east, EastType &&east_type) {
@@ -345,7 +441,7 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
     const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
-    extrapolate<Q>(p, f, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
+    extrapolate<Q, SH>(p, f, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
   } else if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = (C)-0.0, sm = (C)-0.0;  // sums from -0, as in moments()
@@ -354,14 +450,15 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
       if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
       if constexpr (Lat<Q>::cx[i] == -1) sm += f[i];
     });
-    const C rho = (s0 + (C)2 * sm) / ((C)1 - uin);
+    // shifted: the weights of the c_x = 0 directions plus twice those of c_x = -1 sum to 1
+    const C rho = SH ? (((C)1 + s0) + (C)2 * sm) / ((C)1 - uin) : (s0 + (C)2 * sm) / ((C)1 - uin);
     nebb_close<Q, 1>(f, rho, uin, (C)0, (C)0);
   } else if (t & NODE_OUTLET) {
     // zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi 2002), all i:
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
     // post-collision populations; the node then holds rho_bc and u_nb exactly.
     const bool nb = x > 0 && !(west_type() & NODE_SOLID);
-    extrapolate<Q>(p, f, nb, rw, false, (C)0, west);
+    extrapolate<Q, SH>(p, f, nb, rw, false, (C)0, west);
   }
 }
 
@@ -375,7 +472,7 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
   pull_periodic<Q, ST>(p, q, f);
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
-  physical_fixups<Q>(
+  physical_fixups<Q, CD::shifted>(
       p, q.x, q.y, q.z, t, f,
       [&](auto I) -> uint8_t {
         constexpr int i = decltype(I)::value;
@@ -421,8 +518,8 @@ __device__ __forceinline__ float recip_rn(float x) {
 // -0, which adds exactly (-0 + x == x for every x), so the compiler drops the first addition; the
 // c_i = 0 terms are left out (0 f_i adds an exact zero).  Both change at most the sign of a zero.
 template <int Q, class C>
-__device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz) {
-  C r = (C)-0.0, jx = (C)-0.0, jy = (C)-0.0, jz = (C)-0.0;
+__device__ __forceinline__ void moment_sums(const C *f, C &r, C &jx, C &jy, C &jz) {
+  r = (C)-0.0, jx = (C)-0.0, jy = (C)-0.0, jz = (C)-0.0;
   static_for<Q>([&](auto I) { r = r + f[decltype(I)::value]; });
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
@@ -441,6 +538,12 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
       if constexpr (Lat<Q>::cz[i] == -1) jz = jz - f[i];
     });
   }
+}
+
+template <int Q, class C>
+__device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz) {
+  C r, jx, jy, jz;
+  moment_sums<Q>(f, r, jx, jy, jz);
   rho = r;
   if constexpr (sizeof(C) == 4) {  // FP32: one correctly rounded reciprocal, then products
     const C inv = recip_rn(r);
@@ -454,6 +557,20 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
   }
 }
 
+// Shifted representation (g_i = f_i - w_i): drho = sum_i g_i, the density deviation, keeps the
+// precision of the small g; rho = 1 + drho; u = (sum_i c_i g_i) / rho, since sum_i c_i w_i = 0.
+template <int Q>
+__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz) {
+  float r, jx, jy, jz;
+  moment_sums<Q>(g, r, jx, jy, jz);
+  drho = r;
+  rho = 1.0f + r;
+  const float inv = recip_rn(rho);
+  ux = jx * inv;
+  uy = jy * inv;
+  uz = (Lat<Q>::D == 3) ? jz * inv : 0.0f;
+}
+
 // REFERENCE Dirichlet overrides, in the reference's order: lid velocity, inlet x-velocity,
 // solid rho = u = 0, fixed-density columns (cpp/d2q9_bgk_lid.cpp:141-146,
 // rust/examples/d2q9_bgk_channel.rs:190-201, rust/examples/d3q27_mrt_lid.rs:323-330).
@@ -464,6 +581,15 @@ __device__ __forceinline__ void overrides(const P &p, uint8_t t, int y, int z, C
   if (t & NODE_INLET) ux = inlet_velocity<C>(p, Lat<Q>::D, y, z);
   if (t & NODE_SOLID) { ux = 0; uy = 0; uz = 0; rho = 0; }
   if (t & NODE_RHO_FIX) rho = (C)p.rho_bc;
+}
+
+// The same overrides in the shifted representation: drho follows rho.
+template <int Q, class P>
+__device__ __forceinline__ void overrides_s(const P &p, uint8_t t, int y, int z, float &drho, float &rho, float &ux,
+                                            float &uy, float &uz) {
+  overrides<Q>(p, t, y, z, rho, ux, uy, uz);
+  if (t & NODE_SOLID) drho = -1.0f;
+  if (t & NODE_RHO_FIX) drho = (float)(p.rho_bc - 1.0);
 }
 
 // c_i . u evaluated as the reference does: (cx ux + cy uy) + cz uz with exact +-1/0 products.
@@ -504,10 +630,40 @@ __device__ __forceinline__ C usq_of(C ux, C uy, C uz) {
   }
 }
 
-// F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153); FP32: f + omega (f_eq - f).
-template <int Q, class C, class P>
+// Shifted equilibrium g_eq,i = f_eq,i - w_i = w_i (drho + rho P_i), P_i = 3 c.u + 4.5 (c.u)^2
+// - 1.5 u.u: every term small, so its FP32 rounding is relative to the deviation, not to w_i.
+// m15_of returns -1.5 u.u.
+template <int Q>
+__device__ __forceinline__ float m15_of(float ux, float uy, float uz) {
+  float u2 = __builtin_fmaf(uy, uy, ux * ux);
+  if constexpr (Lat<Q>::D == 3) u2 = __builtin_fmaf(uz, uz, u2);
+  return -1.5f * u2;
+}
+template <int Q, int i>
+__device__ __forceinline__ float geq_i(float drho, float rho, float ux, float uy, float uz, float m15) {
+  float P;
+  if constexpr (Lat<Q>::cx[i] == 0 && Lat<Q>::cy[i] == 0 && Lat<Q>::cz[i] == 0) {
+    P = m15;
+  } else {
+    const float eu = cdotu<Q, i>(ux, uy, uz);
+    P = __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), m15);
+  }
+  return (float)Lat<Q>::w[i] * __builtin_fmaf(rho, P, drho);
+}
+
+// F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153); FP32: f + omega (f_eq - f);
+// SH (shifted representation): g + omega (g_eq - g).
+template <int Q, bool SH = false, class C, class P>
 __device__ __forceinline__ void collide_bgk(const P &p, const C *f, C rho, C ux, C uy, C uz,
-                                            C *out) {
+                                            C *out, C drho = (C)0) {
+  if constexpr (SH) {
+    const float omega = (float)p.omega, m15 = m15_of<Q>(ux, uy, uz);
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      out[i] = __builtin_fmaf(omega, geq_i<Q, i>(drho, rho, ux, uy, uz, m15) - f[i], f[i]);
+    });
+    return;
+  }
   const C omega = (C)p.omega, om1 = (C)1 - omega;
   const C usq = usq_of<Q>(ux, uy, uz);
   static_for<Q>([&](auto I) {
@@ -586,6 +742,25 @@ __device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
   meq[26] = v2 + rc6;
 }
 
+// The same table less its value at rest (rho = 1, u = 0), from drho = rho - 1: the rows of the
+// shifted representation.
+__device__ __forceinline__ void meq_reference_s(float dr, float r, float ux, float uy, float uz, float *meq) {
+  const float cs2 = 1.0f / 3.0f, cs4 = cs2 * cs2, cs6 = cs2 * cs2 * cs2;
+  const float rvx = r * ux, rvy = r * uy, rvz = r * uz, rc2 = cs2 * r, rc4 = cs4 * r;
+  const float vx2 = ux * ux, vy2 = uy * uy, vz2 = uz * uz, v2 = vx2 + vy2 + vz2;
+  meq[0] = dr; meq[1] = rvx; meq[2] = rvy; meq[3] = rvz;
+  meq[4] = rvx * uy; meq[5] = rvx * uz; meq[6] = rvy * uz;
+  meq[7] = dr + rvx * ux + rvy * uy + rvz * uz;
+  meq[8] = r * (vx2 - vy2); meq[9] = r * (vx2 - vz2);
+  meq[10] = rc2 * ux; meq[11] = rc2 * ux; meq[12] = rc2 * uy; meq[13] = rc2 * uz;
+  meq[14] = rc2 * uy; meq[15] = rc2 * uz; meq[16] = 0;
+  meq[17] = cs4 * dr + rc2 * (vx2 + vy2); meq[18] = cs4 * dr + rc2 * (vx2 + vz2);
+  meq[19] = cs4 * dr + rc2 * (vy2 + vz2);
+  meq[20] = rc2 * uy * uz; meq[21] = rc2 * ux * uz; meq[22] = rc2 * ux * uy;
+  meq[23] = rc4 * ux; meq[24] = rc4 * uy; meq[25] = rc4 * uz;
+  meq[26] = v2 + cs6 * dr;
+}
+
 // FP32 MRT, the relaxation below (out = f - M^-1 S (M f - meq), S acting in the collision's
 // moment basis) through the tensor-product structure of D3Q27:
 //   mu = P f, the monomial moments sum_i cx^a cy^b cz^c f_i (a, b, c in 0..2): three passes of the
@@ -659,12 +834,16 @@ constexpr bool raw_rows_match() {
 static_assert(raw_rows_match(), "monomial tables disagree with MRAW");
 }  // namespace d27
 
-template <int CO, class P>
+// SH: f holds g = f - w_i and the raw equilibrium moments are taken less their value at rest
+// (meq(rho, u) - meq(1, 0) = c_0 drho + rho (the u terms), c_0 the constant column), so the
+// deviations are differences of small numbers; the passes below are linear and map g to g'.
+template <int CO, bool SH = false, class P>
 __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, float rho, float ux,
-                                                 float uy, float uz, float *out) {
+                                                 float uy, float uz, float *out, float drho = 0.0f) {
   float d[27];  // the equilibrium raw moments, then the deviations
   if (CO == CO_MRT_RAW && p.meq_mode == 0) {
-    meq_reference<float>(rho, ux, uy, uz, d);
+    if constexpr (SH) meq_reference_s(drho, rho, ux, uy, uz, d);
+    else meq_reference<float>(rho, ux, uy, uz, d);
   } else {
     const float phi[10] = {1.0f, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
     static_for<27>([&](auto K) {
@@ -673,9 +852,15 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
       static_for<10>([&](auto J) {
         constexpr int j = decltype(J)::value;
         constexpr float c = (float)MEQ_RAW[k][j];
-        if constexpr (c != 0.0f) acc = __builtin_fmaf(c, phi[j], acc);
+        if constexpr (c != 0.0f && !(SH && j == 0)) acc = __builtin_fmaf(c, phi[j], acc);
       });
-      d[k] = rho * acc;
+      if constexpr (SH) {
+        constexpr float c0 = (float)MEQ_RAW[k][0];
+        if constexpr (c0 != 0.0f) d[k] = __builtin_fmaf(rho, acc, c0 * drho);
+        else d[k] = rho * acc;
+      } else {
+        d[k] = rho * acc;
+      }
     });
   }
   float v[27];
@@ -745,11 +930,11 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
 }
 
 // m = M f; m* = m - s (m - meq); out = M^-1 m*   (rust/examples/d3q27_mrt_lid.rs:336,385-387)
-template <int CO, class C, class P>
+template <int CO, bool SH = false, class C, class P>
 __device__ __forceinline__ void collide_mrt_scalar(const P &p, const C *f, C rho, C ux, C uy,
-                                                   C uz, C *out) {
+                                                   C uz, C *out, C drho = (C)0) {
   if constexpr (sizeof(C) == 4) {
-    collide_mrt_fp32<CO>(p, f, rho, ux, uy, uz, out);
+    collide_mrt_fp32<CO, SH>(p, f, rho, ux, uy, uz, out, drho);
     return;
   }
   C m[27], meq[27];
@@ -803,9 +988,11 @@ static_assert(pair_rep(8) > 0 && pair_rep(9) == -1, "D3Q19 has 9 opposite pairs"
 // D3Q19 MRT in the orthogonal basis M19 (d'Humieres et al. 2002): m = M19 f, projected
 // equilibrium moments meq = M19 f_eq (closed form MEQ19), m* = m - s (m - meq),
 // out = M19^T diag(1 / |row|^2) m*.  FP32: out = f - M19^T ((s / |row|^2) (m - meq)), FMAs, both
-// transforms in the pair form above (half the terms of the dense rows).
-template <class C, class P>
-__device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
+// transforms in the pair form above (half the terms of the dense rows).  SH: f holds g = f - w_i
+// and meq is taken less its value at rest, as in collide_mrt_fp32.
+template <bool SH = false, class C, class P>
+__device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out,
+                                              C drho = (C)0) {
   const C phi[10] = {(C)1, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
   C d[19], ps[9], pd[9];  // FP32: sums and differences of the opposite pairs
   if constexpr (sizeof(C) == 4)
@@ -820,7 +1007,7 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
     static_for<10>([&](auto J) {
       constexpr int j = decltype(J)::value;
       constexpr C c = (C)MEQ19[k][j];
-      if constexpr (c != (C)0) {
+      if constexpr (c != (C)0 && !(SH && j == 0)) {
         if constexpr (sizeof(C) == 4) meq = __builtin_fmaf(c, phi[j], meq);
         else meq = meq + c * phi[j];
       }
@@ -836,7 +1023,13 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
     } else {
       static_for<19>([&](auto I) { m = add_int_term<C, M19[k][decltype(I)::value]>(m, f[decltype(I)::value]); });
     }
-    meq = rho * meq;
+    if constexpr (SH) {
+      constexpr float c0 = (float)MEQ19[k][0];
+      if constexpr (c0 != 0.0f) meq = __builtin_fmaf(rho, meq, c0 * drho);
+      else meq = rho * meq;
+    } else {
+      meq = rho * meq;
+    }
     if constexpr (sizeof(C) == 4) d[k] = (p.s32[k] * (1.0f / (float)M19_NORM2[k])) * (m - meq);
     else d[k] = m - (m - meq) * (C)p.s[k];  // m*
   });
@@ -874,27 +1067,43 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
   }
 }
 
-template <int Q, int CO, class C, class P>
-__device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out) {
-  if constexpr (CO == CO_BGK) collide_bgk<Q>(p, f, rho, ux, uy, uz, out);
-  else if constexpr (Q == 19) collide_mrt19(p, f, rho, ux, uy, uz, out);
-  else collide_mrt_scalar<CO>(p, f, rho, ux, uy, uz, out);
+// SH: f and out in the shifted representation (f - w_i), drho = rho - 1 kept separately.
+template <int Q, int CO, bool SH = false, class C, class P>
+__device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out, C drho = (C)0) {
+  if constexpr (CO == CO_BGK) collide_bgk<Q, SH>(p, f, rho, ux, uy, uz, out, drho);
+  else if constexpr (Q == 19) collide_mrt19<SH>(p, f, rho, ux, uy, uz, out, drho);
+  else collide_mrt_scalar<CO, SH>(p, f, rho, ux, uy, uz, out, drho);
 }
 
+// Stores the node's Q populations: out in the compute representation, rnd the node's rounding
+// bits (node_bits; shifted codecs only).
 template <int Q, int ST, bool NT = false, class P>
-__device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename Codec<ST, Q>::C *out) {
+__device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename Codec<ST, Q>::C *out, uint32_t rnd) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   T *fout = (T *)p.fout;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     T *dst = fout + (size_t)i * (size_t)p.stride + n;
+    const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
     // NT: non-temporal stores for lattices too large for the caches (the next step reads these
     // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s
     // (4-byte) and from 5.32 to 5.62 TB/s (2-byte) on MI355X, and the 3D step kernels by 3-5 %; a
     // lattice that stays in the caches between steps runs faster with ordinary stores.
-    if constexpr (NT) __builtin_nontemporal_store(CD::template enc<i>(out[i]), dst);
-    else *dst = CD::template enc<i>(out[i]);
+    if constexpr (NT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+  });
+}
+
+// Stores populations given unshifted (initial states), rounded to nearest.
+template <int Q, int ST, class P>
+__device__ __forceinline__ void store_all_f(const P &p, uint32_t n, const typename Codec<ST, Q>::C *f) {
+  using CD = Codec<ST, Q>;
+  using T = typename CD::T;
+  T *fout = (T *)p.fout;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    fout[(size_t)i * (size_t)p.stride + n] = CD::template enc_f<i>(f[i]);
   });
 }
 

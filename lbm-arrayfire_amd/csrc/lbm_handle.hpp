@@ -16,6 +16,12 @@
 struct lbm_batch_ctx;
 
 namespace lbm {
+// The handle's small device scratch: 3 reduction results (monitors, force) in 4 doubles, then the
+// two 32-bit step-counter words of the stochastic rounding.
+constexpr size_t RED_BYTES = 5 * sizeof(double);
+}
+
+namespace lbm {
 struct DevPool;
 }
 
@@ -34,7 +40,7 @@ struct lbm_ctx {
   int cur = 0;
   uint8_t *types = nullptr;
   double *partials = nullptr;
-  double *red = nullptr;
+  double *red = nullptr;      // 3 reduction results, then (at red + 4) the two step-counter words
   double *fields = nullptr;  // 4N doubles, allocated on first use
   float *mrt_tab = nullptr;  // MFMA MRT per-lane operands
   void *probe = nullptr;     // sensor node list + gathered values
@@ -54,10 +60,13 @@ struct lbm_ctx {
   size_t pop_bytes() const { return (size_t)N * (size_t)Q * (size_t)esize; }       // packed
   size_t buf_bytes() const { return (size_t)stride * (size_t)Q * (size_t)esize; }   // allocated
 
+  uint32_t *step_ctr() const { return reinterpret_cast<uint32_t *>(red + 4); }
   lbm::StepParams params(int from) const {
     lbm::StepParams p = base;
     p.fin = f[from];
     p.fout = f[from ^ 1];
+    p.step_in = step_ctr() + from;
+    p.step_out = step_ctr() + (from ^ 1);
     return p;
   }
 };

@@ -24,6 +24,11 @@ struct StepParams {
   int64_t N;                // nodes
   int64_t stride;           // elements between consecutive population arrays (>= N)
   const float *mrt_tab;     // MFMA MRT: per-lane A operands, 64 x MFMA_TAB floats
+  // Step counter of the stochastic rounding (shifted storages): a pair of device words, one per
+  // buffer orientation.  A launch reads *step_in and writes the count after it to *step_out, the
+  // word the next launch (the other orientation) reads.
+  const uint32_t *step_in;
+  uint32_t *step_out;
   double omega;
   double u_wall[3];
   double u_in;

@@ -22,12 +22,22 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fields(StepParams p, double *
   if (uz_o) uz_o[q.n] = r[3];
 }
 
+// An initial state restarts the step count of the stochastic rounding (both words of the pair).
+template <class P>
+__device__ __forceinline__ void reset_step_count(const P &p) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *p.step_out = 0;
+    *const_cast<uint32_t *>(p.step_in) = 0;
+  }
+}
+
 // f = f_eq(rho, u) (cpp/d2q9_bgk_lid.cpp:100-107) into both buffers.
 template <int Q, int ST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_init_fields(StepParams p, const double *rho_i,
                                                                const double *ux_i, const double *uy_i,
                                                                const double *uz_i) {
   using C = typename Codec<ST, Q>::C;
+  reset_step_count(p);
   const NodePos q = node_of_thread(p);
   if (!q.valid) return;
   const C r = rho_i ? (C)rho_i[q.n] : (C)1, vx = ux_i ? (C)ux_i[q.n] : (C)0,
@@ -38,10 +48,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_fields(StepParams p, con
     constexpr int i = decltype(I)::value;
     out[i] = feq_i<Q, i>(r, vx, vy, vz, usq);
   });
-  store_all<Q, ST>(p, q.n, out);
+  store_all_f<Q, ST>(p, q.n, out);
   StepParams p2 = p;
   p2.fout = const_cast<void *>(p.fin);
-  store_all<Q, ST>(p2, q.n, out);
+  store_all_f<Q, ST>(p2, q.n, out);
 }
 
 // f = f_eq(rho0, u0 + noise), the noise drawn on the device.  SplitMix64 is counter based: draw
@@ -59,6 +69,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_equilibrium(StepParams p
                                                                     double u0y, double u0z, uint64_t seed,
                                                                     double amp) {
   using C = typename Codec<ST, Q>::C;
+  reset_step_count(p);
   const NodePos q = node_of_thread(p);
   if (!q.valid) return;
   constexpr int D = Lat<Q>::D;
@@ -75,10 +86,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_equilibrium(StepParams p
     constexpr int i = decltype(I)::value;
     out[i] = feq_i<Q, i>(r, vx, vy, vz, usq);
   });
-  store_all<Q, ST>(p, q.n, out);
+  store_all_f<Q, ST>(p, q.n, out);
   StepParams p2 = p;
   p2.fout = const_cast<void *>(p.fin);
-  store_all<Q, ST>(p2, q.n, out);
+  store_all_f<Q, ST>(p2, q.n, out);
 }
 
 hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double rho0, const double *u0,
@@ -95,6 +106,7 @@ hipError_t launch_init_equilibrium(const Config &c, const StepParams &p, double 
 template <int Q, int ST, int CO>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, double rho0) {
   using C = typename Codec<ST, Q>::C;
+  reset_step_count(p);
   const NodePos q = node_of_thread(p);
   if (!q.valid) return;
   C out[Q];
@@ -109,10 +121,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, 
     static_for<Q>([&](auto I) { f[decltype(I)::value] = (C)rho0 / (C)27; });
     collide<Q, CO>(p, f, (C)rho0, (C)0, (C)0, (C)0, out);
   }
-  store_all<Q, ST>(p, q.n, out);
+  store_all_f<Q, ST>(p, q.n, out);
   StepParams p2 = p;
   p2.fout = const_cast<void *>(p.fin);
-  store_all<Q, ST>(p2, q.n, out);
+  store_all_f<Q, ST>(p2, q.n, out);
 }
 
 // Wavefront + workgroup reduction of (mass, kinetic energy, nonfinite) in FP64 over the
@@ -136,7 +148,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_monitors(StepParams p, double
     bool finite = true;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      const double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+      double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+      if constexpr (CD::shifted) v += Lat<Q>::w[i];  // f = g + w_i, exact in FP64
       finite = finite && __builtin_isfinite(v);
       r += v;
       jx += (double)Lat<Q>::cx[i] * v;
@@ -191,7 +204,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_force(StepParams p, int x0, i
         sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
         const uint8_t ts = p.type[(uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz)];
         if ((ts & NODE_SOLID) && sx >= x0 && sx < x1 && sy >= y0 && sy < y1 && sz >= z0 && sz < z1) {
-          double m = 2.0 * (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+          double fi = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+          if constexpr (CD::shifted) fi += Lat<Q>::w[i];
+          double m = 2.0 * fi;
           if (ts & NODE_MOVING)
             m -= 6.0 * Lat<Q>::w[i] * rw * (cx * p.u_wall[0] + cy * p.u_wall[1] + cz * p.u_wall[2]);
           fx += cx * m;

@@ -82,6 +82,8 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
   float *L = lds[wv];
 
   // ---- phase 1: node per lane -----------------------------------------------------------
+  constexpr bool SH = CD::shifted;  // f holds g = f - w_i (2-byte storages)
+  const uint32_t key = step_begin<SH>(p, tile, 1);
   const NodePos q = node_of_tile(p, tile);
   const uint8_t t = q.valid ? p.type[q.n] : (uint8_t)NODE_SOLID;
   const bool active = q.valid && !(BC == BC_PHYS && (t & NODE_SOLID));
@@ -95,19 +97,33 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
     float row[ROW];
     float dev[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
-      moments<27>(f, rho, ux, uy, uz);
+      C drho = 0, dr0 = 0;
+      if constexpr (SH) moments_s<27>(f, drho, rho, ux, uy, uz);
+      else moments<27>(f, rho, ux, uy, uz);
       const C r0 = rho, x0 = ux, y0 = uy, z0 = uz;
-      if constexpr (BC == BC_REF) overrides<27>(p, t, q.y, q.z, rho, ux, uy, uz);
+      dr0 = drho;
+      if constexpr (BC == BC_REF) {
+        if constexpr (SH) overrides_s<27>(p, t, q.y, q.z, drho, rho, ux, uy, uz);
+        else overrides<27>(p, t, q.y, q.z, rho, ux, uy, uz);
+      }
       // conserved rows: rho_f - rho and rho_f u_f - rho u, exactly 0 without an override
-      dev[0] = r0 - rho;
+      dev[0] = SH ? dr0 - drho : r0 - rho;
       dev[1] = r0 * x0 - rho * ux;
       dev[2] = r0 * y0 - rho * uy;
       dev[3] = r0 * z0 - rho * uz;
-      const C usq = usq_of<27>(ux, uy, uz);
-      static_for<27>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        row[i] = f[i] - feq_i<27, i>(rho, ux, uy, uz, usq);
-      });
+      if constexpr (SH) {
+        const float m15 = m15_of<27>(ux, uy, uz);
+        static_for<27>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          row[i] = f[i] - geq_i<27, i>(drho, rho, ux, uy, uz, m15);
+        });
+      } else {
+        const C usq = usq_of<27>(ux, uy, uz);
+        static_for<27>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          row[i] = f[i] - feq_i<27, i>(rho, ux, uy, uz, usq);
+        });
+      }
       float d26 = 0.f;
       if (p.meq_fix26) {  // the reference table's row 26 against M f_eq
         const float phi[10] = {1.0f, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
@@ -115,11 +131,16 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
         static_for<10>([&](auto J) {
           constexpr int j = decltype(J)::value;
           constexpr float cf = (float)MEQ_RAW[26][j];
-          if constexpr (cf != 0.0f) acc = __builtin_fmaf(cf, phi[j], acc);
+          if constexpr (cf != 0.0f && !(SH && j == 0)) acc = __builtin_fmaf(cf, phi[j], acc);
         });
-        float ref[27];
-        meq_reference<float>(rho, ux, uy, uz, ref);
-        d26 = rho * acc - ref[26];
+        if constexpr (SH) {
+          // both rows less their common value at rest (rho / 27): rho (u terms) - u.u
+          d26 = rho * acc - ((ux * ux + uy * uy) + uz * uz);
+        } else {
+          float ref[27];
+          meq_reference<float>(rho, ux, uy, uz, ref);
+          d26 = rho * acc - ref[26];
+        }
       }
       row[27] = d26;
     } else {
@@ -231,7 +252,7 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
       });
     }
   }
-  store_all<27, ST>(p, q.n, out);
+  store_all<27, ST>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
 }
 
 template <int ST, int BC, int ENG>

@@ -51,6 +51,11 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   using C = typename CD::C;
+  constexpr bool SH = CD::shifted;  // populations as g = f - w_i, stored with stochastic rounding
+  // step t0 + s - 1 is sub-step s (the single-step kernels' key for that step)
+  const uint32_t t0 = SH ? *p.step_in : 0u;
+  if constexpr (SH)
+    if (tile == 0 && threadIdx.x == 0) *p.step_out = t0 + K;
 
   __syncthreads();  // a workgroup that walks several tiles: the previous tile's reads are done
   if constexpr (LIST) {
@@ -118,15 +123,21 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   // is not updated (PHYSICAL walls keep their populations).
   auto update = [&](const C *src, int pos, C *out, auto with_fixups) -> bool {
     const uint8_t t = lt[pos];
-    C f[Q], rho, ux, uy, uz;
+    C f[Q], rho, ux, uy, uz, drho = 0;
     static_for<Q>([&](auto I) {
       constexpr int q = decltype(I)::value;
       f[q] = src[q * RN + pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
     });
     if constexpr (BC == BC_REF) {
-      moments<Q>(f, rho, ux, uy, uz);
-      overrides<Q>(p, t, (t & NODE_INLET) ? (int)(lxy[pos] >> 16) : 0, 0, rho, ux, uy, uz);
-      collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
+      const int yin = (t & NODE_INLET) ? (int)(lxy[pos] >> 16) : 0;
+      if constexpr (SH) {
+        moments_s<Q>(f, drho, rho, ux, uy, uz);
+        overrides_s<Q>(p, t, yin, 0, drho, rho, ux, uy, uz);
+      } else {
+        moments<Q>(f, rho, ux, uy, uz);
+        overrides<Q>(p, t, yin, 0, rho, ux, uy, uz);
+      }
+      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
       if (t & NODE_SOLID) {  // full-way bounce-back: out[opp(j)] = f[j], j >= 1
         static_for<Q - 1>([&](auto J) {
           constexpr int q = decltype(J)::value + 1;
@@ -138,7 +149,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       if (t & NODE_SOLID) return false;
       if (decltype(with_fixups)::value && (t & NODE_SLOW)) {
         const uint32_t xy = lxy[pos];
-        physical_fixups<Q>(
+        physical_fixups<Q, SH>(
             p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
@@ -150,33 +161,42 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
             [&]() -> uint8_t { return lt[pos + 1]; });
       }
-      moments<Q>(f, rho, ux, uy, uz);
-      collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
+      if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz);
+      else moments<Q>(f, rho, ux, uy, uz);
+      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
       return true;
     }
   };
-  // the rounding a store and a reload apply
-  auto round_trip = [&](auto I, C v) -> C {
+  // the rounding a store and a reload apply; rnd: the node's rounding bits in this sub-step
+  auto round_trip = [&](auto I, C v, uint32_t rnd) -> C {
     constexpr int q = decltype(I)::value;
     T w;
-    CD::template st<q>(&w, 0, v);
+    CD::template st<q>(&w, 0, v, pop_bits<q>(rnd));
     return CD::template ld<q>(&w, 0);
   };
-  auto keep = [&](C *dst, int pos, const C *out) {
-    static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value]); });
+  // node_bits of region position pos in the sub-step with key k
+  auto bits_of = [&](int pos, uint32_t k) -> uint32_t {
+    if constexpr (!SH) return 0u;
+    const uint32_t xy = lxy[pos];
+    return node_bits((xy & 0xffffu) + (uint32_t)p.nx * (xy >> 16), k);
+  };
+  auto keep = [&](C *dst, int pos, const C *out, uint32_t k) {
+    const uint32_t rnd = bits_of(pos, k);
+    static_for<Q>([&](auto I) { dst[decltype(I)::value * RN + pos] = round_trip(I, out[decltype(I)::value], rnd); });
   };
 
   T *fout = (T *)p.fout;
   // rows dealt top-down when the held rows are at the region's end, so that they come first
   const bool flip = BC == BC_PHYS && (p.wrap_cut & 2) && y0 + RY > p.ny;
   // sub-step K updates exactly the interior: write the part inside the lattice
-  auto write_out = [&](int i, int j, const C *out) {
+  auto write_out = [&](int i, int j, const C *out, uint32_t k) {
     const int x = x0 + i, y = y0 + j;
     if (x < p.nx && y < p.ny) {
       const uint32_t n = (uint32_t)x + (uint32_t)p.nx * (uint32_t)y;
+      const uint32_t rnd = SH ? node_bits(n, k) : 0u;
       static_for<Q>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        CD::template st<q>(fout + (size_t)q * (size_t)p.stride, n, out[q]);
+        CD::template st<q>(fout + (size_t)q * (size_t)p.stride, n, out[q], pop_bits<q>(rnd));
       });
     }
   };
@@ -186,6 +206,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
   static_for<K>([&](auto SI) {
     constexpr int s = decltype(SI)::value + 1;
     constexpr int WX = RX - 2 * s, W = WX * (RY - 2 * s), NW = (W + NT - 1) / NT;
+    const uint32_t key = SH ? step_key(t0 + (uint32_t)(s - 1)) : 0u;
     const C *src = lf + (DB ? ((s - 1) & 1) * Q * RN : 0);
     C *dst = lf + (DB ? (s & 1) * Q * RN : 0);
     if constexpr (LIST) {
@@ -207,7 +228,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
         const uint32_t xy = lxy[pos];
         const uint8_t t = (uint8_t)(((w >> 28) & 1u ? NODE_INLET : 0) | ((w >> 29) & 1u ? NODE_OUTLET : 0) |
                                     ((w >> 30) & 1u ? NODE_WALLNB : 0));
-        physical_fixups<Q>(
+        physical_fixups<Q, SH>(
             p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
@@ -218,10 +239,12 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             [&]() -> uint8_t { return (w >> 27) & 1u ? NODE_SOLID : 0; },
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
             [&]() -> uint8_t { return (w >> 31) & 1u ? NODE_SOLID : 0; });
-        moments<Q>(f, rho, ux, uy, uz);
-        collide<Q, CO_BGK>(p, f, rho, ux, uy, uz, out);
-        if constexpr (s == K) write_out(i, j, out);
-        else keep(dst, pos, out);
+        C drho = 0;
+        if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz);
+        else moments<Q>(f, rho, ux, uy, uz);
+        collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
+        if constexpr (s == K) write_out(i, j, out, key);
+        else keep(dst, pos, out, key);
       }
     }
     C res[DB ? 1 : NW][Q];
@@ -243,15 +266,15 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       C *out = res[DB ? 0 : k];
       upd[k] = update(src, pos, out, std::integral_constant<bool, !LIST>{});
       if (!upd[k]) return;
-      if constexpr (s == K) write_out(i, j, out);
-      else if constexpr (DB) keep(dst, pos, out);
+      if constexpr (s == K) write_out(i, j, out, key);
+      else if constexpr (DB) keep(dst, pos, out, key);
     });
     if constexpr (s < K) {
       __syncthreads();  // DB: the copy written is complete; otherwise: every read of sub-step s is done
       if constexpr (!DB) {
         static_for<NW>([&](auto KI) {
           constexpr int k = decltype(KI)::value;
-          if (upd[k]) keep(dst, wpos[k], res[k]);
+          if (upd[k]) keep(dst, wpos[k], res[k], key);
         });
         __syncthreads();
       }

@@ -9,18 +9,27 @@ namespace lbm {
 
 // One fused time step: pull stream + boundary handling + collision (+ full-way bounce-back at
 // solid nodes in REFERENCE mode), A -> B buffers.  Replaces cpp/d2q9_bgk_lid.cpp:124-155.
+// Shifted storages compute on g = f - w_i throughout and store with stochastic rounding.
 template <int Q, int ST, int BC, int CO, bool NT, class P>
 __device__ __forceinline__ void step_tile(const P &p, int tile) {
-  using C = typename Codec<ST, Q>::C;
+  using CD = Codec<ST, Q>;
+  using C = typename CD::C;
+  constexpr bool SH = CD::shifted;
+  const uint32_t key = step_begin<SH>(p, tile, 1);
   const NodePos q = node_of_tile(p, tile);
   if (!q.valid) return;
   const uint8_t t = p.type[q.n];
-  C f[Q], out[Q], rho, ux, uy, uz;
+  C f[Q], out[Q], rho, ux, uy, uz, drho = 0;
   if constexpr (BC == BC_REF) {
     pull_periodic<Q, ST>(p, q, f);
-    moments<Q>(f, rho, ux, uy, uz);
-    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
-    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+    if constexpr (SH) {
+      moments_s<Q>(f, drho, rho, ux, uy, uz);
+      overrides_s<Q>(p, t, q.y, q.z, drho, rho, ux, uy, uz);
+    } else {
+      moments<Q>(f, rho, ux, uy, uz);
+      overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+    }
+    collide<Q, CO, SH>(p, f, rho, ux, uy, uz, out, drho);
     if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK: out[opp(j)] = f[j], j >= 1
       static_for<Q - 1>([&](auto J) {
         constexpr int j = decltype(J)::value + 1;
@@ -30,29 +39,45 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
   } else {
     pull_physical<Q, ST>(p, q, t, f);
     if (t & NODE_SOLID) return;  // walls keep their populations; nothing writes them
-    moments<Q>(f, rho, ux, uy, uz);
-    collide<Q, CO>(p, f, rho, ux, uy, uz, out);
+    if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz);
+    else moments<Q>(f, rho, ux, uy, uz);
+    collide<Q, CO, SH>(p, f, rho, ux, uy, uz, out, drho);
   }
-  store_all<Q, ST, NT>(p, q.n, out);
+  store_all<Q, ST, NT>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
 }
 
 // rho, u the reference holds after a step at node q (cpp/d2q9_bgk_lid.cpp:131-146): moments of the
 // pulled populations with the overrides (REFERENCE), or of the boundary-completed pulled
-// populations (PHYSICAL; 0 at solid nodes).  out = {rho, ux, uy, uz}.
+// populations (PHYSICAL; 0 at solid nodes).  out = {rho, ux, uy, uz}; the shifted storages report
+// rho as 1 + drho in FP64.
 template <int Q, int ST, int BC, class P>
 __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, double *out) {
-  using C = typename Codec<ST, Q>::C;
+  using CD = Codec<ST, Q>;
+  using C = typename CD::C;
   const uint8_t t = p.type[q.n];
   C f[Q], rho = 0, ux = 0, uy = 0, uz = 0;
-  if constexpr (BC == BC_REF) {
-    pull_periodic<Q, ST>(p, q, f);
-    moments<Q>(f, rho, ux, uy, uz);
-    overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
-  } else if (!(t & NODE_SOLID)) {
-    pull_physical<Q, ST>(p, q, t, f);
-    moments<Q>(f, rho, ux, uy, uz);
+  if constexpr (CD::shifted) {
+    C drho = -1;
+    if constexpr (BC == BC_REF) {
+      pull_periodic<Q, ST>(p, q, f);
+      moments_s<Q>(f, drho, rho, ux, uy, uz);
+      overrides_s<Q>(p, t, q.y, q.z, drho, rho, ux, uy, uz);
+    } else if (!(t & NODE_SOLID)) {
+      pull_physical<Q, ST>(p, q, t, f);
+      moments_s<Q>(f, drho, rho, ux, uy, uz);
+    }
+    out[0] = 1.0 + (double)drho;
+  } else {
+    if constexpr (BC == BC_REF) {
+      pull_periodic<Q, ST>(p, q, f);
+      moments<Q>(f, rho, ux, uy, uz);
+      overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
+    } else if (!(t & NODE_SOLID)) {
+      pull_physical<Q, ST>(p, q, t, f);
+      moments<Q>(f, rho, ux, uy, uz);
+    }
+    out[0] = (double)rho;
   }
-  out[0] = (double)rho;
   out[1] = (double)ux;
   out[2] = (double)uy;
   out[3] = (double)uz;

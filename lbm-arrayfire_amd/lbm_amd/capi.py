@@ -331,7 +331,7 @@ class Lattice:
 
     def set_state(self, b):
         b = np.ascontiguousarray(b, dtype=np.uint8).reshape(-1)
-        want = self.N * self.Q * STORAGE_BYTES[self.cfg.get("storage", F32)]  # lbm_state_bytes
+        want = self.N * self.Q * STORAGE_BYTES[self.cfg.get("storage", F32)] + 8  # lbm_state_bytes
         if b.size != want:
             raise ValueError(f"state has {b.size} bytes, the lattice holds {want}")
         check(self.lib.lbm_set_state(self.h, b.ctypes.data_as(ctypes.c_void_p)))

@@ -33,6 +33,14 @@
 //                       closed form and the sign of the coupling.
 //
 // Shifted storage: F16S holds (f - w_i) * 2^15 in IEEE half, BF16S holds f - w_i in bfloat16.
+// Neither has a reference counterpart; this restates the kernels' own algorithm for them: the
+// state is held as g = f - w_i, every step computes on g (density deviation drho = sum g, rho =
+// 1 + drho, equilibria as deviations w_i (drho + rho P_i)) and rounds g to the storage with
+// stochastic rounding, restated bit for bit from the gfx950 conversions (f2h_sr, f2bf_sr) with
+// the kernels' random bits (mix32 of the node index and the step count).  Initial states are
+// rounded to nearest.  Test switch: desc.reserved bit 0 makes the steps round to nearest too
+// (the rounding whose frozen errors the stochastic rounding removes), for tests that show the
+// difference; the library has no such mode.
 //
 // FP32 mode contracts the equilibrium and the BGK relaxation into FMAs (see feq()) and takes
 // u = j * (1/rho), matching the kernels; FP64 mode keeps every operation separately rounded and
@@ -161,6 +169,48 @@ float bf2f(uint16_t b) {
   return v;
 }
 
+// Stochastic rounding as v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32 do it (pinned against MI355X
+// outputs by tests/golden/conversions/sr_conversions.npz): |x| is cut to 13 bits below the
+// half-precision ulp of its binade (the subnormal ulp 2^-24 below 2^-14), r >> 19 is added, and the sum is cut to
+// the ulp; values from 65520 up give inf.  bfloat16: r >> 16 added to the magnitude bits, cut to
+// their upper half.  The sign is kept.
+uint16_t f2h_sr(float f, uint32_t r) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+  int e = (int)(ax >> 23) - 127;
+  uint64_t m = ax & 0x7fffffu;
+  if (ax >> 23) m |= 0x800000u;
+  else e = -126;
+  const int ulp_e = e >= -14 ? e - 10 : -24;           // log2 of the half ulp of |x|
+  const int sh = (ulp_e - 13) - (e - 23);              // m in units of 2^(ulp_e - 13)
+  uint64_t t = sh >= 0 ? (sh >= 64 ? 0 : m >> sh) : m << (-sh);
+  t += r >> 19;
+  const uint64_t k = t >> 13;                           // |result| / 2^ulp_e
+  const uint64_t bits = ulp_e == -24 ? k : ((uint64_t)(e + 14) << 10) + k;
+  return (uint16_t)(sign | (bits >= 0x7c00u ? 0x7c00u : (uint32_t)bits));
+}
+uint16_t f2bf_sr(float f, uint32_t r) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = x & 0x80000000u, ax = x & 0x7fffffffu;
+  if (ax > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+  return (uint16_t)((sign | (ax + (r >> 16))) >> 16);
+}
+
+// Random bits of the shifted storages' stochastic rounding, as the kernels draw them: node n in
+// step t takes mix32(n ^ mix32(t + 0x9E3779B9)), population i those bits rotated left by 7 i.
+uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+uint32_t rotl32(uint32_t h, int k) { return k == 0 ? h : (h << k) | (h >> (32 - k)); }
+
 // ---- MRT matrices ------------------------------------------------------------------------
 // Raw monomial rows in the reference's order (rust/examples/d3q27_mrt_lid.rs:167-193).
 void raw_moment_matrix(const Lattice &L, double M[27][27]) {
@@ -251,10 +301,14 @@ struct Sim {
   double Md[27][27];
   bool mrt = false;
   T w[27];
+  // shifted storage (F16S, BF16S): F holds g = f - w_i and the arithmetic is on deviations
+  bool sh = false;
+  uint32_t tstep = 0;  // step count of the stochastic rounding
 
   Sim(const oracle_desc &dd, const uint8_t *types) : d(dd), L(make_lattice(dd.lattice)), type(types) {
     nx = d.nx; ny = d.ny; nz = d.nz;
     N = (int64_t)nx * ny * nz;
+    sh = std::is_same<T, float>::value && (d.storage == F16S || d.storage == BF16S);
     for (int i = 0; i < L.Q; ++i) w[i] = (T)L.w[i];
     if (d.collision != BGK) {
       mrt = true;
@@ -313,6 +367,12 @@ struct Sim {
     if (t & SOLID) { ux = 0; uy = 0; uz = 0; rho = 0; }
     if (t & RHO_FIX) rho = (T)d.rho_bc;
   }
+  // the same in the shifted form: drho follows rho
+  void overrides_s(int64_t n, int y, int z, T &drho, T &rho, T &ux, T &uy, T &uz) const {
+    overrides(n, y, z, rho, ux, uy, uz);
+    if (type[n] & SOLID) drho = (T)-1;
+    if (type[n] & RHO_FIX) drho = (T)(d.rho_bc - 1.0);
+  }
 
   // Equilibrium.  FP64: the reference's expression term by term (cpp/d2q9_bgk_lid.cpp:149-151).
   // FP32: the contracted form w rho (A + eu (3 + 4.5 eu)), A = 1 - 1.5 u.u, as the reference's
@@ -337,6 +397,27 @@ struct Sim {
     } else {
       return (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
     }
+  }
+
+  // Shifted equilibrium (FP32 only): f_eq,i - w_i = w_i (drho + rho P_i), P_i = eu (3 + 4.5 eu)
+  // - 1.5 u.u; m15 = -1.5 u.u.
+  T m15_term(T ux, T uy, T uz) const {
+    T u2 = std::fma(uy, uy, ux * ux);
+    if (L.D == 3) u2 = std::fma(uz, uz, u2);
+    return (T)-1.5 * u2;
+  }
+  T geq(int i, T drho, T rho, T ux, T uy, T uz, T m15) const {
+    T P = m15;
+    if (L.cx[i] != 0 || L.cy[i] != 0 || L.cz[i] != 0) {
+      T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
+      if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
+      P = std::fma(eu, std::fma((T)4.5, eu, (T)3), m15);
+    }
+    return w[i] * std::fma(rho, P, drho);
+  }
+  void bgk_s(const T *S, T drho, T rho, T ux, T uy, T uz, T *out) const {
+    const T omega = (T)d.omega, m = m15_term(ux, uy, uz);
+    for (int i = 0; i < L.Q; ++i) out[i] = std::fma(omega, geq(i, drho, rho, ux, uy, uz, m) - S[i], S[i]);
   }
 
   // BGK relaxation (cpp/d2q9_bgk_lid.cpp:153): omega feq + (1 - omega) f; FP32 f + omega (feq - f).
@@ -396,13 +477,77 @@ struct Sim {
     }
   }
 
-  void moments(const T *S, T &rho, T &ux, T &uy, T &uz) const {
-    T r = 0, jx = 0, jy = 0, jz = 0;
+  // The shifted forms: S holds g = f - w_i, drho = rho - 1; the reference table less its value at
+  // rest, and the projected equilibrium moments as M g_eq.
+  void meq_reference_s(T dr, T r, T ux, T uy, T uz, T *meq) const {
+    const T cs2 = (T)1 / (T)3, cs4 = cs2 * cs2, cs6 = cs2 * cs2 * cs2;
+    const T rvx = r * ux, rvy = r * uy, rvz = r * uz, rc2 = cs2 * r, rc4 = cs4 * r;
+    const T vx2 = ux * ux, vy2 = uy * uy, vz2 = uz * uz, v2 = vx2 + vy2 + vz2;
+    meq[0] = dr; meq[1] = rvx; meq[2] = rvy; meq[3] = rvz;
+    meq[4] = rvx * uy; meq[5] = rvx * uz; meq[6] = rvy * uz;
+    meq[7] = dr + rvx * ux + rvy * uy + rvz * uz;
+    meq[8] = r * (vx2 - vy2); meq[9] = r * (vx2 - vz2);
+    meq[10] = rc2 * ux; meq[11] = rc2 * ux; meq[12] = rc2 * uy; meq[13] = rc2 * uz;
+    meq[14] = rc2 * uy; meq[15] = rc2 * uz; meq[16] = 0;
+    meq[17] = cs4 * dr + rc2 * (vx2 + vy2); meq[18] = cs4 * dr + rc2 * (vx2 + vz2);
+    meq[19] = cs4 * dr + rc2 * (vy2 + vz2);
+    meq[20] = rc2 * uy * uz; meq[21] = rc2 * ux * uz; meq[22] = rc2 * ux * uy;
+    meq[23] = rc4 * ux; meq[24] = rc4 * uy; meq[25] = rc4 * uz;
+    meq[26] = v2 + cs6 * dr;
+  }
+  void mrt_collide_s(const T *S, T drho, T rho, T ux, T uy, T uz, T *out) const {
+    const int Q = L.Q;
+    T m[27], meq[27], mc[27];
+    for (int k = 0; k < Q; ++k) {
+      T acc = 0;
+      for (int i = 0; i < Q; ++i) acc += S[i] * M[k][i];
+      m[k] = acc;
+    }
+    if (Q == 27 && d.collision == MRT_RAW && d.meq_mode == 0) {
+      meq_reference_s(drho, rho, ux, uy, uz, meq);
+    } else {
+      T ge[27];
+      const T m15 = m15_term(ux, uy, uz);
+      for (int i = 0; i < Q; ++i) ge[i] = geq(i, drho, rho, ux, uy, uz, m15);
+      for (int k = 0; k < Q; ++k) {
+        T acc = 0;
+        for (int i = 0; i < Q; ++i) acc += ge[i] * M[k][i];
+        meq[k] = acc;
+      }
+    }
+    for (int k = 0; k < Q; ++k) mc[k] = m[k] - (m[k] - meq[k]) * s[k];
+    for (int i = 0; i < Q; ++i) {
+      T acc = 0;
+      for (int k = 0; k < Q; ++k) acc += mc[k] * Minv[i][k];
+      out[i] = acc;
+    }
+  }
+  void collide_s(const T *S, T drho, T rho, T ux, T uy, T uz, T *out) const {
+    if (mrt) mrt_collide_s(S, drho, rho, ux, uy, uz, out);
+    else bgk_s(S, drho, rho, ux, uy, uz, out);
+  }
+
+  void moment_sums(const T *S, T &r, T &jx, T &jy, T &jz) const {
+    r = 0, jx = 0, jy = 0, jz = 0;
     for (int i = 0; i < L.Q; ++i) r += S[i];
     for (int i = 0; i < L.Q; ++i) jx += (T)L.cx[i] * S[i];
     for (int i = 0; i < L.Q; ++i) jy += (T)L.cy[i] * S[i];
     if (L.D == 3)
       for (int i = 0; i < L.Q; ++i) jz += (T)L.cz[i] * S[i];
+  }
+  // shifted: drho = sum g, rho = 1 + drho, u = (sum c g) * (1 / rho) (FP32 only)
+  void moments_s(const T *S, T &drho, T &rho, T &ux, T &uy, T &uz) const {
+    T r, jx, jy, jz;
+    moment_sums(S, r, jx, jy, jz);
+    drho = r;
+    rho = (T)1 + r;
+    const T inv = (T)1 / rho;
+    ux = jx * inv; uy = jy * inv; uz = (L.D == 3) ? jz * inv : (T)0;
+  }
+
+  void moments(const T *S, T &rho, T &ux, T &uy, T &uz) const {
+    T r, jx, jy, jz;
+    moment_sums(S, r, jx, jy, jz);
     rho = r;
     if (sizeof(T) == 4) {  // FP32 as the kernels: the correctly rounded 1/rho, then products
       const T inv = (T)1 / r;
@@ -413,13 +558,14 @@ struct Sim {
   }
 
   // ---- REFERENCE step, multi-pass as the reference's ArrayFire program ----
-  std::vector<T> S, B, RHO, UX, UY, UZ;
+  std::vector<T> S, B, RHO, DRHO, UX, UY, UZ;
   void ref_step() {
     const int Q = L.Q;
     const int64_t NQ = N * Q, nON = (int64_t)on.size();
     S.resize((size_t)NQ);
     B.resize((size_t)(nON * (Q - 1)));
     RHO.resize((size_t)N); UX.resize((size_t)N); UY.resize((size_t)N); UZ.resize((size_t)N);
+    if (sh) DRHO.resize((size_t)N);
     // K1 gather: F_streamed = F(nb_index)
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < NQ; ++p) S[(size_t)p] = F[(size_t)nb[(size_t)p]];
@@ -434,10 +580,16 @@ struct Sim {
     for (int64_t n = 0; n < N; ++n) {
       T Sn[27];
       for (int i = 0; i < Q; ++i) Sn[i] = S[(size_t)(n + N * i)];
-      T rho, ux, uy, uz;
-      moments(Sn, rho, ux, uy, uz);
+      T rho, ux, uy, uz, drho = 0;
       const int y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
-      overrides(n, y, z, rho, ux, uy, uz);
+      if (sh) {
+        moments_s(Sn, drho, rho, ux, uy, uz);
+        overrides_s(n, y, z, drho, rho, ux, uy, uz);
+        DRHO[(size_t)n] = drho;
+      } else {
+        moments(Sn, rho, ux, uy, uz);
+        overrides(n, y, z, rho, ux, uy, uz);
+      }
       RHO[(size_t)n] = rho; UX[(size_t)n] = ux; UY[(size_t)n] = uy; UZ[(size_t)n] = uz;
     }
     // K4/K7 collision at every node
@@ -445,7 +597,8 @@ struct Sim {
     for (int64_t n = 0; n < N; ++n) {
       T Sn[27], out[27];
       for (int i = 0; i < Q; ++i) Sn[i] = S[(size_t)(n + N * i)];
-      if (mrt) mrt_collide(Sn, RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
+      if (sh) collide_s(Sn, DRHO[(size_t)n], RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
+      else if (mrt) mrt_collide(Sn, RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
       else bgk(Sn, RHO[(size_t)n], UX[(size_t)n], UY[(size_t)n], UZ[(size_t)n], out);
       for (int i = 0; i < Q; ++i) F[(size_t)(n + N * i)] = out[i];
     }
@@ -501,11 +654,21 @@ struct Sim {
       // neighbour's populations as the pull reads them, rho_e and u_e their moments; f_eq(rho_bc,
       // u_in) when that neighbour is solid or absent.  The node then holds rho_e and u_in.
       const T uin = inlet_u(y, z);
-      T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
+      T g[27], rn = (T)1, drn = 0, vx = 0, vy = 0, vz = 0;
       const bool nb = x + 1 < nx && !(type[n + 1] & SOLID);
       if (nb) {
         for (int i = 0; i < L.Q; ++i) g[i] = Fin[(size_t)(n + 1 + N * i)];
-        moments(g, rn, vx, vy, vz);
+        if (sh) moments_s(g, drn, rn, vx, vy, vz);
+        else moments(g, rn, vx, vy, vz);
+      }
+      if (sh) {  // the same with the equilibria as deviations from w_i
+        const T rb = nb ? rn : rw, drb = nb ? drn : (T)(d.rho_bc - 1.0);
+        const T mb = m15_term(uin, (T)0, (T)0), mg = m15_term(vx, vy, vz);
+        for (int i = 0; i < L.Q; ++i) {
+          const T eb = geq(i, drb, rb, uin, (T)0, (T)0, mb);
+          f[i] = nb ? g[i] + (eb - geq(i, drn, rn, vx, vy, vz, mg)) : eb;
+        }
+        return;
       }
       const T rb = nb ? rn : rw;
       const T us_b = usq_term(uin, (T)0, (T)0), us_g = usq_term(vx, vy, vz);
@@ -520,7 +683,8 @@ struct Sim {
         if (L.cx[i] == 0) s0 += f[i];
         if (L.cx[i] == -1) sm += f[i];
       }
-      const T rho = (s0 + (T)2 * sm) / ((T)1 - uin);
+      // shifted: the weights of the c_x = 0 directions plus twice those of c_x = -1 sum to 1
+      const T rho = sh ? (((T)1 + s0) + (T)2 * sm) / ((T)1 - uin) : (s0 + (T)2 * sm) / ((T)1 - uin);
       nebb(f, 1, rho, uin, (T)0, (T)0);
     } else if (t & OUTLET) {
       // Zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi, Chinese Physics
@@ -528,13 +692,20 @@ struct Sim {
       // u_nb)), g = the upstream neighbour's populations as the pull reads them (its
       // post-collision state of the previous step).  The node then holds rho_bc and u_nb exactly.
       // f_eq is linear in rho: evaluated as g + (rho_bc - rho_nb) f_eq(1, u_nb), FP32 as one FMA.
-      T g[27], rn = (T)1, vx = 0, vy = 0, vz = 0;
+      T g[27], rn = (T)1, drn = 0, vx = 0, vy = 0, vz = 0;
       const bool nb = x > 0 && !(type[n - 1] & SOLID);
       if (nb) {
         for (int i = 0; i < L.Q; ++i) g[i] = Fin[(size_t)(n - 1 + N * i)];
-        moments(g, rn, vx, vy, vz);
+        if (sh) moments_s(g, drn, rn, vx, vy, vz);
+        else moments(g, rn, vx, vy, vz);
       }
       const T usq = usq_term(vx, vy, vz);
+      if (sh) {  // rho_bc - rho_nb between deviations; f_eq(1, u_nb) unshifted
+        const T drbc = (T)(d.rho_bc - 1.0), dr = drbc - drn, mg = m15_term(vx, vy, vz);
+        for (int i = 0; i < L.Q; ++i)
+          f[i] = nb ? std::fma(dr, feq(i, (T)1, vx, vy, vz, usq), g[i]) : geq(i, drbc, rw, vx, vy, vz, mg);
+        return;
+      }
       const T dr = rw - rn;
       for (int i = 0; i < L.Q; ++i) {
         if (nb) {
@@ -559,27 +730,56 @@ struct Sim {
         for (int i = 0; i < Q; ++i) Fnext[(size_t)(n + N * i)] = F[(size_t)(n + N * i)];
         continue;
       }
-      T f[27], out[27], rho, ux, uy, uz;
+      T f[27], out[27], rho, ux, uy, uz, drho;
       phys_pull(F, x, y, z, f);
-      moments(f, rho, ux, uy, uz);
-      if (mrt) mrt_collide(f, rho, ux, uy, uz, out);
-      else bgk(f, rho, ux, uy, uz, out);
+      if (sh) {
+        moments_s(f, drho, rho, ux, uy, uz);
+        collide_s(f, drho, rho, ux, uy, uz, out);
+      } else {
+        moments(f, rho, ux, uy, uz);
+        if (mrt) mrt_collide(f, rho, ux, uy, uz, out);
+        else bgk(f, rho, ux, uy, uz, out);
+      }
       for (int i = 0; i < Q; ++i) Fnext[(size_t)(n + N * i)] = out[i];
     }
     F.swap(Fnext);
   }
 
-  // storage codec applied to the whole state (shifted FP16/BF16)
-  void codec() {
+  // Shifted storages: round g to the storage and back, to nearest (initial states) or
+  // stochastically with random bits r (steps).
+  float round_g(int i, float g) const {
+    (void)i;
+    return d.storage == F16S ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g));
+  }
+  float round_g_sr(float g, uint32_t r) const {
+    return d.storage == F16S ? h2f(f2h_sr(g * 32768.0f, r)) * (1.0f / 32768.0f) : bf2f(f2bf_sr(g, r));
+  }
+  // after a step: every population through the storage with this step's random bits (values
+  // a step did not change are representable and come back unchanged)
+  void codec_step() {
     if constexpr (std::is_same<T, float>::value) {
-      if (d.storage != F16S && d.storage != BF16S) return;
       const int Q = L.Q;
+      const uint32_t key = mix32(tstep + 0x9E3779B9u);
 #pragma omp parallel for schedule(static)
+      for (int64_t n = 0; n < N; ++n) {
+        const uint32_t h = mix32((uint32_t)n ^ key);
+        for (int i = 0; i < Q; ++i) {
+          float &v = F[(size_t)(n + N * i)];
+          v = (d.reserved & 1) ? round_g(i, v) : round_g_sr(v, rotl32(h, (7 * i) & 31));
+        }
+      }
+    }
+    ++tstep;
+  }
+  // an initial state given unshifted (F holds f): F becomes g = f - w_i rounded to nearest
+  void encode_initial() {
+    if constexpr (std::is_same<T, float>::value) {
+      if (!sh) return;
+      const int Q = L.Q;
       for (int64_t n = 0; n < N; ++n)
         for (int i = 0; i < Q; ++i) {
           float &v = F[(size_t)(n + N * i)];
-          const float g = v - w[i];
-          v = (d.storage == F16S ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g))) + w[i];
+          v = round_g(i, v - w[i]);
         }
     }
   }
@@ -587,36 +787,24 @@ struct Sim {
   void step() {
     if (d.bc_mode == 0) ref_step();
     else phys_step();
-    codec();
+    if (sh) codec_step();
   }
 
-  // state in/out as doubles; shifted storages use decode = half + w in double (exact)
+  // state in/out as doubles; shifted storages hold g and decode to g + w_i in double (exact)
   void load(const double *f) {
     F.resize((size_t)(N * L.Q));
     for (int i = 0; i < L.Q; ++i)
       for (int64_t n = 0; n < N; ++n) {
         const double v = f[n + N * i];
-        if (d.storage == F16S || d.storage == BF16S) {
-          const float g = (float)(v - (double)(float)L.w[i]);
-          const float hv = (d.storage == F16S) ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g));
-          F[(size_t)(n + N * i)] = (T)(hv + (float)L.w[i]);
-        } else {
-          F[(size_t)(n + N * i)] = (T)v;
-        }
+        if (sh) F[(size_t)(n + N * i)] = (T)round_g(i, (float)(v - (double)(float)L.w[i]));
+        else F[(size_t)(n + N * i)] = (T)v;
       }
   }
   void store(double *f) const {
     for (int i = 0; i < L.Q; ++i)
       for (int64_t n = 0; n < N; ++n) {
         const T v = F[(size_t)(n + N * i)];
-        if (d.storage == F16S || d.storage == BF16S) {
-          const float fv = (float)v;
-          const float g = fv - (float)L.w[i];
-          const float hv = (d.storage == F16S) ? h2f(f2h(g * 32768.0f)) * (1.0f / 32768.0f) : bf2f(f2bf(g));
-          f[n + N * i] = (double)hv + (double)(float)L.w[i];
-        } else {
-          f[n + N * i] = (double)v;
-        }
+        f[n + N * i] = sh ? (double)v + (double)(float)L.w[i] : (double)v;
       }
   }
 };
@@ -632,8 +820,9 @@ bool desc_ok(const oracle_desc *d) {
 }
 
 template <class T>
-int run_impl(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32c) {
+int run_impl(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32c, uint32_t step0) {
   Sim<T> sim(*d, types);
+  sim.tstep = step0;
   sim.load(f);
   if (d->bc_mode == 0 && !sim.build_pull_map(f32c != 0)) return 3;  // rounded index out of range
   for (int64_t t = 0; t < nsteps; ++t) sim.step();
@@ -655,7 +844,7 @@ int init_fields_impl(const oracle_desc *d, const uint8_t *types, const double *r
     for (int i = 0; i < Q; ++i) out[i] = sim.feq(i, r, vx, vy, vz, usq);
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
-  sim.codec();
+  sim.encode_initial();
   sim.store(f);
   return 0;
 }
@@ -677,7 +866,7 @@ int init_reference_impl(const oracle_desc *d, const uint8_t *types, double rho0,
     }
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
-  sim.codec();
+  sim.encode_initial();
   sim.store(f);
   return 0;
 }
@@ -692,22 +881,29 @@ int fields_impl(const oracle_desc *d, const uint8_t *types, const double *f, dou
     for (int y = 0; y < sim.ny; ++y)
       for (int x = 0; x < sim.nx; ++x) {
         const int64_t n = sim.idx(x, y, z);
-        T S[27], r, vx, vy, vz;
+        T S[27], r, vx, vy, vz, dr = -1;
         if (d->bc_mode == 0) {
           for (int i = 0; i < Q; ++i) {
             const int64_t src = sim.idx(Sim<T>::wrap(x - sim.L.cx[i], sim.nx), Sim<T>::wrap(y - sim.L.cy[i], sim.ny),
                                         Sim<T>::wrap(z - sim.L.cz[i], sim.nz));
             S[i] = sim.F[(size_t)(src + sim.N * i)];
           }
-          sim.moments(S, r, vx, vy, vz);
-          sim.overrides(n, y, z, r, vx, vy, vz);
+          if (sim.sh) {
+            sim.moments_s(S, dr, r, vx, vy, vz);
+            sim.overrides_s(n, y, z, dr, r, vx, vy, vz);
+          } else {
+            sim.moments(S, r, vx, vy, vz);
+            sim.overrides(n, y, z, r, vx, vy, vz);
+          }
         } else if (types[n] & SOLID) {
           r = 0; vx = 0; vy = 0; vz = 0;
         } else {
           sim.phys_pull(sim.F, x, y, z, S);
-          sim.moments(S, r, vx, vy, vz);
+          if (sim.sh) sim.moments_s(S, dr, r, vx, vy, vz);
+          else sim.moments(S, r, vx, vy, vz);
         }
-        if (rho) rho[n] = (double)r;
+        // shifted storages report rho = 1 + drho in FP64, as the kernels do
+        if (rho) rho[n] = sim.sh ? 1.0 + (double)dr : (double)r;
         if (ux) ux[n] = (double)vx;
         if (uy) uy[n] = (double)vy;
         if (uz) uz[n] = (double)vz;
@@ -721,7 +917,7 @@ double time_impl(const oracle_desc *d, const uint8_t *types, int64_t nsteps, int
   const int Q = sim.L.Q;
   sim.F.assign((size_t)(sim.N * Q), (T)0);
   for (int i = 0; i < Q; ++i)
-    for (int64_t n = 0; n < sim.N; ++n) sim.F[(size_t)(n + sim.N * i)] = sim.w[i];
+    for (int64_t n = 0; n < sim.N; ++n) sim.F[(size_t)(n + sim.N * i)] = sim.sh ? (T)0 : sim.w[i];
   if (d->bc_mode == 0) sim.build_pull_map(false);
   for (int64_t t = 0; t < warmup; ++t) sim.step();  // warm-up (page in temporaries)
   const auto t0 = std::chrono::steady_clock::now();
@@ -918,10 +1114,16 @@ int oracle_init_reference(const oracle_desc *d, const uint8_t *types, double rho
                    : init_reference_impl<float>(d, types, rho0, f);
 }
 
-int oracle_run(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32_index_compat) {
+// step0: the step count the run starts from (the shifted storages' random bits depend on it; a
+// lattice's count restarts at 0 with every initial state).
+int oracle_run_from(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32_index_compat,
+                    uint32_t step0) {
   if (!desc_ok(d)) return 1;
-  return is_f64(d) ? run_impl<double>(d, types, f, nsteps, f32_index_compat)
-                   : run_impl<float>(d, types, f, nsteps, f32_index_compat);
+  return is_f64(d) ? run_impl<double>(d, types, f, nsteps, f32_index_compat, step0)
+                   : run_impl<float>(d, types, f, nsteps, f32_index_compat, step0);
+}
+int oracle_run(const oracle_desc *d, const uint8_t *types, double *f, int64_t nsteps, int f32_index_compat) {
+  return oracle_run_from(d, types, f, nsteps, f32_index_compat, 0);
 }
 
 int oracle_fields(const oracle_desc *d, const uint8_t *types, const double *f, double *rho, double *ux,
@@ -940,6 +1142,9 @@ double oracle_time_steps(const oracle_desc *d, const uint8_t *types, int64_t nst
 
 // Codecs exposed for tests (checked against numpy's float16 conversion).
 uint16_t oracle_f2h(float f) { return f2h(f); }
+uint16_t oracle_f2h_sr(float f, uint32_t r) { return f2h_sr(f, r); }
+uint16_t oracle_f2bf_sr(float f, uint32_t r) { return f2bf_sr(f, r); }
+uint32_t oracle_mix32(uint32_t x) { return mix32(x); }
 float oracle_h2f(uint16_t h) { return h2f(h); }
 uint16_t oracle_f2bf(float f) { return f2bf(f); }
 float oracle_bf2f(uint16_t b) { return bf2f(b); }

@@ -51,6 +51,13 @@ def lib():
         L.oracle_init_fields.argtypes = [dsc, u8p, dp, dp, dp, dp, dp]
         L.oracle_init_reference.argtypes = [dsc, u8p, ctypes.c_double, dp]
         L.oracle_run.argtypes = [dsc, u8p, dp, ctypes.c_int64, ctypes.c_int]
+        L.oracle_run_from.argtypes = [dsc, u8p, dp, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32]
+        L.oracle_f2h_sr.argtypes = [ctypes.c_float, ctypes.c_uint32]
+        L.oracle_f2h_sr.restype = ctypes.c_uint16
+        L.oracle_f2bf_sr.argtypes = [ctypes.c_float, ctypes.c_uint32]
+        L.oracle_f2bf_sr.restype = ctypes.c_uint16
+        L.oracle_mix32.argtypes = [ctypes.c_uint32]
+        L.oracle_mix32.restype = ctypes.c_uint32
         L.oracle_fields.argtypes = [dsc, u8p, dp, dp, dp, dp, dp]
         L.oracle_time_steps.argtypes = [dsc, u8p, ctypes.c_int64, ctypes.c_int64]
         L.oracle_time_steps.restype = ctypes.c_double
@@ -94,6 +101,8 @@ def make_desc(cfg):
     d.meq_mode = cfg.get("meq_mode", 0)
     d.inlet_profile = cfg.get("inlet_profile", 0)
     d.inlet_closure = cfg.get("inlet_closure", 0)
+    # test switch of the oracle alone: shifted storages round to nearest in the steps
+    d.reserved = 1 if cfg.get("oracle_round_nearest", False) else 0
     d.omega = cfg.get("omega", 1.0)
     s = cfg.get("s", [1.0] * 27)
     for k in range(27):
@@ -131,10 +140,13 @@ def init_reference(cfg, types, rho0=1.0):
     return f
 
 
-def run(cfg, types, f, nsteps, f32_index_compat=False):
+def run(cfg, types, f, nsteps, f32_index_compat=False, step0=0):
+    """nsteps steps from populations f.  step0: the lattice's step count at f (the shifted
+    storages' stochastic rounding draws on it; 0 after an initial state)."""
     d = make_desc(cfg)
     out = np.array(f, dtype=np.float64, copy=True)
-    rc = lib().oracle_run(ctypes.byref(d), _u8(types), _dp(out), int(nsteps), int(f32_index_compat))
+    rc = lib().oracle_run_from(ctypes.byref(d), _u8(types), _dp(out), int(nsteps), int(f32_index_compat),
+                               int(step0))
     if rc == 3:
         raise ValueError("f32_index_compat: a pull index rounded through float falls outside the lattice")
     assert rc == 0, rc
@@ -189,6 +201,18 @@ def f2h(x):
 
 def h2f(h):
     return lib().oracle_h2f(int(h))
+
+
+def f2h_sr(x, r):
+    return lib().oracle_f2h_sr(float(x), int(r))
+
+
+def f2bf_sr(x, r):
+    return lib().oracle_f2bf_sr(float(x), int(r))
+
+
+def mix32(x):
+    return lib().oracle_mix32(int(x))
 
 
 def f2bf(x):

@@ -25,7 +25,7 @@ def test_header_matches_binding_list():
 def test_library_exports_every_declared_symbol(lbm_lib):
     missing = [s for s in declared_functions() if not hasattr(lbm_lib, s)]
     assert not missing, missing
-    assert lbm_lib.lbm_abi_version() == 2
+    assert lbm_lib.lbm_abi_version() == 3
 
 
 def test_desc_layout_matches_header():

@@ -40,7 +40,7 @@ def test_d3q27_mrt_lid_128_re400_20000_steps():
     runs = {}
     for tag, st, en in (("fp64", F64, MRT_SCALAR), ("fp32_scalar", F32, MRT_SCALAR), ("fp32_mfma2", F32, MRT_MFMA_F32),
                         ("fp32_fused", F32, MRT_MFMA_FUSED), ("fp32_bf16x3", F32, MRT_MFMA_BF16X3),
-                        ("f16s_scalar", F16S, MRT_SCALAR)):
+                        ("f16s_scalar", F16S, MRT_SCALAR), ("f16s_bf16x3", F16S, MRT_MFMA_BF16X3)):
         cfg, types = cases.d3q27_mrt_lid_gs(n, re=400.0, storage=st, engine=en)
         runs[tag] = run_fields(cfg, types, steps, lambda lat: lat.init_fields())
     errs = {k: errors(v, runs["fp64"]) for k, v in runs.items() if k != "fp64"}
@@ -49,10 +49,12 @@ def test_d3q27_mrt_lid_128_re400_20000_steps():
     assert max(errs["fp32_scalar"]) <= 5e-5
     for k in ("fp32_mfma2", "fp32_fused", "fp32_bf16x3"):
         assert max(errs[k]) <= 1e-4, k
-    # F16S: each step rounds every population to 2^-11 of its deviation from w_i (about 4e-5 of f
-    # where u is 0.05), and at tau = 0.548 these errors add up over 20,000 steps to a few percent of
-    # u; the configuration asks for FP32, the bound only records the level
-    assert max(errs["f16s_scalar"]) <= 5e-2
+    # F16S: each step rounds every population to 2^-11 of its deviation from w_i.  Rounded to
+    # nearest, the errors of this nearly steady flow repeat every step and added up to 3e-2 in u;
+    # computed on the deviations and rounded stochastically they stay at the level of one step's
+    # rounding (measured 2.3e-4)
+    for k in ("f16s_scalar", "f16s_bf16x3"):
+        assert max(errs[k]) <= 1e-3, k
 
 
 def test_sphere_duct_shifted_storage_and_drag():
@@ -77,7 +79,7 @@ def test_sphere_duct_shifted_storage_and_drag():
         dcd = abs(res[tag][1] / res["fp64"][1] - 1)
         print(f"sphere duct {nx}x{ny}x{nz} r {r} Re 50, {steps} steps, {tag} vs FP64: rho {er:.2e}, u {eu:.2e}, "
               f"C_d {res[tag][1]:.4f} vs {res['fp64'][1]:.4f} ({dcd:.1e})")
-        bound = {"fp32": 1e-4, "f16s": 1e-3, "bf16s": 2e-2}[tag]
-        if not (er <= bound and eu <= bound and dcd <= {"fp32": 1e-4, "f16s": 5e-3, "bf16s": 5e-2}[tag]):
+        bound = {"fp32": 1e-4, "f16s": 1e-3, "bf16s": 5e-3}[tag]
+        if not (er <= bound and eu <= bound and dcd <= {"fp32": 1e-4, "f16s": 1e-3, "bf16s": 2e-2}[tag]):
             bad.append(tag)
     assert not bad, bad

@@ -19,6 +19,9 @@ from lbm_amd.capi import BC_PHYSICAL, BC_REFERENCE, BGK, F16S, F32, F64, BF16S, 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
@@ -742,3 +745,90 @@ def test_f32_index_compat_refuses_an_index_past_the_array():
     assert int(np.float32(9 * nx * ny - 1)) == 9 * nx * ny
     with pytest.raises(ValueError, match="outside"):
         orc.run(cfg, np.zeros(nx * ny, np.uint8), np.zeros(9 * nx * ny), 1, f32_index_compat=True)
+
+
+# ---- shifted storages: stochastic rounding and arithmetic on deviations ------------------------
+
+def test_stochastic_rounding_matches_mi355x():
+    """The oracle's restatement of v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32 equals the conversions'
+    outputs on MI355X for 16,384 (x, r) pairs (tests/golden/conversions/sr_conversions.npz: normal and
+    subnormal half magnitudes, zeros, the largest halves, overflow).  Taking the random bits from
+    the bottom of r instead of the top (a plausible misreading) fails it."""
+    d = np.load(os.path.join(GOLDEN, "conversions", "sr_conversions.npz"))
+    x, r = d["x"], d["r"]
+    got16 = np.array([orc.f2h_sr(a, b) for a, b in zip(x, r)], np.uint16)
+    gotbf = np.array([orc.f2bf_sr(a, b) for a, b in zip(x, r)], np.uint16)
+    assert np.array_equal(got16, d["f16"]) and np.array_equal(gotbf, d["bf16"])
+    low = np.array([orc.f2h_sr(a, (int(b) & 0x1FFF) << 19) for a, b in zip(x[:2000], r[:2000])], np.uint16)
+    assert (low != d["f16"][:2000]).mean() > 0.2
+
+
+def test_stochastic_rounding_is_unbiased_and_exact_on_representable_values():
+    """Over all 2^13 (half) or 2^16 (bfloat16) values of the bits the conversion uses, the mean of
+    the rounded value is x (the defining property of stochastic rounding): exactly for normal
+    halves and bfloat16, within 2^-13 of the ulp for half subnormals (whose bits below that are
+    cut first); and a value the format holds comes back unchanged for every r."""
+    rng = np.random.default_rng(5)
+    for x in np.float32(rng.uniform(-3e3, 3e3, 6)).tolist() + [np.float32(3.3e-6), np.float32(-7.7e-7)]:
+        vals = [orc.h2f(orc.f2h_sr(x, k << 19)) for k in range(1 << 13)]
+        ulp = 2.0 ** -24 if abs(x) < 2.0 ** -14 else 2.0 ** (np.floor(np.log2(abs(x))) - 10)
+        tol = 0.0 if abs(x) >= 2.0 ** -14 else ulp * 2.0 ** -13
+        assert abs(np.mean(np.array(vals, np.float64)) - np.float64(np.float32(x))) <= tol
+    for x in np.float32(rng.uniform(-1, 1, 4)).tolist():
+        vals = [orc.bf2f(orc.f2bf_sr(x, k << 16)) for k in range(1 << 16)]
+        assert abs(np.mean(np.array(vals, np.float64)) - np.float64(np.float32(x))) <= 1e-12
+    for h in (0x3C00, 0x0001, 0x83FF, 0x7BFF, 0x1234):
+        x = orc.h2f(h)
+        assert all(orc.f2h_sr(x, int(r)) == h for r in rng.integers(0, 2 ** 32, 50, dtype=np.uint64))
+
+
+def _lid_2d(side, re, storage, **extra):
+    nu = 0.05 * (side - 2) / re
+    cfg = dict(lattice=9, nx=side, ny=side, nz=1, collision=BGK, storage=storage, bc_mode=BC_PHYSICAL,
+               omega=cases.omega_from_nu(nu), u_wall=(0.05, 0.0, 0.0), **extra)
+    return cfg, geo.lid_cavity_2d_physical(side, side).reshape(-1)
+
+
+def test_shifted_storage_holds_a_long_lid_run_to_fp64():
+    """D2Q9 lid cavity 128^2 at tau = 1 (Re 40), 4,000 steps from rest: the F16S state computed on
+    deviations and stored with stochastic rounding stays within 1e-3 rel-L2 of the FP64 run in u
+    and 1e-5 in rho, and its mean density within 1e-6 (measured 3.1e-4, 3.6e-6, 4.1e-7).  Rounded to
+    nearest instead (the oracle's test switch), the frozen rounding errors of the quasi-steady flow
+    act as a source: the mean density drifts 40 times as far (1.7e-5) and u more than twice as
+    far."""
+    cfg64, types = _lid_2d(128, 40.0, F64)
+    fluid = (types & 1) == 0
+    ref = orc.fields(cfg64, types, orc.run(cfg64, types, orc.init_fields(cfg64, types), 4000))
+    out = {}
+    for rn in (False, True):
+        cfg, _ = _lid_2d(128, 40.0, F16S, oracle_round_nearest=rn)
+        got = orc.fields(cfg, types, orc.run(cfg, types, orc.init_fields(cfg, types), 4000))
+        e_u = np.linalg.norm(np.stack(got[1:3]) - np.stack(ref[1:3])) / np.linalg.norm(np.stack(ref[1:3]))
+        out[rn] = (rel_l2(got[0], ref[0]), e_u, abs((got[0] - ref[0])[fluid].mean()))
+    (er, eu, dm), (_, eu_rn, _) = out[False], out[True]
+    assert eu <= 1e-3 and er <= 1e-5 and dm <= 1e-6, out
+    (_, eu_rn, dm_rn) = out[True]
+    assert dm_rn > 5e-6 and eu_rn > 2 * eu, out
+
+
+@pytest.mark.parametrize("collision", [BGK, MRT_GS], ids=["bgk", "mrt_gs"])
+def test_shifted_step_is_the_fp64_step_to_the_storage_rounding(collision):
+    """One step on deviations, from a representable F16S state of a D3Q27 sphere duct, rounded to
+    nearest: every population is within half a storage ulp of the FP64 step from the same state,
+    plus 2e-9 for the FP32 arithmetic on deviations up to 0.03 (measured 1.2e-9).  The unshifted
+    FP32 step, whose rounding is relative to f ~ w_i, misses the same bound by up to 1e-7 -- the
+    error that the arithmetic on deviations removes; a sign or weight wrong in the shifted
+    equilibrium would move whole populations."""
+    cfg, types = cases.d3q27_sphere_channel(20, 10, 10, (6, 5, 5), 2, re=5.0, storage=F16S, collision=collision)
+    types = types.reshape(-1)
+    rng = np.random.default_rng(8)
+    N = types.size
+    u = [0.03 * rng.standard_normal(N) for _ in range(3)]
+    f16 = orc.init_fields(cfg, types, 1 + 1e-3 * rng.standard_normal(N), *u)
+    a = orc.run(dict(cfg, oracle_round_nearest=True), types, f16, 1).reshape(27, -1)
+    ref = orc.run(dict(cfg, storage=F64), types, f16, 1).reshape(27, -1)
+    b32 = orc.run(dict(cfg, storage=F32), types, f16, 1).reshape(27, -1)
+    _, w, _ = orc.tables(27)
+    half_ulp = np.abs(ref - w[:, None]).clip(2.0 ** -29) * 2.0 ** -11
+    assert np.all(np.abs(a - ref) <= half_ulp + 2e-9)
+    assert np.abs(b32 - ref).max() > 2e-8
