@@ -40,10 +40,16 @@ template <int ST, int Q> struct Codec;
 //                    initial states);
 //   st<I>(p, k, v, r)  p[k] = enc<I>(v, r).
 // shifted: the compute value is the deviation g = f - w_i, not f (the 2-byte storages).
+// S: the compute scale.  Shifted codecs hand out S g (S an exact power of two), and every step
+// computes in those units: each operation on S-scaled operands rounds to S times its unscaled
+// result, so a step in scaled units equals the unscaled step bit for bit, while loads and stores
+// need no scaling multiply.  Quantities that do not scale (rho, u) stay unscaled; rho S and the
+// density deviation S drho enter the equilibria.
 template <int Q> struct Codec<ST_F64, Q> {
   using T = double;
   using C = double;
   static constexpr bool shifted = false;
+  static constexpr float S = 1.0f;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
   template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
@@ -53,6 +59,7 @@ template <int Q> struct Codec<ST_F32, Q> {
   using T = float;
   using C = float;
   static constexpr bool shifted = false;
+  static constexpr float S = 1.0f;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
   template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
@@ -61,7 +68,7 @@ template <int Q> struct Codec<ST_F32, Q> {
 // Shifted half storage: the stored value is (f - w_i) * 2^15 in IEEE half.  The exact
 // power-of-two scale keeps the small deviations of the low-weight D3Q27 directions out of the
 // half-precision subnormal range, so every stored value carries 11 significant bits.
-// The step kernels compute on g = f - w_i itself (decoded exactly: h * 2^-15) and store it with
+// The compute value is the stored half itself, 2^15 g (S = 2^15), stored with
 // stochastic rounding (v_cvt_sr_f16_f32: round up with probability equal to the fraction of the
 // half ulp that g exceeds the value below, from the random bits r).  A population whose state
 // does not change from one step to the next would otherwise round the same way every step, and in
@@ -79,12 +86,15 @@ template <int Q> struct Codec<ST_F16S, Q> {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr float S = F16S_SCALE;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
     const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
-    return (float)__builtin_bit_cast(_Float16, v) * F16S_INV_SCALE;
+    return (float)__builtin_bit_cast(_Float16, v);
   }
+  // the conversion writes the low half of its destination and keeps the high half, which no
+  // store reads: the destination starts as the value's own register (no zeroing move)
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
-    const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32((half2_t){0, 0}, v * F16S_SCALE, r, false);
+    const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2_t, v), v, r, false);
     return __builtin_bit_cast(uint16_t, h[0]);
   }
   // (f - w_i) * 2^15 as one FMA: the scale is a power of two, so this rounds like the separate
@@ -101,12 +111,13 @@ template <int Q> struct Codec<ST_BF16S, Q> {
   using T = uint16_t;
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr float S = 1.0f;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
     const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
     return __builtin_bit_cast(float, (uint32_t)v << 16);
   }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
-    const bf162_t h = __builtin_amdgcn_cvt_sr_bf16_f32((bf162_t){0, 0}, v, r, false);
+    const bf162_t h = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf162_t, v), v, r, false);
     return __builtin_bit_cast(uint16_t, h[0]);
   }
   template <int I> __device__ static __forceinline__ T enc_f(C v) {
@@ -256,16 +267,23 @@ __device__ __forceinline__ void pull_periodic(const P &p, const NodePos &q,
   using T = typename CD::T;
   const T *fin = (const T *)p.fin;
   const int xm = q.x == 0 ? p.nx - 1 : q.x - 1, xp = q.x == p.nx - 1 ? 0 : q.x + 1;
-  const int ym = q.y == 0 ? p.ny - 1 : q.y - 1, yp = q.y == p.ny - 1 ? 0 : q.y + 1;
-  const int zm = q.z == 0 ? p.nz - 1 : q.z - 1, zp = q.z == p.nz - 1 ? 0 : q.z + 1;
+  // source rows as the node's row plus wrapped y and z steps (modulo 2^32, exact once summed):
+  // additions instead of one 32-bit multiply (a quarter-rate instruction) per (y, z) pair
+  const uint32_t nx = (uint32_t)p.nx, nxy = nx * (uint32_t)p.ny;
+  const uint32_t row = q.n - (uint32_t)q.x;
+  const uint32_t dym = q.y == 0 ? nxy - nx : 0u - nx, dyp = q.y == p.ny - 1 ? nx - nxy : nx;
+  const uint32_t dzm = q.z == 0 ? nxy * (uint32_t)(p.nz - 1) : 0u - nxy;
+  const uint32_t dzp = q.z == p.nz - 1 ? 0u - nxy * (uint32_t)(p.nz - 1) : nxy;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
     const int sx = cx == 1 ? xm : (cx == -1 ? xp : q.x);
-    const int sy = cy == 1 ? ym : (cy == -1 ? yp : q.y);
-    const int sz = cz == 1 ? zm : (cz == -1 ? zp : q.z);
-    const uint32_t src = (uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz);
-    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, src);
+    uint32_t r = row;
+    if constexpr (cz == 1) r += dzm;
+    if constexpr (cz == -1) r += dzp;
+    if constexpr (cy == 1) r += dym;
+    if constexpr (cy == -1) r += dyp;
+    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, r + (uint32_t)sx);
   });
 }
 
@@ -294,7 +312,8 @@ __device__ __forceinline__ C c_times(C u) {
 template <int Q, class C>
 __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz);
 template <int Q>
-__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz);
+__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz,
+                                          float S);
 template <int Q, int i, class C>
 __device__ __forceinline__ C cdotu(C ux, C uy, C uz);
 template <int Q, int i, class C>
@@ -346,10 +365,10 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   west(IC<j>)      population j of the node at x - 1;  west_type()  its type
 //   east(IC<j>)      population j of the node at x + 1;  east_type()  its type
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
-// 6 w_i rho_w c_i.u_w when the source is a moving wall.
+// 6 w_i rho_w c_i.u_w when the source is a moving wall (rho_w taken in compute units, S rho_bc).
 template <int Q, class C, class SrcType, class Own, class P>
-__device__ __forceinline__ void wall_links(const P &p, C *f, SrcType &&src_type, Own &&own) {
-  const C rw = (C)p.rho_bc;
+__device__ __forceinline__ void wall_links(const P &p, C *f, C S, SrcType &&src_type, Own &&own) {
+  const C rw = (C)p.rho_bc * S;
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
     constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
@@ -375,25 +394,26 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, SrcType &&src_type,
 // and rho_g, u_g their moments.  rho_b < 0 takes rho_b = rho_g (velocity inlet), otherwise u_b =
 // u_g (pressure outlet: f_eq is linear in rho, so g_i + (rho_b - rho_g) f_eq,i(1, u_g), one FMA in
 // FP32).  Without a fluid neighbour the node takes f_eq(rho_b, u_b) (rho_b = rho_bc).
-// SH: populations in the shifted representation (f - w_i); the same rules, with the equilibria
-// as deviations from w_i and the density differences taken between density deviations.
+// SH: populations in the shifted representation (f - w_i, in compute units S); the same rules,
+// with the equilibria as deviations from w_i and the density differences taken between density
+// deviations (all S-scaled).
 template <int Q, bool SH = false, class C, class Nbr, class P>
-__device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, bool set_u, C ubx, Nbr &&nbr) {
+__device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rho_b, bool set_u, C ubx, Nbr &&nbr) {
   C g[Q], rn = 1, drn = 0, vx = 0, vy = 0, vz = 0;
   if (nb) {
     static_for<Q>([&](auto I) { g[decltype(I)::value] = nbr(I); });
-    if constexpr (SH) moments_s<Q>(g, drn, rn, vx, vy, vz);
+    if constexpr (SH) moments_s<Q>(g, drn, rn, vx, vy, vz, S);
     else moments<Q>(g, rn, vx, vy, vz);
   }
   if constexpr (SH) {
-    const float drbc = (float)(p.rho_bc - 1.0);
+    const float drbc = (float)(p.rho_bc - 1.0) * S, rbc = (float)p.rho_bc * S;
     if (set_u) {  // velocity inlet
-      const float rb = nb ? rn : (float)p.rho_bc, drb = nb ? drn : drbc;
+      const float rb = nb ? rn * S : rbc, drb = nb ? drn : drbc;
       const float mb = m15_of<Q>(ubx, 0.0f, 0.0f), mg = m15_of<Q>(vx, vy, vz);
       static_for<Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         const float eb = geq_i<Q, i>(drb, rb, ubx, 0.0f, 0.0f, mb);
-        f[i] = nb ? g[i] + (eb - geq_i<Q, i>(drn, rn, vx, vy, vz, mg)) : eb;
+        f[i] = nb ? g[i] + (eb - geq_i<Q, i>(drn, rn * S, vx, vy, vz, mg)) : eb;
       });
       return;
     }
@@ -402,7 +422,7 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, 
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       if (nb) f[i] = __builtin_fmaf(dr, feq_i<Q, i>(1.0f, vx, vy, vz, usq), g[i]);
-      else f[i] = geq_i<Q, i>(drbc, (float)p.rho_bc, vx, vy, vz, mg);
+      else f[i] = geq_i<Q, i>(drbc, rbc, vx, vy, vz, mg);
     });
     return;
   }
@@ -430,18 +450,19 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, bool nb, C rho_b, 
   });
 }
 
+// S: the compute scale of the populations (Codec::S; 1 unless shifted).
 template <int Q, bool SH = false, class C, class SrcType, class Own, class West, class WestType, class East,
           class EastType, class P>
-__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f,
+__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type, East &&east, EastType &&east_type) {
   const C rw = (C)p.rho_bc;
   if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
-    wall_links<Q>(p, f, src_type, own);
+    wall_links<Q>(p, f, S, src_type, own);
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
     const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
-    extrapolate<Q, SH>(p, f, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
+    extrapolate<Q, SH>(p, f, S, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
   } else if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = (C)-0.0, sm = (C)-0.0;  // sums from -0, as in moments()
@@ -450,15 +471,17 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
       if constexpr (Lat<Q>::cx[i] == 0) s0 += f[i];
       if constexpr (Lat<Q>::cx[i] == -1) sm += f[i];
     });
-    // shifted: the weights of the c_x = 0 directions plus twice those of c_x = -1 sum to 1
-    const C rho = SH ? (((C)1 + s0) + (C)2 * sm) / ((C)1 - uin) : (s0 + (C)2 * sm) / ((C)1 - uin);
-    nebb_close<Q, 1>(f, rho, uin, (C)0, (C)0);
+    // shifted: the weights of the c_x = 0 directions plus twice those of c_x = -1 sum to 1 (the
+    // sums unscaled exactly; nebb_close is homogeneous in (f, rho), so it takes S rho)
+    const C rho = SH ? (((C)1 + s0 * ((C)1 / S)) + (C)2 * (sm * ((C)1 / S))) / ((C)1 - uin)
+                     : (s0 + (C)2 * sm) / ((C)1 - uin);
+    nebb_close<Q, 1>(f, SH ? rho * S : rho, uin, (C)0, (C)0);
   } else if (t & NODE_OUTLET) {
     // zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi 2002), all i:
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
     // post-collision populations; the node then holds rho_bc and u_nb exactly.
     const bool nb = x > 0 && !(west_type() & NODE_SOLID);
-    extrapolate<Q, SH>(p, f, nb, rw, false, (C)0, west);
+    extrapolate<Q, SH>(p, f, S, nb, rw, false, (C)0, west);
   }
 }
 
@@ -473,7 +496,7 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
   physical_fixups<Q, CD::shifted>(
-      p, q.x, q.y, q.z, t, f,
+      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S,
       [&](auto I) -> uint8_t {
         constexpr int i = decltype(I)::value;
         constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
@@ -557,15 +580,18 @@ __device__ __forceinline__ void moments(const C *f, C &rho, C &ux, C &uy, C &uz)
   }
 }
 
-// Shifted representation (g_i = f_i - w_i): drho = sum_i g_i, the density deviation, keeps the
-// precision of the small g; rho = 1 + drho; u = (sum_i c_i g_i) / rho, since sum_i c_i w_i = 0.
+// Shifted representation (g_i = f_i - w_i, in compute units S): S drho = sum_i S g_i, the
+// density deviation, keeps the precision of the small g; rho = 1 + drho; u = (sum_i c_i g_i) /
+// rho, since sum_i c_i w_i = 0.  drho is returned in compute units, rho and u unscaled; the
+// unscaling is folded into one FMA and the reciprocal (exact powers of two).
 template <int Q>
-__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz) {
+__device__ __forceinline__ void moments_s(const float *g, float &drho, float &rho, float &ux, float &uy, float &uz,
+                                          float S) {
   float r, jx, jy, jz;
   moment_sums<Q>(g, r, jx, jy, jz);
   drho = r;
-  rho = 1.0f + r;
-  const float inv = recip_rn(rho);
+  rho = __builtin_fmaf(r, 1.0f / S, 1.0f);
+  const float inv = recip_rn(rho) * (1.0f / S);
   ux = jx * inv;
   uy = jy * inv;
   uz = (Lat<Q>::D == 3) ? jz * inv : 0.0f;
@@ -583,13 +609,13 @@ __device__ __forceinline__ void overrides(const P &p, uint8_t t, int y, int z, C
   if (t & NODE_RHO_FIX) rho = (C)p.rho_bc;
 }
 
-// The same overrides in the shifted representation: drho follows rho.
+// The same overrides in the shifted representation: drho (compute units S) follows rho.
 template <int Q, class P>
 __device__ __forceinline__ void overrides_s(const P &p, uint8_t t, int y, int z, float &drho, float &rho, float &ux,
-                                            float &uy, float &uz) {
+                                            float &uy, float &uz, float S) {
   overrides<Q>(p, t, y, z, rho, ux, uy, uz);
-  if (t & NODE_SOLID) drho = -1.0f;
-  if (t & NODE_RHO_FIX) drho = (float)(p.rho_bc - 1.0);
+  if (t & NODE_SOLID) drho = -S;
+  if (t & NODE_RHO_FIX) drho = (float)(p.rho_bc - 1.0) * S;
 }
 
 // c_i . u evaluated as the reference does: (cx ux + cy uy) + cz uz with exact +-1/0 products.
@@ -743,8 +769,9 @@ __device__ __forceinline__ void meq_reference(C r, C ux, C uy, C uz, C *meq) {
 }
 
 // The same table less its value at rest (rho = 1, u = 0), from drho = rho - 1: the rows of the
-// shifted representation.
-__device__ __forceinline__ void meq_reference_s(float dr, float r, float ux, float uy, float uz, float *meq) {
+// shifted representation, in compute units S (dr and r given as S drho and S rho; the one term
+// without a density factor, u.u in row 26, is scaled here).
+__device__ __forceinline__ void meq_reference_s(float dr, float r, float ux, float uy, float uz, float *meq, float S) {
   const float cs2 = 1.0f / 3.0f, cs4 = cs2 * cs2, cs6 = cs2 * cs2 * cs2;
   const float rvx = r * ux, rvy = r * uy, rvz = r * uz, rc2 = cs2 * r, rc4 = cs4 * r;
   const float vx2 = ux * ux, vy2 = uy * uy, vz2 = uz * uz, v2 = vx2 + vy2 + vz2;
@@ -758,7 +785,7 @@ __device__ __forceinline__ void meq_reference_s(float dr, float r, float ux, flo
   meq[19] = cs4 * dr + rc2 * (vy2 + vz2);
   meq[20] = rc2 * uy * uz; meq[21] = rc2 * ux * uz; meq[22] = rc2 * ux * uy;
   meq[23] = rc4 * ux; meq[24] = rc4 * uy; meq[25] = rc4 * uz;
-  meq[26] = v2 + cs6 * dr;
+  meq[26] = v2 * S + cs6 * dr;
 }
 
 // FP32 MRT, the relaxation below (out = f - M^-1 S (M f - meq), S acting in the collision's
@@ -837,12 +864,47 @@ static_assert(raw_rows_match(), "monomial tables disagree with MRAW");
 // SH: f holds g = f - w_i and the raw equilibrium moments are taken less their value at rest
 // (meq(rho, u) - meq(1, 0) = c_0 drho + rho (the u terms), c_0 the constant column), so the
 // deviations are differences of small numbers; the passes below are linear and map g to g'.
+// In compute units S: f, rho and drho come scaled by S (S the codec's compute scale).
+// The monomial moments mu = P f of a node's populations (grid order: slot a + 3 b + 9 c holds
+// sum_i cx^a cy^b cz^c f_i with exponent 2 for a squared component): three passes of the 1D
+// transform.  mu[0] = sum f, mu[1], mu[3], mu[9] = the momentum components.
+__device__ __forceinline__ void mrt27_monomials(const float *f, float *v) {
+  static_for<27>([&](auto G) { v[decltype(G)::value] = f[d27::kGrid.dir[decltype(G)::value]]; });
+  d27::forward_pass<1>(v);
+  d27::forward_pass<3>(v);
+  d27::forward_pass<9>(v);
+}
+
+// rho and u from the monomial moments: the FP32 MRT step takes them from the forward transform it
+// needs anyway instead of separate sums (the summation order of the tensor-product passes, not the
+// ascending one).  SH: drho in compute units S, as moments_s.
+template <bool SH>
+__device__ __forceinline__ void moments_mono(const float *v, float &drho, float &rho, float &ux, float &uy, float &uz,
+                                             float S) {
+  if constexpr (SH) {
+    drho = v[0];
+    rho = __builtin_fmaf(v[0], 1.0f / S, 1.0f);
+    const float inv = recip_rn(rho) * (1.0f / S);
+    ux = v[1] * inv;
+    uy = v[3] * inv;
+    uz = v[9] * inv;
+  } else {
+    rho = v[0];
+    const float inv = recip_rn(rho);
+    ux = v[1] * inv;
+    uy = v[3] * inv;
+    uz = v[9] * inv;
+  }
+}
+
+// mono: the monomial moments of f (mrt27_monomials), or null to compute them here.
 template <int CO, bool SH = false, class P>
 __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, float rho, float ux,
-                                                 float uy, float uz, float *out, float drho = 0.0f) {
+                                                 float uy, float uz, float *out, float drho = 0.0f,
+                                                 float S = 1.0f, const float *mono = nullptr) {
   float d[27];  // the equilibrium raw moments, then the deviations
   if (CO == CO_MRT_RAW && p.meq_mode == 0) {
-    if constexpr (SH) meq_reference_s(drho, rho, ux, uy, uz, d);
+    if constexpr (SH) meq_reference_s(drho, rho, ux, uy, uz, d, S);
     else meq_reference<float>(rho, ux, uy, uz, d);
   } else {
     const float phi[10] = {1.0f, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
@@ -864,10 +926,8 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
     });
   }
   float v[27];
-  static_for<27>([&](auto G) { v[decltype(G)::value] = f[d27::kGrid.dir[decltype(G)::value]]; });
-  d27::forward_pass<1>(v);
-  d27::forward_pass<3>(v);
-  d27::forward_pass<9>(v);
+  if (mono) static_for<27>([&](auto G) { v[decltype(G)::value] = mono[decltype(G)::value]; });
+  else mrt27_monomials(f, v);
   static_for<27>([&](auto K) {
     constexpr int k = decltype(K)::value;
     float raw = v[MRT_RAW_MONO[k][0]];
@@ -932,9 +992,9 @@ __device__ __forceinline__ void collide_mrt_fp32(const P &p, const float *f, flo
 // m = M f; m* = m - s (m - meq); out = M^-1 m*   (rust/examples/d3q27_mrt_lid.rs:336,385-387)
 template <int CO, bool SH = false, class C, class P>
 __device__ __forceinline__ void collide_mrt_scalar(const P &p, const C *f, C rho, C ux, C uy,
-                                                   C uz, C *out, C drho = (C)0) {
+                                                   C uz, C *out, C drho = (C)0, float S = 1.0f) {
   if constexpr (sizeof(C) == 4) {
-    collide_mrt_fp32<CO, SH>(p, f, rho, ux, uy, uz, out, drho);
+    collide_mrt_fp32<CO, SH>(p, f, rho, ux, uy, uz, out, drho, S);
     return;
   }
   C m[27], meq[27];
@@ -1067,12 +1127,16 @@ __device__ __forceinline__ void collide_mrt19(const P &p, const C *f, C rho, C u
   }
 }
 
-// SH: f and out in the shifted representation (f - w_i), drho = rho - 1 kept separately.
+// SH: f and out in the shifted representation (f - w_i) in compute units S (the codec's scale),
+// drho = rho - 1 kept separately in the same units, rho unscaled.  The collisions are homogeneous
+// of degree one in (f, rho, drho) but for the reference table's u.u term (meq_reference_s), so
+// they take S rho.
 template <int Q, int CO, bool SH = false, class C, class P>
-__device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out, C drho = (C)0) {
-  if constexpr (CO == CO_BGK) collide_bgk<Q, SH>(p, f, rho, ux, uy, uz, out, drho);
-  else if constexpr (Q == 19) collide_mrt19<SH>(p, f, rho, ux, uy, uz, out, drho);
-  else collide_mrt_scalar<CO, SH>(p, f, rho, ux, uy, uz, out, drho);
+__device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C uy, C uz, C *out, C drho, C S) {
+  const C rs = SH ? rho * S : rho;
+  if constexpr (CO == CO_BGK) collide_bgk<Q, SH>(p, f, rs, ux, uy, uz, out, drho);
+  else if constexpr (Q == 19) collide_mrt19<SH>(p, f, rs, ux, uy, uz, out, drho);
+  else collide_mrt_scalar<CO, SH>(p, f, rs, ux, uy, uz, out, drho, (float)S);
 }
 
 // Stores the node's Q populations: out in the compute representation, rnd the node's rounding
@@ -1082,9 +1146,20 @@ __device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   T *fout = (T *)p.fout;
+  uint32_t off = n * (uint32_t)sizeof(T);
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    T *dst = fout + (size_t)i * (size_t)p.stride + n;
+    // 2-byte storages: the population's (uniform) base plus a 32-bit byte offset, the scalar-base
+    // store form with one offset register for all Q stores (lbm_create keeps these lattices
+    // below 2^31 nodes).  The empty asm hides that the offsets are equal: the compiler would
+    // otherwise form fout + offset once and add each population's base to it in 64-bit vector
+    // arithmetic, one instruction per store.  Wider storages take 64-bit addresses.
+    T *base = fout + (size_t)i * (size_t)p.stride;
+    T *dst = base + n;
+    if constexpr (sizeof(T) == 2) {
+      asm volatile("" : "+v"(off));
+      dst = (T *)((char *)base + off);
+    }
     const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
     // NT: non-temporal stores for lattices too large for the caches (the next step reads these
     // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s

@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_reference(StepParams p, 
   } else {
     C f[Q];
     static_for<Q>([&](auto I) { f[decltype(I)::value] = (C)rho0 / (C)27; });
-    collide<Q, CO>(p, f, (C)rho0, (C)0, (C)0, (C)0, out);
+    collide<Q, CO>(p, f, (C)rho0, (C)0, (C)0, (C)0, out, (C)0, (C)1);
   }
   store_all_f<Q, ST>(p, q.n, out);
   StepParams p2 = p;
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_monitors(StepParams p, double
     bool finite = true;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+      double v = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n) * (1.0 / CD::S);
       if constexpr (CD::shifted) v += Lat<Q>::w[i];  // f = g + w_i, exact in FP64
       finite = finite && __builtin_isfinite(v);
       r += v;
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_force(StepParams p, int x0, i
         sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
         const uint8_t ts = p.type[(uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz)];
         if ((ts & NODE_SOLID) && sx >= x0 && sx < x1 && sy >= y0 && sy < y1 && sz >= z0 && sz < z1) {
-          double fi = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n);
+          double fi = (double)CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, q.n) * (1.0 / CD::S);
           if constexpr (CD::shifted) fi += Lat<Q>::w[i];
           double m = 2.0 * fi;
           if (ts & NODE_MOVING)

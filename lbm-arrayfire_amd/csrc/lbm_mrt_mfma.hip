@@ -98,24 +98,26 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
     float dev[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
       C drho = 0, dr0 = 0;
-      if constexpr (SH) moments_s<27>(f, drho, rho, ux, uy, uz);
+      if constexpr (SH) moments_s<27>(f, drho, rho, ux, uy, uz, CD::S);
       else moments<27>(f, rho, ux, uy, uz);
       const C r0 = rho, x0 = ux, y0 = uy, z0 = uz;
       dr0 = drho;
       if constexpr (BC == BC_REF) {
-        if constexpr (SH) overrides_s<27>(p, t, q.y, q.z, drho, rho, ux, uy, uz);
+        if constexpr (SH) overrides_s<27>(p, t, q.y, q.z, drho, rho, ux, uy, uz, CD::S);
         else overrides<27>(p, t, q.y, q.z, rho, ux, uy, uz);
       }
-      // conserved rows: rho_f - rho and rho_f u_f - rho u, exactly 0 without an override
+      // conserved rows: rho_f - rho and rho_f u_f - rho u, exactly 0 without an override (in the
+      // compute units of the populations: S rho for the shifted storages)
+      const C rs0 = SH ? r0 * CD::S : r0, rs = SH ? rho * CD::S : rho;
       dev[0] = SH ? dr0 - drho : r0 - rho;
-      dev[1] = r0 * x0 - rho * ux;
-      dev[2] = r0 * y0 - rho * uy;
-      dev[3] = r0 * z0 - rho * uz;
+      dev[1] = rs0 * x0 - rs * ux;
+      dev[2] = rs0 * y0 - rs * uy;
+      dev[3] = rs0 * z0 - rs * uz;
       if constexpr (SH) {
         const float m15 = m15_of<27>(ux, uy, uz);
         static_for<27>([&](auto I) {
           constexpr int i = decltype(I)::value;
-          row[i] = f[i] - geq_i<27, i>(drho, rho, ux, uy, uz, m15);
+          row[i] = f[i] - geq_i<27, i>(drho, rs, ux, uy, uz, m15);
         });
       } else {
         const C usq = usq_of<27>(ux, uy, uz);
@@ -135,7 +137,7 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
         });
         if constexpr (SH) {
           // both rows less their common value at rest (rho / 27): rho (u terms) - u.u
-          d26 = rho * acc - ((ux * ux + uy * uy) + uz * uz);
+          d26 = rs * acc - ((ux * ux + uy * uy) + uz * uz) * CD::S;
         } else {
           float ref[27];
           meq_reference<float>(rho, ux, uy, uz, ref);

@@ -131,13 +131,13 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
     if constexpr (BC == BC_REF) {
       const int yin = (t & NODE_INLET) ? (int)(lxy[pos] >> 16) : 0;
       if constexpr (SH) {
-        moments_s<Q>(f, drho, rho, ux, uy, uz);
-        overrides_s<Q>(p, t, yin, 0, drho, rho, ux, uy, uz);
+        moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
+        overrides_s<Q>(p, t, yin, 0, drho, rho, ux, uy, uz, CD::S);
       } else {
         moments<Q>(f, rho, ux, uy, uz);
         overrides<Q>(p, t, yin, 0, rho, ux, uy, uz);
       }
-      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
+      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho, (C)CD::S);
       if (t & NODE_SOLID) {  // full-way bounce-back: out[opp(j)] = f[j], j >= 1
         static_for<Q - 1>([&](auto J) {
           constexpr int q = decltype(J)::value + 1;
@@ -150,7 +150,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       if (decltype(with_fixups)::value && (t & NODE_SLOW)) {
         const uint32_t xy = lxy[pos];
         physical_fixups<Q, SH>(
-            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f, (C)CD::S,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
               return lt[pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
@@ -161,9 +161,9 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
             [&]() -> uint8_t { return lt[pos + 1]; });
       }
-      if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz);
+      if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
       else moments<Q>(f, rho, ux, uy, uz);
-      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
+      collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho, (C)CD::S);
       return true;
     }
   };
@@ -229,7 +229,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
         const uint8_t t = (uint8_t)(((w >> 28) & 1u ? NODE_INLET : 0) | ((w >> 29) & 1u ? NODE_OUTLET : 0) |
                                     ((w >> 30) & 1u ? NODE_WALLNB : 0));
         physical_fixups<Q, SH>(
-            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f,
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f, (C)CD::S,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
               return (uint8_t)(((w >> (10 + q)) & 1u ? NODE_SOLID : 0) | ((w >> (18 + q)) & 1u ? NODE_MOVING : 0));
@@ -240,9 +240,9 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
             [&](auto J) -> C { return src[decltype(J)::value * RN + pos + 1]; },
             [&]() -> uint8_t { return (w >> 31) & 1u ? NODE_SOLID : 0; });
         C drho = 0;
-        if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz);
+        if constexpr (SH) moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
         else moments<Q>(f, rho, ux, uy, uz);
-        collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho);
+        collide<Q, CO_BGK, SH>(p, f, rho, ux, uy, uz, out, drho, (C)CD::S);
         if constexpr (s == K) write_out(i, j, out, key);
         else keep(dst, pos, out, key);
       }
