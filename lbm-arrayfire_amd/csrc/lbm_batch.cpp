@@ -52,9 +52,12 @@ struct Group {
   int phase = 0;               // multi-step groups: the next launch uses parameter block [phase]
   int grid = 0;
   int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
+  int nt = 0;                  // single-step launches with non-temporal stores (3D scalar groups too
+                               // large for the caches)
 
   BatchArgs args(int phase, int launch_grid) const {
-    return BatchArgs{(const BatchTable *)d_tab, phase, launch_grid >= (multi_k ? total_multi : total_tiles) ? 1 : 0};
+    return BatchArgs{(const BatchTable *)d_tab, phase, launch_grid >= (multi_k ? total_multi : total_tiles) ? 1 : 0,
+                     multi_k ? 0 : nt};
   }
   void release() {
     if (uploaded) (void)hipEventSynchronize(uploaded);
@@ -184,6 +187,11 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     hdr->total_tiles = g.total_tiles;
     hdr->total_multi = g.total_multi;
     hdr->stream = group_bytes > ((size_t)256 << 20) ? 1 : 0;
+    // such groups also store non-temporally, as single lattices over the cache size do (3D scalar
+    // kernels); the captured launches carry the flag, so a change re-captures them
+    const int nt = hdr->stream && g.cfg.Q != 9 && !uses_mfma(g.cfg) ? 1 : 0;
+    if (nt != g.nt) recapture = true;
+    g.nt = nt;
     hdr->prefix = (const int32_t *)(g.d_tab + g.prefix_off);
     hdr->prefix_multi = (const int32_t *)(g.d_tab + g.pmulti_off);
     hdr->members[0] = (const StepParams *)(g.d_tab + g.par_off[0]);

@@ -113,7 +113,7 @@ size_t lbm_footprint_bytes(const lbm_desc *d) {
   int32_t gx, gy, gz, nt;
   tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
   return 2 * (size_t)population_stride((int64_t)N, storage_bytes(d->storage)) * (size_t)d->lattice *
-             (size_t)storage_bytes(d->storage) + N +
+             (size_t)storage_bytes(d->storage) + types_block_bytes(d->lattice, d->bc_mode, (int64_t)N) +
          (size_t)nt * 3 * sizeof(double) + RED_BYTES;
 }
 
@@ -176,7 +176,8 @@ std::vector<size_t> pooled_blocks(const lbm_desc *d, bool fields) {
   tile_grid(d->nx, d->ny, d->nz, gx, gy, gz, nt);
   const size_t buf = DevPool::size_class((size_t)population_stride((int64_t)N, storage_bytes(d->storage)) *
                                          (size_t)d->lattice * (size_t)storage_bytes(d->storage));
-  std::vector<size_t> b = {buf, buf, DevPool::size_class(N), DevPool::size_class((size_t)nt * 3 * sizeof(double)),
+  std::vector<size_t> b = {buf, buf, DevPool::size_class(types_block_bytes(d->lattice, d->bc_mode, (int64_t)N)),
+                           DevPool::size_class((size_t)nt * 3 * sizeof(double)),
                            DevPool::size_class(RED_BYTES)};
   if (d->collision != LBM_BGK && d->lattice == 27) b.push_back(DevPool::size_class(64 * MFMA_TAB * sizeof(float)));
   if (fields) b.push_back(DevPool::size_class(field_scratch_bytes((int64_t)N)));
@@ -288,7 +289,8 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   for (int b = 0; b < 2; ++b)
     if (dmalloc(&h->f[b], h->buf_bytes()) != hipSuccess)
       return cleanup(LBM_ENOMEM, "population buffer allocation failed (" + std::to_string(h->buf_bytes()) + " bytes)");
-  if (dmalloc((void **)&h->types, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type allocation failed");
+  if (dmalloc((void **)&h->types, types_block_bytes(h->Q, d->bc_mode, h->N)) != hipSuccess)
+    return cleanup(LBM_ENOMEM, "node-type allocation failed");
   if (dmalloc((void **)&h->partials, (size_t)p.ntiles * 3 * sizeof(double)) != hipSuccess ||
       dmalloc((void **)&h->red, RED_BYTES) != hipSuccess)
     return cleanup(LBM_ENOMEM, "monitor scratch allocation failed");
@@ -300,7 +302,9 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
     uint8_t *raw = nullptr;
     if (dmalloc((void **)&raw, (size_t)h->N) != hipSuccess) return cleanup(LBM_ENOMEM, "node-type staging failed");
     hipError_t e = hipMemcpyAsync(raw, node_type, (size_t)h->N, hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, h->stream);
+    uint32_t *links = wall_links_kept(h->Q, d->bc_mode) ? (uint32_t *)(h->types + wall_links_offset(h->N)) : nullptr;
+    if (e == hipSuccess) e = launch_prepare_types(h->Q, d->bc_mode, d->nx, d->ny, d->nz, raw, h->types, links, h->stream);
+    p.links = links;
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (pool)
       pool->put(raw, (size_t)h->N);
@@ -344,7 +348,7 @@ lbm_status lbm_destroy(lbm_handle h) {
         (void)hipFree(ptr);
     };
     for (int b = 0; b < 2; ++b) dfree(h->f[b], h->buf_bytes());
-    dfree(h->types, (size_t)h->N);
+    dfree(h->types, types_block_bytes(h->Q, h->d.bc_mode, h->N));
     dfree(h->partials, (size_t)h->base.ntiles * 3 * sizeof(double));
     dfree(h->red, RED_BYTES);
     dfree(h->mrt_tab, 64 * MFMA_TAB * sizeof(float));
@@ -368,6 +372,18 @@ static lbm_status ensure_fields(lbm_handle h) {
   return LBM_OK;
 }
 
+namespace {
+// Both buffers' wall copies after a state was written from outside the step kernels.
+hipError_t prime_walls(lbm_ctx *h) {
+  if (!lbm::wall_links_kept(h->Q, h->d.bc_mode)) return hipSuccess;
+  for (int b = 0; b < 2; ++b) {
+    const hipError_t e = lbm::launch_prime_wall_copies(h->cfg, h->params(b), h->strm());
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+}  // namespace
+
 lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, const double *uy,
                            const double *uz) {
   return lbm::abi_guard([&]() -> lbm_status {
@@ -384,6 +400,7 @@ lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, co
         dev[k] = dst;
       }
     HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
+    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -397,6 +414,7 @@ lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], u
     HIPCHK(hipSetDevice(h->device));
     const double u[3] = {u0 ? u0[0] : 0.0, u0 ? u0[1] : 0.0, u0 ? u0[2] : 0.0};
     HIPCHK(launch_init_equilibrium(h->cfg, h->params(h->cur), rho0, u, seed, noise_amp, h->strm()));
+    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -408,6 +426,7 @@ lbm_status lbm_init_reference(lbm_handle h, double rho0) {
     if (!h) return fail(LBM_EINVAL, "handle is NULL");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->strm()));
+    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -604,6 +623,7 @@ lbm_status lbm_set_state(lbm_handle h, const void *src) {
     std::memcpy(&ctr[0], (const char *)src + h->pop_bytes(), sizeof(uint32_t));
     ctr[1] = ctr[0];
     HIPCHK(hipMemcpyAsync(h->step_ctr(), ctr, sizeof(ctr), hipMemcpyHostToDevice, h->strm()));
+    HIPCHK(prime_walls(h));  // a saved state already holds them; populations set by hand may not
     HIPCHK(hipStreamSynchronize(h->strm()));
     return LBM_OK;
   });

@@ -259,6 +259,38 @@ __device__ __forceinline__ void for_batch_tiles(const BatchArgs &b, F &&f) {
 }
 
 // ---- pulls -------------------------------------------------------------------------------
+// The periodic source x - c_i (mod dims) of each population of node q: src(IC<i>) is its node
+// index.  Source rows are the node's row plus wrapped y and z steps (modulo 2^32, exact once
+// summed): additions instead of one 32-bit multiply (a quarter-rate instruction) per (y, z) pair.
+template <int Q>
+struct PullSources {
+  uint32_t row, dym, dyp, dzm, dzp;
+  int xm, x, xp;
+  template <class P>
+  __device__ __forceinline__ PullSources(const P &p, const NodePos &q) {
+    const uint32_t nx = (uint32_t)p.nx, nxy = nx * (uint32_t)p.ny;
+    row = q.n - (uint32_t)q.x;
+    dym = q.y == 0 ? nxy - nx : 0u - nx;
+    dyp = q.y == p.ny - 1 ? nx - nxy : nx;
+    dzm = q.z == 0 ? nxy * (uint32_t)(p.nz - 1) : 0u - nxy;
+    dzp = q.z == p.nz - 1 ? 0u - nxy * (uint32_t)(p.nz - 1) : nxy;
+    xm = q.x == 0 ? p.nx - 1 : q.x - 1;
+    x = q.x;
+    xp = q.x == p.nx - 1 ? 0 : q.x + 1;
+  }
+  template <int i>
+  __device__ __forceinline__ uint32_t operator()(IC<i>) const {
+    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+    const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
+    uint32_t r = row;
+    if constexpr (cz == 1) r += dzm;
+    if constexpr (cz == -1) r += dzp;
+    if constexpr (cy == 1) r += dym;
+    if constexpr (cy == -1) r += dyp;
+    return r + (uint32_t)sx;
+  }
+};
+
 // Periodic pull: f_i(x) = F_i(x - c_i mod dims).
 template <int Q, int ST, class P>
 __device__ __forceinline__ void pull_periodic(const P &p, const NodePos &q,
@@ -266,24 +298,10 @@ __device__ __forceinline__ void pull_periodic(const P &p, const NodePos &q,
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   const T *fin = (const T *)p.fin;
-  const int xm = q.x == 0 ? p.nx - 1 : q.x - 1, xp = q.x == p.nx - 1 ? 0 : q.x + 1;
-  // source rows as the node's row plus wrapped y and z steps (modulo 2^32, exact once summed):
-  // additions instead of one 32-bit multiply (a quarter-rate instruction) per (y, z) pair
-  const uint32_t nx = (uint32_t)p.nx, nxy = nx * (uint32_t)p.ny;
-  const uint32_t row = q.n - (uint32_t)q.x;
-  const uint32_t dym = q.y == 0 ? nxy - nx : 0u - nx, dyp = q.y == p.ny - 1 ? nx - nxy : nx;
-  const uint32_t dzm = q.z == 0 ? nxy * (uint32_t)(p.nz - 1) : 0u - nxy;
-  const uint32_t dzp = q.z == p.nz - 1 ? 0u - nxy * (uint32_t)(p.nz - 1) : nxy;
+  const PullSources<Q> src(p, q);
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
-    const int sx = cx == 1 ? xm : (cx == -1 ? xp : q.x);
-    uint32_t r = row;
-    if constexpr (cz == 1) r += dzm;
-    if constexpr (cz == -1) r += dzp;
-    if constexpr (cy == 1) r += dym;
-    if constexpr (cy == -1) r += dyp;
-    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, r + (uint32_t)sx);
+    f[i] = CD::template ld<i>(fin + (size_t)i * (size_t)p.stride, src(I));
   });
 }
 
@@ -366,22 +384,36 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   east(IC<j>)      population j of the node at x + 1;  east_type()  its type
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
 // 6 w_i rho_w c_i.u_w when the source is a moving wall (rho_w taken in compute units, S rho_bc).
-template <int Q, class C, class SrcType, class Own, class P>
-__device__ __forceinline__ void wall_links(const P &p, C *f, C S, SrcType &&src_type, Own &&own) {
+// COPIED (3D lattices stepped from global memory): the step that produced this node's populations
+// also stored each resting-wall value f_opp(i) into the slot i of the solid source (store_wall_copies),
+// so the periodic pull has already read it; only moving-wall links remain, told apart through
+// src_type on nodes whose wall link mask lm has bit 31 set.
+template <int Q, bool COPIED, class C, class SrcType, class Own, class P>
+__device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint32_t lm, SrcType &&src_type, Own &&own) {
   const C rw = (C)p.rho_bc * S;
+  auto moving_term = [&](auto I) -> C {
+    constexpr int i = decltype(I)::value;
+    C cu = (C)Lat<Q>::cx[i] * (C)p.u_wall[0] + (C)Lat<Q>::cy[i] * (C)p.u_wall[1];
+    if (Lat<Q>::D == 3) cu = cu + (C)Lat<Q>::cz[i] * (C)p.u_wall[2];
+    return (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
+  };
+  if constexpr (COPIED) {
+    if (lm & 0x80000000u) {
+      static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        if (((lm >> (i - 1)) & 1u) && (src_type(IC<i>{}) & NODE_MOVING))
+          f[i] = own(IC<Lat<Q>::opp[i]>{}) + moving_term(IC<i>{});
+      });
+    }
+    return;
+  }
   static_for<Q>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
-    constexpr int io = Lat<Q>::opp[i];
     if constexpr (i != 0) {
       const uint8_t ts = src_type(I);
       if (ts & NODE_SOLID) {
-        C v = own(IC<io>{});
-        if (ts & NODE_MOVING) {
-          C cu = (C)cx * (C)p.u_wall[0] + (C)cy * (C)p.u_wall[1];
-          if (Lat<Q>::D == 3) cu = cu + (C)cz * (C)p.u_wall[2];
-          v = v + (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
-        }
+        C v = own(IC<Lat<Q>::opp[i]>{});
+        if (ts & NODE_MOVING) v = v + moving_term(I);
         f[i] = v;
       }
     }
@@ -450,15 +482,16 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rh
   });
 }
 
-// S: the compute scale of the populations (Codec::S; 1 unless shifted).
-template <int Q, bool SH = false, class C, class SrcType, class Own, class West, class WestType, class East,
-          class EastType, class P>
-__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S,
+// S: the compute scale of the populations (Codec::S; 1 unless shifted).  COPIED, lm: resting-wall
+// values already pulled from the solid slots, and the node's wall link mask (wall_links).
+template <int Q, bool SH = false, bool COPIED = false, class C, class SrcType, class Own, class West, class WestType,
+          class East, class EastType, class P>
+__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S, uint32_t lm,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type, East &&east, EastType &&east_type) {
   const C rw = (C)p.rho_bc;
   if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
-    wall_links<Q>(p, f, S, src_type, own);
+    wall_links<Q, COPIED>(p, f, S, lm, src_type, own);
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
     const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
@@ -485,18 +518,31 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   }
 }
 
+// 3D PHYSICAL lattices keep their resting-wall bounce-back values in the solid nodes' slots: the
+// step that writes a wall neighbour's populations also writes f_opp(i) into slot i of each solid
+// source x - c_i (the one slot of that node the neighbour pulls), so the next plain pull reads the
+// half-way bounce-back value there.  The copy is the stored value itself (same rounding), so the
+// pulled value equals the node's own f_opp(i) bit for bit.  Each slot has one writer.
+template <int Q>
+constexpr bool wall_copies() { return Q >= 19; }
+
 // PHYSICAL-mode pull from global memory.  The periodic pull is issued by every node before its
-// type is known; nodes flagged SLOW then apply the boundary rules.
+// type is known; nodes flagged SLOW then apply the boundary rules.  lm returns the node's wall
+// link mask (3D lattices; 0 for other nodes).
 template <int Q, int ST, class P>
 __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint8_t t,
-                                             typename Codec<ST, Q>::C *f) {
+                                             typename Codec<ST, Q>::C *f, uint32_t &lm) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   pull_periodic<Q, ST>(p, q, f);
+  lm = 0;
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
-  physical_fixups<Q, CD::shifted>(
-      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S,
+  constexpr bool COPIED = wall_copies<Q>();
+  if constexpr (COPIED)
+    if (t & NODE_WALLNB) lm = p.links[q.n];
+  physical_fixups<Q, CD::shifted, COPIED>(
+      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S, lm,
       [&](auto I) -> uint8_t {
         constexpr int i = decltype(I)::value;
         constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
@@ -1140,34 +1186,60 @@ __device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C u
 }
 
 // Stores the node's Q populations: out in the compute representation, rnd the node's rounding
-// bits (node_bits; shifted codecs only).
+// bits (node_bits; shifted codecs only).  lm: the node's wall link mask (3D PHYSICAL lattices, 0
+// otherwise); each population opp(i) whose link i has a resting solid source is also stored, right
+// after its own store, into slot i of that solid node (wall_copies), q the node.  Wavefronts with
+// no wall neighbour take the loop without the copies.
 template <int Q, int ST, bool NT = false, class P>
-__device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename Codec<ST, Q>::C *out, uint32_t rnd) {
+__device__ __forceinline__ void store_all(const P &p, const NodePos &q, const typename Codec<ST, Q>::C *out,
+                                          uint32_t rnd, uint32_t lm = 0) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   T *fout = (T *)p.fout;
-  uint32_t off = n * (uint32_t)sizeof(T);
-  static_for<Q>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    // 2-byte storages: the population's (uniform) base plus a 32-bit byte offset, the scalar-base
-    // store form with one offset register for all Q stores (lbm_create keeps these lattices
-    // below 2^31 nodes).  The empty asm hides that the offsets are equal: the compiler would
-    // otherwise form fout + offset once and add each population's base to it in 64-bit vector
-    // arithmetic, one instruction per store.  Wider storages take 64-bit addresses.
-    T *base = fout + (size_t)i * (size_t)p.stride;
-    T *dst = base + n;
-    if constexpr (sizeof(T) == 2) {
-      asm volatile("" : "+v"(off));
-      dst = (T *)((char *)base + off);
+  const uint32_t n = q.n;
+  auto store_pops = [&](auto WITH_COPIES) {
+    constexpr bool COPY = decltype(WITH_COPIES)::value != 0;
+    uint32_t off = n * (uint32_t)sizeof(T);
+    // the copies' source offsets are recomputed here (the empty asm keeps the compiler from
+    // holding the pull's through the collision, where registers are scarcest)
+    NodePos qc = q;
+    if constexpr (COPY) asm volatile("" : "+v"(qc.n), "+v"(qc.x), "+v"(qc.y), "+v"(qc.z));
+    static_for<Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      // 2-byte storages: the population's (uniform) base plus a 32-bit byte offset, the
+      // scalar-base store form with one offset register for all Q stores (lbm_create keeps these
+      // lattices below 2^31 nodes).  The empty asm hides that the offsets are equal: the compiler
+      // would otherwise form fout + offset once and add each population's base to it in 64-bit
+      // vector arithmetic, one instruction per store.  Wider storages take 64-bit addresses.
+      T *base = fout + (size_t)i * (size_t)p.stride;
+      T *dst = base + n;
+      if constexpr (sizeof(T) == 2) {
+        asm volatile("" : "+v"(off));
+        dst = (T *)((char *)base + off);
+      }
+      const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
+      // NT: non-temporal stores for lattices too large for the caches (the next step reads these
+      // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s
+      // (4-byte) and from 5.32 to 5.62 TB/s (2-byte) on MI355X, and the 3D step kernels by 3-5 %; a
+      // lattice that stays in the caches between steps runs faster with ordinary stores.
+      if constexpr (NT) __builtin_nontemporal_store(v, dst);
+      else *dst = v;
+      if constexpr (COPY && i != 0) {
+        constexpr int l = Lat<Q>::opp[i];  // the link whose bounce-back value this population is
+        if ((lm >> (l - 1)) & 1u) {
+          const PullSources<Q> src(p, qc);
+          fout[(size_t)l * (size_t)p.stride + src(IC<l>{})] = v;
+        }
+      }
+    });
+  };
+  if constexpr (wall_copies<Q>()) {
+    if (__builtin_amdgcn_ballot_w64(lm != 0) != 0) {
+      store_pops(IC<1>{});
+      return;
     }
-    const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
-    // NT: non-temporal stores for lattices too large for the caches (the next step reads these
-    // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s
-    // (4-byte) and from 5.32 to 5.62 TB/s (2-byte) on MI355X, and the 3D step kernels by 3-5 %; a
-    // lattice that stays in the caches between steps runs faster with ordinary stores.
-    if constexpr (NT) __builtin_nontemporal_store(v, dst);
-    else *dst = v;
-  });
+  }
+  store_pops(IC<0>{});
 }
 
 // Stores populations given unshifted (initial states), rounded to nearest.

@@ -12,6 +12,10 @@ struct StepParams {
   const void *fin;          // populations read (post-collision of the previous step)
   void *fout;               // populations written
   const uint8_t *type;      // node types (with NODE_SLOW in PHYSICAL mode)
+  // PHYSICAL 3D lattices: per node, bit i - 1 set when population i (1 <= i < Q) is pulled from a
+  // solid node, bit 31 when one of those solid sources is a moving wall (wall_link_mask); null
+  // otherwise.  Read only by wall-neighbour nodes, in place of one node-type load per link.
+  const uint32_t *links;
   int32_t nx, ny, nz;
   int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile
   int32_t ntiles;
@@ -83,9 +87,11 @@ struct BatchArgs {
   const BatchTable *tab;
   int32_t phase;     // which orientation this launch steps
   int32_t one_tile;  // the grid has one workgroup per tile (selects the loop-free kernel)
+  int32_t nt;        // 3D scalar groups too large for the caches: non-temporal population stores
 };
 hipError_t launch_step_batch(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
+hipError_t launch_step_batch_scalar_nt(const Config &c, const BatchArgs &b, int grid, hipStream_t s);  // Q = 19, 27
 hipError_t launch_step_batch_mrt_mfma(const Config &c, const BatchArgs &b, int grid, hipStream_t s);
 // Several D2Q9 BGK steps per launch from an LDS tile (2D lattices, FP32-arithmetic storages).
 // multi_steps_per_launch = the steps one launch of the given shape advances, 0 if c or the shape
@@ -115,9 +121,20 @@ void build_mfma_table(int collision, const double *s, int engine, float *tab);
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);
 // out = in & 0x1f, plus NODE_SLOW on non-solid nodes that are inlet/outlet nodes or pull from a
-// solid node, and NODE_WALLNB on the latter (PHYSICAL mode).
+// solid node, and NODE_WALLNB on the latter (PHYSICAL mode); links (if not null): the wall link
+// masks (StepParams::links).
 hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
-                                hipStream_t s);
+                                uint32_t *links, hipStream_t s);
+// Fills the wall copies (3D PHYSICAL lattices; see wall_copies in lbm_device.hpp) of the buffer
+// p.fout from its fluid populations; nothing to do for other lattices.
+hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s);
+// The node-type block of a lattice: N type bytes, then (wall_links_kept) the 32-bit wall link masks
+// from byte offset wall_links_offset(N).
+inline bool wall_links_kept(int Q, int bc_mode) { return bc_mode == BC_PHYS && Q >= 19; }
+inline size_t wall_links_offset(int64_t N) { return ((size_t)N + 255) & ~(size_t)255; }
+inline size_t types_block_bytes(int Q, int bc_mode, int64_t N) {
+  return wall_links_kept(Q, bc_mode) ? wall_links_offset(N) + 4 * (size_t)N : (size_t)N;
+}
 // out[a * n + k] = fields[a * N + nodes[k]] for a = rho, ux, uy, uz.
 hipError_t launch_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out,
                                 hipStream_t s);

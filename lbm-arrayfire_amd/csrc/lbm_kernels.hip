@@ -257,35 +257,69 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_reduce_partials(const double 
 
 template <int Q>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_prepare_types(int bc_mode, int nx, int ny, int nz,
-                                                                 const uint8_t *in, uint8_t *out) {
+                                                                 const uint8_t *in, uint8_t *out, uint32_t *links) {
   const int64_t n = (int64_t)blockIdx.x * BLOCK_THREADS + threadIdx.x;
   if (n >= (int64_t)nx * ny * nz) return;
   uint8_t t = in[n] & 0x1f;
+  uint32_t mask = 0;  // wall link mask (StepParams::links)
   if (bc_mode == BC_PHYS && !(t & NODE_SOLID)) {
     const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
-    bool wall = false;
     static_for<Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       int sx = x - Lat<Q>::cx[i], sy = y - Lat<Q>::cy[i], sz = z - Lat<Q>::cz[i];
       sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
       sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
       sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
-      if (in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)] & NODE_SOLID) wall = true;
+      const uint8_t ts = in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)];
+      if (i != 0 && (ts & NODE_SOLID)) mask |= (1u << (i - 1)) | ((ts & NODE_MOVING) ? 0x80000000u : 0u);
     });
-    if (wall) t |= NODE_SLOW | NODE_WALLNB;
+    if (mask) t |= NODE_SLOW | NODE_WALLNB;
     if (t & (NODE_INLET | NODE_OUTLET)) t |= NODE_SLOW;
   }
   out[n] = t;
+  if (links) links[n] = mask;
 }
 
 hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
-                                hipStream_t s) {
+                                uint32_t *links, hipStream_t s) {
   const int64_t N = (int64_t)nx * ny * nz;
   const dim3 grid((unsigned)((N + BLOCK_THREADS - 1) / BLOCK_THREADS));
-  if (Q == 9) hipLaunchKernelGGL(k_prepare_types<9>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
-  else if (Q == 19) hipLaunchKernelGGL(k_prepare_types<19>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
-  else hipLaunchKernelGGL(k_prepare_types<27>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out);
+  if (Q == 9) hipLaunchKernelGGL(k_prepare_types<9>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out, links);
+  else if (Q == 19) hipLaunchKernelGGL(k_prepare_types<19>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out, links);
+  else hipLaunchKernelGGL(k_prepare_types<27>, grid, dim3(BLOCK_THREADS), 0, s, bc_mode, nx, ny, nz, in, out, links);
   return hipGetLastError();
+}
+
+// Fills the wall copies of a state written from outside the step kernels (initial states,
+// restored populations): slot i of each resting solid source x - c_i of a wall neighbour x takes
+// x's stored population opp(i), in the buffer p.fout, as a step leaves it (store_all).  Reads
+// fluid slots and writes solid ones only.
+template <int Q, int ST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_prime_wall_copies(StepParams p) {
+  using T = typename Codec<ST, Q>::T;
+  const NodePos q = node_of_thread(p);
+  if constexpr (wall_copies<Q>()) {
+    if (!q.valid || !(p.type[q.n] & NODE_WALLNB)) return;
+    const uint32_t lm = p.links[q.n];
+    T *f = (T *)p.fout;
+    const PullSources<Q> src(p, q);
+    static_for<Q - 1>([&](auto I) {
+      constexpr int i = decltype(I)::value + 1;
+      if ((lm >> (i - 1)) & 1u)
+        f[(size_t)i * (size_t)p.stride + src(IC<i>{})] = f[(size_t)Lat<Q>::opp[i] * (size_t)p.stride + q.n];
+    });
+  }
+}
+
+hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s) {
+  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
+    constexpr int Q = decltype(Qc)::value;
+    if constexpr (wall_copies<Q>()) {
+      hipLaunchKernelGGL((k_prime_wall_copies<Q, decltype(STc)::value>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
+      return hipGetLastError();
+    }
+    return hipSuccess;
+  });
 }
 
 // Gathers rho/u of selected nodes from the four field arrays (sensors).
@@ -313,6 +347,7 @@ hipError_t launch_step_scalar(const Config &c, const StepParams &p, hipStream_t 
 }
 
 hipError_t launch_step_batch_scalar(const Config &c, const BatchArgs &b, int grid, hipStream_t s) {
+  if (b.nt) return launch_step_batch_scalar_nt(c, b, grid, s);  // lbm_kernels_nt.hip
   return with_full(c, [&](auto Qc, auto STc, auto BCc, auto COc) -> hipError_t {
     constexpr int Q = decltype(Qc)::value, ST = decltype(STc)::value, BC = decltype(BCc)::value,
                   CO = decltype(COc)::value;

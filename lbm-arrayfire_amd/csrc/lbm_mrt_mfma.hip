@@ -88,11 +88,12 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
   const uint8_t t = q.valid ? p.type[q.n] : (uint8_t)NODE_SOLID;
   const bool active = q.valid && !(BC == BC_PHYS && (t & NODE_SOLID));
   C f[27];
+  uint32_t lm = 0;  // wall link mask (PHYSICAL wall neighbours)
   {
     C rho = 0, ux = 0, uy = 0, uz = 0;
     if (q.valid) {
       if constexpr (BC == BC_REF) pull_periodic<27, ST>(p, q, f);
-      else pull_physical<27, ST>(p, q, t, f);
+      else pull_physical<27, ST>(p, q, t, f, lm);
     }
     float row[ROW];
     float dev[4] = {0.f, 0.f, 0.f, 0.f};
@@ -254,7 +255,7 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
       });
     }
   }
-  store_all<27, ST>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
+  store_all<27, ST>(p, q, out, SH ? node_bits(q.n, key) : 0u, lm);
 }
 
 template <int ST, int BC, int ENG>

@@ -24,6 +24,7 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
   // transform (80 fewer additions a node than the separate sums)
   constexpr bool MONO = Q == 27 && CO != CO_BGK && sizeof(C) == 4;
   float mono[MONO ? 27 : 1];
+  uint32_t lm = 0;  // wall link mask (3D PHYSICAL wall neighbours)
   if constexpr (BC == BC_REF) {
     pull_periodic<Q, ST>(p, q, f);
     if constexpr (MONO) {
@@ -47,8 +48,8 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
       });
     }
   } else {
-    pull_physical<Q, ST>(p, q, t, f);
-    if (t & NODE_SOLID) return;  // walls keep their populations; nothing writes them
+    pull_physical<Q, ST>(p, q, t, f, lm);
+    if (t & NODE_SOLID) return;  // walls are written only by their neighbours' wall copies
     if constexpr (MONO) {
       mrt27_monomials(f, mono);
       moments_mono<SH>(mono, drho, rho, ux, uy, uz, CD::S);
@@ -59,7 +60,7 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
       collide<Q, CO, SH>(p, f, rho, ux, uy, uz, out, drho, (C)CD::S);
     }
   }
-  store_all<Q, ST, NT>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
+  store_all<Q, ST, NT>(p, q, out, SH ? node_bits(q.n, key) : 0u, lm);
 }
 
 // rho, u the reference holds after a step at node q (cpp/d2q9_bgk_lid.cpp:131-146): moments of the
@@ -79,7 +80,8 @@ __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, dou
       moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
       overrides_s<Q>(p, t, q.y, q.z, drho, rho, ux, uy, uz, CD::S);
     } else if (!(t & NODE_SOLID)) {
-      pull_physical<Q, ST>(p, q, t, f);
+      uint32_t lm;
+      pull_physical<Q, ST>(p, q, t, f, lm);
       moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
     }
     out[0] = 1.0 + (double)drho * (1.0 / CD::S);
@@ -89,7 +91,8 @@ __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, dou
       moments<Q>(f, rho, ux, uy, uz);
       overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
     } else if (!(t & NODE_SOLID)) {
-      pull_physical<Q, ST>(p, q, t, f);
+      uint32_t lm;
+      pull_physical<Q, ST>(p, q, t, f, lm);
       moments<Q>(f, rho, ux, uy, uz);
     }
     out[0] = (double)rho;
@@ -106,10 +109,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_step(StepParams p) {
   step_tile<Q, ST, BC, CO, NT>(p, NT ? stream_tile((int)blockIdx.x, p.gx) : xcd_tile((int)blockIdx.x, p.ntiles));
 }
 
-// Persistent batched step: every member of one configuration group, one launch.
-template <int Q, int ST, int BC, int CO, bool ONE>
+// Persistent batched step: every member of one configuration group, one launch.  NT: non-temporal
+// population stores (groups whose buffers cannot stay in the caches, which take their tiles in
+// stream order).
+template <int Q, int ST, int BC, int CO, bool ONE, bool NT = false>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_step_batch(BatchArgs b) {
-  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO, false>(p, tile); });
+  for_batch_tiles<false, ONE>(b, [&](const auto &p, int tile) { step_tile<Q, ST, BC, CO, NT>(p, tile); });
 }
 
 // ---- dispatch ----------------------------------------------------------------------------

@@ -40,7 +40,7 @@ def fixtures():
     cfg, t = cases.d3q27_mrt_lid_gs(5, re=10.0, storage=F64)
     out["d3q27_mrt_gs_lid_ref_5"] = (cfg, t, 2)
     cfg, t = cases.d3q27_sphere_channel(12, 7, 7, (4, 3, 3), 1, storage=F64, collision=MRT_GS)
-    out["d3q27_sphere_mrt_phys_12x7x7"] = (cfg, t, 2)
+    out["d3q27_sphere_mrt_phys_12x7x7"] = (dict(cfg, inlet_closure=0), t, 2)  # Zou-He inlet
     return out
 
 

@@ -220,10 +220,28 @@ def step_physical(F, types, cfg):
     return np.where(solid[None], F, out)
 
 
+def wall_copies(F, types, cfg):
+    """3D PHYSICAL lattices: slot i of a solid node s holds population opp(i) of its fluid
+    neighbour s + c_i (the storage convention of the library's 3D kernels; the step itself never
+    reads these slots)."""
+    Q = cfg["lattice"]
+    if cfg.get("bc_mode", 0) == 0 or Q < 19:
+        return F
+    c, _, opp = lattice(Q)
+    solid = (types & SOLID) != 0
+    F = F.copy()
+    for i in range(1, Q):
+        back = (-c[i][2], -c[i][1], -c[i][0])  # out[s] = in[s + c_i]
+        fluid_nb = np.roll(~solid, shift=back, axis=(0, 1, 2))
+        F[i] = np.where(solid & fluid_nb, np.roll(F[opp[i]], shift=back, axis=(0, 1, 2)), F[i])
+    return F
+
+
 def run(F, types, cfg, nsteps):
     step = step_reference if cfg.get("bc_mode", 0) == 0 else step_physical
+    F = wall_copies(F, types, cfg)
     for _ in range(nsteps):
-        F = step(F, types, cfg)
+        F = wall_copies(step(F, types, cfg), types, cfg)
     return F
 
 
