@@ -189,7 +189,9 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     hdr->stream = group_bytes > ((size_t)256 << 20) ? 1 : 0;
     // such groups also store non-temporally, as single lattices over the cache size do (3D scalar
     // kernels); the captured launches carry the flag, so a change re-captures them
-    const int nt = hdr->stream && g.cfg.Q != 9 && !uses_mfma(g.cfg) ? 1 : 0;
+    int nt = hdr->stream && g.cfg.Q != 9 && !uses_mfma(g.cfg) ? 1 : 0;
+    for (const lbm_ctx *m : g.members)
+      if (!rows_fill_lines(m->d.nx, m->esize)) nt = 0;  // partial lines stay in the L2 to merge
     if (nt != g.nt) recapture = true;
     g.nt = nt;
     hdr->prefix = (const int32_t *)(g.d_tab + g.prefix_off);

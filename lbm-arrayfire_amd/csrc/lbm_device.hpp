@@ -385,11 +385,11 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
 // 6 w_i rho_w c_i.u_w when the source is a moving wall (rho_w taken in compute units, S rho_bc).
 // COPIED (3D lattices stepped from global memory): the step that produced this node's populations
-// also stored each resting-wall value f_opp(i) into the slot i of the solid source (store_wall_copies),
-// so the periodic pull has already read it; only moving-wall links remain, told apart through
-// src_type on nodes whose wall link mask lm has bit 31 set.
+// also stored each resting-wall value f_opp(i) into the slot i of the solid source (wall_copies in
+// store_all), so the periodic pull has already read it; only moving-wall links remain, on nodes
+// of type NODE_WALLMOV (t), told apart through src_type.
 template <int Q, bool COPIED, class C, class SrcType, class Own, class P>
-__device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint32_t lm, SrcType &&src_type, Own &&own) {
+__device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, SrcType &&src_type, Own &&own) {
   const C rw = (C)p.rho_bc * S;
   auto moving_term = [&](auto I) -> C {
     constexpr int i = decltype(I)::value;
@@ -398,11 +398,11 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint32_t lm, S
     return (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
   };
   if constexpr (COPIED) {
-    if (lm & 0x80000000u) {
+    if (t & NODE_WALLMOV) {
       static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        if (((lm >> (i - 1)) & 1u) && (src_type(IC<i>{}) & NODE_MOVING))
-          f[i] = own(IC<Lat<Q>::opp[i]>{}) + moving_term(IC<i>{});
+        const uint8_t ts = src_type(IC<i>{});
+        if ((ts & NODE_SOLID) && (ts & NODE_MOVING)) f[i] = own(IC<Lat<Q>::opp[i]>{}) + moving_term(IC<i>{});
       });
     }
     return;
@@ -482,16 +482,16 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rh
   });
 }
 
-// S: the compute scale of the populations (Codec::S; 1 unless shifted).  COPIED, lm: resting-wall
-// values already pulled from the solid slots, and the node's wall link mask (wall_links).
+// S: the compute scale of the populations (Codec::S; 1 unless shifted).  COPIED: resting-wall
+// values already pulled from the solid slots (wall_links).
 template <int Q, bool SH = false, bool COPIED = false, class C, class SrcType, class Own, class West, class WestType,
           class East, class EastType, class P>
-__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S, uint32_t lm,
+__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S,
                                                 SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type, East &&east, EastType &&east_type) {
   const C rw = (C)p.rho_bc;
   if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
-    wall_links<Q, COPIED>(p, f, S, lm, src_type, own);
+    wall_links<Q, COPIED>(p, f, S, t, src_type, own);
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
     const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
@@ -539,10 +539,11 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
   constexpr bool COPIED = wall_copies<Q>();
+  // the mask is used by the stores at the end of the step, not here (no wait on it in the pull)
   if constexpr (COPIED)
     if (t & NODE_WALLNB) lm = p.links[q.n];
   physical_fixups<Q, CD::shifted, COPIED>(
-      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S, lm,
+      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S,
       [&](auto I) -> uint8_t {
         constexpr int i = decltype(I)::value;
         constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
@@ -1228,7 +1229,9 @@ __device__ __forceinline__ void store_all(const P &p, const NodePos &q, const ty
         constexpr int l = Lat<Q>::opp[i];  // the link whose bounce-back value this population is
         if ((lm >> (l - 1)) & 1u) {
           const PullSources<Q> src(p, qc);
-          fout[(size_t)l * (size_t)p.stride + src(IC<l>{})] = v;
+          T *cp = fout + (size_t)l * (size_t)p.stride + src(IC<l>{});
+          if constexpr (NT) __builtin_nontemporal_store(v, cp);
+          else *cp = v;
         }
       }
     });

@@ -13,8 +13,8 @@ struct StepParams {
   void *fout;               // populations written
   const uint8_t *type;      // node types (with NODE_SLOW in PHYSICAL mode)
   // PHYSICAL 3D lattices: per node, bit i - 1 set when population i (1 <= i < Q) is pulled from a
-  // solid node, bit 31 when one of those solid sources is a moving wall (wall_link_mask); null
-  // otherwise.  Read only by wall-neighbour nodes, in place of one node-type load per link.
+  // solid node (the wall link mask); null otherwise.  Read only by wall-neighbour nodes, to store
+  // their wall copies.
   const uint32_t *links;
   int32_t nx, ny, nz;
   int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile
@@ -121,10 +121,13 @@ void build_mfma_table(int collision, const double *s, int engine, float *tab);
 hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, double *ux, double *uy,
                          double *uz, hipStream_t s);
 // out = in & 0x1f, plus NODE_SLOW on non-solid nodes that are inlet/outlet nodes or pull from a
-// solid node, and NODE_WALLNB on the latter (PHYSICAL mode); links (if not null): the wall link
-// masks (StepParams::links).
+// solid node, NODE_WALLNB on the latter, NODE_WALLMOV on those with a moving solid source (PHYSICAL
+// mode); links (if not null): the wall link masks (StepParams::links).
 hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
                                 uint32_t *links, hipStream_t s);
+// A lattice row (nx populations of one array) is a whole number of 128-byte cache lines, so that
+// wavefronts of 64 consecutive nodes write whole lines (non-temporal stores only then).
+inline bool rows_fill_lines(int nx, int esize) { return ((int64_t)nx * esize) % 128 == 0; }
 // Fills the wall copies (3D PHYSICAL lattices; see wall_copies in lbm_device.hpp) of the buffer
 // p.fout from its fluid populations; nothing to do for other lattices.
 hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s);

@@ -271,7 +271,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_prepare_types(int bc_mode, in
       sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
       sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
       const uint8_t ts = in[sx + (int64_t)nx * (sy + (int64_t)ny * sz)];
-      if (i != 0 && (ts & NODE_SOLID)) mask |= (1u << (i - 1)) | ((ts & NODE_MOVING) ? 0x80000000u : 0u);
+      if (i != 0 && (ts & NODE_SOLID)) {
+        mask |= 1u << (i - 1);
+        if (ts & NODE_MOVING) t |= NODE_WALLMOV;
+      }
     });
     if (mask) t |= NODE_SLOW | NODE_WALLNB;
     if (t & (NODE_INLET | NODE_OUTLET)) t |= NODE_SLOW;

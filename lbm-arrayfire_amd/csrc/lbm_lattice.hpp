@@ -15,9 +15,9 @@ namespace lbm {
 constexpr uint8_t NODE_SOLID = 0x01, NODE_MOVING = 0x02, NODE_INLET = 0x04, NODE_OUTLET = 0x08,
                   NODE_RHO_FIX = 0x10;
 // Internal: a non-solid node whose pull needs the boundary path (some pull source is solid, or
-// the node is an inlet/outlet), and, of those, the ones with a solid pull source.  Set by the
-// library at create time, PHYSICAL mode only.
-constexpr uint8_t NODE_SLOW = 0x80, NODE_WALLNB = 0x40;
+// the node is an inlet/outlet), of those the ones with a solid pull source, and of those the
+// ones with a moving solid pull source.  Set by the library at create time, PHYSICAL mode only.
+constexpr uint8_t NODE_SLOW = 0x80, NODE_WALLNB = 0x40, NODE_WALLMOV = 0x20;
 
 enum Storage { ST_F64 = 0, ST_F32 = 1, ST_F16S = 2, ST_BF16S = 3 };
 enum Collision { CO_BGK = 0, CO_MRT_RAW = 1, CO_MRT_GS = 2 };
