@@ -262,13 +262,15 @@ def _rounded(d):
     return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
 
 
-def serving_leg(njobs, scale, sides="mixed", accuracy_every=10, modes=("realtime", "saturated", "sequential")):
+def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
+                modes=("realtime", "saturated", "sequential", "plain")):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
     continuous-batching server, in real time and saturated, against the same jobs run one at a
-    time with the plain solver.  Accuracy: every accuracy_every-th job is served again (alone in a
-    server of its own kind of traffic: the sample, saturated) with its fields kept, and compared
-    with the same job run in FP64 storage."""
+    time: "sequential" with the solver the server runs (same storage and kernels), "plain" with the
+    plain solver (FP32 storage, one fused launch per step).  Accuracy: every accuracy_every-th job
+    is served again (alone in a server of its own kind of traffic: the sample, saturated) with its
+    fields kept, and compared with the same job run in FP64 storage."""
     import numpy as np
 
     from lbm_amd import serving
@@ -279,11 +281,17 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10, modes=("realtime
     rt = serving.serve(jobs, budget, realtime=True, progress=prog) if "realtime" in modes else None
     sat = serving.serve(jobs, budget, realtime=False, progress=prog) if "saturated" in modes else None
     seq = serving.sequential(jobs, progress=prog) if "sequential" in modes else None
+    plain = serving.sequential(jobs, progress=prog, plain=True) if "plain" in modes else None
     served = sat or rt
     out = {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
            "storage": "F16S", "realtime": rt and _rounded(rt), "saturated": sat and _rounded(sat),
-           "one_at_a_time": seq and _rounded(seq),
+           "one_at_a_time": seq and _rounded({k: v for k, v in seq.items() if not isinstance(v, str)}),
+           "plain_one_at_a_time": plain and _rounded({k: v for k, v in plain.items() if not isinstance(v, str)}),
            "throughput_vs_one_at_a_time": round(served["mlups"] / seq["mlups"], 3) if served and seq else None,
+           "throughput_vs_plain_solver": round(served["mlups"] / plain["mlups"], 3) if served and plain else None,
+           "baselines": {"one_at_a_time": "each job alone with the server's solver: same storage, same kernels, "
+                                          "multi-step launches where they apply",
+                         "plain_solver": "each job alone, FP32 storage, one fused launch per step"},
            "results_equal_alone_and_in_mix": "bitwise (tests/test_gpu_batch_serve.py)",
            "total_lattice_updates": sum(j["nodes"] * j["steps"] for j in jobs)}
     if accuracy_every:
@@ -439,8 +447,9 @@ def main():
     ap.add_argument("--serve-only", action="store_true",
                     help="only run the serving leg of the mixed mix and print it (with --serve-jobs, "
                          "--serve-scale, --serve-modes)")
-    ap.add_argument("--serve-modes", default="realtime,saturated,sequential",
-                    help="serving runs of --serve-only: realtime, saturated, sequential")
+    ap.add_argument("--serve-modes", default="realtime,saturated,sequential,plain",
+                    help="serving runs of --serve-only: realtime, saturated, sequential (same solver), plain "
+                         "(FP32, one launch per step), accuracy (the FP64 comparison of a sample)")
     ap.add_argument("--accuracy", choices=["sphere", "lid128"], default=None,
                     help="only run the accuracy comparison of one configuration against FP64 and print it")
     ap.add_argument("--serve-jobs", type=int, default=120,
@@ -459,8 +468,9 @@ def main():
         print(json.dumps(accuracy_run(args.accuracy)))
         return
     if args.serve_only:
-        print(json.dumps(serving_leg(args.serve_jobs, args.serve_scale, "mixed", accuracy_every=50,
-                                     modes=tuple(args.serve_modes.split(",")))))
+        modes = tuple(args.serve_modes.split(","))
+        print(json.dumps(serving_leg(args.serve_jobs, args.serve_scale, "mixed",
+                                     accuracy_every=50 if "accuracy" in modes else 0, modes=modes)))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "

@@ -18,7 +18,8 @@ import time
 import numpy as np
 
 from . import cases, geometry as geo
-from .capi import BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, INLET_EXTRAPOLATED, INLET_UNIFORM, MRT_GS, Lattice, Server
+from .capi import (BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, F32, INLET_EXTRAPOLATED, INLET_UNIFORM, MRT_GS, Lattice,
+                   Server)
 
 NU_MIN = 1.0 / 150.0
 U_LAT = 0.05
@@ -136,27 +137,50 @@ def fp64_errors(job, fields, steps=None):
     return e_rho, float(np.linalg.norm(ua - ub) / max(np.linalg.norm(ub), 1e-300))
 
 
-def sequential(jobs, progress=0):
-    """The plain solver, one job at a time in arrival order (no idle time between jobs)."""
+def sequential(jobs, progress=0, plain=False):
+    """One job at a time in arrival order (no idle time between jobs), each on a lattice of its own.
+    plain=False: the same solver the server runs (the job's storage, multi-step launches where they
+    apply); plain=True: the plain solver, FP32 storage and one fused launch per step (the
+    reference's one-lattice-per-program shape on this library's single-step kernel)."""
+    import os
     t0 = time.perf_counter()
     lups = 0
     t_setup = t_run = t_end = 0.0
-    for k, j in enumerate(jobs):
-        if progress and k and k % progress == 0:
-            print(f"sequential: {k}/{len(jobs)} jobs done at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                  flush=True)
-        a = time.perf_counter()
-        with Lattice(j["cfg"], j["types"]) as lat:
-            lat.init_equilibrium(1.0, j["u0"], j["seed"], 1e-4)
-            b = time.perf_counter()
-            lat.step(j["steps"])
-            lat.monitors()
-            c = time.perf_counter()
-        d = time.perf_counter()
-        t_setup += b - a
-        t_run += c - b
-        t_end += d - c
-        lups += j["nodes"] * j["steps"]
+    saved = os.environ.get("LBM_MULTISTEP")
+    if plain:
+        os.environ["LBM_MULTISTEP"] = "off"  # read by lbm_create
+    try:
+        for k, j in enumerate(jobs):
+            if progress and k and k % progress == 0:
+                print(f"sequential{' plain' if plain else ''}: {k}/{len(jobs)} jobs done at "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            a = time.perf_counter()
+            cfg = dict(j["cfg"], storage=F32) if plain else j["cfg"]
+            b, c = _run_alone(cfg, j)
+            d = time.perf_counter()
+            t_setup += b - a
+            t_run += c - b
+            t_end += d - c
+            lups += j["nodes"] * j["steps"]
+    finally:
+        if plain:
+            if saved is None:
+                os.environ.pop("LBM_MULTISTEP", None)
+            else:
+                os.environ["LBM_MULTISTEP"] = saved
     wall = time.perf_counter() - t0
     return dict(wall_s=wall, jobs=len(jobs), jobs_per_s=len(jobs) / wall, mlups=lups / wall / 1e6,
-                setup_s=t_setup, run_s=t_run, release_s=t_end)
+                setup_s=t_setup, run_s=t_run, release_s=t_end, storage="F32" if plain else "job's",
+                launches="one per step" if plain else "multi-step where it applies")
+
+
+def _run_alone(cfg, j):
+    """Creates, initialises, steps and releases one job's lattice; returns the times at which the
+    steps started and ended."""
+    with Lattice(cfg, j["types"]) as lat:
+        lat.init_equilibrium(1.0, j["u0"], j["seed"], 1e-4)
+        b = time.perf_counter()
+        lat.step(j["steps"])
+        lat.monitors()
+        c = time.perf_counter()
+    return b, c

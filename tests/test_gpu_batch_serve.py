@@ -50,10 +50,13 @@ def make(spec):
     return lat
 
 
-@pytest.mark.parametrize("graphs", ["1", "0"])
-def test_batch_matches_individual_stepping(monkeypatch, graphs):
-    """Replayed two-step graphs (default) and direct launches (LBM_BATCH_GRAPH=0) alike."""
+@pytest.mark.parametrize("graphs,concurrent", [("1", "1"), ("0", "1"), ("1", "0")])
+def test_batch_matches_individual_stepping(monkeypatch, graphs, concurrent):
+    """Replayed two-step graphs (default) and direct launches (LBM_BATCH_GRAPH=0) alike, with the
+    configuration groups on streams of their own (default) or all on the batch's stream
+    (LBM_BATCH_CONCURRENT=0)."""
     monkeypatch.setenv("LBM_BATCH_GRAPH", graphs)
+    monkeypatch.setenv("LBM_BATCH_CONCURRENT", concurrent)
     specs = member_specs()
     alone = []
     for sp in specs:
