@@ -8,8 +8,10 @@ with a cylinder/sphere of radius 5-15 % of the height; steps uniform in [1000, 2
 ``step_scale``; FP32 arithmetic with FP16-shifted storage (``storage`` selects another).
 
 Lattice-velocity policy (the reference fixes u and derives nu from Re): u = 0.05 for lids and
-inlets, nu = u L / Re, clamped to nu >= 1/150 (tau >= 0.52) so every job is stable; the Reynolds
-number actually run is recorded with the job.
+inlets, nu = u L / Re, clamped to nu >= 1/150 (tau >= 0.52), and for the D2Q9 BGK channels with a
+cylinder to nu >= 1/60 (tau >= 0.55): of the 1,000-job mix, all 52 such channels with tau <= 0.535
+diverged within 2,200 steps, in FP32 and FP16-shifted storage alike, while no lid and no D3Q27 MRT
+job did.  The Reynolds number actually run is recorded with the job.
 """
 import math
 import sys
@@ -22,6 +24,7 @@ from .capi import (BC_PHYSICAL, BGK, D2Q9, D3Q27, F16S, F32, INLET_EXTRAPOLATED,
                    Server)
 
 NU_MIN = 1.0 / 150.0
+NU_MIN_BGK_CHANNEL = 1.0 / 60.0
 U_LAT = 0.05
 
 
@@ -42,6 +45,8 @@ def make_job(rng, step_scale=1.0, sides="mixed", storage=F16S):
     lid = rng.random() < 0.5
     steps = max(1, int(round(int(rng.integers(1000, 20001)) * step_scale)))
     L = side - 2
+    if not d3 and not lid:
+        nu_min = max(nu_min, NU_MIN_BGK_CHANNEL)
     nu = max(U_LAT * L / re, nu_min)
     re_run = U_LAT * L / nu
     om = cases.omega_from_nu(nu)
@@ -116,7 +121,16 @@ def serve(jobs, budget_bytes, steps_per_tick=200, realtime=True, keep=(), progre
                     for r in res])
     lups = sum(r["nodes"] * r["steps"] for r in res)
     bad = sum(1 for r in res if r["status"] != 0)
-    out = dict(wall_s=wall, jobs=len(res), failed=bad, p50_s=float(np.percentile(lat, 50)),
+    failures = []
+    for r in res:
+        if r["status"] != 0:
+            j = jobs[ids[r["job_id"]]]
+            c = j["cfg"]
+            failures.append(dict(job=ids[r["job_id"]], status=int(r["status"]), steps_done=int(r["steps"]),
+                                 steps=j["steps"], kind=("D2Q9 BGK" if c["lattice"] == D2Q9 else "D3Q27 MRT-GS") +
+                                 (" lid" if "u_wall" in c else " channel"), side=c["nx"], re=j["re"],
+                                 re_run=j["re_run"], tau=1.0 / c["omega"]))
+    out = dict(wall_s=wall, jobs=len(res), failed=bad, failures=failures, p50_s=float(np.percentile(lat, 50)),
                p99_s=float(np.percentile(lat, 99)), jobs_per_s=len(res) / wall, mlups=lups / wall / 1e6,
                server=stats)
     if keep:
