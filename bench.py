@@ -259,7 +259,12 @@ def cpu_baseline(name, target_s=10.0):
 
 
 def _rounded(d):
-    return {k: (_rounded(v) if isinstance(v, dict) else round(v, 4)) for k, v in d.items()}
+    """Floats of a (nested) result rounded to 4 decimals; other values as they are."""
+    if isinstance(d, dict):
+        return {k: _rounded(v) for k, v in d.items()}
+    if isinstance(d, (list, tuple)):
+        return [_rounded(v) for v in d]
+    return round(d, 4) if isinstance(d, float) else d
 
 
 def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
