@@ -281,7 +281,10 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
     from lbm_amd import serving
 
     jobs = serving.make_mix(njobs, step_scale=scale, sides=sides)
-    budget = 64 << 30
+    # admission budget: 4 GiB of resident jobs measured best on the mixed mix (1,000 jobs, steps x 0.2:
+    # 31.8k MLUPS and p50 36.7 s, against 30.5k and 53.4 s with 64 GiB, where hundreds of jobs share
+    # every tick)
+    budget = 4 << 30
     prog = 100 if njobs >= 500 else 0
     rt = serving.serve(jobs, budget, realtime=True, progress=prog) if "realtime" in modes else None
     sat = serving.serve(jobs, budget, realtime=False, progress=prog) if "saturated" in modes else None

@@ -16,12 +16,15 @@
 // two steps are captured in a hipGraph of its own and replayed; the graphs are re-captured when
 // the set of groups changes, a table is reallocated (grown) or a launch shape changes.
 //
-// Groups are independent of each other, so with two or more groups every batch step call forks:
-// each group runs its steps on a stream of its own, joined back into the batch's stream at the
-// end, and the device overlaps the groups' launches (a D2Q9 multi-step group, bound by its
-// on-chip work, beside a D3Q27 group bound by HBM traffic: 1.03-1.27x over running them one after
-// the other on MI355X, by lattice sizes).  LBM_BATCH_CONCURRENT=0 keeps every group on the
-// batch's stream.
+// Groups are cut to the device's last-level cache: members of one configuration are dealt first
+// fit into groups whose two population buffers per member total at most the bin size
+// (LBM_BATCH_BIN_MIB, default 160 MiB of the 256 MB Infinity Cache; 0 = one group per
+// configuration), a member larger than that in a group of its own.  A batch step call runs each
+// group's steps back to back, so a group's working set stays in the cache across its steps, as a
+// medium lattice stepped alone does; tiny members still share one launch per step.  When all
+// groups fit the bin size together, they run concurrently instead: each on a stream of its own,
+// forked from and joined back into the batch's stream (LBM_BATCH_CONCURRENT=0 keeps them on the
+// batch's stream), and the device overlaps their launches.
 //
 // While a lattice is a member all of its work goes to the batch's stream, which keeps member
 // reads, writes and batch steps in one order.  Joining and leaving are ordered against the
@@ -61,6 +64,8 @@ struct Group {
   int tuned = 0;               // grid chosen by lbm_batch_tune, 0 = default
   int nt = 0;                  // single-step launches with non-temporal stores (3D scalar groups too
                                // large for the caches)
+  size_t bytes_used = 0;       // both population buffers of every member
+  int size_class = 0;          // members' node counts within one factor of 4 (LBM_BATCH_SIZE_CLASSES)
   hipStream_t gs = nullptr;    // the group's own stream (concurrent groups)
   hipEvent_t done = nullptr;   // the group's work of one call, recorded on gs
   hipGraphExec_t graph = nullptr;  // two single steps (phase 0 then 1)
@@ -93,6 +98,20 @@ struct Group {
   }
 };
 
+// Members whose node counts lie within one factor of 4 of each other share a size class: lattices
+// of similar size step well in one launch (8 D3Q27 MRT 64^3 lattices: 1.7x their rate one after
+// the other), while a launch over very different sizes ends each step on its largest member's
+// tail (8 lattices of sides 40-110 batched: 0.93x one after the other; as 4 concurrent batches:
+// 1.14x).
+int size_class_of(int64_t nodes) {
+  int c = 0;
+  while (nodes >= 4) {
+    nodes >>= 2;
+    ++c;
+  }
+  return c;
+}
+
 bool same_cfg(const Config &a, const Config &b) {
   return a.Q == b.Q && a.storage == b.storage && a.bc == b.bc && a.collision == b.collision &&
          uses_mfma(a) == uses_mfma(b) && (!uses_mfma(a) || a.engine == b.engine);
@@ -110,6 +129,9 @@ struct lbm_batch_ctx {
   hipEvent_t fork = nullptr;  // the batch stream's point where concurrent groups start
   bool use_graph = true;   // LBM_BATCH_GRAPH=0: direct launches instead of graph replays
   bool concurrent = true;  // LBM_BATCH_CONCURRENT=0: every group on the batch's stream
+  size_t bin_bytes = 0;  // LBM_BATCH_BIN_MIB: group size limit in MiB, 0 (default) = none
+  size_t stream_bytes = (size_t)256 << 20;  // LBM_BATCH_STREAM_MIB: groups above take tiles in stream order
+  bool size_classes = true;  // LBM_BATCH_SIZE_CLASSES=0: one group per configuration
   int64_t steps = 0;
   lbm::BatchCounters counters{};
 
@@ -150,30 +172,40 @@ lbm_status reserve(Group &g, int n, bool *grown) {
 // Regroups the members and rewrites the group tables.  Members' cur is current (the batch phase
 // is folded in by enqueue), so the tables restart at phase 0.
 lbm_status rebuild(lbm_batch_ctx *b) {
-  bool recapture = false;
-  for (auto &g : b->groups) g.members.clear();
+  for (auto &g : b->groups) {
+    g.members.clear();
+    g.bytes_used = 0;
+  }
+  // first fit, in member order, into groups of one configuration and size class within the bin size
   for (lbm_ctx *m : b->members) {
+    const size_t mb = 2 * m->buf_bytes();
+    const int sc = b->size_classes ? size_class_of(m->N) : 0;
     Group *grp = nullptr;
     for (auto &g : b->groups)
-      if (same_cfg(g.cfg, m->cfg)) grp = &g;
+      if (same_cfg(g.cfg, m->cfg) && g.size_class == sc &&
+          (b->bin_bytes == 0 || g.bytes_used == 0 || g.bytes_used + mb <= b->bin_bytes)) {
+        grp = &g;
+        break;
+      }
     if (!grp) {
       b->groups.emplace_back();
       grp = &b->groups.back();
       grp->cfg = m->cfg;
-      recapture = true;
+      grp->size_class = sc;
     }
     grp->members.push_back(m);
+    grp->bytes_used += mb;
   }
   for (size_t gi = 0; gi < b->groups.size();) {
     if (b->groups[gi].members.empty()) {
       b->groups[gi].release();
       b->groups.erase(b->groups.begin() + (long)gi);
-      recapture = true;
     } else {
       ++gi;
     }
   }
   for (auto &g : b->groups) {
+    bool recapture = false;  // this group's captured steps no longer match its table or shape
     const int n = (int)g.members.size();
     bool grown = false;
     lbm_status st = reserve(g, n, &grown);
@@ -208,7 +240,7 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     hdr->nmembers = n;
     hdr->total_tiles = g.total_tiles;
     hdr->total_multi = g.total_multi;
-    hdr->stream = group_bytes > ((size_t)256 << 20) ? 1 : 0;
+    hdr->stream = group_bytes > b->stream_bytes ? 1 : 0;
     // such groups also store non-temporally, as single lattices over the cache size do (3D scalar
     // kernels); the captured launches carry the flag, so a change re-captures them
     int nt = hdr->stream && g.cfg.Q != 9 && !uses_mfma(g.cfg) ? 1 : 0;
@@ -234,8 +266,8 @@ lbm_status rebuild(lbm_batch_ctx *b) {
     else if (grid < need || grid > 2 * need) grid = need + need / 8 + 1;
     if (grid != g.grid) recapture = true;
     g.grid = grid;
+    if (recapture) g.drop_graph();
   }
-  if (recapture) b->drop_graph();
   b->dirty = false;
   ++b->counters.rebuilds;
   return LBM_OK;
@@ -322,7 +354,9 @@ lbm_status enqueue(lbm_batch_ctx *b, int64_t nsteps) {
   }
   if (b->groups.empty() || nsteps == 0) return LBM_OK;
   b->steps += nsteps;
-  if (b->groups.size() == 1 || !b->concurrent) {
+  size_t total = 0;
+  for (const auto &g : b->groups) total += g.bytes_used;
+  if (b->groups.size() == 1 || !b->concurrent || (b->bin_bytes && total > b->bin_bytes)) {
     for (auto &g : b->groups) {
       lbm_status st = enqueue_group(b, g, nsteps, b->stream);
       if (st != LBM_OK) return st;
@@ -370,6 +404,9 @@ lbm_status lbm_batch_create(lbm_batch *out) {
     }
     if (const char *e = std::getenv("LBM_BATCH_GRAPH")) b->use_graph = std::atoi(e) != 0;
     if (const char *e = std::getenv("LBM_BATCH_CONCURRENT")) b->concurrent = std::atoi(e) != 0;
+    if (const char *e = std::getenv("LBM_BATCH_BIN_MIB")) b->bin_bytes = (size_t)std::max(0, std::atoi(e)) << 20;
+    if (const char *e = std::getenv("LBM_BATCH_STREAM_MIB")) b->stream_bytes = (size_t)std::max(0, std::atoi(e)) << 20;
+    if (const char *e = std::getenv("LBM_BATCH_SIZE_CLASSES")) b->size_classes = std::atoi(e) != 0;
     *out = b;
     return LBM_OK;
   });
@@ -424,7 +461,9 @@ lbm_status lbm_batch_step_timed(lbm_batch b, int64_t nsteps, double *ms) {
       if (st != LBM_OK) return st;
     }
     if (nsteps >= 2 && b->use_graph) {  // capture outside the timed region
-      const bool conc = b->groups.size() > 1 && b->concurrent;
+      size_t total = 0;
+      for (const auto &g : b->groups) total += g.bytes_used;
+      const bool conc = b->groups.size() > 1 && b->concurrent && !(b->bin_bytes && total > b->bin_bytes);
       for (auto &g : b->groups)
         if (!g.multi_k && g.phase == 0 && (!conc || g.gs)) {
           lbm_status st = ensure_graph(b, g, conc ? g.gs : b->stream);
