@@ -268,7 +268,7 @@ def _rounded(d):
 
 
 def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
-                modes=("realtime", "saturated", "sequential", "plain")):
+                modes=("realtime", "saturated", "sequential", "plain"), storage="auto"):
     """Serving mix (Poisson arrivals at 50 jobs/s, half D2Q9 BGK 64^2-2048^2, half D3Q27 MRT
     32^3-192^3, FP16-shifted storage; sides="small": 32^2-256^2 and 16^3-48^3) through the
     continuous-batching server, in real time and saturated, against the same jobs run one at a
@@ -280,7 +280,9 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
 
     from lbm_amd import serving
 
-    jobs = serving.make_mix(njobs, step_scale=scale, sides=sides)
+    from lbm_amd.capi import F16S
+
+    jobs = serving.make_mix(njobs, step_scale=scale, sides=sides, storage="auto" if storage == "auto" else F16S)
     # admission budget: 4 GiB of resident jobs measured best on the mixed mix (1,000 jobs, steps x 0.2:
     # 31.8k MLUPS and p50 36.7 s, against 30.5k and 53.4 s with 64 GiB, where hundreds of jobs share
     # every tick)
@@ -292,7 +294,8 @@ def serving_leg(njobs, scale, sides="mixed", accuracy_every=10,
     plain = serving.sequential(jobs, progress=prog, plain=True) if "plain" in modes else None
     served = sat or rt
     out = {"mix": sides, "jobs": njobs, "arrival_rate_per_s": 50.0, "step_scale": scale, "budget_bytes": budget,
-           "storage": "F16S", "realtime": rt and _rounded(rt), "saturated": sat and _rounded(sat),
+           "storage": "D2Q9 FP32, D3Q27 F16S" if storage == "auto" else "F16S",
+           "realtime": rt and _rounded(rt), "saturated": sat and _rounded(sat),
            "one_at_a_time": seq and _rounded({k: v for k, v in seq.items() if not isinstance(v, str)}),
            "plain_one_at_a_time": plain and _rounded({k: v for k, v in plain.items() if not isinstance(v, str)}),
            "throughput_vs_one_at_a_time": round(served["mlups"] / seq["mlups"], 3) if served and seq else None,
@@ -460,6 +463,8 @@ def main():
                          "(FP32, one launch per step), accuracy (the FP64 comparison of a sample)")
     ap.add_argument("--accuracy", choices=["sphere", "lid128"], default=None,
                     help="only run the accuracy comparison of one configuration against FP64 and print it")
+    ap.add_argument("--serve-storage", choices=["auto", "f16s"], default="auto",
+                    help="storage of the serving mix's jobs: auto (D2Q9 FP32, D3Q27 FP16-shifted) or f16s (all)")
     ap.add_argument("--serve-jobs", type=int, default=120,
                     help="jobs of the serving mix (0 skips the serving leg)")
     ap.add_argument("--serve-scale", type=float, default=0.05,
@@ -478,7 +483,8 @@ def main():
     if args.serve_only:
         modes = tuple(args.serve_modes.split(","))
         print(json.dumps(serving_leg(args.serve_jobs, args.serve_scale, "mixed",
-                                     accuracy_every=50 if "accuracy" in modes else 0, modes=modes)))
+                                     accuracy_every=50 if "accuracy" in modes else 0, modes=modes,
+                                     storage=args.serve_storage)))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no "
