@@ -564,7 +564,9 @@ def main():
                      "traffic_gbs": None if not traffic else round(traffic / launch_s / 1e9, 1),
                      "traffic_frac": None if not traffic else round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic_over_algorithmic": None if ratio is None else round(ratio, 4),
-                     "traffic_source": None if not tr else "profiles/traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+                     "traffic_source": None if not tr else ("profiles/traffic.json (rocprofv3 FETCH_SIZE, WRITE_SIZE: the L2's "
+                                                             "fills from and writes to the memory fabric, Infinity "
+                                                             "Cache hits included)"),
                      "kernel": kernel_name(cfg, r["steps_per_launch"], r["nt"]),
                      "steps_per_launch": r["steps_per_launch"], "avg_launch_us": round(r["launch_us"], 3),
                      "algorithmic_bytes_per_launch": r["bytes_per_launch"],
