@@ -57,8 +57,9 @@ typedef enum { LBM_D2Q9 = 9, LBM_D3Q19 = 19, LBM_D3Q27 = 27 } lbm_lattice;
 typedef enum { LBM_BGK = 0, LBM_MRT_RAW = 1, LBM_MRT_GS = 2 } lbm_collision;
 
 /* Population storage in HBM.  F16S stores the shifted value (f - w_i) * 2^15 as IEEE half,
- * BF16S stores f - w_i as bfloat16 (both round to nearest even) with FP32 collision arithmetic;
- * F64 computes in FP64, all others in FP32.  Lattices hold fewer than 2^32 nodes, and F16S /
+ * BF16S stores f - w_i as bfloat16, with FP32 collision arithmetic on the deviations; the steps
+ * round them stochastically (random bits from the node index and the lattice's step count), the
+ * initial states to nearest even.  F64 computes in FP64, all others in FP32.  Lattices hold fewer than 2^32 nodes, and F16S /
  * BF16S lattices fewer than 2^31 (lbm_create returns LBM_EINVAL otherwise). */
 typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_storage;
 
@@ -67,7 +68,9 @@ typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_stora
  * (cpp/d2q9_bgk_lid.cpp:94-98,141-155; rust/examples/d2q9_bgk_channel.rs:190-201).
  * PHYSICAL: half-way (and moving-wall) bounce-back at links into solid nodes, non-equilibrium
  * bounce-back (Zou-He) velocity inlet, zero-gradient outlet at a reference density (no reference
- * counterpart). */
+ * counterpart).  In PHYSICAL 3D lattices slot i of a solid node s holds population opp(i) of its
+ * fluid neighbour s + c_i (the half-way bounce-back value that node pulls next step); the steps
+ * and every state setter keep these slots, and other solid slots keep their values. */
 typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
 /* How the MRT moment transforms are evaluated.  The MFMA engines evaluate the collision in
@@ -232,7 +235,8 @@ lbm_status lbm_monitors(lbm_handle h, double *mass, double *kinetic_energy, int6
  * reports no forces).  Blocks. */
 lbm_status lbm_force(lbm_handle h, const int32_t box[6], double force[3]);
 
-/* ---- batching: many lattices, one persistent launch per configuration per step -------- */
+/* ---- batching: many lattices, one launch per group (kernel configuration and size class, node
+ * counts within a factor of 4) per step, the groups on concurrent streams ------------------- */
 /* No reference counterpart (each reference program steps one lattice).  Members keep their own
  * state and stay readable through their handles (lbm_get_*, lbm_monitors, lbm_set_*); while a
  * lattice is a member its work runs on the batch's stream and lbm_step on it is refused.  Stepping
