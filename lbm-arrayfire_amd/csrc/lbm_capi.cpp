@@ -218,16 +218,20 @@ lbm_status create_lattice(const lbm_desc *d, const uint8_t *node_type, hipStream
   p.ny = d->ny;
   p.nz = d->nz;
   tile_grid(d->nx, d->ny, d->nz, p.gx, p.gy, p.gz, p.ntiles);
+  p.flat = flat_tiles(d->nx) ? 1 : 0;
+  make_fastdiv((uint32_t)d->nx, p.div_nx);
+  make_fastdiv((uint32_t)d->ny, p.div_ny);
   p.inlet_profile = d->inlet_profile;
   p.inlet_closure = d->inlet_closure;
   p.meq_fix26 = d->collision == LBM_MRT_RAW && d->meq_mode == LBM_MEQ_REFERENCE ? 1 : 0;
   p.meq_mode = d->meq_mode;
   // Non-temporal population stores for 3D lattices stepped on their own whose buffers exceed the
   // 256 MB Infinity Cache (measured on D3Q27: +2 to +6 % from 192^3 F16S / 160^3 FP32 up, -1.4 %
-  // at 160^3 F16S, -18 % at 128^3 F16S), and whose rows are whole 128-byte lines: a wavefront's
-  // stores then fill whole lines, while on other row widths each store leaves two partial lines,
-  // which non-temporal writes send to memory as they are (D3Q27 BGK F16S, nx = 193 x 192 x 192:
-  // 15.2k MLUPS non-temporal against 32.2k with ordinary stores, which the L2 merges).
+  // at 160^3 F16S, -18 % at 128^3 F16S), and whose wavefronts' stores fill whole 128-byte lines
+  // (rows_fill_lines: flat tiles, or rows of whole lines).  A wavefront over a row that starts
+  // inside a line leaves two partial lines per store, which non-temporal writes send to memory as
+  // they are (64 x 4 tiles of a 193 x 192 x 192 D3Q27 BGK F16S lattice: 15.2k MLUPS non-temporal
+  // against 32.2k with ordinary stores, which the L2 merges).
   // LBM_NT_STORE=0 / 1 forces them off / on.
   h->cfg.nt = h->buf_bytes() > ((size_t)256 << 20) && rows_fill_lines(d->nx, h->esize) ? 1 : 0;
   if (const char *e = std::getenv("LBM_NT_STORE")) h->cfg.nt = std::atoi(e) != 0 ? 1 : 0;

@@ -189,10 +189,26 @@ struct NodePos {
   bool valid;
 };
 
+__device__ __forceinline__ uint32_t fastdiv(uint32_t n, uint32_t m, uint32_t s1, uint32_t s2) {
+  const uint32_t t = __umulhi(m, n);
+  return (t + ((n - t) >> s1)) >> s2;
+}
+
 template <class P>
 __device__ __forceinline__ NodePos node_of_tile(const P &p, int tile) {
-  const int bx = tile % p.gx, t2 = tile / p.gx, by = t2 % p.gy, bz = t2 / p.gy;
   NodePos q;
+  if (p.flat) {  // 256 consecutive node indices per tile (rows not a multiple of 64 wide)
+    const uint32_t n = (uint32_t)tile * (uint32_t)BLOCK_THREADS + threadIdx.x;
+    const uint32_t yz = fastdiv(n, p.div_nx[0], p.div_nx[1], p.div_nx[2]);
+    const uint32_t z = fastdiv(yz, p.div_ny[0], p.div_ny[1], p.div_ny[2]);
+    q.x = (int)(n - yz * (uint32_t)p.nx);
+    q.y = (int)(yz - z * (uint32_t)p.ny);
+    q.z = (int)z;
+    q.n = n;
+    q.valid = (int64_t)n < p.N;
+    return q;
+  }
+  const int bx = tile % p.gx, t2 = tile / p.gx, by = t2 % p.gy, bz = t2 / p.gy;
   q.x = bx * TILE_X + (int)(threadIdx.x & (TILE_X - 1));
   q.y = by * TILE_Y + (int)(threadIdx.x / TILE_X);
   q.z = bz;

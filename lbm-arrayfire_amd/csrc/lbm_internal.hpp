@@ -17,7 +17,9 @@ struct StepParams {
   // their wall copies.
   const uint32_t *links;
   int32_t nx, ny, nz;
-  int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile
+  int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile (flat lattices: ntiles x 1 x 1)
+  int32_t flat;             // tiles of 256 consecutive node indices (rows not a multiple of 64 wide)
+  uint32_t div_nx[3], div_ny[3];  // exact division by nx and by ny (FastDiv: multiplier, shifts)
   int32_t ntiles;
   int32_t inlet_profile;
   int32_t inlet_closure;    // 0 Zou-He (non-equilibrium bounce-back), 1 extrapolated (Guo et al.)
@@ -59,7 +61,31 @@ constexpr int MFMA_TAB = 48;
 // padded.  LBM_POP_PAD=0 in the environment disables the padding.
 int64_t population_stride(int64_t N, int esize);
 
+// Exact unsigned 32-bit division by an invariant d >= 1 through one high multiply (Granlund &
+// Montgomery, "Division by invariant integers using multiplication", 1994, fig. 4.1):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> min(l, 1))) >>
+// max(l - 1, 0) with t = mulhi(m, n), for every 32-bit n.
+inline void make_fastdiv(uint32_t d, uint32_t out[3]) {
+  int l = 0;
+  while ((1ull << l) < (unsigned long long)d) ++l;
+  out[0] = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+  out[1] = (uint32_t)(l < 1 ? l : 1);
+  out[2] = (uint32_t)(l > 0 ? l - 1 : 0);
+}
+// Lattices whose rows are not a multiple of 64 nodes wide are cut into tiles of 256 consecutive
+// node indices ("flat" tiles): every lane of a wavefront then holds a node, where 64 x 4 tiles
+// leave the lanes past each row's end idle (a 193-wide row takes 4 wavefronts for 193 nodes; the
+// kernels cost per wavefront, so D3Q27 BGK F16S ran 32.4k MLUPS at 193 x 192 x 192 against 43.4k
+// at 192).
+inline bool flat_tiles(int nx) { return nx % TILE_X != 0; }
 inline void tile_grid(int nx, int ny, int nz, int32_t &gx, int32_t &gy, int32_t &gz, int32_t &nt) {
+  if (flat_tiles(nx)) {
+    const int64_t N = (int64_t)nx * ny * nz;
+    nt = (int32_t)((N + BLOCK_THREADS - 1) / BLOCK_THREADS);
+    gx = nt;
+    gy = gz = 1;
+    return;
+  }
   gx = (nx + TILE_X - 1) / TILE_X;
   gy = (ny + TILE_Y - 1) / TILE_Y;
   gz = nz;
@@ -125,9 +151,13 @@ hipError_t launch_fields(const Config &c, const StepParams &p, double *rho, doub
 // mode); links (if not null): the wall link masks (StepParams::links).
 hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, const uint8_t *in, uint8_t *out,
                                 uint32_t *links, hipStream_t s);
-// A lattice row (nx populations of one array) is a whole number of 128-byte cache lines, so that
-// wavefronts of 64 consecutive nodes write whole lines (non-temporal stores only then).
-inline bool rows_fill_lines(int nx, int esize) { return ((int64_t)nx * esize) % 128 == 0; }
+// A wavefront's 64 consecutive stores of one population array fill whole 128-byte lines: its
+// nodes are 64 consecutive indices (flat tiles, or rows a multiple of 64 nodes wide) and a row of
+// whole lines starts each 64-node run (non-temporal stores only then).
+inline bool rows_fill_lines(int nx, int esize) {
+  return nx % 64 != 0 /* flat tiles: 64 consecutive indices, 64 * esize >= 128 bytes */ ||
+         ((int64_t)nx * esize) % 128 == 0;
+}
 // Fills the wall copies (3D PHYSICAL lattices; see wall_copies in lbm_device.hpp) of the buffer
 // p.fout from its fluid populations; nothing to do for other lattices.
 hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s);
