@@ -189,11 +189,6 @@ struct NodePos {
   bool valid;
 };
 
-__device__ __forceinline__ uint32_t fastdiv(uint32_t n, uint32_t m, uint32_t s1, uint32_t s2) {
-  const uint32_t t = __umulhi(m, n);
-  return (t + ((n - t) >> s1)) >> s2;
-}
-
 template <class P>
 __device__ __forceinline__ NodePos node_of_tile(const P &p, int tile) {
   NodePos q;

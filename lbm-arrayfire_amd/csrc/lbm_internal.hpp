@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "lbm_fastdiv.hpp"
 #include "lbm_lattice.hpp"
 
 namespace lbm {
@@ -61,17 +62,6 @@ constexpr int MFMA_TAB = 48;
 // padded.  LBM_POP_PAD=0 in the environment disables the padding.
 int64_t population_stride(int64_t N, int esize);
 
-// Exact unsigned 32-bit division by an invariant d >= 1 through one high multiply (Granlund &
-// Montgomery, "Division by invariant integers using multiplication", 1994, fig. 4.1):
-// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> min(l, 1))) >>
-// max(l - 1, 0) with t = mulhi(m, n), for every 32-bit n.
-inline void make_fastdiv(uint32_t d, uint32_t out[3]) {
-  int l = 0;
-  while ((1ull << l) < (unsigned long long)d) ++l;
-  out[0] = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
-  out[1] = (uint32_t)(l < 1 ? l : 1);
-  out[2] = (uint32_t)(l > 0 ? l - 1 : 0);
-}
 // Lattices whose rows are not a multiple of 64 nodes wide are cut into tiles of 256 consecutive
 // node indices ("flat" tiles): every lane of a wavefront then holds a node, where 64 x 4 tiles
 // leave the lanes past each row's end idle (a 193-wide row takes 4 wavefronts for 193 nodes; the
