@@ -380,18 +380,6 @@ static lbm_status ensure_fields(lbm_handle h) {
   return LBM_OK;
 }
 
-namespace {
-// Both buffers' wall copies after a state was written from outside the step kernels.
-hipError_t prime_walls(lbm_ctx *h) {
-  if (!lbm::wall_links_kept(h->Q, h->d.bc_mode)) return hipSuccess;
-  for (int b = 0; b < 2; ++b) {
-    const hipError_t e = lbm::launch_prime_wall_copies(h->cfg, h->params(b), h->strm());
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-}  // namespace
-
 lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, const double *uy,
                            const double *uz) {
   return lbm::abi_guard([&]() -> lbm_status {
@@ -408,7 +396,6 @@ lbm_status lbm_init_fields(lbm_handle h, const double *rho, const double *ux, co
         dev[k] = dst;
       }
     HIPCHK(launch_init_fields(h->cfg, h->params(h->cur), dev[0], dev[1], dev[2], dev[3], h->strm()));
-    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -422,7 +409,6 @@ lbm_status lbm_init_equilibrium(lbm_handle h, double rho0, const double u0[3], u
     HIPCHK(hipSetDevice(h->device));
     const double u[3] = {u0 ? u0[0] : 0.0, u0 ? u0[1] : 0.0, u0 ? u0[2] : 0.0};
     HIPCHK(launch_init_equilibrium(h->cfg, h->params(h->cur), rho0, u, seed, noise_amp, h->strm()));
-    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -434,7 +420,6 @@ lbm_status lbm_init_reference(lbm_handle h, double rho0) {
     if (!h) return fail(LBM_EINVAL, "handle is NULL");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(launch_init_reference(h->cfg, h->params(h->cur), rho0, h->strm()));
-    HIPCHK(prime_walls(h));
     HIPCHK(hipStreamSynchronize(h->strm()));
     h->steps = 0;
     return LBM_OK;
@@ -631,7 +616,6 @@ lbm_status lbm_set_state(lbm_handle h, const void *src) {
     std::memcpy(&ctr[0], (const char *)src + h->pop_bytes(), sizeof(uint32_t));
     ctr[1] = ctr[0];
     HIPCHK(hipMemcpyAsync(h->step_ctr(), ctr, sizeof(ctr), hipMemcpyHostToDevice, h->strm()));
-    HIPCHK(prime_walls(h));  // a saved state already holds them; populations set by hand may not
     HIPCHK(hipStreamSynchronize(h->strm()));
     return LBM_OK;
   });

@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include "lbm_internal.hpp"
@@ -34,7 +35,8 @@ __device__ __forceinline__ void static_for(F &&f) {
 // ---- storage codecs ----------------------------------------------------------------------
 template <int ST, int Q> struct Codec;
 // Each codec converts one population between storage (T) and compute (C) types:
-//   ld<I>(p, k)      the compute value of population I stored at p[k];
+//   ld<I>(p, k)      the compute value of population I stored at p[k] (dec<I>(raw<I>(p, k)));
+//   raw<I>(p, k)     the stored form at p[k];  dec<I>(t)  its compute value;
 //   enc<I>(v, r)     the stored form of compute value v (r: random bits, shifted codecs only);
 //   enc_f<I>(f)      the stored form of a population f given unshifted, rounded to nearest (the
 //                    initial states);
@@ -51,6 +53,8 @@ template <int Q> struct Codec<ST_F64, Q> {
   static constexpr bool shifted = false;
   static constexpr float S = 1.0f;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ T raw(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ C dec(T v) { return v; }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
   template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
   template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t) { p[k] = v; }
@@ -61,6 +65,8 @@ template <int Q> struct Codec<ST_F32, Q> {
   static constexpr bool shifted = false;
   static constexpr float S = 1.0f;
   template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ T raw(const T *p, uint32_t k) { return p[k]; }
+  template <int I> __device__ static __forceinline__ C dec(T v) { return v; }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t) { return v; }
   template <int I> __device__ static __forceinline__ T enc_f(C v) { return v; }
   template <int I> __device__ static __forceinline__ void st(T *p, uint32_t k, C v, uint32_t) { p[k] = v; }
@@ -87,10 +93,11 @@ template <int Q> struct Codec<ST_F16S, Q> {
   using C = float;
   static constexpr bool shifted = true;
   static constexpr float S = F16S_SCALE;
-  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
-    const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
-    return (float)__builtin_bit_cast(_Float16, v);
+  template <int I> __device__ static __forceinline__ T raw(const T *p, uint32_t k) {
+    return *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
   }
+  template <int I> __device__ static __forceinline__ C dec(T v) { return (float)__builtin_bit_cast(_Float16, v); }
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return dec<I>(raw<I>(p, k)); }
   // the conversion writes the low half of its destination and keeps the high half, which no
   // store reads: the destination starts as the value's own register (no zeroing move)
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
@@ -112,10 +119,11 @@ template <int Q> struct Codec<ST_BF16S, Q> {
   using C = float;
   static constexpr bool shifted = true;
   static constexpr float S = 1.0f;
-  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) {
-    const T v = *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
-    return __builtin_bit_cast(float, (uint32_t)v << 16);
+  template <int I> __device__ static __forceinline__ T raw(const T *p, uint32_t k) {
+    return *(const T *)((const char *)p + (uint32_t)(k * (uint32_t)sizeof(T)));
   }
+  template <int I> __device__ static __forceinline__ C dec(T v) { return __builtin_bit_cast(float, (uint32_t)v << 16); }
+  template <int I> __device__ static __forceinline__ C ld(const T *p, uint32_t k) { return dec<I>(raw<I>(p, k)); }
   template <int I> __device__ static __forceinline__ T enc(C v, uint32_t r) {
     const bf162_t h = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf162_t, v), v, r, false);
     return __builtin_bit_cast(uint16_t, h[0]);
@@ -395,12 +403,24 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   east(IC<j>)      population j of the node at x + 1;  east_type()  its type
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
 // 6 w_i rho_w c_i.u_w when the source is a moving wall (rho_w taken in compute units, S rho_bc).
-// COPIED (3D lattices stepped from global memory): the step that produced this node's populations
-// also stored each resting-wall value f_opp(i) into the slot i of the solid source (wall_copies in
-// store_all), so the periodic pull has already read it; only moving-wall links remain, on nodes
-// of type NODE_WALLMOV (t), told apart through src_type.
-template <int Q, bool COPIED, class C, class SrcType, class Own, class P>
-__device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, SrcType &&src_type, Own &&own) {
+// MaskedCodec (not void: 3D lattices stepped from global memory, whose storage codec it is): lm,
+// the node's wall link mask, marks the links
+// whose source is solid (bit i - 1 for population i), so no source type is read for a resting
+// wall.  The masked links reload the node's own populations in place of what the periodic pull
+// read from the solid slots (those slots hold the rest equilibrium); moving-wall links, on nodes
+// of type NODE_WALLMOV (t) only, are told apart through src_type.  Unmasked (D2Q9): every link's
+// source type is read.
+template <int Q>
+constexpr uint32_t links_with_c(int a, int sg) {  // link mask bits (i - 1) of the directions with c_a = sg
+  uint32_t m = 0;
+  for (int i = 1; i < Q; ++i)
+    if ((a == 0 ? Lat<Q>::cx[i] : a == 1 ? Lat<Q>::cy[i] : Lat<Q>::cz[i]) == sg) m |= 1u << (i - 1);
+  return m;
+}
+template <int Q, class MaskedCodec, class C, class SrcType, class Own, class P>
+__device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, uint32_t lm, SrcType &&src_type,
+                                           Own &&own) {
+  constexpr bool MASKED = !std::is_void<MaskedCodec>::value;
   const C rw = (C)p.rho_bc * S;
   auto moving_term = [&](auto I) -> C {
     constexpr int i = decltype(I)::value;
@@ -408,12 +428,47 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, Src
     if (Lat<Q>::D == 3) cu = cu + (C)Lat<Q>::cz[i] * (C)p.u_wall[2];
     return (((C)6 * (C)Lat<Q>::w[i]) * rw) * cu;
   };
-  if constexpr (COPIED) {
+  if constexpr (MASKED) {
+    // own(IC<j>) hands out population j of the node in storage form here (MaskedCodec::dec: its
+    // compute value).  The links are reloaded by group: the nine directions with c_a = s for
+    // (a, s) = (x, +1), (x, -1), (y, +1), ... -- the links of one planar wall are one group.  A
+    // group that no lane of the wavefront still needs is skipped, within a group every load is
+    // issued before any is decoded (one memory round trip per group), and the links a group
+    // reloaded are cleared from the lane's mask for the groups after it.
+    uint32_t todo = lm;
+    static_for<6>([&](auto Gk) {
+      constexpr int a = (int)decltype(Gk)::value / 2, sg = ((int)decltype(Gk)::value % 2) ? -1 : 1;
+      constexpr uint32_t gm = links_with_c<Q>(a, sg);
+      const uint32_t mine = todo & gm;
+      if (__builtin_amdgcn_ballot_w64(mine != 0) == 0) return;
+      decltype(own(IC<0>{})) r[Q];
+      static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        if constexpr ((gm >> (i - 1)) & 1u) {
+          r[i] = 0;
+          if ((mine >> (i - 1)) & 1u) r[i] = own(IC<Lat<Q>::opp[i]>{});
+        }
+      });
+      static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        if constexpr ((gm >> (i - 1)) & 1u) {  // all loads before any decode
+          auto v = r[i];
+          asm volatile("" : "+v"(v));
+          r[i] = v;
+        }
+      });
+      static_for<Q - 1>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        if constexpr ((gm >> (i - 1)) & 1u)
+          if ((mine >> (i - 1)) & 1u) f[i] = MaskedCodec::template dec<Lat<Q>::opp[i]>(r[i]);
+      });
+      todo &= ~gm;
+    });
     if (t & NODE_WALLMOV) {
       static_for<Q - 1>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
-        const uint8_t ts = src_type(IC<i>{});
-        if ((ts & NODE_SOLID) && (ts & NODE_MOVING)) f[i] = own(IC<Lat<Q>::opp[i]>{}) + moving_term(IC<i>{});
+        if ((lm >> (i - 1)) & 1u)
+          if (src_type(IC<i>{}) & NODE_MOVING) f[i] = f[i] + moving_term(IC<i>{});
       });
     }
     return;
@@ -493,16 +548,16 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rh
   });
 }
 
-// S: the compute scale of the populations (Codec::S; 1 unless shifted).  COPIED: resting-wall
-// values already pulled from the solid slots (wall_links).
-template <int Q, bool SH = false, bool COPIED = false, class C, class SrcType, class Own, class West, class WestType,
+// S: the compute scale of the populations (Codec::S; 1 unless shifted).  MaskedCodec: wall links
+// from the node's link mask lm, own in that codec's storage form (wall_links).
+template <int Q, bool SH = false, class MaskedCodec = void, class C, class SrcType, class Own, class West, class WestType,
           class East, class EastType, class P>
-__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, C *f, C S,
-                                                SrcType &&src_type, Own &&own, West &&west,
+__device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z, uint8_t t, uint32_t lm, C *f,
+                                                C S, SrcType &&src_type, Own &&own, West &&west,
                                                 WestType &&west_type, East &&east, EastType &&east_type) {
   const C rw = (C)p.rho_bc;
-  if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their neighbours' types
-    wall_links<Q, COPIED>(p, f, S, t, src_type, own);
+  if (t & NODE_WALLNB)  // only nodes with a solid pull source look at their links
+    wall_links<Q, MaskedCodec>(p, f, S, t, lm, src_type, own);
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
     const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
@@ -529,32 +584,35 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   }
 }
 
-// 3D PHYSICAL lattices keep their resting-wall bounce-back values in the solid nodes' slots: the
-// step that writes a wall neighbour's populations also writes f_opp(i) into slot i of each solid
-// source x - c_i (the one slot of that node the neighbour pulls), so the next plain pull reads the
-// half-way bounce-back value there.  The copy is the stored value itself (same rounding), so the
-// pulled value equals the node's own f_opp(i) bit for bit.  Each slot has one writer.
+// 3D PHYSICAL lattices take their wall links from a per-node link mask (wall_links, MaskedCodec),
+// and the solid nodes of a wavefront that holds a fluid node store the rest equilibrium f_i = w_i
+// (0 in the shifted storages) every step, in the same store instructions as the fluid nodes: a
+// wavefront whose solid lanes stored nothing would leave every line it writes partly unwritten,
+// and partial lines cost a read-modify-write in memory (a 192^3 cavity with x-walls stepped at
+// 0.63x the rate of one without).  Wavefronts of solid nodes only store nothing.  Every step
+// kernel's wavefront is an aligned block of 64 consecutive node indices, so the rule reads: a
+// solid node takes w_i iff its block holds a non-solid node.  Nothing reads the solid slots: the
+// links that point at them are reloaded.
 template <int Q>
-constexpr bool wall_copies() { return Q >= 19; }
+constexpr bool masked_walls() { return Q >= 19; }
 
 // PHYSICAL-mode pull from global memory.  The periodic pull is issued by every node before its
-// type is known; nodes flagged SLOW then apply the boundary rules.  lm returns the node's wall
-// link mask (3D lattices; 0 for other nodes).
+// type is known; nodes flagged SLOW then apply the boundary rules (wall neighbours of 3D lattices
+// after loading their link mask).
 template <int Q, int ST, class P>
 __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint8_t t,
-                                             typename Codec<ST, Q>::C *f, uint32_t &lm) {
+                                             typename Codec<ST, Q>::C *f) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   pull_periodic<Q, ST>(p, q, f);
-  lm = 0;
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
-  constexpr bool COPIED = wall_copies<Q>();
-  // the mask is used by the stores at the end of the step, not here (no wait on it in the pull)
-  if constexpr (COPIED)
+  constexpr bool MASKED = masked_walls<Q>();
+  uint32_t lm = 0;
+  if constexpr (MASKED)
     if (t & NODE_WALLNB) lm = p.links[q.n];
-  physical_fixups<Q, CD::shifted, COPIED>(
-      p, q.x, q.y, q.z, t, f, (typename CD::C)CD::S,
+  physical_fixups<Q, CD::shifted, std::conditional_t<MASKED, CD, void>>(
+      p, q.x, q.y, q.z, t, lm, f, (typename CD::C)CD::S,
       [&](auto I) -> uint8_t {
         constexpr int i = decltype(I)::value;
         constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
@@ -566,9 +624,10 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
         sz = sz < 0 ? sz + p.nz : (sz >= p.nz ? sz - p.nz : sz);
         return p.type[(uint32_t)sx + (uint32_t)p.nx * ((uint32_t)sy + (uint32_t)p.ny * (uint32_t)sz)];
       },
-      [&](auto J) {
+      [&](auto J) {  // MASKED: the stored form (the wall links decode it)
         constexpr int j = decltype(J)::value;
-        return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n);
+        if constexpr (MASKED) return CD::template raw<j>(fin + (size_t)j * (size_t)p.stride, q.n);
+        else return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n);
       },
       [&](auto J) {
         constexpr int j = decltype(J)::value;
@@ -580,6 +639,17 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
         return CD::template ld<j>(fin + (size_t)j * (size_t)p.stride, q.n + 1);
       },
       [&]() -> uint8_t { return p.type[q.n + 1]; });
+}
+
+// The rest equilibrium in the compute representation: w_i, or 0 for the shifted storages.
+template <int Q, int ST>
+__device__ __forceinline__ void rest_populations(typename Codec<ST, Q>::C *out) {
+  using CD = Codec<ST, Q>;
+  using C = typename CD::C;
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    out[i] = CD::shifted ? (C)0 : (C)Lat<Q>::w[i];
+  });
 }
 
 // ---- moments, overrides, collisions ------------------------------------------------------
@@ -1198,62 +1268,34 @@ __device__ __forceinline__ void collide(const P &p, const C *f, C rho, C ux, C u
 }
 
 // Stores the node's Q populations: out in the compute representation, rnd the node's rounding
-// bits (node_bits; shifted codecs only).  lm: the node's wall link mask (3D PHYSICAL lattices, 0
-// otherwise); each population opp(i) whose link i has a resting solid source is also stored, right
-// after its own store, into slot i of that solid node (wall_copies), q the node.  Wavefronts with
-// no wall neighbour take the loop without the copies.
+// bits (node_bits; shifted codecs only).
 template <int Q, int ST, bool NT = false, class P>
-__device__ __forceinline__ void store_all(const P &p, const NodePos &q, const typename Codec<ST, Q>::C *out,
-                                          uint32_t rnd, uint32_t lm = 0) {
+__device__ __forceinline__ void store_all(const P &p, uint32_t n, const typename Codec<ST, Q>::C *out, uint32_t rnd) {
   using CD = Codec<ST, Q>;
   using T = typename CD::T;
   T *fout = (T *)p.fout;
-  const uint32_t n = q.n;
-  auto store_pops = [&](auto WITH_COPIES) {
-    constexpr bool COPY = decltype(WITH_COPIES)::value != 0;
-    uint32_t off = n * (uint32_t)sizeof(T);
-    // the copies' source offsets are recomputed here (the empty asm keeps the compiler from
-    // holding the pull's through the collision, where registers are scarcest)
-    NodePos qc = q;
-    if constexpr (COPY) asm volatile("" : "+v"(qc.n), "+v"(qc.x), "+v"(qc.y), "+v"(qc.z));
-    static_for<Q>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      // 2-byte storages: the population's (uniform) base plus a 32-bit byte offset, the
-      // scalar-base store form with one offset register for all Q stores (lbm_create keeps these
-      // lattices below 2^31 nodes).  The empty asm hides that the offsets are equal: the compiler
-      // would otherwise form fout + offset once and add each population's base to it in 64-bit
-      // vector arithmetic, one instruction per store.  Wider storages take 64-bit addresses.
-      T *base = fout + (size_t)i * (size_t)p.stride;
-      T *dst = base + n;
-      if constexpr (sizeof(T) == 2) {
-        asm volatile("" : "+v"(off));
-        dst = (T *)((char *)base + off);
-      }
-      const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
-      // NT: non-temporal stores for lattices too large for the caches (the next step reads these
-      // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s
-      // (4-byte) and from 5.32 to 5.62 TB/s (2-byte) on MI355X, and the 3D step kernels by 3-5 %; a
-      // lattice that stays in the caches between steps runs faster with ordinary stores.
-      if constexpr (NT) __builtin_nontemporal_store(v, dst);
-      else *dst = v;
-      if constexpr (COPY && i != 0) {
-        constexpr int l = Lat<Q>::opp[i];  // the link whose bounce-back value this population is
-        if ((lm >> (l - 1)) & 1u) {
-          const PullSources<Q> src(p, qc);
-          T *cp = fout + (size_t)l * (size_t)p.stride + src(IC<l>{});
-          if constexpr (NT) __builtin_nontemporal_store(v, cp);
-          else *cp = v;
-        }
-      }
-    });
-  };
-  if constexpr (wall_copies<Q>()) {
-    if (__builtin_amdgcn_ballot_w64(lm != 0) != 0) {
-      store_pops(IC<1>{});
-      return;
+  uint32_t off = n * (uint32_t)sizeof(T);
+  static_for<Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    // 2-byte storages: the population's (uniform) base plus a 32-bit byte offset, the scalar-base
+    // store form with one offset register for all Q stores (lbm_create keeps these lattices below
+    // 2^31 nodes).  The empty asm hides that the offsets are equal: the compiler would otherwise
+    // form fout + offset once and add each population's base to it in 64-bit vector arithmetic,
+    // one instruction per store.  Wider storages take 64-bit addresses.
+    T *base = fout + (size_t)i * (size_t)p.stride;
+    T *dst = base + n;
+    if constexpr (sizeof(T) == 2) {
+      asm volatile("" : "+v"(off));
+      dst = (T *)((char *)base + off);
     }
-  }
-  store_pops(IC<0>{});
+    const T v = CD::template enc<i>(out[i], pop_bits<i>(rnd));
+    // NT: non-temporal stores for lattices too large for the caches (the next step reads these
+    // populations from HBM anyway).  They raised the bare D3Q27 pull from 5.35 to 5.56 TB/s
+    // (4-byte) and from 5.32 to 5.62 TB/s (2-byte) on MI355X, and the 3D step kernels by 3-5 %; a
+    // lattice that stays in the caches between steps runs faster with ordinary stores.
+    if constexpr (NT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
+  });
 }
 
 // Stores populations given unshifted (initial states), rounded to nearest.

@@ -14,8 +14,8 @@ struct StepParams {
   void *fout;               // populations written
   const uint8_t *type;      // node types (with NODE_SLOW in PHYSICAL mode)
   // PHYSICAL 3D lattices: per node, bit i - 1 set when population i (1 <= i < Q) is pulled from a
-  // solid node (the wall link mask); null otherwise.  Read only by wall-neighbour nodes, to store
-  // their wall copies.
+  // solid node (the wall link mask); null otherwise.  Read only by wall-neighbour nodes, which
+  // reload those links from their own populations.
   const uint32_t *links;
   int32_t nx, ny, nz;
   int32_t gx, gy, gz;       // tile grid: 64 x 4 x 1 nodes per tile (flat lattices: ntiles x 1 x 1)
@@ -148,9 +148,6 @@ inline bool rows_fill_lines(int nx, int esize) {
   return nx % 64 != 0 /* flat tiles: 64 consecutive indices, 64 * esize >= 128 bytes */ ||
          ((int64_t)nx * esize) % 128 == 0;
 }
-// Fills the wall copies (3D PHYSICAL lattices; see wall_copies in lbm_device.hpp) of the buffer
-// p.fout from its fluid populations; nothing to do for other lattices.
-hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s);
 // The node-type block of a lattice: N type bytes, then (wall_links_kept) the 32-bit wall link masks
 // from byte offset wall_links_offset(N).
 inline bool wall_links_kept(int Q, int bc_mode) { return bc_mode == BC_PHYS && Q >= 19; }

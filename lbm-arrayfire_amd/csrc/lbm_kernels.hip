@@ -293,38 +293,6 @@ hipError_t launch_prepare_types(int Q, int bc_mode, int nx, int ny, int nz, cons
   return hipGetLastError();
 }
 
-// Fills the wall copies of a state written from outside the step kernels (initial states,
-// restored populations): slot i of each resting solid source x - c_i of a wall neighbour x takes
-// x's stored population opp(i), in the buffer p.fout, as a step leaves it (store_all).  Reads
-// fluid slots and writes solid ones only.
-template <int Q, int ST>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_prime_wall_copies(StepParams p) {
-  using T = typename Codec<ST, Q>::T;
-  const NodePos q = node_of_thread(p);
-  if constexpr (wall_copies<Q>()) {
-    if (!q.valid || !(p.type[q.n] & NODE_WALLNB)) return;
-    const uint32_t lm = p.links[q.n];
-    T *f = (T *)p.fout;
-    const PullSources<Q> src(p, q);
-    static_for<Q - 1>([&](auto I) {
-      constexpr int i = decltype(I)::value + 1;
-      if ((lm >> (i - 1)) & 1u)
-        f[(size_t)i * (size_t)p.stride + src(IC<i>{})] = f[(size_t)Lat<Q>::opp[i] * (size_t)p.stride + q.n];
-    });
-  }
-}
-
-hipError_t launch_prime_wall_copies(const Config &c, const StepParams &p, hipStream_t s) {
-  return with_storage(c, [&](auto Qc, auto STc) -> hipError_t {
-    constexpr int Q = decltype(Qc)::value;
-    if constexpr (wall_copies<Q>()) {
-      hipLaunchKernelGGL((k_prime_wall_copies<Q, decltype(STc)::value>), dim3(p.ntiles), dim3(BLOCK_THREADS), 0, s, p);
-      return hipGetLastError();
-    }
-    return hipSuccess;
-  });
-}
-
 // Gathers rho/u of selected nodes from the four field arrays (sensors).
 __global__ void k_gather_fields(const double *fields, int64_t N, const int64_t *nodes, int n, double *out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -88,13 +88,14 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
   const uint8_t t = q.valid ? p.type[q.n] : (uint8_t)NODE_SOLID;
   const bool active = q.valid && !(BC == BC_PHYS && (t & NODE_SOLID));
   C f[27];
-  uint32_t lm = 0;  // wall link mask (PHYSICAL wall neighbours)
+  bool any_fluid;
   {
     C rho = 0, ux = 0, uy = 0, uz = 0;
     if (q.valid) {
       if constexpr (BC == BC_REF) pull_periodic<27, ST>(p, q, f);
-      else pull_physical<27, ST>(p, q, t, f, lm);
+      else pull_physical<27, ST>(p, q, t, f);
     }
+    any_fluid = __builtin_amdgcn_ballot_w64(!(t & NODE_SOLID)) != 0;  // in this wavefront
     float row[ROW];
     float dev[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
@@ -246,16 +247,19 @@ __device__ __forceinline__ void mrt_mfma_tile(const Params &p, int tile, float (
     static_for<27>([&](auto I) { out[decltype(I)::value] = f[decltype(I)::value] - src[decltype(I)::value]; });
   }
   __syncthreads();  // the LDS rows are free for the next tile of a persistent workgroup
-  if (!active) return;
-  if constexpr (BC == BC_REF) {
-    if (t & NODE_SOLID) {  // F(REFLECTED) = BOUNCEDBACK, as the scalar kernel
+  if (!q.valid) return;
+  if (t & NODE_SOLID) {
+    if constexpr (BC == BC_REF) {  // F(REFLECTED) = BOUNCEDBACK, as the scalar kernel
       static_for<26>([&](auto J) {
         constexpr int jj = decltype(J)::value + 1;
         out[Lat<27>::opp[jj]] = f[jj];
       });
+    } else {  // the rest equilibrium, stored with the fluid lanes, as the scalar kernel
+      if (!any_fluid) return;
+      rest_populations<27, ST>(out);
     }
   }
-  store_all<27, ST>(p, q, out, SH ? node_bits(q.n, key) : 0u, lm);
+  store_all<27, ST>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
 }
 
 template <int ST, int BC, int ENG>

@@ -150,7 +150,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
       if (decltype(with_fixups)::value && (t & NODE_SLOW)) {
         const uint32_t xy = lxy[pos];
         physical_fixups<Q, SH>(
-            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f, (C)CD::S,
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, 0u, f, (C)CD::S,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
               return lt[pos - Lat<Q>::cx[q] - Lat<Q>::cy[q] * RX];
@@ -229,7 +229,7 @@ __device__ __forceinline__ void multi_tile(const P &p, int gxm, int tile, typena
         const uint8_t t = (uint8_t)(((w >> 28) & 1u ? NODE_INLET : 0) | ((w >> 29) & 1u ? NODE_OUTLET : 0) |
                                     ((w >> 30) & 1u ? NODE_WALLNB : 0));
         physical_fixups<Q, SH>(
-            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, f, (C)CD::S,
+            p, (int)(xy & 0xffffu), (int)(xy >> 16), 0, t, 0u, f, (C)CD::S,
             [&](auto I) -> uint8_t {
               constexpr int q = decltype(I)::value;
               return (uint8_t)(((w >> (10 + q)) & 1u ? NODE_SOLID : 0) | ((w >> (18 + q)) & 1u ? NODE_MOVING : 0));

@@ -24,7 +24,6 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
   // transform (80 fewer additions a node than the separate sums)
   constexpr bool MONO = Q == 27 && CO != CO_BGK && sizeof(C) == 4;
   float mono[MONO ? 27 : 1];
-  uint32_t lm = 0;  // wall link mask (3D PHYSICAL wall neighbours)
   if constexpr (BC == BC_REF) {
     pull_periodic<Q, ST>(p, q, f);
     if constexpr (MONO) {
@@ -48,9 +47,17 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
       });
     }
   } else {
-    pull_physical<Q, ST>(p, q, t, f, lm);
-    if (t & NODE_SOLID) return;  // walls are written only by their neighbours' wall copies
-    if constexpr (MONO) {
+    pull_physical<Q, ST>(p, q, t, f);
+    // the wavefront holds a fluid node (a wavefront is an aligned block of 64 node indices); after
+    // the pull, which is issued before the node type is known
+    const bool any_fluid = __builtin_amdgcn_ballot_w64(!(t & NODE_SOLID)) != 0;
+    if (t & NODE_SOLID) {
+      // 3D: the rest equilibrium, stored with the fluid lanes (masked_walls); a wavefront of solid
+      // nodes only leaves its lines untouched.  2D: not written.
+      if constexpr (!masked_walls<Q>()) return;
+      if (!any_fluid) return;
+      rest_populations<Q, ST>(out);
+    } else if constexpr (MONO) {
       mrt27_monomials(f, mono);
       moments_mono<SH>(mono, drho, rho, ux, uy, uz, CD::S);
       collide_mrt_fp32<CO, SH>(p, f, SH ? rho * CD::S : rho, ux, uy, uz, out, drho, CD::S, mono);
@@ -60,7 +67,7 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
       collide<Q, CO, SH>(p, f, rho, ux, uy, uz, out, drho, (C)CD::S);
     }
   }
-  store_all<Q, ST, NT>(p, q, out, SH ? node_bits(q.n, key) : 0u, lm);
+  store_all<Q, ST, NT>(p, q.n, out, SH ? node_bits(q.n, key) : 0u);
 }
 
 // rho, u the reference holds after a step at node q (cpp/d2q9_bgk_lid.cpp:131-146): moments of the
@@ -80,8 +87,7 @@ __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, dou
       moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
       overrides_s<Q>(p, t, q.y, q.z, drho, rho, ux, uy, uz, CD::S);
     } else if (!(t & NODE_SOLID)) {
-      uint32_t lm;
-      pull_physical<Q, ST>(p, q, t, f, lm);
+      pull_physical<Q, ST>(p, q, t, f);
       moments_s<Q>(f, drho, rho, ux, uy, uz, CD::S);
     }
     out[0] = 1.0 + (double)drho * (1.0 / CD::S);
@@ -91,8 +97,7 @@ __device__ __forceinline__ void fields_of_node(const P &p, const NodePos &q, dou
       moments<Q>(f, rho, ux, uy, uz);
       overrides<Q>(p, t, q.y, q.z, rho, ux, uy, uz);
     } else if (!(t & NODE_SOLID)) {
-      uint32_t lm;
-      pull_physical<Q, ST>(p, q, t, f, lm);
+      pull_physical<Q, ST>(p, q, t, f);
       moments<Q>(f, rho, ux, uy, uz);
     }
     out[0] = (double)rho;

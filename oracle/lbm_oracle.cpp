@@ -26,10 +26,11 @@
 //  * PHYSICAL step      no reference counterpart: half-way / moving-wall bounce-back, velocity
 //                       inlet by Zou-He (non-equilibrium bounce-back) or by non-equilibrium
 //                       extrapolation, zero-gradient outlet at a reference density
-//                       (non-equilibrium extrapolation of every population).  3D lattices store
-//                       the kernels' wall copies: slot i of a solid node s holds population
-//                       opp(i) of its fluid neighbour s + c_i (wall_copies); solid slots with no
-//                       such neighbour keep their values.
+//                       (non-equilibrium extrapolation of every population).  In 3D lattices a
+//                       solid node takes the rest equilibrium w_i each step when its aligned
+//                       block of 64 node indices holds a non-solid node (the kernels' wavefronts
+//                       store whole blocks) and keeps its values otherwise; in 2D lattices solid
+//                       nodes keep their values.  No step reads them.
 //  * Shan-Chen          cpp/d2q9_shanchen.cpp:39-181 with its work-in-progress defects corrected
 //                       (oracle_sc_run); pinned in tests/test_shanchen.py by a numpy restatement,
 //                       the single-fluid reduction at G = 0, conservation, a point-perturbation
@@ -723,14 +724,22 @@ struct Sim {
   }
 
   std::vector<T> Fnext;
+  std::vector<char> block_fluid;  // per aligned block of 64 node indices: holds a non-solid node
   void phys_step() {
     const int Q = L.Q;
     Fnext.resize(F.size());
+    if (block_fluid.empty()) {
+      block_fluid.assign((size_t)((N + 63) / 64), 0);
+      for (int64_t n = 0; n < N; ++n)
+        if (!(type[n] & SOLID)) block_fluid[(size_t)(n / 64)] = 1;
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < N; ++n) {
       const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
       if (type[n] & SOLID) {
-        for (int i = 0; i < Q; ++i) Fnext[(size_t)(n + N * i)] = F[(size_t)(n + N * i)];
+        for (int i = 0; i < Q; ++i)
+          Fnext[(size_t)(n + N * i)] =
+              Q >= 19 && block_fluid[(size_t)(n / 64)] ? (sh ? (T)0 : w[i]) : F[(size_t)(n + N * i)];
         continue;
       }
       T f[27], out[27], rho, ux, uy, uz, drho;
@@ -791,28 +800,6 @@ struct Sim {
     if (d.bc_mode == 0) ref_step();
     else phys_step();
     if (sh) codec_step();
-    wall_copies();
-  }
-
-  // 3D PHYSICAL lattices: slot i of every solid source s = x - c_i of a fluid node x takes x's
-  // population opp(i) (the value x's half-way bounce-back pulls from there next step; equal to
-  // the node's own value, so the step above does not read these slots).
-  std::vector<std::pair<int64_t, int64_t>> copy_list;  // (solid slot, fluid slot), built once
-  bool copy_list_built = false;
-  void wall_copies() {
-    if (d.bc_mode == 0 || L.Q < 19) return;
-    if (!copy_list_built) {
-      for (int64_t n = 0; n < N; ++n) {
-        if (type[n] & SOLID) continue;
-        const int x = (int)(n % nx), y = (int)((n / nx) % ny), z = (int)(n / ((int64_t)nx * ny));
-        for (int i = 1; i < L.Q; ++i) {
-          const int64_t src = idx(wrap(x - L.cx[i], nx), wrap(y - L.cy[i], ny), wrap(z - L.cz[i], nz));
-          if (type[src] & SOLID) copy_list.emplace_back(src + N * i, n + N * L.opp[i]);
-        }
-      }
-      copy_list_built = true;
-    }
-    for (const auto &c : copy_list) F[(size_t)c.first] = F[(size_t)c.second];
   }
 
   // state in/out as doubles; shifted storages hold g and decode to g + w_i in double (exact)
@@ -849,7 +836,6 @@ int run_impl(const oracle_desc *d, const uint8_t *types, double *f, int64_t nste
   Sim<T> sim(*d, types);
   sim.tstep = step0;
   sim.load(f);
-  sim.wall_copies();  // as the library's state setters fill them
   if (d->bc_mode == 0 && !sim.build_pull_map(f32c != 0)) return 3;  // rounded index out of range
   for (int64_t t = 0; t < nsteps; ++t) sim.step();
   sim.store(f);
@@ -871,7 +857,6 @@ int init_fields_impl(const oracle_desc *d, const uint8_t *types, const double *r
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
   sim.encode_initial();
-  sim.wall_copies();
   sim.store(f);
   return 0;
 }
@@ -894,7 +879,6 @@ int init_reference_impl(const oracle_desc *d, const uint8_t *types, double rho0,
     for (int i = 0; i < Q; ++i) sim.F[(size_t)(n + sim.N * i)] = out[i];
   }
   sim.encode_initial();
-  sim.wall_copies();
   sim.store(f);
   return 0;
 }

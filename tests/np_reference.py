@@ -217,31 +217,23 @@ def step_physical(F, types, cfg):
     rho = f.sum(0)
     u = np.einsum("ia,i...->a...", c.astype(float), f) / np.where(solid, 1.0, rho)
     out = collide(f, rho, u, cfg, c, w)
-    return np.where(solid[None], F, out)
-
-
-def wall_copies(F, types, cfg):
-    """3D PHYSICAL lattices: slot i of a solid node s holds population opp(i) of its fluid
-    neighbour s + c_i (the storage convention of the library's 3D kernels; the step itself never
-    reads these slots)."""
-    Q = cfg["lattice"]
-    if cfg.get("bc_mode", 0) == 0 or Q < 19:
-        return F
-    c, _, opp = lattice(Q)
-    solid = (types & SOLID) != 0
-    F = F.copy()
-    for i in range(1, Q):
-        back = (-c[i][2], -c[i][1], -c[i][0])  # out[s] = in[s + c_i]
-        fluid_nb = np.roll(~solid, shift=back, axis=(0, 1, 2))
-        F[i] = np.where(solid & fluid_nb, np.roll(F[opp[i]], shift=back, axis=(0, 1, 2)), F[i])
-    return F
+    # solid nodes: in 3D lattices those whose aligned block of 64 node indices holds a non-solid
+    # node take the rest equilibrium w_i; the others, and those of 2D lattices, keep their values
+    new_solid = F
+    if Q >= 19:
+        flat = solid.reshape(-1)
+        pad = (-flat.size) % 64
+        blocks = np.concatenate([flat, np.ones(pad, bool)]).reshape(-1, 64)
+        fluid_block = np.repeat(~blocks.all(axis=1), 64)[:flat.size].reshape(solid.shape)
+        rest = np.broadcast_to(np.asarray(w)[:, None, None, None], F.shape)
+        new_solid = np.where(fluid_block[None], rest, F)
+    return np.where(solid[None], new_solid, out)
 
 
 def run(F, types, cfg, nsteps):
     step = step_reference if cfg.get("bc_mode", 0) == 0 else step_physical
-    F = wall_copies(F, types, cfg)
     for _ in range(nsteps):
-        F = wall_copies(step(F, types, cfg), types, cfg)
+        F = step(F, types, cfg)
     return F
 
 
