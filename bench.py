@@ -444,6 +444,49 @@ def copy_calibration(mib):
                       "copy_gbs": gbs}))
 
 
+def wall_layouts(side, steps, storage_name):
+    """The cost of walls by layout: one D3Q27 BGK PHYSICAL side^3 lattice per node-type layout (no
+    walls; solid planes x = 0 and x = side-1; x = 0 only; y and z faces; the lid cavity), each
+    stepped 20 times, then 3 x ``steps`` steps timed, best of three.  steps = 0: 10 steps per
+    layout and no timing (the run rocprofv3 counts)."""
+    import numpy as np
+
+    from lbm_amd import Lattice, cases, geometry as geo
+    from lbm_amd.capi import BC_PHYSICAL, BGK, D3Q27, F16S, F32
+
+    S = side
+    st = {"f16s": F16S, "fp32": F32}[storage_name]
+    cfg = dict(lattice=D3Q27, nx=S, ny=S, nz=S, collision=BGK, storage=st, bc_mode=BC_PHYSICAL,
+               omega=cases.omega_from_nu(0.05 * S / 500), u_wall=(0.05, 0.0, 0.0))
+    z, y, x = np.meshgrid(np.arange(S), np.arange(S), np.arange(S), indexing="ij")
+    layouts = {
+        "no_walls": np.zeros((S, S, S), np.uint8),
+        "x_walls": np.where((x == 0) | (x == S - 1), geo.NODE_SOLID, 0).astype(np.uint8),
+        "one_x_wall": np.where(x == 0, geo.NODE_SOLID, 0).astype(np.uint8),
+        "y_z_walls": np.where((y == 0) | (y == S - 1) | (z == 0) | (z == S - 1), geo.NODE_SOLID, 0).astype(np.uint8),
+        "lid_cavity": geo.lid_cavity_3d_physical(S, S, S),
+    }
+    out = {}
+    for name, t in layouts.items():
+        print(f"wall layout {name}, {S}^3 {storage_name}", file=sys.stderr, flush=True)
+        with Lattice(cfg, t.reshape(-1)) as lat:
+            lat.init_equilibrium(1.0, (0.0, 0.0, 0.0), 3, 1e-4)
+            if steps == 0:
+                lat.step(10)
+                lat.sync()
+                continue
+            lat.step(20)
+            lat.sync()
+            best = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                lat.step(steps)
+                lat.sync()
+                best = min(best, time.perf_counter() - t0)
+            out[name] = round(S ** 3 * steps / best / 1e6, 1)
+    return {"wall_layouts_mlups": out, "side": S, "storage": storage_name, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,7 +516,15 @@ def main():
                     help="only run 6 plain streaming copies of MIB MiB (out = -in, float32): the "
                          "known-size kernel whose FETCH_SIZE / WRITE_SIZE counters scale the step "
                          "kernel's counters to bytes")
+    ap.add_argument("--wall-layouts", type=int, default=0, metavar="SIDE",
+                    help="only time D3Q27 BGK PHYSICAL SIDE^3 lattices with five wall layouts "
+                         "(--wall-steps 0: ten untimed steps each, for counters)")
+    ap.add_argument("--wall-steps", type=int, default=100)
+    ap.add_argument("--wall-storage", choices=["f16s", "fp32"], default="f16s")
     args = ap.parse_args()
+    if args.wall_layouts > 0:
+        print(json.dumps(wall_layouts(args.wall_layouts, args.wall_steps, args.wall_storage)))
+        return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
