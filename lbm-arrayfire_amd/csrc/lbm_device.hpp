@@ -403,13 +403,12 @@ __device__ __forceinline__ void nebb_close(C *f, C rho, C ux, C uy, C uz) {
 //   east(IC<j>)      population j of the node at x + 1;  east_type()  its type
 // Half-way bounce-back of the links whose source is solid: f_i = f_opp(i) of this node, plus
 // 6 w_i rho_w c_i.u_w when the source is a moving wall (rho_w taken in compute units, S rho_bc).
-// MaskedCodec (not void: 3D lattices stepped from global memory, whose storage codec it is): lm,
-// the node's wall link mask, marks the links
-// whose source is solid (bit i - 1 for population i), so no source type is read for a resting
-// wall.  The masked links reload the node's own populations in place of what the periodic pull
-// read from the solid slots (those slots hold the rest equilibrium); moving-wall links, on nodes
-// of type NODE_WALLMOV (t) only, are told apart through src_type.  Unmasked (D2Q9): every link's
-// source type is read.
+// MaskedCodec (not void: lattices stepped from global memory, whose storage codec it is): lm, the
+// node's wall link mask, marks the links whose source is solid (bit i - 1 for population i), so
+// no source type is read for a resting wall.  The masked links reload the node's own populations
+// in place of what the periodic pull read from the solid slots; moving-wall links, on nodes of
+// type NODE_WALLMOV (t) only, are told apart through src_type.  Unmasked (the multi-step kernel's
+// LDS tile): every link's source type is read.
 template <int Q>
 constexpr uint32_t links_with_c(int a, int sg) {  // link mask bits (i - 1) of the directions with c_a = sg
   uint32_t m = 0;
@@ -439,6 +438,7 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, uin
     static_for<6>([&](auto Gk) {
       constexpr int a = (int)decltype(Gk)::value / 2, sg = ((int)decltype(Gk)::value % 2) ? -1 : 1;
       constexpr uint32_t gm = links_with_c<Q>(a, sg);
+      if constexpr (gm == 0) return;  // D2Q9: no c_z
       const uint32_t mine = todo & gm;
       if (__builtin_amdgcn_ballot_w64(mine != 0) == 0) return;
       decltype(own(IC<0>{})) r[Q];
@@ -584,8 +584,9 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
   }
 }
 
-// 3D PHYSICAL lattices take their wall links from a per-node link mask (wall_links, MaskedCodec),
-// and the solid nodes of a wavefront that holds a fluid node store the rest equilibrium f_i = w_i
+// PHYSICAL lattices stepped from global memory take their wall links from a per-node link mask
+// (wall_links, MaskedCodec).  In 3D lattices the solid nodes of a wavefront that holds a fluid
+// node also store the rest equilibrium f_i = w_i
 // (0 in the shifted storages) every step, in the same store instructions as the fluid nodes: a
 // wavefront whose solid lanes stored nothing would leave every line it writes partly unwritten,
 // and partial lines cost a read-modify-write in memory (a 192^3 cavity with x-walls stepped at
@@ -594,11 +595,11 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
 // solid node takes w_i iff its block holds a non-solid node.  Nothing reads the solid slots: the
 // links that point at them are reloaded.
 template <int Q>
-constexpr bool masked_walls() { return Q >= 19; }
+constexpr bool rest_at_solids() { return Q >= 19; }
 
 // PHYSICAL-mode pull from global memory.  The periodic pull is issued by every node before its
-// type is known; nodes flagged SLOW then apply the boundary rules (wall neighbours of 3D lattices
-// after loading their link mask).
+// type is known; nodes flagged SLOW then apply the boundary rules (wall neighbours after loading
+// their link mask).
 template <int Q, int ST, class P>
 __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint8_t t,
                                              typename Codec<ST, Q>::C *f) {
@@ -607,10 +608,9 @@ __device__ __forceinline__ void pull_physical(const P &p, const NodePos &q, uint
   pull_periodic<Q, ST>(p, q, f);
   if (!(t & NODE_SLOW)) return;
   const T *fin = (const T *)p.fin;
-  constexpr bool MASKED = masked_walls<Q>();
+  constexpr bool MASKED = true;  // every PHYSICAL lattice keeps its link masks (wall_links_kept)
   uint32_t lm = 0;
-  if constexpr (MASKED)
-    if (t & NODE_WALLNB) lm = p.links[q.n];
+  if (t & NODE_WALLNB) lm = p.links[q.n];
   physical_fixups<Q, CD::shifted, std::conditional_t<MASKED, CD, void>>(
       p, q.x, q.y, q.z, t, lm, f, (typename CD::C)CD::S,
       [&](auto I) -> uint8_t {

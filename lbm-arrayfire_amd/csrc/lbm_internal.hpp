@@ -13,7 +13,7 @@ struct StepParams {
   const void *fin;          // populations read (post-collision of the previous step)
   void *fout;               // populations written
   const uint8_t *type;      // node types (with NODE_SLOW in PHYSICAL mode)
-  // PHYSICAL 3D lattices: per node, bit i - 1 set when population i (1 <= i < Q) is pulled from a
+  // PHYSICAL lattices: per node, bit i - 1 set when population i (1 <= i < Q) is pulled from a
   // solid node (the wall link mask); null otherwise.  Read only by wall-neighbour nodes, which
   // reload those links from their own populations.
   const uint32_t *links;
@@ -150,7 +150,7 @@ inline bool rows_fill_lines(int nx, int esize) {
 }
 // The node-type block of a lattice: N type bytes, then (wall_links_kept) the 32-bit wall link masks
 // from byte offset wall_links_offset(N).
-inline bool wall_links_kept(int Q, int bc_mode) { return bc_mode == BC_PHYS && Q >= 19; }
+inline bool wall_links_kept(int Q, int bc_mode) { return (void)Q, bc_mode == BC_PHYS; }
 inline size_t wall_links_offset(int64_t N) { return ((size_t)N + 255) & ~(size_t)255; }
 inline size_t types_block_bytes(int Q, int bc_mode, int64_t N) {
   return wall_links_kept(Q, bc_mode) ? wall_links_offset(N) + 4 * (size_t)N : (size_t)N;

@@ -52,9 +52,9 @@ __device__ __forceinline__ void step_tile(const P &p, int tile) {
     // the pull, which is issued before the node type is known
     const bool any_fluid = __builtin_amdgcn_ballot_w64(!(t & NODE_SOLID)) != 0;
     if (t & NODE_SOLID) {
-      // 3D: the rest equilibrium, stored with the fluid lanes (masked_walls); a wavefront of solid
-      // nodes only leaves its lines untouched.  2D: not written.
-      if constexpr (!masked_walls<Q>()) return;
+      // 3D: the rest equilibrium, stored with the fluid lanes (rest_at_solids); a wavefront of
+      // solid nodes only leaves its lines untouched.  2D: not written.
+      if constexpr (!rest_at_solids<Q>()) return;
       if (!any_fluid) return;
       rest_populations<Q, ST>(out);
     } else if constexpr (MONO) {
