@@ -146,8 +146,9 @@ int lbm_abi_version(void);
 lbm_status lbm_set_device(int device);
 /* Thread-local message of the last failed call ("" if none). */
 const char *lbm_last_error(void);
-/* Device bytes a lattice with this descriptor holds (both population buffers, node types,
- * monitor scratch).  Used for admission by HBM footprint. */
+/* Device bytes a lattice with this descriptor holds (both population buffers, node types, in
+ * PHYSICAL mode the 4-byte wall link masks, monitor scratch).  Used for admission by HBM
+ * footprint. */
 size_t lbm_footprint_bytes(const lbm_desc *d);
 
 /* ---- one lattice ------------------------------------------------------------------------ */
