@@ -361,6 +361,10 @@ template <int Q>
 __device__ __forceinline__ float m15_of(float ux, float uy, float uz);
 template <int Q, int i>
 __device__ __forceinline__ float geq_i(float drho, float rho, float ux, float uy, float uz, float m15);
+template <int Q, int i>
+__device__ __forceinline__ float peq_i(float ux, float uy, float uz, float m15);
+template <int Q, int i, class C>
+__device__ __forceinline__ C qeq_i(C ux, C uy, C uz, C usq);
 
 // Non-equilibrium bounce-back closure of the unknowns (c_x == SGN) of an x-face node with
 // prescribed rho, u; transverse momentum corrected through N_a = sum_{c_x=0} f c_a - (2/3) rho u_a
@@ -489,8 +493,10 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, uin
 // Non-equilibrium extrapolation (Guo, Zheng & Shi, Chinese Physics 11 (2002) 366) from a
 // neighbour whose populations nbr(IC<j>) the node would have pulled had the closure not replaced
 // them: f_i = f_eq,i(rho_b, u_b) + (g_i - f_eq,i(rho_g, u_g)) with g the neighbour's populations
-// and rho_g, u_g their moments.  rho_b < 0 takes rho_b = rho_g (velocity inlet), otherwise u_b =
-// u_g (pressure outlet: f_eq is linear in rho, so g_i + (rho_b - rho_g) f_eq,i(1, u_g), one FMA in
+// and rho_g, u_g their moments.  rho_b < 0 takes rho_b = rho_g (velocity inlet: the two equilibria
+// share w_i rho_g, and their difference is taken inside the bracket, g_i + w_i rho_g (qeq_i(u_b) -
+// qeq_i(u_g)), about half the arithmetic of the two full equilibria), otherwise u_b = u_g
+// (pressure outlet: f_eq is linear in rho, so g_i + (rho_b - rho_g) f_eq,i(1, u_g), one FMA in
 // FP32).  Without a fluid neighbour the node takes f_eq(rho_b, u_b) (rho_b = rho_bc).
 // SH: populations in the shifted representation (f - w_i, in compute units S); the same rules,
 // with the equilibria as deviations from w_i and the density differences taken between density
@@ -506,13 +512,21 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rh
   if constexpr (SH) {
     const float drbc = (float)(p.rho_bc - 1.0) * S, rbc = (float)p.rho_bc * S;
     if (set_u) {  // velocity inlet
-      const float rb = nb ? rn * S : rbc, drb = nb ? drn : drbc;
-      const float mb = m15_of<Q>(ubx, 0.0f, 0.0f), mg = m15_of<Q>(vx, vy, vz);
-      static_for<Q>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        const float eb = geq_i<Q, i>(drb, rb, ubx, 0.0f, 0.0f, mb);
-        f[i] = nb ? g[i] + (eb - geq_i<Q, i>(drn, rn * S, vx, vy, vz, mg)) : eb;
-      });
+      const float mb = m15_of<Q>(ubx, 0.0f, 0.0f);
+      if (nb) {
+        // rho_b = rho_g: the density deviations cancel, f_i = g_i + w_i rho_g (P_b,i - P_g,i)
+        const float rs = rn * S, mg = m15_of<Q>(vx, vy, vz);
+        static_for<Q>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          const float d = peq_i<Q, i>(ubx, 0.0f, 0.0f, mb) - peq_i<Q, i>(vx, vy, vz, mg);
+          f[i] = __builtin_fmaf((float)Lat<Q>::w[i] * rs, d, g[i]);
+        });
+      } else {
+        static_for<Q>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          f[i] = geq_i<Q, i>(drbc, rbc, ubx, 0.0f, 0.0f, mb);
+        });
+      }
       return;
     }
     const float usq = usq_of<Q>(vx, vy, vz), mg = m15_of<Q>(vx, vy, vz);
@@ -525,13 +539,21 @@ __device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rh
     return;
   }
   if (set_u) {  // velocity inlet: u_b = (ubx, 0, 0), rho_b = rho_g
-    const C rb = nb ? rn : (C)p.rho_bc;
-    const C us_b = usq_of<Q>(ubx, (C)0, (C)0), us_g = usq_of<Q>(vx, vy, vz);
-    static_for<Q>([&](auto I) {
-      constexpr int i = decltype(I)::value;
-      const C eb = feq_i<Q, i>(rb, ubx, (C)0, (C)0, us_b);
-      f[i] = nb ? g[i] + (eb - feq_i<Q, i>(rn, vx, vy, vz, us_g)) : eb;
-    });
+    const C us_b = usq_of<Q>(ubx, (C)0, (C)0);
+    if (nb) {  // f_i = g_i + w_i rho_g (qeq_i(u_b) - qeq_i(u_g))
+      const C us_g = usq_of<Q>(vx, vy, vz);
+      static_for<Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        const C d = qeq_i<Q, i>(ubx, (C)0, (C)0, us_b) - qeq_i<Q, i>(vx, vy, vz, us_g);
+        if constexpr (sizeof(C) == 4) f[i] = __builtin_fmaf((C)Lat<Q>::w[i] * rn, d, g[i]);
+        else f[i] = g[i] + ((C)Lat<Q>::w[i] * rn) * d;
+      });
+    } else {
+      static_for<Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        f[i] = feq_i<Q, i>((C)p.rho_bc, ubx, (C)0, (C)0, us_b);
+      });
+    }
     return;
   }
   const C usq = usq_of<Q>(vx, vy, vz);
@@ -758,17 +780,22 @@ __device__ __forceinline__ C cdotu(C ux, C uy, C uz) {
 // term by term (cpp/d2q9_bgk_lid.cpp:149-151); FP32 uses the multiply-add form
 // w_i rho (A + eu (3 + 4.5 eu)) with A = 1 - 1.5 u.u, three FMAs per direction.  usq_of returns u.u
 // in FP64 and A in FP32.
+// The bracket of f_eq,i: f_eq,i = (w_i rho) qeq_i.
 template <int Q, int i, class C>
-__device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq) {
+__device__ __forceinline__ C qeq_i(C ux, C uy, C uz, C usq) {
   const C eu = cdotu<Q, i>(ux, uy, uz);
   if constexpr (sizeof(C) == 4 && Lat<Q>::cx[i] == 0 && Lat<Q>::cy[i] == 0 && Lat<Q>::cz[i] == 0) {
-    return ((C)Lat<Q>::w[i] * rho) * usq;  // eu = 0: the FMAs below return A exactly
+    return usq;  // eu = 0: the FMAs below return A exactly
   } else if constexpr (sizeof(C) == 4) {
     const C t = __builtin_fmaf(4.5f, eu, 3.0f);
-    return ((C)Lat<Q>::w[i] * rho) * __builtin_fmaf(eu, t, usq);
+    return __builtin_fmaf(eu, t, usq);
   } else {
-    return ((C)Lat<Q>::w[i] * rho) * ((((C)1 + (C)3 * eu) + (C)4.5 * (eu * eu)) - (C)1.5 * usq);
+    return (((C)1 + (C)3 * eu) + (C)4.5 * (eu * eu)) - (C)1.5 * usq;
   }
+}
+template <int Q, int i, class C>
+__device__ __forceinline__ C feq_i(C rho, C ux, C uy, C uz, C usq) {
+  return ((C)Lat<Q>::w[i] * rho) * qeq_i<Q, i>(ux, uy, uz, usq);
 }
 
 template <int Q, class C>
@@ -794,15 +821,17 @@ __device__ __forceinline__ float m15_of(float ux, float uy, float uz) {
   return -1.5f * u2;
 }
 template <int Q, int i>
-__device__ __forceinline__ float geq_i(float drho, float rho, float ux, float uy, float uz, float m15) {
-  float P;
+__device__ __forceinline__ float peq_i(float ux, float uy, float uz, float m15) {  // P_i
   if constexpr (Lat<Q>::cx[i] == 0 && Lat<Q>::cy[i] == 0 && Lat<Q>::cz[i] == 0) {
-    P = m15;
+    return m15;
   } else {
     const float eu = cdotu<Q, i>(ux, uy, uz);
-    P = __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), m15);
+    return __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), m15);
   }
-  return (float)Lat<Q>::w[i] * __builtin_fmaf(rho, P, drho);
+}
+template <int Q, int i>
+__device__ __forceinline__ float geq_i(float drho, float rho, float ux, float uy, float uz, float m15) {
+  return (float)Lat<Q>::w[i] * __builtin_fmaf(rho, peq_i<Q, i>(ux, uy, uz, m15), drho);
 }
 
 // F = omega f_eq + (1 - omega) f  (cpp/d2q9_bgk_lid.cpp:149-153); FP32: f + omega (f_eq - f);

@@ -392,16 +392,18 @@ struct Sim {
       return u2;
     }
   }
-  T feq(int i, T rho, T ux, T uy, T uz, T usq) const {
+  // the bracket of f_eq,i = (w_i rho) qeq
+  T qeq(int i, T ux, T uy, T uz, T usq) const {
     T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
     if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
     if constexpr (std::is_same<T, float>::value) {
       const T t = std::fma(4.5f, eu, 3.0f);
-      return (w[i] * rho) * std::fma(eu, t, usq);
+      return std::fma(eu, t, usq);
     } else {
-      return (w[i] * rho) * ((((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq);
+      return (((T)1 + (T)3 * eu) + (T)4.5 * (eu * eu)) - (T)1.5 * usq;
     }
   }
+  T feq(int i, T rho, T ux, T uy, T uz, T usq) const { return (w[i] * rho) * qeq(i, ux, uy, uz, usq); }
 
   // Shifted equilibrium (FP32 only): f_eq,i - w_i = w_i (drho + rho P_i), P_i = eu (3 + 4.5 eu)
   // - 1.5 u.u; m15 = -1.5 u.u.
@@ -410,14 +412,17 @@ struct Sim {
     if (L.D == 3) u2 = std::fma(uz, uz, u2);
     return (T)-1.5 * u2;
   }
-  T geq(int i, T drho, T rho, T ux, T uy, T uz, T m15) const {
+  T peq(int i, T ux, T uy, T uz, T m15) const {  // P_i
     T P = m15;
     if (L.cx[i] != 0 || L.cy[i] != 0 || L.cz[i] != 0) {
       T eu = (T)L.cx[i] * ux + (T)L.cy[i] * uy;
       if (L.D == 3) eu = eu + (T)L.cz[i] * uz;
       P = std::fma(eu, std::fma((T)4.5, eu, (T)3), m15);
     }
-    return w[i] * std::fma(rho, P, drho);
+    return P;
+  }
+  T geq(int i, T drho, T rho, T ux, T uy, T uz, T m15) const {
+    return w[i] * std::fma(rho, peq(i, ux, uy, uz, m15), drho);
   }
   void bgk_s(const T *S, T drho, T rho, T ux, T uy, T uz, T *out) const {
     const T omega = (T)d.omega, m = m15_term(ux, uy, uz);
@@ -656,7 +661,9 @@ struct Sim {
       // Extrapolated velocity inlet (Guo, Zheng & Shi, Chinese Physics 11 (2002) 366): every
       // population f_eq(rho_e, u_in) + (g - f_eq(rho_e, u_e)), g = the downstream (x + 1)
       // neighbour's populations as the pull reads them, rho_e and u_e their moments; f_eq(rho_bc,
-      // u_in) when that neighbour is solid or absent.  The node then holds rho_e and u_in.
+      // u_in) when that neighbour is solid or absent.  The node then holds rho_e and u_in.  The
+      // two equilibria share w_i rho_e, so their difference is formed inside the bracket:
+      // g_i + w_i rho_e (qeq_i(u_in) - qeq_i(u_e)) (shifted: P_i in place of qeq_i).
       const T uin = inlet_u(y, z);
       T g[27], rn = (T)1, drn = 0, vx = 0, vy = 0, vz = 0;
       const bool nb = x + 1 < nx && !(type[n + 1] & SOLID);
@@ -666,19 +673,21 @@ struct Sim {
         else moments(g, rn, vx, vy, vz);
       }
       if (sh) {  // the same with the equilibria as deviations from w_i
-        const T rb = nb ? rn : rw, drb = nb ? drn : (T)(d.rho_bc - 1.0);
         const T mb = m15_term(uin, (T)0, (T)0), mg = m15_term(vx, vy, vz);
-        for (int i = 0; i < L.Q; ++i) {
-          const T eb = geq(i, drb, rb, uin, (T)0, (T)0, mb);
-          f[i] = nb ? g[i] + (eb - geq(i, drn, rn, vx, vy, vz, mg)) : eb;
-        }
+        for (int i = 0; i < L.Q; ++i)
+          f[i] = nb ? std::fma(w[i] * rn, peq(i, uin, (T)0, (T)0, mb) - peq(i, vx, vy, vz, mg), g[i])
+                    : geq(i, (T)(d.rho_bc - 1.0), rw, uin, (T)0, (T)0, mb);
         return;
       }
-      const T rb = nb ? rn : rw;
       const T us_b = usq_term(uin, (T)0, (T)0), us_g = usq_term(vx, vy, vz);
       for (int i = 0; i < L.Q; ++i) {
-        const T eb = feq(i, rb, uin, (T)0, (T)0, us_b);
-        f[i] = nb ? g[i] + (eb - feq(i, rn, vx, vy, vz, us_g)) : eb;
+        if (!nb) {
+          f[i] = feq(i, rw, uin, (T)0, (T)0, us_b);
+          continue;
+        }
+        const T dq = qeq(i, uin, (T)0, (T)0, us_b) - qeq(i, vx, vy, vz, us_g);
+        if constexpr (std::is_same<T, float>::value) f[i] = std::fma(w[i] * rn, dq, g[i]);
+        else f[i] = g[i] + (w[i] * rn) * dq;
       }
     } else if (t & INLET) {  // velocity inlet (Zou-He), unknowns c_x = +1
       const T uin = inlet_u(y, z);
