@@ -68,8 +68,10 @@ typedef enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16S = 2, LBM_BF16S = 3 } lbm_stora
  * (cpp/d2q9_bgk_lid.cpp:94-98,141-155; rust/examples/d2q9_bgk_channel.rs:190-201).
  * PHYSICAL: half-way (and moving-wall) bounce-back at links into solid nodes, non-equilibrium
  * bounce-back (Zou-He) velocity inlet, zero-gradient outlet at a reference density (no reference
- * counterpart).  Solid nodes' populations play no part in a PHYSICAL step: each step of a 3D
- * lattice sets them to the rest equilibrium f_i = w_i, and 2D lattices leave them as they are. */
+ * counterpart).  Solid nodes' populations play no part in a PHYSICAL step.  Each step of a 3D
+ * lattice sets a solid node to the rest equilibrium f_i = w_i when its aligned block of 64 node
+ * indices holds a non-solid node (the kernels write whole blocks); other solid nodes, and those
+ * of 2D lattices, keep their values. */
 typedef enum { LBM_BC_REFERENCE = 0, LBM_BC_PHYSICAL = 1 } lbm_bc_mode;
 
 /* How the MRT moment transforms are evaluated.  The MFMA engines evaluate the collision in
