@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 from lbm_amd import Batch, Lattice, LbmError, Server, cases, geometry as geo
-from lbm_amd.capi import BF16S, F16S, F32, F64, MRT_GS, MRT_MFMA_BF16X3, MRT_MFMA_F32, MRT_MFMA_FUSED, MRT_SCALAR
+from lbm_amd.capi import (BC_PHYSICAL, BF16S, BGK, D3Q27, F16S, F32, F64, MRT_GS, MRT_MFMA_BF16X3, MRT_MFMA_F32,
+                          MRT_MFMA_FUSED, MRT_SCALAR)
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +38,14 @@ def member_specs():
     specs.append((cfg, t, ("ref",)))
     cfg, t = cases.d3q19_sphere_channel(40, 20, 20, (12, 10, 10), 4, re=10.0, storage=F16S, collision=MRT_GS)
     specs.append((cfg, t, ("eq", (0.05, 0, 0), 13, 1e-4)))
+    # PHYSICAL lid cavities 64 nodes wide (64 x 4 tiles, wavefronts of solid and fluid lanes, and
+    # of solid lanes only): BGK FP32, and MRT-GS F16S on the fused MFMA engine
+    om = cases.omega_from_nu(0.05 * 62 / 100.0)
+    lid = geo.lid_cavity_3d_physical(64, 20, 12)
+    base = dict(lattice=D3Q27, nx=64, ny=20, nz=12, bc_mode=BC_PHYSICAL, omega=om, u_wall=(0.05, 0.0, 0.0))
+    specs.append((dict(base, collision=BGK, storage=F32), lid, ("eq", (0.0, 0, 0), 17, 1e-4)))
+    specs.append((dict(base, collision=MRT_GS, storage=F16S, s=cases.shear_mrt_rates(om), mrt_engine=MRT_MFMA_FUSED),
+                  lid, ("eq", (0.0, 0, 0), 19, 1e-4)))
     return specs
 
 
