@@ -501,11 +501,19 @@ __device__ __forceinline__ void wall_links(const P &p, C *f, C S, uint8_t t, uin
 // SH: populations in the shifted representation (f - w_i, in compute units S); the same rules,
 // with the equilibria as deviations from w_i and the density differences taken between density
 // deviations (all S-scaled).
-template <int Q, bool SH = false, class C, class Nbr, class P>
-__device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool nb, C rho_b, bool set_u, C ubx, Nbr &&nbr) {
+template <int Q, bool SH = false, class C, class Nbr, class NbrType, class P>
+__device__ __forceinline__ void extrapolate(const P &p, C *f, C S, bool has_nb, C rho_b, bool set_u, C ubx, Nbr &&nbr,
+                                            NbrType &&nbr_type) {
   C g[Q], rn = 1, drn = 0, vx = 0, vy = 0, vz = 0;
-  if (nb) {
+  bool nb = false;  // the neighbour exists and is not solid
+  if (has_nb) {
+    // its type and its populations are read together, one round trip (the populations are in
+    // bounds whatever the type; a solid neighbour's go unused)
+    const uint8_t nt = nbr_type();
     static_for<Q>([&](auto I) { g[decltype(I)::value] = nbr(I); });
+    nb = !(nt & NODE_SOLID);
+  }
+  if (nb) {
     if constexpr (SH) moments_s<Q>(g, drn, rn, vx, vy, vz, S);
     else moments<Q>(g, rn, vx, vy, vz);
   }
@@ -582,8 +590,7 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
     wall_links<Q, MaskedCodec>(p, f, S, t, lm, src_type, own);
   if ((t & NODE_INLET) && p.inlet_closure == 1) {
     // extrapolated velocity inlet: the node holds rho of its downstream neighbour and u_in
-    const bool nb = x + 1 < p.nx && !(east_type() & NODE_SOLID);
-    extrapolate<Q, SH>(p, f, S, nb, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east);
+    extrapolate<Q, SH>(p, f, S, x + 1 < p.nx, (C)-1, true, inlet_velocity<C>(p, Lat<Q>::D, y, z), east, east_type);
   } else if (t & NODE_INLET) {
     const C uin = inlet_velocity<C>(p, Lat<Q>::D, y, z);
     C s0 = (C)-0.0, sm = (C)-0.0;  // sums from -0, as in moments()
@@ -601,8 +608,7 @@ __device__ __forceinline__ void physical_fixups(const P &p, int x, int y, int z,
     // zero-gradient outlet by non-equilibrium extrapolation (Guo, Zheng & Shi 2002), all i:
     // f_i = f_eq,i(rho_bc, u_nb) + (g_i - f_eq,i(rho_nb, u_nb)), g = the upstream neighbour's
     // post-collision populations; the node then holds rho_bc and u_nb exactly.
-    const bool nb = x > 0 && !(west_type() & NODE_SOLID);
-    extrapolate<Q, SH>(p, f, S, nb, rw, false, (C)0, west);
+    extrapolate<Q, SH>(p, f, S, x > 0, rw, false, (C)0, west, west_type);
   }
 }
 
